@@ -1,0 +1,702 @@
+/*
+ * mhd2d_oracle.c -- TEST INFRASTRUCTURE ONLY (see mhd2d_oracle.h).
+ *
+ * Plain C99, FP64, single thread.  A restatement, written for this project,
+ * of the reference's hot path so that the HIP build can be checked against
+ * it.  The reference itself cannot be built in this image (it needs FFTW3,
+ * which is absent), so this file is pinned by the tests in
+ * tests/test_oracle_*.py against closed forms, brute-force dense solves,
+ * numpy.fft and invariants instead.
+ *
+ * Citations are file:line in chenshihelio/MHD2DCFDFFTW_Open.
+ */
+#include "mhd2d_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_PI 3.14159265358979323846264338327950288
+
+/* field indices, macros.h:4-11 */
+enum { RHO = 0, UX, UY, UZ, BX, BY, BZ, PR };
+
+/* ------------------------------------------------------------------------ */
+/* coefficients: initialize.c:572-631 with the alphas of parameter.h:24-34   */
+/* ------------------------------------------------------------------------ */
+void orc_coefs_init(orc_coefs *c, double alpha1, double alpha2,
+                    double alphaF, double alphaSF)
+{
+    double s1 = 0.0, s2 = 0.0;  /* ALPHASUB1 (parameter.h:26); initialize.c:33 */
+    c->alpha1 = alpha1;
+    c->a1 = (alpha1 + 9.) / 6.;
+    c->b1 = (32. * alpha1 - 9.) / 15.;
+    c->c1 = (-3. * alpha1 + 1.) / 10.;
+    c->alphaS1 = s1;
+    c->aS1 = (1. - s1) / 12.;
+    c->bS1 = (3. * s1 - 4.) / 6.;
+    c->cS1 = -1.5 * s1;
+    c->dS1 = (4. + 5. * s1) / 6.;
+    c->eS1 = (3. * s1 - 1.) / 12.;
+
+    c->alpha2 = alpha2;
+    c->a2 = (6. - 9. * alpha2) / 4.;
+    c->b2 = (-3. + 24. * alpha2) / 5.;
+    c->c2 = (2. - 11. * alpha2) / 20.;
+    c->alphaS2 = s2;
+    c->aS2 = -(1. + s2) / 12.;
+    c->bS2 = (s2 + 4.) / 3.;
+    c->cS2 = (s2 - 5.) / 2.;
+    c->dS2 = (4. - 5. * s2) / 3.;
+    c->eS2 = (11. * s2 - 1.) / 12.;
+
+    c->alphaF = alphaF;
+    c->aF = (11. + 10. * alphaF) / 16.;
+    c->bF = (15. + 34. * alphaF) / 32.;
+    c->cF = (-3. + 6. * alphaF) / 16.;
+    c->dF = (1. - 2. * alphaF) / 32.;
+    c->alphaSF = alphaSF;
+    c->aSF = (alphaSF - 1.) / 16.;
+    c->bSF = (1. - alphaSF) / 4.;
+    c->cSF = (3. * alphaSF + 5.) / 8.;
+    c->dSF = (3. * alphaSF + 1.) / 4.;
+    c->eSF = (alphaSF - 1.) / 16.;
+}
+
+void orc_coefs_default(orc_coefs *c)
+{
+    orc_coefs_init(c, 0.375, 0.375, 0.49, 0.95);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Thomas algorithm without pivoting, mathlib.c:8-30                          */
+/* lo[i] multiplies x[i] in row i+1; up[i] multiplies x[i+1] in row i.       */
+/* ------------------------------------------------------------------------ */
+void orc_thomas(int n, const double *lo, const double *diag, const double *up,
+                const double *rhs, double *x)
+{
+    double *b = (double *)malloc(sizeof(double) * n);
+    double *r = (double *)malloc(sizeof(double) * n);
+    memcpy(b, diag, sizeof(double) * n);
+    memcpy(r, rhs, sizeof(double) * n);
+    for (int i = 1; i < n; i++) {
+        double m = lo[i - 1] / b[i - 1];
+        b[i] = b[i] - m * up[i - 1];
+        r[i] = r[i] - m * r[i - 1];
+    }
+    x[n - 1] = r[n - 1] / b[n - 1];
+    for (int i = n - 2; i >= 0; i--)
+        x[i] = (r[i] - up[i] * x[i + 1]) / b[i];
+    free(b);
+    free(r);
+}
+
+/* periodic-branch matrices: initialize.c:633-654 (1st), 710-731 (2nd),
+ * 788-808 (filter): unit diagonal, alpha inside, alphaSub on the two
+ * couplings of rows 0 and N-1. */
+static void tri_periodic(int n, double alpha, double alphaSub,
+                         double *lo, double *diag, double *up)
+{
+    for (int i = 0; i < n; i++) diag[i] = 1.0;
+    for (int i = 0; i < n - 2; i++) lo[i] = alpha;
+    lo[n - 2] = alphaSub;
+    up[0] = alphaSub;
+    for (int i = 1; i < n - 1; i++) up[i] = alpha;
+}
+
+/* value of the line at index i in [-2, N+1], halos outside [0, N) */
+static inline double at(const double *A, const double *H0, const double *H1,
+                        int N, int i)
+{
+    if (i < 0) return H0[i + 2];         /* H0[0] = f[-2], H0[1] = f[-1] */
+    if (i >= N) return H1[i - N];        /* H1[0] = f[N],  H1[1] = f[N+1] */
+    return A[i];
+}
+
+/* derivCFD1st, periodic branch, mathlib.c:46-86 */
+void orc_line_deriv1_c(const orc_coefs *c, int N, double DX, const double *A,
+                       const double *H0, const double *H1, double *X)
+{
+    double *B = (double *)malloc(sizeof(double) * N);
+    double *lo = (double *)malloc(sizeof(double) * N);
+    double *dg = (double *)malloc(sizeof(double) * N);
+    double *up = (double *)malloc(sizeof(double) * N);
+    /* row 0: explicit stencil over f[-2..2], mathlib.c:50-52 */
+    B[0] = 1. / DX * (c->aS1 * H0[0] + c->bS1 * H0[1] + c->cS1 * A[0] +
+                      c->dS1 * A[1] + c->eS1 * A[2]);
+    /* rows 1..N-2: 8th-order compact right-hand side, mathlib.c:55-77 */
+    for (int i = 1; i <= N - 2; i++) {
+        double f1 = at(A, H0, H1, N, i + 1) - at(A, H0, H1, N, i - 1);
+        double f2 = at(A, H0, H1, N, i + 2) - at(A, H0, H1, N, i - 2);
+        double f3 = at(A, H0, H1, N, i + 3) - at(A, H0, H1, N, i - 3);
+        B[i] = 1. / DX * (c->a1 / 2 * f1 + c->b1 / 4 * f2 + c->c1 / 6 * f3);
+    }
+    /* row N-1: mirror of row 0, mathlib.c:80-82 */
+    B[N - 1] = -1. / DX * (c->aS1 * H1[1] + c->bS1 * H1[0] + c->cS1 * A[N - 1] +
+                           c->dS1 * A[N - 2] + c->eS1 * A[N - 3]);
+    tri_periodic(N, c->alpha1, c->alphaS1, lo, dg, up);
+    orc_thomas(N, lo, dg, up, B, X);
+    free(B); free(lo); free(dg); free(up);
+}
+
+/* derivCFD2nd, periodic branch, mathlib.c:178-218 */
+static void line_deriv2_c(const orc_coefs *c, int N, double DX, const double *A,
+                          const double *H0, const double *H1, double *X)
+{
+    double DX2 = DX * DX;
+    double *B = (double *)malloc(sizeof(double) * N);
+    double *lo = (double *)malloc(sizeof(double) * N);
+    double *dg = (double *)malloc(sizeof(double) * N);
+    double *up = (double *)malloc(sizeof(double) * N);
+    B[0] = 1. / DX2 * (c->aS2 * H0[0] + c->bS2 * H0[1] + c->cS2 * A[0] +
+                       c->dS2 * A[1] + c->eS2 * A[2]);
+    for (int i = 1; i <= N - 2; i++) {
+        double fi = at(A, H0, H1, N, i);
+        double s1 = at(A, H0, H1, N, i + 1) - 2 * fi + at(A, H0, H1, N, i - 1);
+        double s2 = at(A, H0, H1, N, i + 2) - 2 * fi + at(A, H0, H1, N, i - 2);
+        double s3 = at(A, H0, H1, N, i + 3) - 2 * fi + at(A, H0, H1, N, i - 3);
+        B[i] = 1. / DX2 * (c->a2 * s1 + c->b2 / 4 * s2 + c->c2 / 9 * s3);
+    }
+    B[N - 1] = 1. / DX2 * (c->aS2 * H1[1] + c->bS2 * H1[0] + c->cS2 * A[N - 1] +
+                           c->dS2 * A[N - 2] + c->eS2 * A[N - 3]);
+    tri_periodic(N, c->alpha2, c->alphaS2, lo, dg, up);
+    orc_thomas(N, lo, dg, up, B, X);
+    free(B); free(lo); free(dg); free(up);
+}
+
+/* filterCFD, periodic branch, mathlib.c:309-354 */
+void orc_line_filter_c(const orc_coefs *c, int N, const double *A,
+                       const double *H0, const double *H1, double *X)
+{
+    double *B = (double *)malloc(sizeof(double) * N);
+    double *lo = (double *)malloc(sizeof(double) * N);
+    double *dg = (double *)malloc(sizeof(double) * N);
+    double *up = (double *)malloc(sizeof(double) * N);
+    B[0] = c->aSF * H0[0] + c->bSF * H0[1] + c->cSF * A[0] + c->dSF * A[1] +
+           c->eSF * A[2];
+    for (int i = 1; i <= N - 2; i++) {
+        B[i] = c->aF * A[i] +
+               c->bF / 2 * (at(A, H0, H1, N, i + 1) + at(A, H0, H1, N, i - 1)) +
+               c->cF / 2 * (at(A, H0, H1, N, i + 2) + at(A, H0, H1, N, i - 2)) +
+               c->dF / 2 * (at(A, H0, H1, N, i + 3) + at(A, H0, H1, N, i - 3));
+    }
+    B[N - 1] = c->aSF * H1[1] + c->bSF * H1[0] + c->cSF * A[N - 1] +
+               c->dSF * A[N - 2] + c->eSF * A[N - 3];
+    tri_periodic(N, c->alphaF, c->alphaSF, lo, dg, up);
+    orc_thomas(N, lo, dg, up, B, X);
+    free(B); free(lo); free(dg); free(up);
+}
+
+void orc_line_deriv1(int N, double DX, const double *A, const double *H0,
+                     const double *H1, double *X)
+{
+    orc_coefs c;
+    orc_coefs_default(&c);
+    orc_line_deriv1_c(&c, N, DX, A, H0, H1, X);
+}
+
+void orc_line_deriv2(int N, double DX, const double *A, const double *H0,
+                     const double *H1, double *X)
+{
+    orc_coefs c;
+    orc_coefs_default(&c);
+    line_deriv2_c(&c, N, DX, A, H0, H1, X);
+}
+
+void orc_line_filter(int N, const double *A, const double *H0,
+                     const double *H1, double *X)
+{
+    orc_coefs c;
+    orc_coefs_default(&c);
+    orc_line_filter_c(&c, N, A, H0, H1, X);
+}
+
+/* ------------------------------------------------------------------------ */
+/* x-direction field operators: calcDerivs.c:9-72, filtering.c:9-37.  The    */
+/* halo of each line is the periodic wrap of the same field (parallel.c:13-24 */
+/* with one rank: X0 <- rows nx-2, nx-1; X1 <- rows 0, 1).                   */
+/* ------------------------------------------------------------------------ */
+typedef enum { OP_D1, OP_D2, OP_F } line_op;
+
+static void x_apply(line_op op, int nx, int ny, double dx, int nv,
+                    const double *in, double *out)
+{
+    orc_coefs c;
+    orc_coefs_default(&c);
+    double *line = (double *)malloc(sizeof(double) * nx);
+    double *sol = (double *)malloc(sizeof(double) * nx);
+    double H0[2], H1[2];
+    for (int v = 0; v < nv; v++) {
+        const double *f = in + (size_t)v * nx * ny;
+        double *g = out + (size_t)v * nx * ny;
+        for (int j = 0; j < ny; j++) {
+            for (int i = 0; i < nx; i++) line[i] = f[(size_t)i * ny + j];
+            H0[0] = line[nx - 2]; H0[1] = line[nx - 1];
+            H1[0] = line[0];      H1[1] = line[1];
+            if (op == OP_D1) orc_line_deriv1_c(&c, nx, dx, line, H0, H1, sol);
+            else if (op == OP_D2) line_deriv2_c(&c, nx, dx, line, H0, H1, sol);
+            else orc_line_filter_c(&c, nx, line, H0, H1, sol);
+            for (int i = 0; i < nx; i++) g[(size_t)i * ny + j] = sol[i];
+        }
+    }
+    free(line);
+    free(sol);
+}
+
+void orc_derivX(int nx, int ny, double dx, int nv, const double *in, double *out)
+{
+    x_apply(OP_D1, nx, ny, dx, nv, in, out);
+}
+
+void orc_derivXX(int nx, int ny, double dx, int nv, const double *in, double *out)
+{
+    x_apply(OP_D2, nx, ny, dx, nv, in, out);
+}
+
+void orc_filterX(int nx, int ny, int nv, double *inout)
+{
+    x_apply(OP_F, nx, ny, 0.0, nv, inout, inout);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FFT: iterative radix-2 complex transform with directly evaluated twiddles */
+/* (stands in for FFTW's r2c/c2r, initialize.c:323-326, which are            */
+/* unnormalised; c2r ignores the imaginary parts of bins 0 and n/2).         */
+/* ------------------------------------------------------------------------ */
+static void cfft(int n, double *re, double *im, int sign)
+{
+    /* bit reversal */
+    for (int i = 1, j = 0; i < n; i++) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            double t = re[i]; re[i] = re[j]; re[j] = t;
+            t = im[i]; im[i] = im[j]; im[j] = t;
+        }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        int half = len >> 1;
+        for (int k = 0; k < half; k++) {
+            /* w = exp(sign * 2 pi i k / len), evaluated per k */
+            double ang = sign * 2.0 * ORC_PI * (double)k / (double)len;
+            double wr = cos(ang), wi = sin(ang);
+            for (int s = 0; s < n; s += len) {
+                int a = s + k, b = s + k + half;
+                double tr = re[b] * wr - im[b] * wi;
+                double ti = re[b] * wi + im[b] * wr;
+                re[b] = re[a] - tr; im[b] = im[a] - ti;
+                re[a] += tr;        im[a] += ti;
+            }
+        }
+    }
+}
+
+void orc_rfft(int n, const double *x, double *re, double *im)
+{
+    double *r = (double *)malloc(sizeof(double) * n);
+    double *i_ = (double *)calloc(n, sizeof(double));
+    memcpy(r, x, sizeof(double) * n);
+    cfft(n, r, i_, -1);
+    for (int k = 0; k <= n / 2; k++) { re[k] = r[k]; im[k] = i_[k]; }
+    free(r);
+    free(i_);
+}
+
+void orc_irfft(int n, const double *re, const double *im, double *x)
+{
+    double *r = (double *)malloc(sizeof(double) * n);
+    double *i_ = (double *)malloc(sizeof(double) * n);
+    int h = n / 2;
+    r[0] = re[0]; i_[0] = 0.0;
+    r[h] = re[h]; i_[h] = 0.0;
+    for (int k = 1; k < h; k++) {
+        r[k] = re[k];      i_[k] = im[k];
+        r[n - k] = re[k];  i_[n - k] = -im[k];
+    }
+    cfft(n, r, i_, +1);
+    memcpy(x, r, sizeof(double) * n);
+    free(r);
+    free(i_);
+}
+
+/* initialize.c:161-177 */
+void orc_filtY_table(int ny, double afy, double *filtY)
+{
+    int nky = ny / 2 + 1;
+    double aj = (5.0 + 6.0 * afy) / 8.0;
+    double bj = (1.0 + 2.0 * afy) / 2.0;
+    double cj = -(1.0 - 2 * afy) / 8.0;
+    for (int k = 0; k < nky; k++) {
+        double w = 2. * ORC_PI * (double)k / ny;
+        filtY[k] = (aj + bj * cos(w) + cj * cos(2 * w)) / (1 + 2 * afy * cos(w));
+    }
+}
+
+typedef enum { Y_D1, Y_D2, Y_F } y_op;
+
+/* derivY calcDerivs.c:75-109, derivYY :114-150, filteringY filtering.c:40-94 */
+static void y_apply(y_op op, int nx, int ny, double Ly, int nv,
+                    const double *in, double *out)
+{
+    int nky = ny / 2 + 1;
+    double *re = (double *)malloc(sizeof(double) * nky);
+    double *im = (double *)malloc(sizeof(double) * nky);
+    double *line = (double *)malloc(sizeof(double) * ny);
+    double *filtY = (double *)malloc(sizeof(double) * nky);
+    orc_filtY_table(ny, 0.45, filtY);  /* afy, initialize.c:44 */
+    for (int v = 0; v < nv; v++) {
+        for (int i = 0; i < nx; i++) {
+            const double *f = in + ((size_t)v * nx + i) * ny;
+            double *g = out + ((size_t)v * nx + i) * ny;
+            memcpy(line, f, sizeof(double) * ny);
+            orc_rfft(ny, line, re, im);
+            for (int k = 0; k < nky; k++) {
+                double a = re[k], b = im[k];
+                if (op == Y_D1) {
+                    double kk = 2 * ORC_PI / Ly * k;
+                    re[k] = -b * kk / ny;
+                    im[k] = a * kk / ny;
+                } else if (op == Y_D2) {
+                    double kk = 2 * ORC_PI / Ly * k;
+                    re[k] = -a * kk * kk / ny;
+                    im[k] = -b * kk * kk / ny;
+                } else {
+                    re[k] = a / ny * filtY[k];
+                    im[k] = b / ny * filtY[k];
+                }
+            }
+            orc_irfft(ny, re, im, line);
+            memcpy(g, line, sizeof(double) * ny);
+        }
+    }
+    free(re); free(im); free(line); free(filtY);
+}
+
+void orc_derivY(int nx, int ny, double Ly, int nv, const double *in, double *out)
+{
+    y_apply(Y_D1, nx, ny, Ly, nv, in, out);
+}
+
+void orc_derivYY(int nx, int ny, double Ly, int nv, const double *in, double *out)
+{
+    y_apply(Y_D2, nx, ny, Ly, nv, in, out);
+}
+
+void orc_filterY(int nx, int ny, int nv, double *inout)
+{
+    y_apply(Y_F, nx, ny, 1.0, nv, inout, inout);
+}
+
+/* ------------------------------------------------------------------------ */
+/* pointwise physics                                                          */
+/* ------------------------------------------------------------------------ */
+/* calcEfield, calcRHS.c:24-43: E = -u x B */
+void orc_efield(int nx, int ny, const double *uu, double *E)
+{
+    size_t n = (size_t)nx * ny;
+    for (size_t p = 0; p < n; p++) {
+        double ux = uu[UX * n + p], uy = uu[UY * n + p], uz = uu[UZ * n + p];
+        double bx = uu[BX * n + p], by = uu[BY * n + p], bz = uu[BZ * n + p];
+        E[0 * n + p] = uz * by - uy * bz;
+        E[1 * n + p] = ux * bz - uz * bx;
+        E[2 * n + p] = uy * bx - ux * by;
+    }
+}
+
+/* calcDivB, calcRHS.c:13-22 */
+void orc_divb(int nx, int ny, const double *dudx, const double *dudy, double *divB)
+{
+    size_t n = (size_t)nx * ny;
+    for (size_t p = 0; p < n; p++) divB[p] = dudx[BX * n + p] + dudy[BY * n + p];
+}
+
+/* calcRHSInnerPoint calcRHS.c:77-142 + addSourceTerm calcRHS.c:498-544 */
+void orc_rhs(int nx, int ny, double gamma, double resistivity, double viscosity,
+             int divbclean, double constDiffDivB,
+             const double *uu, const double *dudx, const double *dudy,
+             const double *dEdx, const double *dEdy,
+             const double *d2udx2, const double *d2udy2,
+             const double *d2Bdx2, const double *d2Bdy2,
+             const double *ddivBdx, const double *ddivBdy, double *dudt)
+{
+    size_t n = (size_t)nx * ny;
+    for (size_t p = 0; p < n; p++) {
+        double rho = uu[RHO * n + p], ux = uu[UX * n + p], uy = uu[UY * n + p];
+        double bx = uu[BX * n + p], by = uu[BY * n + p], bz = uu[BZ * n + p];
+        double pr = uu[PR * n + p];
+        double divU = dudx[UX * n + p] + dudy[UY * n + p];
+        double jx = dudy[BZ * n + p];
+        double jy = -dudx[BZ * n + p];
+        double jz = dudx[BY * n + p] - dudy[BX * n + p];
+
+        dudt[RHO * n + p] = -rho * divU - ux * dudx[RHO * n + p] - uy * dudy[RHO * n + p];
+        dudt[UX * n + p] = -ux * dudx[UX * n + p] - uy * dudy[UX * n + p]
+                           - dudx[PR * n + p] / rho + jy * bz / rho - jz * by / rho;
+        dudt[UY * n + p] = -ux * dudx[UY * n + p] - uy * dudy[UY * n + p]
+                           - dudy[PR * n + p] / rho + jz * bx / rho - jx * bz / rho;
+        dudt[UZ * n + p] = -ux * dudx[UZ * n + p] - uy * dudy[UZ * n + p]
+                           + jx * by / rho - jy * bx / rho;
+        /* induction as -curl E, calcRHS.c:135-137 */
+        dudt[BX * n + p] = -dEdy[2 * n + p];
+        dudt[BY * n + p] = dEdx[2 * n + p];
+        dudt[BZ * n + p] = dEdy[0 * n + p] - dEdx[1 * n + p];
+        dudt[PR * n + p] = -ux * dudx[PR * n + p] - uy * dudy[PR * n + p] - gamma * pr * divU;
+
+        if (viscosity > 0) {  /* calcRHS.c:501-515 */
+            for (int c = 0; c < 3; c++)
+                dudt[(UX + c) * n + p] += viscosity / rho *
+                    (d2udx2[c * n + p] + d2udy2[c * n + p]);
+        }
+        if (resistivity > 0) {  /* calcRHS.c:517-531 */
+            for (int c = 0; c < 3; c++)
+                dudt[(BX + c) * n + p] += resistivity *
+                    (d2Bdx2[c * n + p] + d2Bdy2[c * n + p]);
+        }
+        if (divbclean == 1) {  /* calcRHS.c:533-543 */
+            dudt[BX * n + p] += constDiffDivB * ddivBdx[p];
+            dudt[BY * n + p] += constDiffDivB * ddivBdy[p];
+        }
+    }
+}
+
+/* rkExecute, timeAdvance.c:29-36 */
+void orc_rk_execute(size_t n, double cc, double dd, double *uu,
+                    const double *dudt, double *dudtRK)
+{
+    for (size_t i = 0; i < n; i++) {
+        uu[i] = uu[i] + cc * dudt[i] + dd * dudtRK[i];
+        dudtRK[i] = dudt[i];
+    }
+}
+
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+
+/* calcDt, timeAdvance.c:38-153 (divBcleaning==2 branch excluded) */
+double orc_calc_dt(int nx, int ny, double dx, double dy, double gamma,
+                   double cfl, int hall, double di, double resistivity,
+                   double viscosity, const double *uu, double dt0)
+{
+    size_t n = (size_t)nx * ny;
+    double dtmin = -1;
+    for (size_t p = 0; p < n; p++) {
+        double rho = uu[RHO * n + p];
+        double cs2 = gamma * uu[PR * n + p] / rho;
+        double cax = uu[BX * n + p] / sqrt(rho);
+        double cay = uu[BY * n + p] / sqrt(rho);
+        double caz = uu[BZ * n + p] / sqrt(rho);
+        double ca2 = cax * cax + cay * cay + caz * caz;
+        double cms2 = cs2 + ca2;
+        double nsx = sqrt(dmax(cms2 * cms2 - 4 * cs2 * cax * cax, 0.));
+        double nsy = sqrt(dmax(cms2 * cms2 - 4 * cs2 * cay * cay, 0.));
+        double cfx = sqrt(cms2 + nsx) / sqrt(2.);
+        double cfy = sqrt(cms2 + nsy) / sqrt(2.);
+        double csx = sqrt(dmax(cms2 - nsx, 0.)) / sqrt(2.);
+        double csy = sqrt(dmax(cms2 - nsy, 0.)) / sqrt(2.);
+        double ux = uu[UX * n + p], uy = uu[UY * n + p];
+        double m;
+        m = dmax(fabs(ux), fabs(ux - cax));
+        m = dmax(fabs(ux - csx), m);
+        m = dmax(fabs(ux - cfx), m);
+        m = dmax(fabs(ux + cax), m);
+        m = dmax(fabs(ux + csx), m);
+        double cmx = dmax(fabs(ux + cfx), m);
+        m = dmax(fabs(uy), fabs(uy - cay));
+        m = dmax(fabs(uy - csy), m);
+        m = dmax(fabs(uy - cfy), m);
+        m = dmax(fabs(uy + cay), m);
+        m = dmax(fabs(uy + csy), m);
+        double cmy = dmax(fabs(uy + cfy), m);
+        if (hall == 1) {  /* whistler limit, timeAdvance.c:97-105 */
+            double ch = di / dmin(dx, dy) / rho *
+                dmax(fabs(uu[BX * n + p]),
+                     dmax(fabs(uu[BY * n + p]), fabs(uu[BZ * n + p])));
+            cmx = dmax(cmx, ch);
+            cmy = dmax(cmy, ch);
+        }
+        double dtx = dx / cmx, dty = dy / cmy;
+        if (resistivity > 0) {
+            dtx = dmin(dtx, dx * dx / resistivity);
+            dty = dmin(dty, dy * dy / resistivity);
+        }
+        if (viscosity > 0) {
+            dtx = dmin(dtx, dx * dx / viscosity);
+            dty = dmin(dty, dy * dy / viscosity);
+        }
+        double it = dmin(dtx, dty);
+        if (dtmin < 0 || it < dtmin) dtmin = it;
+    }
+    dtmin = dtmin * cfl;
+    /* hysteresis, timeAdvance.c:144-151 */
+    if (dt0 < 0.98 * dtmin || dt0 > 1.02 * dtmin) return dtmin;
+    return dt0;
+}
+
+/* total energy: sum of 1/2 rho u^2 + p/(gamma-1) + 1/2 B^2 times the cell
+ * area (no such diagnostic exists in the reference). */
+double orc_energy(int nx, int ny, double dx, double dy, double gamma,
+                  const double *uu)
+{
+    size_t n = (size_t)nx * ny;
+    double s = 0.0;
+    for (size_t p = 0; p < n; p++) {
+        double rho = uu[RHO * n + p];
+        double u2 = uu[UX * n + p] * uu[UX * n + p] + uu[UY * n + p] * uu[UY * n + p] +
+                    uu[UZ * n + p] * uu[UZ * n + p];
+        double b2 = uu[BX * n + p] * uu[BX * n + p] + uu[BY * n + p] * uu[BY * n + p] +
+                    uu[BZ * n + p] * uu[BZ * n + p];
+        s += 0.5 * rho * u2 + uu[PR * n + p] / (gamma - 1.0) + 0.5 * b2;
+    }
+    return s * dx * dy;
+}
+
+/* ------------------------------------------------------------------------ */
+/* run driver: the stage order of mhd.c:154-206                               */
+/* ------------------------------------------------------------------------ */
+struct orc_sim {
+    int nx, ny, nvar, hall, divbclean;
+    double Lx, Ly, dx, dy, gamma, cfl, di, resistivity, viscosity, constDiffDivB;
+    double dt, time;
+    long steps;
+    double cc[3], dd[3];
+    double *uu, *dudx, *dudy, *E, *dEdx, *dEdy, *dudt, *dudtRK;
+    double *d2udx2, *d2udy2, *d2Bdx2, *d2Bdy2, *divB, *ddivBdx, *ddivBdy;
+};
+
+orc_sim *orc_sim_create(int nx, int ny, double Lx, double Ly, double gamma,
+                        double cfl, int hall, double di, double resistivity,
+                        double viscosity, int divbclean)
+{
+    orc_sim *s = (orc_sim *)calloc(1, sizeof(orc_sim));
+    size_t n = (size_t)nx * ny;
+    s->nx = nx; s->ny = ny; s->nvar = 8;
+    s->Lx = Lx; s->Ly = Ly;
+    s->dx = Lx / nx;             /* periodic x, initialize.c:125-128 */
+    s->dy = Ly / ny;             /* initialize.c:135 */
+    s->gamma = gamma; s->cfl = cfl; s->hall = hall; s->di = di;
+    s->resistivity = resistivity; s->viscosity = viscosity;
+    s->divbclean = divbclean;
+    /* initialize.c:17 and :154 */
+    s->constDiffDivB = 2.0 * dmax(s->dx, s->dy) * dmax(s->dx, s->dy);
+    s->uu = (double *)calloc(8 * n, sizeof(double));
+    s->dudx = (double *)calloc(8 * n, sizeof(double));
+    s->dudy = (double *)calloc(8 * n, sizeof(double));
+    s->dudt = (double *)calloc(8 * n, sizeof(double));
+    s->dudtRK = (double *)calloc(8 * n, sizeof(double));
+    s->E = (double *)calloc(3 * n, sizeof(double));
+    s->dEdx = (double *)calloc(3 * n, sizeof(double));
+    s->dEdy = (double *)calloc(3 * n, sizeof(double));
+    s->d2udx2 = (double *)calloc(3 * n, sizeof(double));
+    s->d2udy2 = (double *)calloc(3 * n, sizeof(double));
+    s->d2Bdx2 = (double *)calloc(3 * n, sizeof(double));
+    s->d2Bdy2 = (double *)calloc(3 * n, sizeof(double));
+    s->divB = (double *)calloc(n, sizeof(double));
+    s->ddivBdx = (double *)calloc(n, sizeof(double));
+    s->ddivBdy = (double *)calloc(n, sizeof(double));
+    return s;
+}
+
+void orc_sim_destroy(orc_sim *s)
+{
+    if (!s) return;
+    free(s->uu); free(s->dudx); free(s->dudy); free(s->dudt); free(s->dudtRK);
+    free(s->E); free(s->dEdx); free(s->dEdy);
+    free(s->d2udx2); free(s->d2udy2); free(s->d2Bdx2); free(s->d2Bdy2);
+    free(s->divB); free(s->ddivBdx); free(s->ddivBdy);
+    free(s);
+}
+
+/* rkInit, timeAdvance.c:20-27 */
+static void rk_init(orc_sim *s, double dt)
+{
+    s->cc[0] = 8.0 / 15.0 * dt; s->dd[0] = 0.0;
+    s->cc[1] = 5.0 / 12.0 * dt; s->dd[1] = -17. / 60. * dt;
+    s->cc[2] = 0.75 * dt;       s->dd[2] = -5. / 12. * dt;
+    memset(s->dudtRK, 0, sizeof(double) * 8 * (size_t)s->nx * s->ny);
+}
+
+static double sim_calc_dt(orc_sim *s, double dt0)
+{
+    return orc_calc_dt(s->nx, s->ny, s->dx, s->dy, s->gamma, s->cfl, s->hall,
+                       s->di, s->resistivity, s->viscosity, s->uu, dt0);
+}
+
+void orc_sim_set_state(orc_sim *s, const double *uu)
+{
+    memcpy(s->uu, uu, sizeof(double) * 8 * (size_t)s->nx * s->ny);
+    s->time = 0.0;
+    s->steps = 0;
+    s->dt = sim_calc_dt(s, 0.0);   /* mhd.c:58 */
+    rk_init(s, s->dt);             /* mhd.c:63 */
+}
+
+void orc_sim_get_state(const orc_sim *s, double *uu)
+{
+    memcpy(uu, s->uu, sizeof(double) * 8 * (size_t)s->nx * s->ny);
+}
+
+void orc_sim_get_derivs(const orc_sim *s, double *dudx, double *dudy)
+{
+    size_t n = 8 * (size_t)s->nx * s->ny;
+    if (dudx) memcpy(dudx, s->dudx, sizeof(double) * n);
+    if (dudy) memcpy(dudy, s->dudy, sizeof(double) * n);
+}
+
+/* one pass of mhd.c:156-198 */
+void orc_sim_stage(orc_sim *s, int iRK)
+{
+    int nx = s->nx, ny = s->ny;
+    size_t n = (size_t)nx * ny;
+    orc_derivX(nx, ny, s->dx, 8, s->uu, s->dudx);            /* mhd.c:160 */
+    orc_derivY(nx, ny, s->Ly, 8, s->uu, s->dudy);            /* mhd.c:161 */
+    if (s->viscosity > 0) {                                  /* mhd.c:163-167 */
+        orc_derivXX(nx, ny, s->dx, 3, s->uu + UX * n, s->d2udx2);
+        orc_derivYY(nx, ny, s->Ly, 3, s->uu + UX * n, s->d2udy2);
+    }
+    /* mhd.c:169-173 also runs these when Hall==1, but their output is read
+     * only when resistivity > 0 (calcRHS.c:517), so they are skipped here. */
+    if (s->resistivity > 0) {
+        orc_derivXX(nx, ny, s->dx, 3, s->uu + BX * n, s->d2Bdx2);
+        orc_derivYY(nx, ny, s->Ly, 3, s->uu + BX * n, s->d2Bdy2);
+    }
+    if (s->divbclean == 1) {                                 /* mhd.c:175-182 */
+        orc_divb(nx, ny, s->dudx, s->dudy, s->divB);
+        orc_derivX(nx, ny, s->dx, 1, s->divB, s->ddivBdx);
+        orc_derivY(nx, ny, s->Ly, 1, s->divB, s->ddivBdy);
+    }
+    orc_efield(nx, ny, s->uu, s->E);                         /* mhd.c:186 */
+    orc_derivX(nx, ny, s->dx, 3, s->E, s->dEdx);             /* mhd.c:188 */
+    orc_derivY(nx, ny, s->Ly, 3, s->E, s->dEdy);             /* mhd.c:189 */
+    orc_rhs(nx, ny, s->gamma, s->resistivity, s->viscosity, s->divbclean,
+            s->constDiffDivB, s->uu, s->dudx, s->dudy, s->dEdx, s->dEdy,
+            s->d2udx2, s->d2udy2, s->d2Bdx2, s->d2Bdy2, s->ddivBdx,
+            s->ddivBdy, s->dudt);                            /* mhd.c:192 */
+    orc_rk_execute(8 * n, s->cc[iRK], s->dd[iRK], s->uu, s->dudt, s->dudtRK);
+    orc_filterX(nx, ny, 8, s->uu);                           /* filtering.c:100 */
+    orc_filterY(nx, ny, 8, s->uu);                           /* filtering.c:101 */
+}
+
+static void sim_step(orc_sim *s, const double *forced_dt)
+{
+    for (int iRK = 0; iRK < 3; iRK++) orc_sim_stage(s, iRK);
+    s->time += s->dt;                                        /* mhd.c:201 */
+    s->steps++;
+    s->dt = forced_dt ? *forced_dt : sim_calc_dt(s, s->dt);  /* mhd.c:205 */
+    rk_init(s, s->dt);                                       /* mhd.c:206 */
+}
+
+void orc_sim_run(orc_sim *s, int nsteps)
+{
+    for (int k = 0; k < nsteps; k++) sim_step(s, NULL);
+}
+
+/* replay a given dt sequence: dts[k] is the dt used after step k */
+void orc_sim_run_fixed_dt(orc_sim *s, int nsteps, const double *dts)
+{
+    for (int k = 0; k < nsteps; k++) sim_step(s, &dts[k]);
+}
+
+double orc_sim_dt(const orc_sim *s) { return s->dt; }
+double orc_sim_time(const orc_sim *s) { return s->time; }
+long orc_sim_steps(const orc_sim *s) { return s->steps; }
