@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""RK3 cell-updates/s of the fused FP64 device path on one MI355X.
+
+Workload (default): 2048 x 2048 periodic box [0, 2pi)^2, 8 fields, FP64,
+seeded broadband Alfvenic turbulence on rho = p = 1, B0 = (1, 0, 0)
+(1 <= |k| <= 16, |k|^(-5/3), rms 0.1, dB from A_z), gamma = 5/3, cfl = 0.3,
+Hall CFL term with d_i = 0.02, 200 RK3 steps.  A step is one full RK3 step
+(3 stages with derivatives, RHS, update, x and y filters, and the CFL dt).
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu]
+
+Prints ONE JSON line.  The state is resident in HBM before the timed region;
+time is taken with hipEvents on the stream the kernels run on, bracketed by
+device synchronisation on both sides.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
+B_ALG_FP64 = 1680.0        # algorithmic bytes per cell-update, 2-sweep schedule, FP64
+WORDS = 8                  # bytes per FP64 word
+
+
+def kernel_alg_bytes(nx, ny, elem=8):
+    """Algorithmic HBM bytes per launch of each kernel class (inputs read once,
+    outputs written once), per cell times the cells of one launch."""
+    n = nx * ny
+    return {
+        # y-sweep: read u (8), write filtered u (8), dudy (8), E (3), dEdy (3)
+        "ysweep": 30 * n * elem,
+        # x-derivative: read u (8) and E (3), write d/dx of both (11)
+        "xderiv": 22 * n * elem,
+        # RHS + RK, average of the three stages: read u, dudx, dudy (30) plus
+        # R_prev in stages 2-3 (8); write u (8) and R_prev in stages 1-2 (8)
+        "rhs_rk": (46 + 54 + 46) / 3.0 * n * elem,
+        # x-filter in place: read 8, write 8
+        "xfilter": 16 * n * elem,
+    }
+
+
+def cpu_baseline(nx, ny, u0, hall, di):
+    """FP64 C restatement (the checker) timed on this host, 1 thread, on a
+    bounded sample of the same workload: one RK3 step of the full grid."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+    s = orc.Sim(nx, ny, hall=hall, di=di)
+    s.set_state(u0)
+    t0 = time.perf_counter()
+    s.run(1)
+    dt = time.perf_counter() - t0
+    return {"value": nx * ny / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"1 RK3 step of the {nx}x{ny} FP64 turbulence state ({dt:.1f} s, "
+                      "single-threaded oracle restatement)"}
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary, if one exists (profiles/traffic_<kernel>_2048.json)."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_2048.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        return json.load(open(p)).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--ny", type=int, default=2048)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--graph", type=int, default=1)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs the whole grid on one GPU; multi-GPU runs "
+              "(--gpus > 1) are not part of it", file=sys.stderr)
+        sys.exit(2)
+
+    import mhd2dcfdfftw_open_amd as m
+
+    nx, ny = args.nx, args.ny
+    hall, di = 1, 0.02
+    prec = m.FP64 if args.precision == "fp64" else m.FP32
+    elem = 8 if prec == m.FP64 else 4
+    u0 = m.turbulence(nx, ny, seed=42, rms=0.1, kmin=1, kmax=16)
+
+    ctx = m.Context(nx, ny, hall=hall, di=di, precision=prec)
+    ctx.set_state(u0)
+    ctx.run(args.warmup, graph=bool(args.graph))
+    ctx.sync()
+    ms = ctx.time_run(args.steps, graph=bool(args.graph))
+    ctx.sync()
+    cells = nx * ny * args.steps
+    value = cells / (ms * 1e-3)
+
+    # per-kernel device time on the kernels' own stream (hipEvents)
+    prof = ctx.profile_run(args.profile_steps)
+    ctx.sync()
+    alg = kernel_alg_bytes(nx, ny, elem)
+    share = {k: prof[k][0] * prof[k][1] for k in alg}
+    dom = max(share, key=share.get)
+    dom_ms = prof[dom][0]
+    achieved = alg[dom] / (dom_ms * 1e-3)
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                "traffic": load_traffic(dom), "avg_launch_ms": dom_ms,
+                "alg_bytes_per_launch": alg[dom]}
+    kernels = {k: {"avg_ms": prof[k][0], "launches_per_step": prof[k][1] / args.profile_steps,
+                   "GB/s": (alg[k] / (prof[k][0] * 1e-3) / 1e9) if k in alg and prof[k][0] > 0
+                   else None}
+               for k in prof}
+    energy, divb = ctx.diagnostics()
+
+    out = {
+        "metric": "RK3 cell-updates/s (plain FP64 path)" if prec == m.FP64 else
+                  "RK3 cell-updates/s (FP32 path)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == m.FP64 else "f32",
+        "data": "synthetic (seeded turbulence initial state, seed 42)",
+        "config": {"workload": f"hall-mhd2d turbulence {nx}x{ny}, 8 fields, "
+                               f"{args.precision}, d_i={di}, 200-step run",
+                   "nx": nx, "ny": ny, "graph": bool(args.graph)},
+        "roofline": roofline,
+        "step_roofline_frac": value * (B_ALG_FP64 * elem / 8) / HBM_PEAK,
+        "kernels": kernels,
+        "diagnostics": {"energy": energy, "max_divb": divb},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(nx, ny, u0, hall, di)
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

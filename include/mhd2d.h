@@ -1,0 +1,170 @@
+/*
+ * mhd2d.h -- C ABI of the MI355X (gfx950) build of the right-hand-side and
+ * RK3 time-advance path of chenshihelio/MHD2DCFDFFTW_Open.
+ *
+ * Two layers:
+ *
+ *  1. Drop-in entry points with the reference's own names and signatures.
+ *     They operate on the reference's host globals (uu, dudx, dudy, Efield,
+ *     nx, ny, ... as defined by its initialize.c) exactly like the object
+ *     files they replace, so its mhd.c stage loop can drive them unchanged.
+ *     Arrays stay resident on the GPU between calls; host copies are
+ *     refreshed at calcDt (the one per-step call mhd.c makes), see
+ *     mhd2d_dropin_set_sync().
+ *
+ *  2. A context API for running whole RK3 steps on the device with fused
+ *     kernels (the timed path).  Plain pointers, sizes and int status codes.
+ *
+ * Layout everywhere: field-major [v][i][j], i along x (stride ny), j along y
+ * (contiguous), fields (rho, ux, uy, uz, Bx, By, Bz, p) (macros.h:4-11).
+ * Grid: periodic in x and y, nx and ny powers of two in [16, 4096].
+ */
+#ifndef MHD2D_H
+#define MHD2D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double real; /* parameter.h:21 */
+
+/* ===================================================================== */
+/* 1. drop-in entry points                                               */
+/* ===================================================================== */
+
+/* replaces calcDerivs.c:9-39 (declared calcDerivs.h:6-8): compact 8th-order
+ * d/dx of fields iVarStartMain.. of arrMain into fields 0..nv-1 of arrDeriv.
+ * The halo arrays are accepted and not read: the periodic x-halo is an index
+ * wrap on the device (parallel.c:13-24 with one rank). */
+void derivX(int nv, real *arrMain, real *arrHalo0, real *arrHalo1,
+            real *arrDeriv, int iVarStartMain, int iVarStartHalo);
+/* replaces calcDerivs.c:41-72 (calcDerivs.h:10-12): compact d2/dx2 */
+void derivXX(int nv, real *arrMain, real *arrHalo0, real *arrHalo1,
+             real *arrDeriv, int iVarStartMain, int iVarStartHalo);
+/* replaces calcDerivs.c:75-109 (calcDerivs.h:14): spectral d/dy */
+void derivY(int nv, real *arrMain, real *arrDeriv, int iVarStartMain);
+/* replaces calcDerivs.c:114-150 (calcDerivs.h:15): spectral d2/dy2 */
+void derivYY(int nv, real *arrMain, real *arrDeriv, int iVarStartMain);
+/* replaces filtering.c:98-102 (filtering.h:6): compact x filter then
+ * spectral y filter of all NVAR fields of uu, in place */
+void filtering(void);
+/* replace calcRHS.c:46-75, :13-22, :24-43 (calcRHS.h:4-6) */
+void calcRHS(void);
+void calcDivB(void);
+void calcEfield(void);
+/* replace timeAdvance.c:20-27, :29-36, :38-153 (timeAdvance.h:6-8) */
+void rkInit(real dt);
+void rkExecute(int iRK);
+real calcDt(real dt0);
+/* replace parallel.c:4, :66, :130 (parallel.h:4-6): with the whole grid on
+ * one GPU the periodic halo is read by index wrap, so these do nothing */
+void sendrecvHaloPointsX(void);
+void sendrecvHaloPointsdivBX(void);
+void sendrecvHaloPointsEfieldX(void);
+/* replace divBcleaning.c:4-10 (divBcleaning.h:10-11) */
+extern real Cr_divBCleaning;
+void calc_Cp_fix_Cr(void);
+
+/* Host-copy policy of the drop-in layer.
+ *  MHD2D_SYNC_STRICT (default): calcDt copies uu, dudx and dudy back to the
+ *    host every step, so mhd.c's checkDivB (mhd.c:342-367) and writeArray
+ *    (writeOutput.c:47-76), which read host memory directly, see current data.
+ *  MHD2D_SYNC_EXPLICIT: host copies are refreshed only by
+ *    mhd2d_dropin_sync_host(); call it before any host-side read. */
+enum { MHD2D_SYNC_STRICT = 0, MHD2D_SYNC_EXPLICIT = 1 };
+void mhd2d_dropin_set_sync(int mode);
+/* copy every device-newer array back to its host array */
+int mhd2d_dropin_sync_host(void);
+/* declare that the host has written the arrays (e.g. a restart): the next
+ * call uploads them again */
+void mhd2d_dropin_invalidate_device(void);
+/* free all device mirrors (end of run) */
+void mhd2d_dropin_release(void);
+
+/* Host storage of the reference globals for builds that do not link the
+ * reference's initialize.c (tests, the standalone driver): allocates uu,
+ * dudx, ... like initialize.c:87-395 for one rank and periodic x, with the
+ * parameters given.  Not part of the reference ABI. */
+int mhd2d_host_initialize(int nx, int ny, double Lx, double Ly, double gamma,
+                          double cfl, int hall, double di, double resistivity,
+                          double viscosity, int divbclean);
+void mhd2d_host_finalize(void);
+
+/* ===================================================================== */
+/* 2. context API (fused device stepping)                                */
+/* ===================================================================== */
+typedef struct mhd2d_ctx mhd2d_ctx;
+
+enum { MHD2D_FP64 = 0, MHD2D_FP32 = 1 };
+
+typedef struct mhd2d_params {
+    int nx, ny;             /* grid, powers of two in [16, 4096] */
+    double Lx, Ly;          /* periodic box; dx = Lx/nx, dy = Ly/ny */
+    double gamma;           /* adiabatic index */
+    double cfl;
+    int hall;               /* 1: whistler CFL limit with di (timeAdvance.c:97) */
+    double di;
+    double resistivity;     /* must be 0 in the context API */
+    double viscosity;       /* must be 0 in the context API */
+    int divbclean;          /* must be 0 in the context API */
+    int precision;          /* MHD2D_FP64 or MHD2D_FP32 (state and arithmetic) */
+} mhd2d_params;
+
+/* status codes */
+enum {
+    MHD2D_OK = 0,
+    MHD2D_EINVAL = 1,      /* unsupported size or parameter */
+    MHD2D_EHIP = 2,        /* HIP runtime error */
+    MHD2D_ENONFINITE = 3,  /* a CFL limit was NaN/inf/<=0 */
+    MHD2D_ENOMEM = 4
+};
+
+int mhd2d_create(const mhd2d_params *p, mhd2d_ctx **out);
+void mhd2d_destroy(mhd2d_ctx *c);
+/* upload a state [8][nx][ny] (FP64 host); resets time, step and dt, and the
+ * first step's dt is calcDt(0) of this state (mhd.c:58) */
+int mhd2d_set_state(mhd2d_ctx *c, const double *uu);
+/* download the state after the last step (applies the pending y filter) */
+int mhd2d_get_state(mhd2d_ctx *c, double *uu);
+/* advance nsteps full RK3 steps (mhd.c:154-206), asynchronously */
+int mhd2d_run(mhd2d_ctx *c, int nsteps);
+/* same, with the step captured once into a hipGraph and replayed */
+int mhd2d_run_graph(mhd2d_ctx *c, int nsteps);
+int mhd2d_sync(mhd2d_ctx *c);
+/* dt of the last step, simulated time, number of steps done */
+int mhd2d_get_scalars(mhd2d_ctx *c, double *dt, double *time, long long *steps);
+/* total energy sum(rho u^2/2 + p/(gamma-1) + B^2/2) dx dy and max |div B|
+ * of the current state, FP64 reductions */
+int mhd2d_diagnostics(mhd2d_ctx *c, double *energy, double *max_divb);
+/* wall time of nsteps on the context's stream (hipEvents), in ms */
+int mhd2d_time_run(mhd2d_ctx *c, int nsteps, int use_graph, double *ms);
+/* average device time per launch of each kernel class over nsteps, in ms:
+ * out[0] y-sweep, [1] x-derivative, [2] RHS+RK, [3] x-filter, [4] dt */
+int mhd2d_profile_run(mhd2d_ctx *c, int nsteps, double *out_ms, int *out_launches);
+/* bytes of device memory held by the context */
+size_t mhd2d_device_bytes(const mhd2d_ctx *c);
+const char *mhd2d_last_error(void);
+
+/* ===================================================================== */
+/* initial states (host, FP64) shared by the device runs and the checker  */
+/* ===================================================================== */
+/* Orszag-Tang vortex on [0,Lx)x[0,Ly) */
+void mhd2d_ic_orszag_tang(int nx, int ny, double Lx, double Ly, double *uu);
+/* rho=p=1, B0=(1,0,0); du and dB summed over integer wavevectors
+ * kmin <= |k| <= kmax with |k|^(-5/3) amplitudes and SplitMix64 phases;
+ * in-plane parts from stream functions (dB from A_z, so div B = 0); each of
+ * du, dB scaled to the given rms */
+void mhd2d_ic_turbulence(int nx, int ny, double Lx, double Ly,
+                         unsigned long long seed, double rms, int kmin,
+                         int kmax, double *uu);
+/* rho=p=1, B0=(1,0,0) plus one linear wave with integer (kx, ky) in
+ * [1,8]^2 drawn from the seed, amplitude amp, random phase */
+void mhd2d_ic_linear_wave(int nx, int ny, double Lx, double Ly,
+                          unsigned long long seed, double amp, double *uu);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MHD2D_H */

@@ -1,0 +1,227 @@
+"""MI355X (gfx950) build of the RHS + RK3 time-advance path of
+chenshihelio/MHD2DCFDFFTW_Open, behind the C ABI declared in include/mhd2d.h.
+
+This module is a thin ctypes view of that ABI for Python callers (tests,
+bench.py).  Everything it runs is the HIP library libmhd2d.so built in-tree
+by ``python -m mhd2dcfdfftw_open_amd.build``; there is no fallback path, and a
+missing library raises.
+
+Layout of every state array: float64 [8, nx, ny], fields
+(rho, ux, uy, uz, Bx, By, Bz, p), x along axis 1, y along axis 2.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libmhd2d.so")
+
+FP64, FP32 = 0, 1
+RHO, UX, UY, UZ, BX, BY, BZ, PR = range(8)
+FIELDS = ("rho", "ux", "uy", "uz", "Bx", "By", "Bz", "p")
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class MHD2DError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("Lx", ctypes.c_double), ("Ly", ctypes.c_double),
+                ("gamma", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("hall", ctypes.c_int), ("di", ctypes.c_double),
+                ("resistivity", ctypes.c_double), ("viscosity", ctypes.c_double),
+                ("divbclean", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+def lib():
+    """Load libmhd2d.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MHD2DError(
+            f"{LIB_PATH} is missing: build it with `python -m mhd2dcfdfftw_open_amd.build` "
+            "(this package has no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+    i, d, p, vp = ctypes.c_int, ctypes.c_double, _dp, ctypes.c_void_p
+    ll = ctypes.c_longlong
+    sigs = {
+        "mhd2d_create": (i, [ctypes.POINTER(Params), ctypes.POINTER(vp)]),
+        "mhd2d_destroy": (None, [vp]),
+        "mhd2d_set_state": (i, [vp, p]),
+        "mhd2d_get_state": (i, [vp, p]),
+        "mhd2d_run": (i, [vp, i]),
+        "mhd2d_run_graph": (i, [vp, i]),
+        "mhd2d_sync": (i, [vp]),
+        "mhd2d_get_scalars": (i, [vp, p, p, ctypes.POINTER(ll)]),
+        "mhd2d_diagnostics": (i, [vp, p, p]),
+        "mhd2d_time_run": (i, [vp, i, i, p]),
+        "mhd2d_profile_run": (i, [vp, i, p, ctypes.POINTER(i)]),
+        "mhd2d_device_bytes": (ctypes.c_size_t, [vp]),
+        "mhd2d_last_error": (ctypes.c_char_p, []),
+        "mhd2d_ic_orszag_tang": (None, [i, i, d, d, p]),
+        "mhd2d_ic_turbulence": (None, [i, i, d, d, ctypes.c_ulonglong, d, i, i, p]),
+        "mhd2d_ic_linear_wave": (None, [i, i, d, d, ctypes.c_ulonglong, d, p]),
+        "mhd2d_host_initialize": (i, [i, i, d, d, d, d, i, d, d, d, i]),
+        "mhd2d_host_finalize": (None, []),
+        "mhd2d_dropin_set_sync": (None, [i]),
+        "mhd2d_dropin_sync_host": (i, []),
+        "mhd2d_dropin_invalidate_device": (None, []),
+        "mhd2d_dropin_release": (None, []),
+        "derivX": (None, [i, p, p, p, p, i, i]),
+        "derivXX": (None, [i, p, p, p, p, i, i]),
+        "derivY": (None, [i, p, p, i]),
+        "derivYY": (None, [i, p, p, i]),
+        "filtering": (None, []),
+        "calcRHS": (None, []),
+        "calcDivB": (None, []),
+        "calcEfield": (None, []),
+        "rkInit": (None, [d]),
+        "rkExecute": (None, [i]),
+        "calcDt": (d, [d]),
+        "sendrecvHaloPointsX": (None, []),
+        "sendrecvHaloPointsdivBX": (None, []),
+        "sendrecvHaloPointsEfieldX": (None, []),
+        "calc_Cp_fix_Cr": (None, []),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _check(rc):
+    if rc != 0:
+        raise MHD2DError(f"mhd2d error {rc}: {lib().mhd2d_last_error().decode()}")
+
+
+# ------------------------------------------------------------------ initial states
+def orszag_tang(nx, ny, Lx=2 * np.pi, Ly=2 * np.pi):
+    uu = np.zeros((8, nx, ny))
+    lib().mhd2d_ic_orszag_tang(nx, ny, Lx, Ly, _ptr(uu))
+    return uu
+
+
+def turbulence(nx, ny, seed=42, rms=0.1, kmin=1, kmax=16, Lx=2 * np.pi, Ly=2 * np.pi):
+    uu = np.zeros((8, nx, ny))
+    lib().mhd2d_ic_turbulence(nx, ny, Lx, Ly, seed, rms, kmin, kmax, _ptr(uu))
+    return uu
+
+
+def linear_wave(nx, ny, seed=42, amp=1e-4, Lx=2 * np.pi, Ly=2 * np.pi):
+    uu = np.zeros((8, nx, ny))
+    lib().mhd2d_ic_linear_wave(nx, ny, Lx, Ly, seed, amp, _ptr(uu))
+    return uu
+
+
+# ------------------------------------------------------------------ context API
+class Context:
+    """Fused device stepping of one grid (mhd2d_create ... mhd2d_destroy)."""
+
+    def __init__(self, nx, ny, Lx=2 * np.pi, Ly=2 * np.pi, gamma=5.0 / 3.0, cfl=0.3,
+                 hall=0, di=0.0, precision=FP64):
+        self.nx, self.ny = nx, ny
+        self.params = Params(nx, ny, Lx, Ly, gamma, cfl, hall, di, 0.0, 0.0, 0, precision)
+        h = ctypes.c_void_p()
+        _check(lib().mhd2d_create(ctypes.byref(self.params), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mhd2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_state(self, uu):
+        uu = np.ascontiguousarray(uu, dtype=np.float64)
+        assert uu.shape == (8, self.nx, self.ny)
+        _check(lib().mhd2d_set_state(self.h, _ptr(uu)))
+
+    def state(self):
+        uu = np.zeros((8, self.nx, self.ny))
+        _check(lib().mhd2d_get_state(self.h, _ptr(uu)))
+        return uu
+
+    def run(self, nsteps, graph=False):
+        f = lib().mhd2d_run_graph if graph else lib().mhd2d_run
+        _check(f(self.h, nsteps))
+
+    def sync(self):
+        _check(lib().mhd2d_sync(self.h))
+
+    def scalars(self):
+        dt, t, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        _check(lib().mhd2d_get_scalars(self.h, ctypes.byref(dt), ctypes.byref(t),
+                                       ctypes.byref(n)))
+        return dt.value, t.value, n.value
+
+    def diagnostics(self):
+        e, m = ctypes.c_double(), ctypes.c_double()
+        _check(lib().mhd2d_diagnostics(self.h, ctypes.byref(e), ctypes.byref(m)))
+        return e.value, m.value
+
+    def time_run(self, nsteps, graph=False):
+        ms = ctypes.c_double()
+        _check(lib().mhd2d_time_run(self.h, nsteps, int(graph), ctypes.byref(ms)))
+        return ms.value
+
+    def profile_run(self, nsteps):
+        out = (ctypes.c_double * 5)()
+        cnt = (ctypes.c_int * 5)()
+        _check(lib().mhd2d_profile_run(self.h, nsteps, out, cnt))
+        names = ("ysweep", "xderiv", "rhs_rk", "xfilter", "dt_finalize")
+        return {k: (out[q], cnt[q]) for q, k in enumerate(names)}
+
+    def device_bytes(self):
+        return lib().mhd2d_device_bytes(self.h)
+
+
+# ------------------------------------------------------------------ drop-in layer
+_GLOBAL_ARRAYS = {  # name -> number of fields (initialize.c:216-294)
+    "uu": "NVAR", "dudx": "NVAR", "dudy": "NVAR", "dudt": "NVAR", "dudtRK": "NVAR",
+    "Efield": 3, "dEdx": 3, "dEdy": 3, "d2udx2": 3, "d2udy2": 3, "d2Bdx2": 3,
+    "d2Bdy2": 3, "divB": 1, "ddivBdx": 1, "ddivBdy": 1,
+}
+
+
+class RefGlobals:
+    """The reference's host globals as defined in libmhd2d.so's own storage
+    (mhd2d_host_initialize), with numpy views of its arrays, so the drop-in
+    entry points can be called exactly as mhd.c calls them."""
+
+    def __init__(self, nx, ny, Lx=2 * np.pi, Ly=2 * np.pi, gamma=5.0 / 3.0, cfl=0.3,
+                 hall=0, di=0.0, resistivity=0.0, viscosity=0.0, divbclean=0):
+        L = lib()
+        _check(L.mhd2d_host_initialize(nx, ny, Lx, Ly, gamma, cfl, hall, di, resistivity,
+                                       viscosity, divbclean))
+        self.nx, self.ny = nx, ny
+        self.nvar = ctypes.c_int.in_dll(L, "NVAR").value
+
+    def ptr(self, name):
+        return ctypes.cast(ctypes.c_void_p.in_dll(lib(), name).value, _dp)
+
+    def array(self, name):
+        nf = _GLOBAL_ARRAYS[name]
+        nf = self.nvar if nf == "NVAR" else nf
+        n = nf * self.nx * self.ny
+        buf = np.ctypeslib.as_array(self.ptr(name), shape=(n,))
+        return buf.reshape(nf, self.nx, self.ny)
+
+    def scalar(self, name, ctype=ctypes.c_double):
+        return ctype.in_dll(lib(), name)
+
+    def close(self):
+        lib().mhd2d_host_finalize()

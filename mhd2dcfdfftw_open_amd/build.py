@@ -1,0 +1,77 @@
+"""In-tree build of the HIP library (gfx950) and the checker.
+
+    python -m mhd2dcfdfftw_open_amd.build          # both libraries
+Outputs (git-ignored, shipped with the tree):
+    mhd2dcfdfftw_open_amd/libmhd2d.so       kernels + context API + drop-in
+                                            entry points + host storage of the
+                                            reference globals (standalone)
+    mhd2dcfdfftw_open_amd/libmhd2d_core.so  the same without the globals, for
+                                            linking against the reference's own
+                                            initialize.o (globals resolved from
+                                            the executable)
+"""
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libmhd2d.so")
+LIB_CORE = os.path.join(PKG, "libmhd2d_core.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-Wall", "-Wno-unused-result", "-I", os.path.join(ROOT, "include")]
+CORE_SOURCES = ["mhd2d_kernels.hip", "mhd2d_tables.cpp", "mhd2d_ctx.cpp",
+                "mhd2d_dropin.cpp", "mhd2d_ic.cpp"]
+GLOBAL_SOURCES = ["mhd2d_globals.cpp"]
+
+
+def _compile(src):
+    out = os.path.join(BUILD, os.path.basename(src) + ".o")
+    cmd = [HIPCC, "-x", "hip", *COMMON, "-c", os.path.join(CSRC, src), "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    msgs = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
+                     if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{msgs}")
+    return out, msgs
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = CORE_SOURCES + GLOBAL_SOURCES
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        results = dict(zip(srcs, ex.map(_compile, srcs)))
+    for s, (_, msgs) in results.items():
+        if msgs and verbose:
+            print(f"[{s}]\n{msgs}")
+    core_objs = [results[s][0] for s in CORE_SOURCES]
+    glob_objs = [results[s][0] for s in GLOBAL_SOURCES]
+    for out, objs in ((LIB, core_objs + glob_objs), (LIB_CORE, core_objs)):
+        tmp = out + ".tmp"
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed for {out}:\n{r.stdout}{r.stderr}")
+        os.replace(tmp, out)
+    return LIB
+
+
+def clean():
+    shutil.rmtree(BUILD, ignore_errors=True)
+    for p in (LIB, LIB_CORE):
+        if os.path.exists(p):
+            os.remove(p)
+
+
+if __name__ == "__main__":
+    if "clean" in sys.argv[1:]:
+        clean()
+    print(build(verbose=True))

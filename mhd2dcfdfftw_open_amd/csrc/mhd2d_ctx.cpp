@@ -1,0 +1,520 @@
+// Context API: whole RK3 steps on the device with the fused schedule
+//   stage s:  y-sweep  (pending y-filter of stage s-1, d/dy of u, E, d/dy of E,
+//                       CFL limits when s == 0)
+//             [s == 0: dt finalize, the reference's calcDt + rkInit]
+//             x-derivative of u and E        (11 compact line solves)
+//             RHS + low-storage RK update    (pointwise)
+//             x-filter of u                  (8 compact line solves)
+// which is the reference order of mhd.c:154-206 with the y-filter of each
+// stage carried into the next stage's y-sweep.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mhd2d.h"
+#include "mhd2d_internal.h"
+#include "mhd2d_tables.h"
+
+using namespace mhd2d;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const char* what, hipError_t e = hipSuccess)
+{
+    g_err = what;
+    if (e != hipSuccess) {
+        g_err += ": ";
+        g_err += hipGetErrorString(e);
+    }
+    return code;
+}
+}  // namespace
+
+#define HIPCHK(expr)                                                        \
+    do {                                                                    \
+        hipError_t e_ = (expr);                                             \
+        if (e_ != hipSuccess) return fail(MHD2D_EHIP, #expr, e_);           \
+    } while (0)
+
+struct mhd2d_ctx {
+    mhd2d_params p{};
+    int elem = 8;
+    size_t n = 0;
+    GridTables tb;
+    hipStream_t stream = nullptr;
+    void* u = nullptr;      // [8][nx][ny]
+    void* dudx = nullptr;   // [11][nx][ny]: 8 state derivatives + 3 of E
+    void* dudy = nullptr;   // [11][nx][ny]
+    void* E = nullptr;      // [3][nx][ny]
+    void* rprev = nullptr;  // [8][nx][ny]
+    RunScalars* rs = nullptr;
+    double* partial = nullptr;
+    int npartial = 0;
+    bool pending_filter = false;
+    size_t bytes = 0;
+    hipGraphExec_t graph = nullptr;
+    bool have_state = false;
+};
+
+namespace {
+
+DtParams dt_params(const mhd2d_ctx* c)
+{
+    DtParams d;
+    d.dx = c->tb.dx;
+    d.dy = c->tb.dy;
+    d.gamma = c->p.gamma;
+    d.cfl = c->p.cfl;
+    d.di = c->p.di;
+    d.resistivity = c->p.resistivity;
+    d.viscosity = c->p.viscosity;
+    d.hall = c->p.hall;
+    return d;
+}
+
+template <typename T>
+hipError_t ysweep(mhd2d_ctx* c, int filter, int derivs, int dt_limits)
+{
+    YSweepArgs<T> a;
+    T* dudy = (T*)c->dudy;
+    a.u = (T*)c->u;
+    a.dudy = dudy;
+    a.E = (T*)c->E;
+    a.dEdy = dudy + 8 * c->n;
+    a.tb = c->tb.yt;
+    a.dp = dt_params(c);
+    a.rs = c->rs;
+    a.nx = c->p.nx;
+    a.ny = c->p.ny;
+    a.filter = filter;
+    a.derivs = derivs;
+    a.dt_limits = dt_limits;
+    return launch_ysweep<T>(a, c->stream);
+}
+
+template <typename T>
+hipError_t xderiv(mhd2d_ctx* c)
+{
+    XSolveArgs<T> a;
+    T* u = (T*)c->u;
+    T* E = (T*)c->E;
+    T* dx = (T*)c->dudx;
+    for (int v = 0; v < 8; v++) a.in[v] = u + v * c->n;
+    for (int v = 0; v < 3; v++) a.in[8 + v] = E + v * c->n;
+    for (int v = 0; v < 11; v++) a.out[v] = dx + v * c->n;
+    a.st = c->tb.st[XD1];
+    a.tb = c->tb.xt[XD1];
+    a.kind = XD1;
+    a.nx = c->p.nx;
+    a.ny = c->p.ny;
+    return launch_xsolve<T>(a, 11, c->stream);
+}
+
+template <typename T>
+hipError_t xfilter(mhd2d_ctx* c)
+{
+    XSolveArgs<T> a;
+    T* u = (T*)c->u;
+    for (int v = 0; v < 8; v++) a.in[v] = a.out[v] = u + v * c->n;
+    a.st = c->tb.st[XFILT];
+    a.tb = c->tb.xt[XFILT];
+    a.kind = XFILT;
+    a.nx = c->p.nx;
+    a.ny = c->p.ny;
+    return launch_xsolve<T>(a, 8, c->stream);
+}
+
+template <typename T>
+hipError_t rhs_rk(mhd2d_ctx* c, int stage)
+{
+    RhsRkArgs<T> a;
+    T* dx = (T*)c->dudx;
+    T* dy = (T*)c->dudy;
+    a.u = (T*)c->u;
+    a.dudx = dx;
+    a.dudy = dy;
+    a.dEdx = dx + 8 * c->n;
+    a.dEdy = dy + 8 * c->n;
+    a.rprev = (T*)c->rprev;
+    a.rs = c->rs;
+    a.gamma = c->p.gamma;
+    a.stage = stage;
+    a.write_rprev = stage < 2;  // rkInit zeroes R_prev before it is read again
+    a.n = c->n;
+    return launch_rhs_rk<T>(a, c->stream);
+}
+
+// one RK3 step; when ev is given, an event is recorded after every launch
+template <typename T>
+hipError_t step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
+{
+    hipError_t e;
+    auto mark = [&](int) {
+        if (ev) (void)hipEventRecord(ev[(*nev)++], c->stream);
+    };
+    for (int s = 0; s < 3; s++) {
+        e = ysweep<T>(c, (s > 0 || c->pending_filter) ? 1 : 0, 1, s == 0 ? 1 : 0);
+        if (e != hipSuccess) return e;
+        mark(0);
+        if (s == 0) {
+            e = launch_dt_finalize(c->rs, c->p.cfl, 0, c->stream);
+            if (e != hipSuccess) return e;
+            mark(0);
+        }
+        e = xderiv<T>(c);
+        if (e != hipSuccess) return e;
+        mark(0);
+        e = rhs_rk<T>(c, s);
+        if (e != hipSuccess) return e;
+        mark(0);
+        e = xfilter<T>(c);
+        if (e != hipSuccess) return e;
+        mark(0);
+    }
+    c->pending_filter = true;
+    return hipSuccess;
+}
+
+template <typename T>
+hipError_t flush(mhd2d_ctx* c)
+{
+    if (!c->pending_filter) return hipSuccess;
+    hipError_t e = ysweep<T>(c, 1, 0, 1);
+    c->pending_filter = false;
+    return e;
+}
+
+hipError_t do_flush(mhd2d_ctx* c)
+{
+    return c->elem == 8 ? flush<double>(c) : flush<float>(c);
+}
+
+hipError_t do_step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
+{
+    return c->elem == 8 ? step<double>(c, ev, nev) : step<float>(c, ev, nev);
+}
+
+__global__ void k_to_float(const double* __restrict__ a, float* __restrict__ b, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+__global__ void k_to_double(const float* __restrict__ a, double* __restrict__ b, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        b[i] = (double)a[i];
+}
+
+int check_params(const mhd2d_params* p)
+{
+    auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+    if (!pow2(p->nx) || !pow2(p->ny) || p->nx < 16 || p->nx > 4096 || p->ny < 16 ||
+        p->ny > 4096)
+        return fail(MHD2D_EINVAL, "nx and ny must be powers of two in [16, 4096]");
+    if (!(p->Lx > 0) || !(p->Ly > 0) || !(p->cfl > 0) || !(p->gamma > 1))
+        return fail(MHD2D_EINVAL, "Lx, Ly, cfl must be > 0 and gamma > 1");
+    if (p->resistivity != 0 || p->viscosity != 0 || p->divbclean != 0)
+        return fail(MHD2D_EINVAL,
+                    "the context API runs the ideal path only (resistivity = viscosity = "
+                    "divbclean = 0); use the drop-in entry points for the dissipative terms");
+    if (p->precision != MHD2D_FP64 && p->precision != MHD2D_FP32)
+        return fail(MHD2D_EINVAL, "precision must be MHD2D_FP64 or MHD2D_FP32");
+    return MHD2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mhd2d_last_error(void) { return g_err.c_str(); }
+
+int mhd2d_create(const mhd2d_params* p, mhd2d_ctx** out)
+{
+    if (!p || !out) return fail(MHD2D_EINVAL, "null argument");
+    *out = nullptr;
+    int rc = check_params(p);
+    if (rc) return rc;
+    mhd2d_ctx* c = new mhd2d_ctx();
+    c->p = *p;
+    c->elem = p->precision == MHD2D_FP64 ? 8 : 4;
+    c->n = (size_t)p->nx * p->ny;
+    hipError_t e = c->tb.build(p->nx, p->ny, p->Lx, p->Ly, c->elem);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(MHD2D_EHIP, "table build", e);
+    }
+    const size_t fb = c->n * c->elem;
+    c->npartial = 1024;
+    size_t sizes[] = {8 * fb, 11 * fb, 11 * fb, 3 * fb, 8 * fb};
+    void** ptrs[] = {&c->u, &c->dudx, &c->dudy, &c->E, &c->rprev};
+    for (int k = 0; k < 5; k++) {
+        e = hipMalloc(ptrs[k], sizes[k]);
+        if (e != hipSuccess) {
+            mhd2d_destroy(c);
+            return fail(MHD2D_ENOMEM, "device allocation", e);
+        }
+        c->bytes += sizes[k];
+    }
+    e = hipMalloc(&c->rs, sizeof(RunScalars));
+    if (e == hipSuccess) e = hipMalloc(&c->partial, 2 * c->npartial * sizeof(double));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        mhd2d_destroy(c);
+        return fail(MHD2D_EHIP, "context setup", e);
+    }
+    *out = c;
+    return MHD2D_OK;
+}
+
+void mhd2d_destroy(mhd2d_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    void* ptrs[] = {c->u, c->dudx, c->dudy, c->E, c->rprev, c->rs, c->partial};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->tb.release();
+    delete c;
+}
+
+size_t mhd2d_device_bytes(const mhd2d_ctx* c) { return c ? c->bytes : 0; }
+
+int mhd2d_set_state(mhd2d_ctx* c, const double* uu)
+{
+    if (!c || !uu) return fail(MHD2D_EINVAL, "null argument");
+    const size_t cnt = 8 * c->n;
+    if (c->elem == 8) {
+        HIPCHK(hipMemcpyAsync(c->u, uu, cnt * 8, hipMemcpyHostToDevice, c->stream));
+    } else {
+        // stage through the R_prev buffer (8 fields of FP32 = 4 of FP64)
+        double* tmp = nullptr;
+        HIPCHK(hipMalloc(&tmp, cnt * 8));
+        HIPCHK(hipMemcpyAsync(tmp, uu, cnt * 8, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_to_float, dim3(2048), dim3(256), 0, c->stream, tmp, (float*)c->u, cnt);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(tmp));
+    }
+    RunScalars z;
+    std::memset(&z, 0, sizeof(z));
+    z.dtmin_bits = 0x7ff0000000000000ull;
+    HIPCHK(hipMemcpyAsync(c->rs, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->pending_filter = false;
+    c->have_state = true;
+    return MHD2D_OK;
+}
+
+int mhd2d_get_state(mhd2d_ctx* c, double* uu)
+{
+    if (!c || !uu) return fail(MHD2D_EINVAL, "null argument");
+    HIPCHK(do_flush(c));
+    const size_t cnt = 8 * c->n;
+    if (c->elem == 8) {
+        HIPCHK(hipMemcpyAsync(uu, c->u, cnt * 8, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        double* tmp = nullptr;
+        HIPCHK(hipMalloc(&tmp, cnt * 8));
+        hipLaunchKernelGGL(k_to_double, dim3(2048), dim3(256), 0, c->stream, (const float*)c->u,
+                           tmp, cnt);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(uu, tmp, cnt * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(tmp));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MHD2D_OK;
+}
+
+int mhd2d_run(mhd2d_ctx* c, int nsteps)
+{
+    if (!c || nsteps < 0) return fail(MHD2D_EINVAL, "bad argument");
+    if (!c->have_state) return fail(MHD2D_EINVAL, "mhd2d_set_state must come first");
+    for (int k = 0; k < nsteps; k++) HIPCHK(do_step(c, nullptr, nullptr));
+    return MHD2D_OK;
+}
+
+int mhd2d_run_graph(mhd2d_ctx* c, int nsteps)
+{
+    if (!c || nsteps < 0) return fail(MHD2D_EINVAL, "bad argument");
+    if (!c->have_state) return fail(MHD2D_EINVAL, "mhd2d_set_state must come first");
+    if (nsteps == 0) return MHD2D_OK;
+    int k = 0;
+    if (!c->pending_filter) {  // the first step differs (no pending y filter)
+        HIPCHK(do_step(c, nullptr, nullptr));
+        k = 1;
+    }
+    if (k == nsteps) return MHD2D_OK;
+    if (!c->graph) {
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = do_step(c, nullptr, nullptr);
+        hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+        if (e != hipSuccess) return fail(MHD2D_EHIP, "capture", e);
+        HIPCHK(e2);
+        HIPCHK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+    }
+    for (; k < nsteps; k++) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    return MHD2D_OK;
+}
+
+int mhd2d_sync(mhd2d_ctx* c)
+{
+    if (!c) return fail(MHD2D_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int nf = 0;
+    HIPCHK(hipMemcpy(&nf, &c->rs->nonfinite, sizeof(int), hipMemcpyDeviceToHost));
+    if (nf) return fail(MHD2D_ENONFINITE, "a CFL limit was not finite (state blew up)");
+    return MHD2D_OK;
+}
+
+int mhd2d_get_scalars(mhd2d_ctx* c, double* dt, double* time, long long* steps)
+{
+    if (!c) return fail(MHD2D_EINVAL, "null argument");
+    RunScalars r;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&r, c->rs, sizeof(r), hipMemcpyDeviceToHost));
+    // rs->time holds the sum of the dts of all steps but the last
+    if (dt) *dt = r.dt;
+    if (time) *time = r.step > 0 ? r.time + r.dt : 0.0;
+    if (steps) *steps = r.step;
+    return MHD2D_OK;
+}
+
+int mhd2d_diagnostics(mhd2d_ctx* c, double* energy, double* max_divb)
+{
+    if (!c) return fail(MHD2D_EINVAL, "null argument");
+    HIPCHK(do_flush(c));
+    // d(Bx)/dx and d(By)/dy of the current state into the E scratch buffer
+    const size_t n = c->n;
+    int nb = c->npartial;
+    if (c->elem == 8) {
+        double* u = (double*)c->u;
+        double* scratch = (double*)c->E;
+        XSolveArgs<double> xa;
+        xa.in[0] = u + BX * n;
+        xa.out[0] = scratch;
+        xa.st = c->tb.st[XD1];
+        xa.tb = c->tb.xt[XD1];
+        xa.kind = XD1;
+        xa.nx = c->p.nx;
+        xa.ny = c->p.ny;
+        HIPCHK(launch_xsolve<double>(xa, 1, c->stream));
+        YApplyArgs<double> ya;
+        ya.in[0] = u + BY * n;
+        ya.out[0] = scratch + n;
+        ya.tb = c->tb.yt;
+        ya.nv = 1;
+        ya.mode = Y_D1;
+        ya.nx = c->p.nx;
+        ya.ny = c->p.ny;
+        HIPCHK(launch_yapply<double>(ya, c->stream));
+        HIPCHK(launch_diag<double>(u, scratch, scratch + n, c->partial, nb, n, c->p.gamma,
+                                   c->stream));
+    } else {
+        float* u = (float*)c->u;
+        float* scratch = (float*)c->E;
+        XSolveArgs<float> xa;
+        xa.in[0] = u + BX * n;
+        xa.out[0] = scratch;
+        xa.st = c->tb.st[XD1];
+        xa.tb = c->tb.xt[XD1];
+        xa.kind = XD1;
+        xa.nx = c->p.nx;
+        xa.ny = c->p.ny;
+        HIPCHK(launch_xsolve<float>(xa, 1, c->stream));
+        YApplyArgs<float> ya;
+        ya.in[0] = u + BY * n;
+        ya.out[0] = scratch + n;
+        ya.tb = c->tb.yt;
+        ya.nv = 1;
+        ya.mode = Y_D1;
+        ya.nx = c->p.nx;
+        ya.ny = c->p.ny;
+        HIPCHK(launch_yapply<float>(ya, c->stream));
+        HIPCHK(launch_diag<float>(u, scratch, scratch + n, c->partial, nb, n, c->p.gamma,
+                                  c->stream));
+    }
+    std::vector<double> h(2 * nb);
+    HIPCHK(hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double e = 0.0, m = 0.0;
+    for (int b = 0; b < nb; b++) {
+        e += h[2 * b];
+        m = std::fmax(m, h[2 * b + 1]);
+    }
+    if (energy) *energy = e * c->tb.dx * c->tb.dy;
+    if (max_divb) *max_divb = m;
+    return MHD2D_OK;
+}
+
+int mhd2d_time_run(mhd2d_ctx* c, int nsteps, int use_graph, double* ms)
+{
+    if (!c || !ms) return fail(MHD2D_EINVAL, "null argument");
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipEventRecord(a, c->stream));
+    int rc = use_graph ? mhd2d_run_graph(c, nsteps) : mhd2d_run(c, nsteps);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(b, c->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float f = 0;
+    HIPCHK(hipEventElapsedTime(&f, a, b));
+    *ms = f;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return MHD2D_OK;
+}
+
+int mhd2d_profile_run(mhd2d_ctx* c, int nsteps, double* out_ms, int* out_launches)
+{
+    if (!c || !out_ms || nsteps < 1) return fail(MHD2D_EINVAL, "bad argument");
+    // per step: 3 stages x (ysweep, xderiv, rhs_rk, xfilter) + one dt finalize;
+    // event k+1 closes the launch opened by event k.
+    const int per_step = 3 * 5 + 1;
+    std::vector<hipEvent_t> ev(per_step + 1);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    double acc[5] = {0, 0, 0, 0, 0};
+    int cnt[5] = {0, 0, 0, 0, 0};
+    // classes in launch order within a step
+    const int cls0[] = {0, 4, 1, 2, 3};
+    const int cls[] = {0, 1, 2, 3};
+    for (int k = 0; k < nsteps; k++) {
+        HIPCHK(hipEventRecord(ev[0], c->stream));
+        int nev = 1;
+        HIPCHK(do_step(c, ev.data(), &nev));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        int idx = 0;
+        for (int s = 0; s < 3; s++) {
+            const int* cl = s == 0 ? cls0 : cls;
+            const int m = s == 0 ? 5 : 4;
+            for (int q = 0; q < m; q++, idx++) {
+                float f = 0;
+                HIPCHK(hipEventElapsedTime(&f, ev[idx], ev[idx + 1]));
+                acc[cl[q]] += f;
+                cnt[cl[q]] += 1;
+            }
+        }
+    }
+    for (int q = 0; q < 5; q++) {
+        out_ms[q] = cnt[q] ? acc[q] / cnt[q] : 0.0;
+        if (out_launches) out_launches[q] = cnt[q];
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return MHD2D_OK;
+}
+
+}  // extern "C"

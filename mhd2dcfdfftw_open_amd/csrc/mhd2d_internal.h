@@ -1,0 +1,177 @@
+// Internal declarations shared by the HIP kernels, the context API and the
+// drop-in layer.  Not part of the public C ABI (include/mhd2d.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace mhd2d {
+
+// field indices of the state, same order as the reference (macros.h:4-11)
+enum Field { RHO = 0, UX, UY, UZ, BX, BY, BZ, PR, NFIELD = 8 };
+
+// ---------------------------------------------------------------- x operators
+enum XKind { XD1 = 0, XD2 = 1, XFILT = 2, NXKIND = 3 };
+
+// Scaled stencil weights of one x operator.  Interior rows:
+//   D1: r = c1 (f+1 - f-1) + c2 (f+2 - f-2) + c3 (f+3 - f-3)
+//   D2: r = c1 (f+1 - 2f + f-1) + c2 (f+2 - 2f + f-2) + c3 (f+3 - 2f + f-3)
+//   F : r = c0 f + c1 (f+1 + f-1) + c2 (f+2 + f-2) + c3 (f+3 + f-3)
+// Row 0: r = sum_k e[k] f[k-2]; row N-1: r = esign * sum_k e[k] f[N+1-k].
+struct XStencil {
+    double c0, c1, c2, c3;
+    double e[5];
+    double esign;
+};
+
+// Thomas factors of one x operator's matrix (computed once, FP64) and the
+// chunk products used to carry the two recurrences across thread chunks.
+//   forward : y_i = r_i - l_i y_{i-1}           (l_0 = 0)
+//   backward: x_i = y_i * invd_i + q_i x_{i+1}  (q_{N-1} = 0)
+struct XTables {
+    const double* l;      // [nx]
+    const double* invd;   // [nx]
+    const double* q;      // [nx]
+    const double* pf;     // [G] product of -l over each chunk
+    const double* qb;     // [G] product of q over each chunk
+};
+
+template <typename T>
+struct XSolveArgs {
+    const T* in[16];
+    T* out[16];
+    XStencil st;
+    XTables tb;
+    int kind;
+    int nx, ny;
+};
+
+// ---------------------------------------------------------------- y operators
+// Spectral multipliers, precomputed in FP64 for k = 0..ny-1 (full complex
+// spectrum, signed wavenumbers):
+//   md [k] = kappa(k) / ny          (first derivative, Nyquist -> 0)
+//   mdd[k] = -kappa(k)^2 / ny       (second derivative, Nyquist kept)
+//   mf [k] = filtY(|k|) / ny        (y filter)
+//   mfd[k] = filtY(|k|) kappa(k) / ny
+// with kappa(k) = 2 pi / Ly * (k <= ny/2 ? k : k - ny) and the FFT twiddles
+// tw[t] = exp(-2 pi i t / ny).
+struct YTables {
+    const double* md;
+    const double* mdd;
+    const double* mf;
+    const double* mfd;
+    const double2* tw;
+};
+
+enum YMode { Y_D1 = 0, Y_D2 = 1, Y_F = 2 };
+
+template <typename T>
+struct YApplyArgs {          // generic multiplier pass (drop-in entry points)
+    const T* in[16];
+    T* out[16];
+    YTables tb;
+    int nv, mode;
+    int nx, ny;
+};
+
+struct DtParams {
+    double dx, dy, gamma, cfl, di, resistivity, viscosity;
+    int hall;
+};
+
+// device-side scalars of one run
+struct RunScalars {
+    unsigned long long dtmin_bits;   // running min of the local dt limits
+    int nonfinite;                   // set when a dt limit is not finite
+    int pad;
+    double dt;                       // dt of the current step
+    double dt_prev;                  // dt of the previous step (0 at start)
+    double time;                     // simulated time
+    double cc[3], dd[3];             // RK3 coefficients times dt
+    long long step;
+};
+
+template <typename T>
+struct YSweepArgs {          // fused y-sweep of the driver
+    T* u;                    // [8][nx][ny], in place (filtered)
+    T* dudy;                 // [8][nx][ny]
+    T* E;                    // [3][nx][ny]
+    T* dEdy;                 // [3][nx][ny]
+    YTables tb;
+    DtParams dp;
+    RunScalars* rs;
+    int nx, ny;
+    int filter;              // apply the pending y filter first
+    int dt_limits;           // reduce the local dt limits into rs
+    int derivs;              // compute the y derivatives and E
+};
+
+template <typename T>
+struct RhsArgs {
+    const T* u;
+    const T* dudx;
+    const T* dudy;
+    const T* dEdx;
+    const T* dEdy;
+    const T* d2udx2;  // optional (viscosity)
+    const T* d2udy2;
+    const T* d2Bdx2;  // optional (resistivity)
+    const T* d2Bdy2;
+    const T* ddivBdx; // optional (diffusive div B cleaning)
+    const T* ddivBdy;
+    T* dudt;
+    double gamma, resistivity, viscosity, constDiffDivB;
+    int divbclean;
+    size_t n;
+};
+
+template <typename T>
+struct RhsRkArgs {           // fused RHS + low-storage RK update (driver)
+    T* u;
+    const T* dudx;
+    const T* dudy;
+    const T* dEdx;
+    const T* dEdy;
+    T* rprev;
+    const RunScalars* rs;
+    double gamma;
+    int stage;
+    int write_rprev;
+    size_t n;
+};
+
+// ---------------------------------------------------------------- launchers
+template <typename T>
+hipError_t launch_xsolve(const XSolveArgs<T>& a, int nv, hipStream_t s);
+template <typename T>
+hipError_t launch_yapply(const YApplyArgs<T>& a, hipStream_t s);
+template <typename T>
+hipError_t launch_ysweep(const YSweepArgs<T>& a, hipStream_t s);
+template <typename T>
+hipError_t launch_rhs(const RhsArgs<T>& a, hipStream_t s);
+template <typename T>
+hipError_t launch_rhs_rk(const RhsRkArgs<T>& a, hipStream_t s);
+template <typename T>
+hipError_t launch_efield(const T* u, T* E, size_t n, hipStream_t s);
+template <typename T>
+hipError_t launch_divb(const T* dudx, const T* dudy, T* divB, size_t n, hipStream_t s);
+template <typename T>
+hipError_t launch_rk(T* u, const T* dudt, T* dudtRK, double cc, double dd, size_t n,
+                     hipStream_t s);
+template <typename T>
+hipError_t launch_dt_limits(const T* u, const DtParams& dp, RunScalars* rs, int nx,
+                            int ny, hipStream_t s);
+hipError_t launch_dt_finalize(RunScalars* rs, double cfl, int first, hipStream_t s);
+hipError_t launch_dt_reset(RunScalars* rs, hipStream_t s);
+template <typename T>
+hipError_t launch_fill(T* p, T v, size_t n, hipStream_t s);
+template <typename T>
+hipError_t launch_diag(const T* u, const T* dudx, const T* dudy, double* partial,
+                       int nblocks, size_t n, double gamma, hipStream_t s);
+
+// chunk length (rows per thread) and strip width used by the x solves
+void xsolve_shape(int nx, int elem_bytes, int* chunk, int* width, int* groups);
+int ysweep_supported(int ny);
+
+}  // namespace mhd2d

@@ -1,0 +1,949 @@
+// Hand-written gfx950 kernels of the RHS / RK3 hot path.
+//
+//  k_xsolve   compact x-derivatives (1st, 2nd) and the compact x-filter:
+//             7-point stencil + constant-matrix tridiagonal solve along x,
+//             batched over (field, 16-column strip).  Each thread owns a chunk
+//             of rows of one column; the Thomas recurrences are carried across
+//             chunks by a segmented wavefront-shuffle scan.  The periodic
+//             x-halo of the reference (parallel.c:13-24) is an index wrap.
+//  k_yapply   spectral y operators (derivative, 2nd derivative, filter) on
+//             two real fields packed into one complex line, radix-8/4/2
+//             Stockham FFT with the line held in registers + LDS.
+//  k_ysweep   fused y-sweep of the driver: pending y-filter, y-derivatives of
+//             the 8 state fields, E = -u x B and its y-derivative, and the
+//             CFL limits of the new state in one pass over each row.
+//  pointwise  E field, div B, RHS, RK update, fused RHS + RK, dt reduction.
+#include "mhd2d_internal.h"
+
+#include <cmath>
+
+namespace mhd2d {
+
+// ============================================================== complex
+template <typename T>
+struct cx {
+    T re, im;
+};
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, double2 w)
+{
+    const T wr = (T)w.x, wi = (T)w.y;
+    return {a.re * wr - a.im * wi, a.re * wi + a.im * wr};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> mul_mi(cx<T> a) { return {a.im, -a.re}; }  // times -i
+template <typename T>
+__device__ __forceinline__ void bfly(cx<T>& a, cx<T>& b)
+{
+    const cx<T> t = a;
+    a = t + b;
+    b = t - b;
+}
+template <typename T>
+__device__ __forceinline__ void cswap(cx<T>& a, cx<T>& b)
+{
+    const cx<T> t = a;
+    a = b;
+    b = t;
+}
+
+// in-order forward DFTs of length 2, 4, 8 (X_k = sum_n x_n e^{-2 pi i nk/R})
+template <typename T, int R>
+__device__ __forceinline__ void dft(cx<T>* v);
+
+template <>
+__device__ __forceinline__ void dft<double, 2>(cx<double>* v) { bfly(v[0], v[1]); }
+template <>
+__device__ __forceinline__ void dft<float, 2>(cx<float>* v) { bfly(v[0], v[1]); }
+
+template <typename T>
+__device__ __forceinline__ void dft4(cx<T>* v)
+{
+    bfly(v[0], v[2]);
+    bfly(v[1], v[3]);
+    v[3] = mul_mi(v[3]);
+    bfly(v[0], v[1]);
+    bfly(v[2], v[3]);
+    cswap(v[1], v[2]);
+}
+template <>
+__device__ __forceinline__ void dft<double, 4>(cx<double>* v) { dft4(v); }
+template <>
+__device__ __forceinline__ void dft<float, 4>(cx<float>* v) { dft4(v); }
+
+template <typename T>
+__device__ __forceinline__ void dft8(cx<T>* v)
+{
+    const T s = (T)0.70710678118654752440084436210485;
+    bfly(v[0], v[4]);
+    bfly(v[1], v[5]);
+    bfly(v[2], v[6]);
+    bfly(v[3], v[7]);
+    v[5] = cx<T>{s * (v[5].re + v[5].im), s * (v[5].im - v[5].re)};   // * e^{-i pi/4}
+    v[6] = mul_mi(v[6]);                                               // * e^{-i pi/2}
+    v[7] = cx<T>{s * (v[7].im - v[7].re), -s * (v[7].re + v[7].im)};  // * e^{-3i pi/4}
+    bfly(v[0], v[2]);
+    bfly(v[1], v[3]);
+    bfly(v[4], v[6]);
+    bfly(v[5], v[7]);
+    v[3] = mul_mi(v[3]);
+    v[7] = mul_mi(v[7]);
+    bfly(v[0], v[1]);
+    bfly(v[2], v[3]);
+    bfly(v[4], v[5]);
+    bfly(v[6], v[7]);
+    cswap(v[1], v[4]);  // bit-reversed -> natural order
+    cswap(v[3], v[6]);
+}
+template <>
+__device__ __forceinline__ void dft<double, 8>(cx<double>* v) { dft8(v); }
+template <>
+__device__ __forceinline__ void dft<float, 8>(cx<float>* v) { dft8(v); }
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+constexpr int ipow8(int p) { return p == 0 ? 1 : 8 * ipow8(p - 1); }
+
+// One Stockham pass (radix 8 except possibly the last, which is 4 or 2).
+// Thread t of a row owns, before the first and after the last pass, the
+// elements at positions t + m*N/8, m = 0..7 ("strided" layout).
+template <typename T, int N, int P>
+__device__ __forceinline__ void fft_pass(cx<T> (&v)[8], int t, cx<T>* lds,
+                                         const double2* __restrict__ tw)
+{
+    constexpr int M = ilog2c(N), P8 = M / 3, REM = M % 3;
+    constexpr int NP = P8 + (REM ? 1 : 0);
+    constexpr int R = (P < P8) ? 8 : (REM == 2 ? 4 : 2);
+    constexpr int B = 8 / R;  // butterflies per thread in this pass
+    constexpr int TPR = N / 8;
+    constexpr int NS = ipow8(P);
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+        const int j = t + b * TPR;
+        if constexpr (NS > 1) {
+            const int k = j & (NS - 1);
+            constexpr int STEP = N / (NS * R);
+#pragma unroll
+            for (int r = 1; r < R; r++) v[b * R + r] = cmul(v[b * R + r], tw[r * k * STEP]);
+        }
+        dft<T, R>(&v[b * R]);
+    }
+    if constexpr (P + 1 < NP) {
+        // intermediate passes are radix 8 (B == 1)
+        constexpr int R2 = (P + 1 < P8) ? 8 : (REM == 2 ? 4 : 2);
+        constexpr int B2 = 8 / R2;
+        {
+            const int j = t;
+            const int base = (j / NS) * NS * R + (j & (NS - 1));
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[base + r * NS] = v[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < B2; b++) {
+            const int j = t + b * TPR;
+#pragma unroll
+            for (int r = 0; r < R2; r++) v[b * R2 + r] = lds[j + r * (N / R2)];
+        }
+        __syncthreads();
+        fft_pass<T, N, P + 1>(v, t, lds, tw);
+    } else if constexpr (B > 1) {
+        // last pass: element (b, r) sits at t + TPR * (b + r * B)
+        cx<T> tmp[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) tmp[m] = v[(m % B) * R + m / B];
+#pragma unroll
+        for (int m = 0; m < 8; m++) v[m] = tmp[m];
+    }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void fft_fwd(cx<T> (&v)[8], int t, cx<T>* lds,
+                                        const double2* __restrict__ tw)
+{
+    fft_pass<T, N, 0>(v, t, lds, tw);
+}
+
+// unnormalised inverse: conj(FFT(conj(x)))
+template <typename T, int N>
+__device__ __forceinline__ void fft_inv(cx<T> (&v)[8], int t, cx<T>* lds,
+                                        const double2* __restrict__ tw)
+{
+#pragma unroll
+    for (int m = 0; m < 8; m++) v[m].im = -v[m].im;
+    fft_pass<T, N, 0>(v, t, lds, tw);
+#pragma unroll
+    for (int m = 0; m < 8; m++) v[m].im = -v[m].im;
+}
+
+// v <- v * (i * s_k)  (imaginary multiplier)
+template <typename T>
+__device__ __forceinline__ cx<T> mul_i_scal(cx<T> a, T s) { return {-a.im * s, a.re * s}; }
+template <typename T>
+__device__ __forceinline__ cx<T> mul_scal(cx<T> a, T s) { return {a.re * s, a.im * s}; }
+
+// ============================================================== dt limits
+// per-cell CFL limit of calcDt, timeAdvance.c:50-136 (divBcleaning==2 excluded)
+__device__ __forceinline__ double cell_dt(double rho, double ux, double uy, double bx,
+                                          double by, double bz, double p,
+                                          const DtParams& dp)
+{
+    const double cs2 = dp.gamma * p / rho;
+    const double sr = sqrt(rho);
+    const double cax = bx / sr, cay = by / sr, caz = bz / sr;
+    const double ca2 = cax * cax + cay * cay + caz * caz;
+    const double cms2 = cs2 + ca2;
+    const double nsx = sqrt(fmax(cms2 * cms2 - 4 * cs2 * cax * cax, 0.));
+    const double nsy = sqrt(fmax(cms2 * cms2 - 4 * cs2 * cay * cay, 0.));
+    const double r2 = 1.0 / sqrt(2.);
+    const double cfx = sqrt(cms2 + nsx) * r2, cfy = sqrt(cms2 + nsy) * r2;
+    const double csx = sqrt(fmax(cms2 - nsx, 0.)) * r2, csy = sqrt(fmax(cms2 - nsy, 0.)) * r2;
+    double cmx = fmax(fmax(fmax(fabs(ux), fabs(ux - cax)), fmax(fabs(ux - csx), fabs(ux - cfx))),
+                      fmax(fmax(fabs(ux + cax), fabs(ux + csx)), fabs(ux + cfx)));
+    double cmy = fmax(fmax(fmax(fabs(uy), fabs(uy - cay)), fmax(fabs(uy - csy), fabs(uy - cfy))),
+                      fmax(fmax(fabs(uy + cay), fabs(uy + csy)), fabs(uy + cfy)));
+    if (dp.hall == 1) {
+        const double ch = dp.di / fmin(dp.dx, dp.dy) / rho * fmax(fabs(bx), fmax(fabs(by), fabs(bz)));
+        cmx = fmax(cmx, ch);
+        cmy = fmax(cmy, ch);
+    }
+    double dtx = dp.dx / cmx, dty = dp.dy / cmy;
+    if (dp.resistivity > 0) {
+        dtx = fmin(dtx, dp.dx * dp.dx / dp.resistivity);
+        dty = fmin(dty, dp.dy * dp.dy / dp.resistivity);
+    }
+    if (dp.viscosity > 0) {
+        dtx = fmin(dtx, dp.dx * dp.dx / dp.viscosity);
+        dty = fmin(dty, dp.dy * dp.dy / dp.viscosity);
+    }
+    return fmin(dtx, dty);
+}
+
+// block-wide min of one value per thread, then one atomic per block.
+// Positive doubles order like their bit patterns, so the running minimum is
+// an integer atomicMin.  A non-finite or non-positive limit raises a flag.
+__device__ __forceinline__ void block_min_dt(double v, RunScalars* rs, double* red)
+{
+    if (!(v > 0.0) || !isfinite(v)) {
+        atomicOr(&rs->nonfinite, 1);
+        v = INFINITY;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        double m = red[0];
+        for (int k = 1; k < nw; k++) m = fmin(m, red[k]);
+        atomicMin(&rs->dtmin_bits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// ============================================================== x solves
+template <typename T, int C, int W, int KIND>
+__global__ void __launch_bounds__(W * 64) k_xsolve(XSolveArgs<T> a)
+{
+    const int G = blockDim.x / W;
+    const int c = threadIdx.x % W;
+    const int g = threadIdx.x / W;
+    const int v = blockIdx.y;
+    const int nx = a.nx;
+    const size_t ny = (size_t)a.ny;
+    const size_t j = (size_t)blockIdx.x * W + c;
+    const T* __restrict__ f = a.in[v] + j;
+    T* __restrict__ o = a.out[v] + j;
+    const int s = g * C;
+
+    __shared__ T sh_in[W * 64];
+    __shared__ T sh_fw[W * 64];
+    __shared__ T sh_bw[W * 64];
+
+    // rows s-3 .. s+C+2 of this column; only the 3-row halos can wrap in x
+    T w[C + 6];
+    {
+        const T* __restrict__ lo = f + (size_t)(s > 0 ? s - 3 : nx - 3) * ny;
+        const T* __restrict__ hi = f + (size_t)(s + C < nx ? s + C : 0) * ny;
+        const T* __restrict__ mid = f + (size_t)s * ny;
+#pragma unroll
+        for (int k = 0; k < 3; k++) w[k] = lo[k * ny];
+#pragma unroll
+        for (int k = 0; k < C; k++) w[k + 3] = mid[k * ny];
+#pragma unroll
+        for (int k = 0; k < 3; k++) w[C + 3 + k] = hi[k * ny];
+    }
+
+    const XStencil& st = a.st;
+    const T c0 = (T)st.c0, c1 = (T)st.c1, c2 = (T)st.c2, c3 = (T)st.c3;
+    const bool first = (g == 0), last = (g == G - 1);
+    T rfirst = 0, rlast = 0;
+    if (first)   // row 0 over f[-2..2]
+        rfirst = (T)st.e[0] * w[1] + (T)st.e[1] * w[2] + (T)st.e[2] * w[3] +
+                 (T)st.e[3] * w[4] + (T)st.e[4] * w[5];
+    if (last)    // row N-1 over f[N+1..N-3]
+        rlast = (T)st.esign * ((T)st.e[0] * w[C + 4] + (T)st.e[1] * w[C + 3] +
+                               (T)st.e[2] * w[C + 2] + (T)st.e[3] * w[C + 1] +
+                               (T)st.e[4] * w[C]);
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+        const T fm3 = w[k], fm2 = w[k + 1], fm1 = w[k + 2], f0 = w[k + 3];
+        const T fp1 = w[k + 4], fp2 = w[k + 5], fp3 = w[k + 6];
+        T r;
+        if constexpr (KIND == XD1) {
+            r = c1 * (fp1 - fm1) + c2 * (fp2 - fm2) + c3 * (fp3 - fm3);
+        } else if constexpr (KIND == XD2) {
+            r = c1 * (fp1 - 2 * f0 + fm1) + c2 * (fp2 - 2 * f0 + fm2) + c3 * (fp3 - 2 * f0 + fm3);
+        } else {
+            r = c0 * f0 + c1 * (fp1 + fm1) + c2 * (fp2 + fm2) + c3 * (fp3 + fm3);
+        }
+        w[k] = r;  // w[k] is not read again by later rows
+    }
+    if (first) w[0] = rfirst;
+    if (last) w[C - 1] = rlast;
+
+    const double* __restrict__ L = a.tb.l + s;
+    const double* __restrict__ ID = a.tb.invd + s;
+    const double* __restrict__ Q = a.tb.q + s;
+
+    // ---- forward recurrence y_i = r_i - l_i y_{i-1}
+    T y = 0;
+#pragma unroll
+    for (int k = 0; k < C; k++) y = w[k] - (T)L[k] * y;
+    sh_in[threadIdx.x] = y;
+    __syncthreads();
+    {
+        const int c2i = threadIdx.x / G, g2 = threadIdx.x % G;
+        T B = sh_in[g2 * W + c2i];
+        T A = (T)a.tb.pf[g2];
+        for (int d = 1; d < G; d <<= 1) {
+            const T Ap = __shfl_up(A, d, G);
+            const T Bp = __shfl_up(B, d, G);
+            if (g2 >= d) {
+                B = B + A * Bp;
+                A = A * Ap;
+            }
+        }
+        sh_fw[g2 * W + c2i] = B;  // y at the last row of chunk g2
+    }
+    __syncthreads();
+    y = (g > 0) ? sh_fw[(g - 1) * W + c] : (T)0;
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+        y = w[k] - (T)L[k] * y;
+        w[k] = y;
+    }
+
+    // ---- backward recurrence x_i = y_i invd_i + q_i x_{i+1}
+    T x = 0;
+#pragma unroll
+    for (int k = C - 1; k >= 0; k--) x = w[k] * (T)ID[k] + (T)Q[k] * x;
+    sh_in[threadIdx.x] = x;
+    __syncthreads();
+    {
+        const int c2i = threadIdx.x / G, g2 = G - 1 - (threadIdx.x % G);
+        const int gr = threadIdx.x % G;  // position in the reversed scan
+        T B = sh_in[g2 * W + c2i];
+        T A = (T)a.tb.qb[g2];
+        for (int d = 1; d < G; d <<= 1) {
+            const T Ap = __shfl_up(A, d, G);
+            const T Bp = __shfl_up(B, d, G);
+            if (gr >= d) {
+                B = B + A * Bp;
+                A = A * Ap;
+            }
+        }
+        sh_bw[g2 * W + c2i] = B;  // x at the first row of chunk g2
+    }
+    __syncthreads();
+    x = (g < G - 1) ? sh_bw[(g + 1) * W + c] : (T)0;
+#pragma unroll
+    for (int k = C - 1; k >= 0; k--) {
+        x = w[k] * (T)ID[k] + (T)Q[k] * x;
+        w[k] = x;
+    }
+    T* __restrict__ po = o + (size_t)s * ny;
+#pragma unroll
+    for (int k = 0; k < C; k++) po[k * ny] = w[k];
+}
+
+void xsolve_shape(int nx, int elem_bytes, int* chunk, int* width, int* groups)
+{
+    int C = nx / 64;
+    if (C < 8) C = 8;
+    int W = 16;
+    if (elem_bytes == 8 && C > 32) W = 8;
+    *chunk = C;
+    *width = W;
+    *groups = nx / C;
+}
+
+template <typename T, int C, int W>
+static hipError_t launch_xsolve_cw(const XSolveArgs<T>& a, int nv, hipStream_t s)
+{
+    const int G = a.nx / C;
+    dim3 grid(a.ny / W, nv), block(W * G);
+    switch (a.kind) {
+    case XD1: hipLaunchKernelGGL((k_xsolve<T, C, W, XD1>), grid, block, 0, s, a); break;
+    case XD2: hipLaunchKernelGGL((k_xsolve<T, C, W, XD2>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((k_xsolve<T, C, W, XFILT>), grid, block, 0, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_xsolve(const XSolveArgs<T>& a, int nv, hipStream_t s)
+{
+    int C, W, G;
+    xsolve_shape(a.nx, sizeof(T), &C, &W, &G);
+    if (a.nx % C || G > 64 || G < 2 || a.ny % W || nv < 1 || nv > 16) return hipErrorInvalidValue;
+    switch (C) {
+    case 8: return launch_xsolve_cw<T, 8, 16>(a, nv, s);
+    case 16: return launch_xsolve_cw<T, 16, 16>(a, nv, s);
+    case 32: return launch_xsolve_cw<T, 32, 16>(a, nv, s);
+    case 64:
+        if (W == 8) return launch_xsolve_cw<T, 64, 8>(a, nv, s);
+        return launch_xsolve_cw<T, 64, 16>(a, nv, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ============================================================== y operators
+template <int N>
+struct YShape {
+    static constexpr int TPR = N / 8;
+    static constexpr int ROWS = (TPR >= 256) ? 1 : 256 / TPR;
+    static constexpr int THREADS = TPR * ROWS;
+};
+
+int ysweep_supported(int ny)
+{
+    return ny >= 8 && ny <= 4096 && (ny & (ny - 1)) == 0;
+}
+
+// generic spectral multiplier on nv fields (drop-in derivY / derivYY / filteringY)
+template <typename T, int N>
+__global__ void __launch_bounds__(YShape<N>::THREADS) k_yapply(YApplyArgs<T> a)
+{
+    constexpr int TPR = YShape<N>::TPR, ROWS = YShape<N>::ROWS;
+    __shared__ cx<T> lds_all[ROWS * N];
+    const int t = threadIdx.x % TPR, rl = threadIdx.x / TPR;
+    const int i_raw = blockIdx.x * ROWS + rl;
+    const bool active = i_raw < a.nx;
+    const size_t row = (size_t)(active ? i_raw : a.nx - 1) * N;
+    const size_t n = (size_t)a.nx * N;
+    cx<T>* lds = lds_all + rl * N;
+    const double* __restrict__ mul =
+        a.mode == Y_D1 ? a.tb.md : (a.mode == Y_D2 ? a.tb.mdd : a.tb.mf);
+    for (int p = 0; p < a.nv; p += 2) {
+        const bool two = (p + 1) < a.nv;
+        const T* fa = a.in[p] + row;
+        const T* fb = two ? a.in[p + 1] + row : nullptr;
+        cx<T> v[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const int k = t + m * TPR;
+            v[m] = cx<T>{fa[k], two ? fb[k] : (T)0};
+        }
+        fft_fwd<T, N>(v, t, lds, a.tb.tw);
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const int k = t + m * TPR;
+            const T s = (T)mul[k];
+            v[m] = (a.mode == Y_D1) ? mul_i_scal(v[m], s) : mul_scal(v[m], s);
+        }
+        fft_inv<T, N>(v, t, lds, a.tb.tw);
+        if (active) {
+            T* oa = a.out[p] + row;
+            T* ob = two ? a.out[p + 1] + row : nullptr;
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                const int k = t + m * TPR;
+                oa[k] = v[m].re;
+                if (two) ob[k] = v[m].im;
+            }
+        }
+        (void)n;
+    }
+}
+
+// fused y-sweep of one RK stage (driver)
+template <typename T, int N>
+__global__ void __launch_bounds__(YShape<N>::THREADS) k_ysweep(YSweepArgs<T> a)
+{
+    constexpr int TPR = YShape<N>::TPR, ROWS = YShape<N>::ROWS;
+    __shared__ cx<T> lds_all[ROWS * N];
+    __shared__ double red[16];
+    const int t = threadIdx.x % TPR, rl = threadIdx.x / TPR;
+    const int i_raw = blockIdx.x * ROWS + rl;
+    const bool active = i_raw < a.nx;
+    const size_t row = (size_t)(active ? i_raw : a.nx - 1) * N;
+    const size_t n = (size_t)a.nx * N;
+    cx<T>* lds = lds_all + rl * N;
+    const double2* __restrict__ tw = a.tb.tw;
+
+    T keep[8][8];  // filtered state at the 8 positions this thread owns
+    // field pairs packed as re/im of one complex line
+    constexpr int PA[4] = {RHO, UY, BX, BZ};
+    constexpr int PB[4] = {UX, UZ, BY, PR};
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const int fa = PA[p], fb = PB[p];
+        T* ua = a.u + fa * n + row;
+        T* ub = a.u + fb * n + row;
+        cx<T> v[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const int k = t + m * TPR;
+            v[m] = cx<T>{ua[k], ub[k]};
+        }
+        if (!a.filter) {
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                keep[fa][m] = v[m].re;
+                keep[fb][m] = v[m].im;
+            }
+        }
+        if (!a.filter && !a.derivs) continue;
+        fft_fwd<T, N>(v, t, lds, tw);
+        cx<T> Z[8];
+        if (a.filter) {
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                Z[m] = v[m];
+                v[m] = mul_scal(v[m], (T)a.tb.mf[t + m * TPR]);
+            }
+            fft_inv<T, N>(v, t, lds, tw);
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                keep[fa][m] = v[m].re;
+                keep[fb][m] = v[m].im;
+            }
+            if (active) {
+#pragma unroll
+                for (int m = 0; m < 8; m++) {
+                    const int k = t + m * TPR;
+                    ua[k] = v[m].re;
+                    ub[k] = v[m].im;
+                }
+            }
+            if (a.derivs) {
+#pragma unroll
+                for (int m = 0; m < 8; m++) v[m] = mul_i_scal(Z[m], (T)a.tb.mfd[t + m * TPR]);
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < 8; m++) v[m] = mul_i_scal(v[m], (T)a.tb.md[t + m * TPR]);
+        }
+        if (a.derivs) {
+            fft_inv<T, N>(v, t, lds, tw);
+            if (active) {
+                T* da = a.dudy + fa * n + row;
+                T* db = a.dudy + fb * n + row;
+#pragma unroll
+                for (int m = 0; m < 8; m++) {
+                    const int k = t + m * TPR;
+                    da[k] = v[m].re;
+                    db[k] = v[m].im;
+                }
+            }
+        }
+    }
+
+    if (a.derivs) {
+        // E = -u x B of the filtered state (calcRHS.c:24-43)
+        T ex[8], ey[8], ez[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            ex[m] = keep[UZ][m] * keep[BY][m] - keep[UY][m] * keep[BZ][m];
+            ey[m] = keep[UX][m] * keep[BZ][m] - keep[UZ][m] * keep[BX][m];
+            ez[m] = keep[UY][m] * keep[BX][m] - keep[UX][m] * keep[BY][m];
+        }
+        if (active) {
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                const int k = t + m * TPR;
+                a.E[0 * n + row + k] = ex[m];
+                a.E[1 * n + row + k] = ey[m];
+                a.E[2 * n + row + k] = ez[m];
+            }
+        }
+        cx<T> v[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) v[m] = cx<T>{ex[m], ey[m]};
+        fft_fwd<T, N>(v, t, lds, tw);
+#pragma unroll
+        for (int m = 0; m < 8; m++) v[m] = mul_i_scal(v[m], (T)a.tb.md[t + m * TPR]);
+        fft_inv<T, N>(v, t, lds, tw);
+        if (active) {
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                const int k = t + m * TPR;
+                a.dEdy[0 * n + row + k] = v[m].re;
+                a.dEdy[1 * n + row + k] = v[m].im;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 8; m++) v[m] = cx<T>{ez[m], (T)0};
+        fft_fwd<T, N>(v, t, lds, tw);
+#pragma unroll
+        for (int m = 0; m < 8; m++) v[m] = mul_i_scal(v[m], (T)a.tb.md[t + m * TPR]);
+        fft_inv<T, N>(v, t, lds, tw);
+        if (active) {
+#pragma unroll
+            for (int m = 0; m < 8; m++) a.dEdy[2 * n + row + t + m * TPR] = v[m].re;
+        }
+    }
+
+    if (a.dt_limits) {
+        double mn = INFINITY;
+        if (active) {
+#pragma unroll
+            for (int m = 0; m < 8; m++)
+                mn = fmin(mn, cell_dt(keep[RHO][m], keep[UX][m], keep[UY][m], keep[BX][m],
+                                      keep[BY][m], keep[BZ][m], keep[PR][m], a.dp));
+        }
+        block_min_dt(mn, a.rs, red);
+    }
+}
+
+template <typename T, int N>
+static hipError_t launch_y_n(const YApplyArgs<T>* ya, const YSweepArgs<T>* ys, hipStream_t s)
+{
+    const int nx = ya ? ya->nx : ys->nx;
+    dim3 grid((nx + YShape<N>::ROWS - 1) / YShape<N>::ROWS), block(YShape<N>::THREADS);
+    if (ya) hipLaunchKernelGGL((k_yapply<T, N>), grid, block, 0, s, *ya);
+    else hipLaunchKernelGGL((k_ysweep<T, N>), grid, block, 0, s, *ys);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_y(int ny, const YApplyArgs<T>* ya, const YSweepArgs<T>* ys, hipStream_t s)
+{
+    switch (ny) {
+    case 8: return launch_y_n<T, 8>(ya, ys, s);
+    case 16: return launch_y_n<T, 16>(ya, ys, s);
+    case 32: return launch_y_n<T, 32>(ya, ys, s);
+    case 64: return launch_y_n<T, 64>(ya, ys, s);
+    case 128: return launch_y_n<T, 128>(ya, ys, s);
+    case 256: return launch_y_n<T, 256>(ya, ys, s);
+    case 512: return launch_y_n<T, 512>(ya, ys, s);
+    case 1024: return launch_y_n<T, 1024>(ya, ys, s);
+    case 2048: return launch_y_n<T, 2048>(ya, ys, s);
+    case 4096: return launch_y_n<T, 4096>(ya, ys, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t launch_yapply(const YApplyArgs<T>& a, hipStream_t s)
+{
+    if (a.nv < 1 || a.nv > 16) return hipErrorInvalidValue;
+    return launch_y<T>(a.ny, &a, nullptr, s);
+}
+
+template <typename T>
+hipError_t launch_ysweep(const YSweepArgs<T>& a, hipStream_t s)
+{
+    return launch_y<T>(a.ny, nullptr, &a, s);
+}
+
+// ============================================================== pointwise
+static inline dim3 pw_grid(size_t n, int block)
+{
+    size_t b = (n + block - 1) / block;
+    if (b > 65535u * 16u) b = 65535u * 16u;
+    return dim3((unsigned)b);
+}
+
+template <typename T>
+__global__ void k_efield(const T* __restrict__ u, T* __restrict__ E, size_t n)
+{
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x) {
+        const T ux = u[UX * n + p], uy = u[UY * n + p], uz = u[UZ * n + p];
+        const T bx = u[BX * n + p], by = u[BY * n + p], bz = u[BZ * n + p];
+        E[0 * n + p] = uz * by - uy * bz;
+        E[1 * n + p] = ux * bz - uz * bx;
+        E[2 * n + p] = uy * bx - ux * by;
+    }
+}
+
+template <typename T>
+__global__ void k_divb(const T* __restrict__ dudx, const T* __restrict__ dudy,
+                       T* __restrict__ divB, size_t n)
+{
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x)
+        divB[p] = dudx[BX * n + p] + dudy[BY * n + p];
+}
+
+// ideal-MHD right-hand side at one cell, calcRHS.c:85-142
+template <typename T>
+__device__ __forceinline__ void rhs_cell(const T* __restrict__ u, const T* __restrict__ dx,
+                                         const T* __restrict__ dy, const T* __restrict__ dEx,
+                                         const T* __restrict__ dEy, size_t n, size_t p,
+                                         T gamma, T (&R)[8], T& rho_out)
+{
+    const T rho = u[RHO * n + p], ux = u[UX * n + p], uy = u[UY * n + p];
+    const T bx = u[BX * n + p], by = u[BY * n + p], bz = u[BZ * n + p], pr = u[PR * n + p];
+    const T divU = dx[UX * n + p] + dy[UY * n + p];
+    const T jx = dy[BZ * n + p];
+    const T jy = -dx[BZ * n + p];
+    const T jz = dx[BY * n + p] - dy[BX * n + p];
+    const T ir = (T)1 / rho;
+    R[RHO] = -rho * divU - ux * dx[RHO * n + p] - uy * dy[RHO * n + p];
+    R[UX] = -ux * dx[UX * n + p] - uy * dy[UX * n + p] - dx[PR * n + p] * ir + jy * bz * ir -
+            jz * by * ir;
+    R[UY] = -ux * dx[UY * n + p] - uy * dy[UY * n + p] - dy[PR * n + p] * ir + jz * bx * ir -
+            jx * bz * ir;
+    R[UZ] = -ux * dx[UZ * n + p] - uy * dy[UZ * n + p] + jx * by * ir - jy * bx * ir;
+    R[BX] = -dEy[2 * n + p];
+    R[BY] = dEx[2 * n + p];
+    R[BZ] = dEy[0 * n + p] - dEx[1 * n + p];
+    R[PR] = -ux * dx[PR * n + p] - uy * dy[PR * n + p] - gamma * pr * divU;
+    rho_out = rho;
+}
+
+template <typename T>
+__global__ void k_rhs(RhsArgs<T> a)
+{
+    const size_t n = a.n;
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x) {
+        T R[8], rho;
+        rhs_cell<T>(a.u, a.dudx, a.dudy, a.dEdx, a.dEdy, n, p, (T)a.gamma, R, rho);
+        // addSourceTerm, calcRHS.c:498-544
+        if (a.viscosity > 0) {
+            for (int c = 0; c < 3; c++)
+                R[UX + c] += (T)a.viscosity / rho * (a.d2udx2[c * n + p] + a.d2udy2[c * n + p]);
+        }
+        if (a.resistivity > 0) {
+            for (int c = 0; c < 3; c++)
+                R[BX + c] += (T)a.resistivity * (a.d2Bdx2[c * n + p] + a.d2Bdy2[c * n + p]);
+        }
+        if (a.divbclean == 1) {
+            R[BX] += (T)a.constDiffDivB * a.ddivBdx[p];
+            R[BY] += (T)a.constDiffDivB * a.ddivBdy[p];
+        }
+#pragma unroll
+        for (int f = 0; f < 8; f++) a.dudt[f * n + p] = R[f];
+    }
+}
+
+template <typename T>
+__global__ void k_rhs_rk(RhsRkArgs<T> a)
+{
+    const size_t n = a.n;
+    const T cc = (T)a.rs->cc[a.stage];
+    const T dd = (T)a.rs->dd[a.stage];
+    const bool use_prev = a.stage > 0;
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x) {
+        T R[8], rho;
+        rhs_cell<T>(a.u, a.dudx, a.dudy, a.dEdx, a.dEdy, n, p, (T)a.gamma, R, rho);
+#pragma unroll
+        for (int f = 0; f < 8; f++) {
+            // rkExecute, timeAdvance.c:31-35 (R_prev is zero in stage 0)
+            const T rp = use_prev ? a.rprev[f * n + p] : (T)0;
+            a.u[f * n + p] = a.u[f * n + p] + cc * R[f] + dd * rp;
+            if (a.write_rprev) a.rprev[f * n + p] = R[f];
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_rk(T* __restrict__ u, const T* __restrict__ dudt, T* __restrict__ dudtRK,
+                     T cc, T dd, size_t n)
+{
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x) {
+        u[p] = u[p] + cc * dudt[p] + dd * dudtRK[p];
+        dudtRK[p] = dudt[p];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_dt_limits(const T* __restrict__ u, DtParams dp,
+                                                   RunScalars* rs, size_t n)
+{
+    __shared__ double red[16];
+    double mn = INFINITY;
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x)
+        mn = fmin(mn, cell_dt(u[RHO * n + p], u[UX * n + p], u[UY * n + p], u[BX * n + p],
+                              u[BY * n + p], u[BZ * n + p], u[PR * n + p], dp));
+    block_min_dt(mn, rs, red);
+}
+
+// dt for the coming step: cfl * global min, with the +-2 % hysteresis of
+// timeAdvance.c:144-151 against the previous step's dt (0 before the first
+// step, mhd.c:58), then rkInit's coefficients (timeAdvance.c:20-27).
+__global__ void k_dt_finalize(RunScalars* rs, double cfl)
+{
+    const double m = __longlong_as_double((long long)rs->dtmin_bits) * cfl;
+    const bool first = rs->step == 0;
+    const double dt0 = first ? 0.0 : rs->dt;
+    const double dt = (dt0 < 0.98 * m || dt0 > 1.02 * m) ? m : dt0;
+    if (!first) rs->time += rs->dt;
+    rs->dt_prev = first ? 0.0 : rs->dt;
+    rs->dt = dt;
+    rs->cc[0] = 8.0 / 15.0 * dt;
+    rs->dd[0] = 0.0;
+    rs->cc[1] = 5.0 / 12.0 * dt;
+    rs->dd[1] = -17. / 60. * dt;
+    rs->cc[2] = 0.75 * dt;
+    rs->dd[2] = -5. / 12. * dt;
+    rs->step += 1;
+    rs->dtmin_bits = 0x7ff0000000000000ull;  // +inf for the next reduction
+}
+
+__global__ void k_dt_reset(RunScalars* rs)
+{
+    rs->dtmin_bits = 0x7ff0000000000000ull;
+    rs->nonfinite = 0;
+}
+
+template <typename T>
+__global__ void k_fill(T* p, T v, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// per-block partial sums (deterministic order): [0] energy density,
+// [1] max |div B| (from the given derivative arrays; skipped if null)
+template <typename T>
+__global__ void __launch_bounds__(256) k_diag(const T* __restrict__ u, const T* __restrict__ dudx,
+                                              const T* __restrict__ dudy, double* partial,
+                                              size_t n, double gamma)
+{
+    __shared__ double se[256], sm[256];
+    double e = 0.0, m = 0.0;
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x) {
+        const double rho = u[RHO * n + p];
+        const double ux = u[UX * n + p], uy = u[UY * n + p], uz = u[UZ * n + p];
+        const double bx = u[BX * n + p], by = u[BY * n + p], bz = u[BZ * n + p];
+        e += 0.5 * rho * (ux * ux + uy * uy + uz * uz) + u[PR * n + p] / (gamma - 1.0) +
+             0.5 * (bx * bx + by * by + bz * bz);
+        if (dudx) m = fmax(m, fabs((double)dudx[BX * n + p] + (double)dudy[BY * n + p]));
+    }
+    se[threadIdx.x] = e;
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            se[threadIdx.x] += se[threadIdx.x + o];
+            sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = se[0];
+        partial[2 * blockIdx.x + 1] = sm[0];
+    }
+}
+
+// ============================================================== launchers
+template <typename T>
+hipError_t launch_efield(const T* u, T* E, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_efield<T>), pw_grid(n, 256), dim3(256), 0, s, u, E, n);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_divb(const T* dudx, const T* dudy, T* divB, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_divb<T>), pw_grid(n, 256), dim3(256), 0, s, dudx, dudy, divB, n);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rhs(const RhsArgs<T>& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_rhs<T>), pw_grid(a.n, 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rhs_rk(const RhsRkArgs<T>& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_rhs_rk<T>), pw_grid(a.n, 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rk(T* u, const T* dudt, T* dudtRK, double cc, double dd, size_t n,
+                     hipStream_t s)
+{
+    hipLaunchKernelGGL((k_rk<T>), pw_grid(n, 256), dim3(256), 0, s, u, dudt, dudtRK, (T)cc,
+                       (T)dd, n);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_dt_limits(const T* u, const DtParams& dp, RunScalars* rs, int nx, int ny,
+                            hipStream_t s)
+{
+    const size_t n = (size_t)nx * ny;
+    size_t b = (n + 255) / 256;
+    if (b > 2048) b = 2048;
+    hipLaunchKernelGGL((k_dt_limits<T>), dim3((unsigned)b), dim3(256), 0, s, u, dp, rs, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_dt_finalize(RunScalars* rs, double cfl, int first, hipStream_t s)
+{
+    (void)first;
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, s, rs, cfl);
+    return hipGetLastError();
+}
+
+hipError_t launch_dt_reset(RunScalars* rs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(1), 0, s, rs);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fill(T* p, T v, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_fill<T>), pw_grid(n, 256), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_diag(const T* u, const T* dudx, const T* dudy, double* partial, int nblocks,
+                       size_t n, double gamma, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_diag<T>), dim3(nblocks), dim3(256), 0, s, u, dudx, dudy, partial, n,
+                       gamma);
+    return hipGetLastError();
+}
+
+// explicit instantiations
+#define MHD2D_INST(T)                                                                        \
+    template hipError_t launch_xsolve<T>(const XSolveArgs<T>&, int, hipStream_t);           \
+    template hipError_t launch_yapply<T>(const YApplyArgs<T>&, hipStream_t);                 \
+    template hipError_t launch_ysweep<T>(const YSweepArgs<T>&, hipStream_t);                 \
+    template hipError_t launch_rhs<T>(const RhsArgs<T>&, hipStream_t);                       \
+    template hipError_t launch_rhs_rk<T>(const RhsRkArgs<T>&, hipStream_t);                  \
+    template hipError_t launch_efield<T>(const T*, T*, size_t, hipStream_t);                 \
+    template hipError_t launch_divb<T>(const T*, const T*, T*, size_t, hipStream_t);         \
+    template hipError_t launch_rk<T>(T*, const T*, T*, double, double, size_t, hipStream_t); \
+    template hipError_t launch_dt_limits<T>(const T*, const DtParams&, RunScalars*, int, int, \
+                                            hipStream_t);                                    \
+    template hipError_t launch_fill<T>(T*, T, size_t, hipStream_t);                          \
+    template hipError_t launch_diag<T>(const T*, const T*, const T*, double*, int, size_t,   \
+                                       double, hipStream_t);
+
+MHD2D_INST(double)
+MHD2D_INST(float)
+
+}  // namespace mhd2d

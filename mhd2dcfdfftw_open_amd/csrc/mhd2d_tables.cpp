@@ -1,0 +1,147 @@
+// One-time FP64 construction of the operator tables (host), uploaded once.
+#include "mhd2d_tables.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace mhd2d {
+
+GridTables::~GridTables() { release(); }
+
+void GridTables::release()
+{
+    if (blob_) (void)hipFree(blob_);
+    blob_ = nullptr;
+}
+
+// Thomas factorisation (no pivoting) of the unit-diagonal tridiagonal matrix
+// of the periodic branch: rows 1..n-2 carry alpha on both sides, rows 0 and
+// n-1 carry alphaSub on their single coupling (initialize.c:633-654 and its
+// 2nd-derivative / filter twins).  Same elimination order as the reference's
+// solver (mathlib.c:17-22), done once instead of per line.
+void GridTables::factor(int n, double alpha, double alphaSub, double* l, double* invd,
+                        double* q)
+{
+    std::vector<double> lo(n), up(n), b(n);
+    for (int i = 0; i < n - 2; i++) lo[i] = alpha;  // coefficient of x[i] in row i+1
+    lo[n - 2] = alphaSub;
+    up[0] = alphaSub;                               // coefficient of x[i+1] in row i
+    for (int i = 1; i < n - 1; i++) up[i] = alpha;
+    b[0] = 1.0;
+    l[0] = 0.0;
+    for (int i = 1; i < n; i++) {
+        const double m = lo[i - 1] / b[i - 1];
+        b[i] = 1.0 - m * up[i - 1];
+        l[i] = m;
+    }
+    for (int i = 0; i < n; i++) invd[i] = 1.0 / b[i];
+    for (int i = 0; i < n - 1; i++) q[i] = -up[i] / b[i];
+    q[n - 1] = 0.0;
+}
+
+hipError_t GridTables::build(int nx_, int ny_, double Lx_, double Ly_, int elem_bytes_,
+                             const SchemeAlphas& al)
+{
+    release();
+    nx = nx_;
+    ny = ny_;
+    Lx = Lx_;
+    Ly = Ly_;
+    elem_bytes = elem_bytes_;
+    dx = Lx / nx;  // periodic x, initialize.c:125-128
+    dy = Ly / ny;  // initialize.c:135
+    xsolve_shape(nx, elem_bytes, &chunk, &width, &groups);
+
+    // ---- stencil weights, formulas of initialize.c:574-630
+    const double a1 = al.alpha1, s1 = al.alphaS1;
+    const double aI1 = (a1 + 9.) / 6., bI1 = (32. * a1 - 9.) / 15., cI1 = (-3. * a1 + 1.) / 10.;
+    const double e1[5] = {(1. - s1) / 12., (3. * s1 - 4.) / 6., -1.5 * s1, (4. + 5. * s1) / 6.,
+                          (3. * s1 - 1.) / 12.};
+    const double a2 = al.alpha2, s2 = al.alphaS2;
+    const double aI2 = (6. - 9. * a2) / 4., bI2 = (-3. + 24. * a2) / 5., cI2 = (2. - 11. * a2) / 20.;
+    const double e2[5] = {-(1. + s2) / 12., (s2 + 4.) / 3., (s2 - 5.) / 2., (4. - 5. * s2) / 3.,
+                          (11. * s2 - 1.) / 12.};
+    const double aF = al.alphaF, sF = al.alphaSF;
+    const double aIF = (11. + 10. * aF) / 16., bIF = (15. + 34. * aF) / 32.,
+                 cIF = (-3. + 6. * aF) / 16., dIF = (1. - 2. * aF) / 32.;
+    const double eF[5] = {(sF - 1.) / 16., (1. - sF) / 4., (3. * sF + 5.) / 8., (3. * sF + 1.) / 4.,
+                          (sF - 1.) / 16.};
+
+    const double idx = 1.0 / dx, idx2 = 1.0 / (dx * dx);
+    // first derivative: mathlib.c:50-82 (row N-1 carries a leading minus)
+    st[XD1] = XStencil{0.0, aI1 / 2 * idx, bI1 / 4 * idx, cI1 / 6 * idx,
+                       {e1[0] * idx, e1[1] * idx, e1[2] * idx, e1[3] * idx, e1[4] * idx}, -1.0};
+    // second derivative: mathlib.c:182-214
+    st[XD2] = XStencil{0.0, aI2 * idx2, bI2 / 4 * idx2, cI2 / 9 * idx2,
+                       {e2[0] * idx2, e2[1] * idx2, e2[2] * idx2, e2[3] * idx2, e2[4] * idx2}, 1.0};
+    // filter: mathlib.c:313-350
+    st[XFILT] = XStencil{aIF, bIF / 2, cIF / 2, dIF / 2, {eF[0], eF[1], eF[2], eF[3], eF[4]}, 1.0};
+
+    // ---- host tables
+    const int G = groups, C = chunk, N = ny;
+    const size_t nxs = (size_t)nx, gs = (size_t)G;
+    const size_t x_doubles = NXKIND * (3 * nxs + 2 * gs);
+    const size_t y_doubles = 4 * (size_t)N + 2 * (size_t)N;  // md, mdd, mf, mfd + tw
+    std::vector<double> h(x_doubles + y_doubles);
+    double* p = h.data();
+    const double alph[NXKIND][2] = {{al.alpha1, al.alphaS1}, {al.alpha2, al.alphaS2},
+                                    {al.alphaF, al.alphaSF}};
+    size_t off_l[NXKIND], off_id[NXKIND], off_q[NXKIND], off_pf[NXKIND], off_qb[NXKIND];
+    size_t o = 0;
+    for (int k = 0; k < NXKIND; k++) {
+        off_l[k] = o;  o += nxs;
+        off_id[k] = o; o += nxs;
+        off_q[k] = o;  o += nxs;
+        off_pf[k] = o; o += gs;
+        off_qb[k] = o; o += gs;
+        factor(nx, alph[k][0], alph[k][1], p + off_l[k], p + off_id[k], p + off_q[k]);
+        for (int g = 0; g < G; g++) {
+            double pf = 1.0, qb = 1.0;
+            for (int i = g * C; i < (g + 1) * C; i++) {
+                pf *= -p[off_l[k] + i];
+                qb *= p[off_q[k] + i];
+            }
+            p[off_pf[k] + g] = pf;
+            p[off_qb[k] + g] = qb;
+        }
+    }
+    // ---- y multipliers and twiddles (long double trig, rounded once)
+    const size_t off_md = o, off_mdd = o + N, off_mf = o + 2 * N, off_mfd = o + 3 * N;
+    const size_t off_tw = o + 4 * N;
+    const long double PI = 3.141592653589793238462643383279502884L;
+    const long double afy = al.afy;
+    const long double aj = (5.0L + 6.0L * afy) / 8.0L, bj = (1.0L + 2.0L * afy) / 2.0L,
+                      cj = -(1.0L - 2.0L * afy) / 8.0L;  // initialize.c:166-168
+    for (int k = 0; k < N; k++) {
+        const int ka = (k <= N / 2) ? k : N - k;          // |k|
+        const int ks = (k < N / 2) ? k : (k == N / 2 ? 0 : k - N);  // signed, Nyquist 0
+        const long double kap = 2.0L * PI / (long double)Ly * ks;
+        const long double kapa = 2.0L * PI / (long double)Ly * ka;
+        const long double w = 2.0L * PI * ka / N;
+        const long double fy = (aj + bj * cosl(w) + cj * cosl(2 * w)) / (1 + 2 * afy * cosl(w));
+        p[off_md + k] = (double)(kap / N);
+        p[off_mdd + k] = (double)(-kapa * kapa / N);
+        p[off_mf + k] = (double)(fy / N);
+        p[off_mfd + k] = (double)(fy * kap / N);
+        const long double ang = 2.0L * PI * k / N;
+        p[off_tw + 2 * k] = (double)cosl(ang);
+        p[off_tw + 2 * k + 1] = (double)(-sinl(ang));
+    }
+
+    hipError_t e = hipMalloc(&blob_, h.size() * sizeof(double));
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(blob_, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    const double* d = (const double*)blob_;
+    for (int k = 0; k < NXKIND; k++)
+        xt[k] = XTables{d + off_l[k], d + off_id[k], d + off_q[k], d + off_pf[k], d + off_qb[k]};
+    yt.md = d + off_md;
+    yt.mdd = d + off_mdd;
+    yt.mf = d + off_mf;
+    yt.mfd = d + off_mfd;
+    yt.tw = (const double2*)(d + off_tw);
+    return hipSuccess;
+}
+
+}  // namespace mhd2d

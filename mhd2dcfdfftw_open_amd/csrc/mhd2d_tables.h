@@ -1,0 +1,44 @@
+// Device-resident operator tables of one grid: stencils and Thomas factors of
+// the three x operators, spectral multipliers and FFT twiddles along y.
+#pragma once
+
+#include "mhd2d_internal.h"
+
+namespace mhd2d {
+
+struct SchemeAlphas {
+    // parameter.h:24-34 (boundaryType 0 uses the inner and "sub" alphas)
+    double alpha1 = 0.375, alphaS1 = 0.0;
+    double alpha2 = 0.375, alphaS2 = 0.0;
+    double alphaF = 0.49, alphaSF = 0.95;
+    double afy = 0.45;  // initialize.c:44
+};
+
+class GridTables {
+public:
+    GridTables() = default;
+    ~GridTables();
+    GridTables(const GridTables&) = delete;
+    GridTables& operator=(const GridTables&) = delete;
+
+    // nx, ny powers of two; dx = Lx/nx; elem_bytes selects the x-solve chunking
+    hipError_t build(int nx, int ny, double Lx, double Ly, int elem_bytes,
+                     const SchemeAlphas& al = SchemeAlphas());
+    void release();
+
+    int nx = 0, ny = 0, elem_bytes = 0;
+    double Lx = 0, Ly = 0, dx = 0, dy = 0;
+    int chunk = 0, width = 0, groups = 0;
+    XStencil st[NXKIND];
+    XTables xt[NXKIND];
+    YTables yt{};
+
+    // host copies (FP64) for inspection and tests
+    static void factor(int n, double alpha, double alphaSub, double* l, double* invd,
+                       double* q);
+
+private:
+    void* blob_ = nullptr;  // one allocation holding every table
+};
+
+}  // namespace mhd2d

@@ -1,0 +1,97 @@
+"""CPU checks of the C ABI: the libraries load and export every symbol that
+include/mhd2d.h declares; the core library leaves the reference globals to
+the executable it is linked into.  No GPU work is launched here."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mhd2d.h")
+PKG = os.path.join(ROOT, "mhd2dcfdfftw_open_amd")
+
+
+def declared_symbols():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    funcs = set(re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", src))
+    funcs -= {"if", "while", "for", "return", "sizeof"}
+    data = set(re.findall(r"extern\s+real\s+(\w+)\s*;", src))
+    return funcs, data
+
+
+def _exported(path):
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True,
+                         text=True, check=True).stdout
+    return {l.split()[-1] for l in out.splitlines() if l.strip()}
+
+
+def test_header_parses():
+    funcs, data = declared_symbols()
+    for f in ("derivX", "derivXX", "derivY", "derivYY", "filtering", "calcRHS", "calcDivB",
+              "calcEfield", "rkInit", "rkExecute", "calcDt", "sendrecvHaloPointsX",
+              "sendrecvHaloPointsdivBX", "sendrecvHaloPointsEfieldX", "calc_Cp_fix_Cr",
+              "mhd2d_create", "mhd2d_run"):
+        assert f in funcs
+    assert data == {"Cr_divBCleaning"}
+
+
+@pytest.mark.parametrize("libname", ["libmhd2d.so", "libmhd2d_core.so"])
+def test_library_exports_every_declared_symbol(libname):
+    path = os.path.join(PKG, libname)
+    if not os.path.exists(path):
+        pytest.skip(f"{libname} not built (run python -m mhd2dcfdfftw_open_amd.build)")
+    exported = _exported(path)
+    funcs, data = declared_symbols()
+    if libname == "libmhd2d_core.so":
+        # storage of the reference globals comes from the reference's initialize.o
+        funcs = funcs - {"mhd2d_host_initialize", "mhd2d_host_finalize"}
+    missing = (funcs | data) - exported
+    assert not missing, f"{libname} lacks {sorted(missing)}"
+
+
+def test_standalone_library_loads_and_defines_reference_globals():
+    path = os.path.join(PKG, "libmhd2d.so")
+    if not os.path.exists(path):
+        pytest.skip("libmhd2d.so not built")
+    L = ctypes.CDLL(path)
+    # storage named and typed as in the reference's initialize.c:15-79
+    for name in ("uu", "dudx", "dudy", "Efield", "dEdx", "dEdy", "dudt", "dudtRK",
+                 "uuHaloX0", "uuHaloX1"):
+        ctypes.c_void_p.in_dll(L, name)
+    for name in ("nx", "ny", "iSize", "jSize", "arrSize", "nky"):
+        ctypes.c_size_t.in_dll(L, name)
+    for name in ("NVAR", "Hall", "divBcleaning", "boundaryTypeX", "npe"):
+        ctypes.c_int.in_dll(L, name)
+    assert abs(ctypes.c_double.in_dll(L, "Cr_divBCleaning").value - 0.18) < 1e-15
+
+
+def test_core_library_leaves_reference_globals_undefined():
+    path = os.path.join(PKG, "libmhd2d_core.so")
+    if not os.path.exists(path):
+        pytest.skip("libmhd2d_core.so not built")
+    out = subprocess.run(["nm", "-D", "--undefined-only", path], capture_output=True,
+                         text=True, check=True).stdout
+    undef = {l.split()[-1] for l in out.splitlines() if l.strip()}
+    for name in ("uu", "dudx", "nx", "ny", "NVAR", "adiabaticIdx", "Efield"):
+        assert name in undef
+
+
+def test_host_initialize_mirrors_reference_setup():
+    import mhd2dcfdfftw_open_amd as m
+    if not os.path.exists(m.LIB_PATH):
+        pytest.skip("libmhd2d.so not built")
+    G = m.RefGlobals(64, 32, Lx=2.0, Ly=4.0, divbclean=0)
+    try:
+        L = m.lib()
+        assert ctypes.c_size_t.in_dll(L, "nky").value == 17       # initialize.c:123
+        assert ctypes.c_double.in_dll(L, "dx").value == 2.0 / 64   # initialize.c:127
+        assert ctypes.c_double.in_dll(L, "dy").value == 4.0 / 32   # initialize.c:135
+        assert ctypes.c_size_t.in_dll(L, "arrSize").value == 8 * 64 * 32
+        # constDiffDivB = 2 max(dx, dy)^2 (initialize.c:17, :154)
+        assert ctypes.c_double.in_dll(L, "constDiffDivB").value == 2.0 * (4.0 / 32) ** 2
+        assert G.array("uu").shape == (8, 64, 32) and not G.array("uu").any()
+    finally:
+        G.close()

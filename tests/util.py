@@ -1,0 +1,38 @@
+"""Shared helpers for the parity tests."""
+import numpy as np
+
+# per-field relative L2 gate of the FP64 device path against the FP64 oracle
+FP64_TOL = 1e-12
+
+
+def rel_l2(a, b, floor=None):
+    """||a-b|| / ||b||; for an identically-zero reference field the absolute
+    L2 is divided by `floor` (the rms scale of the state) instead."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    d = np.linalg.norm(a - b)
+    if nb == 0.0:
+        return d / (floor if floor else 1.0) / np.sqrt(b.size)
+    return d / nb
+
+
+def field_errors(a, b):
+    """per-field relative L2 of two [nv, nx, ny] arrays"""
+    scale = np.sqrt(np.mean(b ** 2)) or 1.0
+    return np.array([rel_l2(a[v], b[v], floor=scale) for v in range(b.shape[0])])
+
+
+def assert_fields_close(a, b, tol=FP64_TOL, what=""):
+    e = field_errors(a, b)
+    assert np.all(e <= tol), f"{what} per-field rel-L2 {e} > {tol}"
+    return e
+
+
+def random_state(nx, ny, seed, positive=True):
+    rng = np.random.default_rng(seed)
+    uu = rng.uniform(-1.0, 1.0, (8, nx, ny))
+    if positive:
+        uu[0] = 1.0 + 0.5 * rng.uniform(0, 1, (nx, ny))
+        uu[7] = 1.0 + 0.5 * rng.uniform(0, 1, (nx, ny))
+    return uu
