@@ -61,6 +61,70 @@ def cpu_baseline(nx, ny, u0, hall, di):
                       "single-threaded oracle restatement)"}
 
 
+COPY_BYTES = 1 << 30
+
+
+def _pmc_rows(path):
+    """rows of a rocprofv3 counter_collection.csv: (kernel name, counter, value)"""
+    import csv
+    out = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            name = r.get("Kernel_Name") or r.get("Kernel-Name") or ""
+            ctr = r.get("Counter_Name") or r.get("Counter-Name") or ""
+            val = r.get("Counter_Value") or r.get("Counter-Value") or "0"
+            out.append((name, ctr, float(val)))
+    return out
+
+
+def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, nx, ny):
+    """HBM bytes per launch of each kernel class from two --pmc passes
+    (FETCH_SIZE, WRITE_SIZE in KiB per dispatch), scaled by what the same
+    counters read for the copy kernel of known size (COPY_BYTES read and
+    COPY_BYTES written per launch)."""
+    def per_kernel(path, counter):
+        acc = {}
+        for name, ctr, val in _pmc_rows(path):
+            if ctr != counter:
+                continue
+            acc.setdefault(name, []).append(val * 1024.0)
+        # median over dispatches: drops the odd launches of the same kernel
+        # (first-stage y-sweep without filter, 1-field diagnostics solves)
+        return {k: float(np.median(v)) for k, v in acc.items()}
+
+    cf = per_kernel(copy_fetch_csv, "FETCH_SIZE")
+    cw = per_kernel(copy_write_csv, "WRITE_SIZE")
+    ck = [k for k in cf if "k_copy" in k]
+    fscale = COPY_BYTES / cf[ck[0]] if ck and cf[ck[0]] > 0 else 1.0
+    ck = [k for k in cw if "k_copy" in k]
+    wscale = COPY_BYTES / cw[ck[0]] if ck and cw[ck[0]] > 0 else 1.0
+    fetch = per_kernel(fetch_csv, "FETCH_SIZE")
+    write = per_kernel(write_csv, "WRITE_SIZE")
+    classes = {"ysweep": "k_ysweep", "xderiv": "k_xsolve", "rhs_rk": "k_rhs_rk",
+               "xfilter": "k_xsolve"}
+    alg = kernel_alg_bytes(nx, ny)
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    for cls, key in classes.items():
+        names = [k for k in fetch if key in k]
+        if cls == "xderiv":
+            names = [k for k in names if ", 0>(" in k]
+        if cls == "xfilter":
+            names = [k for k in names if ", 2>(" in k]
+        if not names:
+            continue
+        k = names[0]
+        rd = fetch[k] * fscale
+        wr = write.get(k, 0.0) * wscale
+        rec = {"kernel": k, "workload": f"{nx}x{ny} FP64 turbulence",
+               "fetch_bytes_per_launch": rd, "write_bytes_per_launch": wr,
+               "bytes_per_launch": rd + wr, "alg_bytes_per_launch": alg[cls],
+               "traffic_over_alg": (rd + wr) / alg[cls],
+               "copy_scale": {"fetch": fscale, "write": wscale}}
+        with open(os.path.join(ROOT, "profiles", f"traffic_{cls}_{nx}.json"), "w") as f:
+            json.dump(rec, f, indent=1)
+        print(json.dumps(rec))
+
+
 def load_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary, if one exists (profiles/traffic_<kernel>_2048.json)."""
@@ -84,7 +148,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the streaming-copy calibration kernel and print its rate")
+    ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV",
+                                                         "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="turn rocprofv3 --pmc counter CSVs into profiles/traffic_*.json")
     args = ap.parse_args()
+
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc, nx=args.nx, ny=args.ny)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the whole grid on one GPU; multi-GPU runs "
@@ -92,6 +165,11 @@ def main():
         sys.exit(2)
 
     import mhd2dcfdfftw_open_amd as m
+
+    if args.copy_probe:
+        print(json.dumps({"copy_probe_bytes": COPY_BYTES,
+                          "copy_gbps": m.copy_probe(COPY_BYTES, 5)}))
+        return
 
     nx, ny = args.nx, args.ny
     hall, di = 1, 0.02
@@ -125,6 +203,8 @@ def main():
                    else None}
                for k in prof}
     energy, divb = ctx.diagnostics()
+    copy_gbps = m.copy_probe(COPY_BYTES, 10)
+    roofline["frac_vs_copy_probe"] = achieved / (copy_gbps * 1e9)
 
     out = {
         "metric": "RK3 cell-updates/s (plain FP64 path)" if prec == m.FP64 else
@@ -146,6 +226,7 @@ def main():
         "roofline": roofline,
         "step_roofline_frac": value * (B_ALG_FP64 * elem / 8) / HBM_PEAK,
         "kernels": kernels,
+        "copy_probe_GBps": copy_gbps,
         "diagnostics": {"energy": energy, "max_divb": divb},
     }
     if not args.no_cpu:

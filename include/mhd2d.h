@@ -145,6 +145,11 @@ int mhd2d_time_run(mhd2d_ctx *c, int nsteps, int use_graph, double *ms);
 int mhd2d_profile_run(mhd2d_ctx *c, int nsteps, double *out_ms, int *out_launches);
 /* bytes of device memory held by the context */
 size_t mhd2d_device_bytes(const mhd2d_ctx *c);
+/* measurement helper: a plain device-to-device streaming copy of `bytes`,
+ * run `iters` times; *gbps = 2 * bytes / average kernel time (read + write).
+ * Used to calibrate the profiler's HBM byte counters and to state the
+ * achievable copy rate beside the 8 TB/s spec peak. */
+int mhd2d_copy_probe(size_t bytes, int iters, double *gbps);
 const char *mhd2d_last_error(void);
 
 /* ===================================================================== */

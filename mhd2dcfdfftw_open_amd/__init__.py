@@ -63,6 +63,7 @@ def lib():
         "mhd2d_time_run": (i, [vp, i, i, p]),
         "mhd2d_profile_run": (i, [vp, i, p, ctypes.POINTER(i)]),
         "mhd2d_device_bytes": (ctypes.c_size_t, [vp]),
+        "mhd2d_copy_probe": (i, [ctypes.c_size_t, i, p]),
         "mhd2d_last_error": (ctypes.c_char_p, []),
         "mhd2d_ic_orszag_tang": (None, [i, i, d, d, p]),
         "mhd2d_ic_turbulence": (None, [i, i, d, d, ctypes.c_ulonglong, d, i, i, p]),
@@ -104,6 +105,13 @@ def _ptr(a):
 def _check(rc):
     if rc != 0:
         raise MHD2DError(f"mhd2d error {rc}: {lib().mhd2d_last_error().decode()}")
+
+
+def copy_probe(nbytes=1 << 30, iters=20):
+    """Achievable streaming-copy rate in GB/s (read + write bytes)."""
+    g = ctypes.c_double()
+    _check(lib().mhd2d_copy_probe(nbytes, iters, ctypes.byref(g)))
+    return g.value
 
 
 # ------------------------------------------------------------------ initial states

@@ -288,6 +288,35 @@ void mhd2d_destroy(mhd2d_ctx* c)
 
 size_t mhd2d_device_bytes(const mhd2d_ctx* c) { return c ? c->bytes : 0; }
 
+int mhd2d_copy_probe(size_t bytes, int iters, double* gbps)
+{
+    if (!gbps || bytes < 16 || iters < 1) return fail(MHD2D_EINVAL, "bad argument");
+    bytes &= ~(size_t)15;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0, e1;
+    HIPCHK(hipMalloc(&a, bytes));
+    HIPCHK(hipMalloc(&b, bytes));
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIPCHK(hipMemsetAsync(a, 0, bytes, s));
+    HIPCHK(launch_copy(a, b, bytes, s));  // warm-up
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+    for (int k = 0; k < iters; k++) HIPCHK(launch_copy(a, b, bytes, s));
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return MHD2D_OK;
+}
+
 int mhd2d_set_state(mhd2d_ctx* c, const double* uu)
 {
     if (!c || !uu) return fail(MHD2D_EINVAL, "null argument");

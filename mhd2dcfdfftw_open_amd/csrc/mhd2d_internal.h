@@ -167,8 +167,10 @@ hipError_t launch_dt_reset(RunScalars* rs, hipStream_t s);
 template <typename T>
 hipError_t launch_fill(T* p, T v, size_t n, hipStream_t s);
 template <typename T>
-hipError_t launch_diag(const T* u, const T* dudx, const T* dudy, double* partial,
+hipError_t launch_diag(const T* u, const T* dbx, const T* dby, double* partial,
                        int nblocks, size_t n, double gamma, hipStream_t s);
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 // chunk length (rows per thread) and strip width used by the x solves
 void xsolve_shape(int nx, int elem_bytes, int* chunk, int* width, int* groups);

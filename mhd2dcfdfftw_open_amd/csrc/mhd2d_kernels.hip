@@ -13,177 +13,12 @@
 //             the 8 state fields, E = -u x B and its y-derivative, and the
 //             CFL limits of the new state in one pass over each row.
 //  pointwise  E field, div B, RHS, RK update, fused RHS + RK, dt reduction.
+#include "mhd2d_fft.h"
 #include "mhd2d_internal.h"
 
 #include <cmath>
 
 namespace mhd2d {
-
-// ============================================================== complex
-template <typename T>
-struct cx {
-    T re, im;
-};
-template <typename T>
-__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
-template <typename T>
-__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
-template <typename T>
-__device__ __forceinline__ cx<T> cmul(cx<T> a, double2 w)
-{
-    const T wr = (T)w.x, wi = (T)w.y;
-    return {a.re * wr - a.im * wi, a.re * wi + a.im * wr};
-}
-template <typename T>
-__device__ __forceinline__ cx<T> mul_mi(cx<T> a) { return {a.im, -a.re}; }  // times -i
-template <typename T>
-__device__ __forceinline__ void bfly(cx<T>& a, cx<T>& b)
-{
-    const cx<T> t = a;
-    a = t + b;
-    b = t - b;
-}
-template <typename T>
-__device__ __forceinline__ void cswap(cx<T>& a, cx<T>& b)
-{
-    const cx<T> t = a;
-    a = b;
-    b = t;
-}
-
-// in-order forward DFTs of length 2, 4, 8 (X_k = sum_n x_n e^{-2 pi i nk/R})
-template <typename T, int R>
-__device__ __forceinline__ void dft(cx<T>* v);
-
-template <>
-__device__ __forceinline__ void dft<double, 2>(cx<double>* v) { bfly(v[0], v[1]); }
-template <>
-__device__ __forceinline__ void dft<float, 2>(cx<float>* v) { bfly(v[0], v[1]); }
-
-template <typename T>
-__device__ __forceinline__ void dft4(cx<T>* v)
-{
-    bfly(v[0], v[2]);
-    bfly(v[1], v[3]);
-    v[3] = mul_mi(v[3]);
-    bfly(v[0], v[1]);
-    bfly(v[2], v[3]);
-    cswap(v[1], v[2]);
-}
-template <>
-__device__ __forceinline__ void dft<double, 4>(cx<double>* v) { dft4(v); }
-template <>
-__device__ __forceinline__ void dft<float, 4>(cx<float>* v) { dft4(v); }
-
-template <typename T>
-__device__ __forceinline__ void dft8(cx<T>* v)
-{
-    const T s = (T)0.70710678118654752440084436210485;
-    bfly(v[0], v[4]);
-    bfly(v[1], v[5]);
-    bfly(v[2], v[6]);
-    bfly(v[3], v[7]);
-    v[5] = cx<T>{s * (v[5].re + v[5].im), s * (v[5].im - v[5].re)};   // * e^{-i pi/4}
-    v[6] = mul_mi(v[6]);                                               // * e^{-i pi/2}
-    v[7] = cx<T>{s * (v[7].im - v[7].re), -s * (v[7].re + v[7].im)};  // * e^{-3i pi/4}
-    bfly(v[0], v[2]);
-    bfly(v[1], v[3]);
-    bfly(v[4], v[6]);
-    bfly(v[5], v[7]);
-    v[3] = mul_mi(v[3]);
-    v[7] = mul_mi(v[7]);
-    bfly(v[0], v[1]);
-    bfly(v[2], v[3]);
-    bfly(v[4], v[5]);
-    bfly(v[6], v[7]);
-    cswap(v[1], v[4]);  // bit-reversed -> natural order
-    cswap(v[3], v[6]);
-}
-template <>
-__device__ __forceinline__ void dft<double, 8>(cx<double>* v) { dft8(v); }
-template <>
-__device__ __forceinline__ void dft<float, 8>(cx<float>* v) { dft8(v); }
-
-constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
-constexpr int ipow8(int p) { return p == 0 ? 1 : 8 * ipow8(p - 1); }
-
-// One Stockham pass (radix 8 except possibly the last, which is 4 or 2).
-// Thread t of a row owns, before the first and after the last pass, the
-// elements at positions t + m*N/8, m = 0..7 ("strided" layout).
-template <typename T, int N, int P>
-__device__ __forceinline__ void fft_pass(cx<T> (&v)[8], int t, cx<T>* lds,
-                                         const double2* __restrict__ tw)
-{
-    constexpr int M = ilog2c(N), P8 = M / 3, REM = M % 3;
-    constexpr int NP = P8 + (REM ? 1 : 0);
-    constexpr int R = (P < P8) ? 8 : (REM == 2 ? 4 : 2);
-    constexpr int B = 8 / R;  // butterflies per thread in this pass
-    constexpr int TPR = N / 8;
-    constexpr int NS = ipow8(P);
-#pragma unroll
-    for (int b = 0; b < B; b++) {
-        const int j = t + b * TPR;
-        if constexpr (NS > 1) {
-            const int k = j & (NS - 1);
-            constexpr int STEP = N / (NS * R);
-#pragma unroll
-            for (int r = 1; r < R; r++) v[b * R + r] = cmul(v[b * R + r], tw[r * k * STEP]);
-        }
-        dft<T, R>(&v[b * R]);
-    }
-    if constexpr (P + 1 < NP) {
-        // intermediate passes are radix 8 (B == 1)
-        constexpr int R2 = (P + 1 < P8) ? 8 : (REM == 2 ? 4 : 2);
-        constexpr int B2 = 8 / R2;
-        {
-            const int j = t;
-            const int base = (j / NS) * NS * R + (j & (NS - 1));
-#pragma unroll
-            for (int r = 0; r < R; r++) lds[base + r * NS] = v[r];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int b = 0; b < B2; b++) {
-            const int j = t + b * TPR;
-#pragma unroll
-            for (int r = 0; r < R2; r++) v[b * R2 + r] = lds[j + r * (N / R2)];
-        }
-        __syncthreads();
-        fft_pass<T, N, P + 1>(v, t, lds, tw);
-    } else if constexpr (B > 1) {
-        // last pass: element (b, r) sits at t + TPR * (b + r * B)
-        cx<T> tmp[8];
-#pragma unroll
-        for (int m = 0; m < 8; m++) tmp[m] = v[(m % B) * R + m / B];
-#pragma unroll
-        for (int m = 0; m < 8; m++) v[m] = tmp[m];
-    }
-}
-
-template <typename T, int N>
-__device__ __forceinline__ void fft_fwd(cx<T> (&v)[8], int t, cx<T>* lds,
-                                        const double2* __restrict__ tw)
-{
-    fft_pass<T, N, 0>(v, t, lds, tw);
-}
-
-// unnormalised inverse: conj(FFT(conj(x)))
-template <typename T, int N>
-__device__ __forceinline__ void fft_inv(cx<T> (&v)[8], int t, cx<T>* lds,
-                                        const double2* __restrict__ tw)
-{
-#pragma unroll
-    for (int m = 0; m < 8; m++) v[m].im = -v[m].im;
-    fft_pass<T, N, 0>(v, t, lds, tw);
-#pragma unroll
-    for (int m = 0; m < 8; m++) v[m].im = -v[m].im;
-}
-
-// v <- v * (i * s_k)  (imaginary multiplier)
-template <typename T>
-__device__ __forceinline__ cx<T> mul_i_scal(cx<T> a, T s) { return {-a.im * s, a.re * s}; }
-template <typename T>
-__device__ __forceinline__ cx<T> mul_scal(cx<T> a, T s) { return {a.re * s, a.im * s}; }
 
 // ============================================================== dt limits
 // per-cell CFL limit of calcDt, timeAdvance.c:50-136 (divBcleaning==2 excluded)
@@ -412,199 +247,210 @@ hipError_t launch_xsolve(const XSolveArgs<T>& a, int nv, hipStream_t s)
 }
 
 // ============================================================== y operators
-template <int N>
+// elements per thread of the row FFTs: 8 keeps an FP64 line plus its
+// spectrum within ~100 VGPRs; FP32 lines take 16
+template <typename T>
+struct YElems {
+    static constexpr int E = sizeof(T) == 8 ? 8 : 16;
+};
+
+template <typename T, int N>
 struct YShape {
-    static constexpr int TPR = N / 8;
-    static constexpr int ROWS = (TPR >= 256) ? 1 : 256 / TPR;
+    static constexpr int E = YElems<T>::E;
+    static constexpr int TPR = FFTShape<N, E>::TPR;
+    static constexpr int ROWS = (TPR >= 128) ? 1 : 128 / TPR;
     static constexpr int THREADS = TPR * ROWS;
+    static constexpr int LDSW = FFTShape<N, E>::LDS;
 };
 
 int ysweep_supported(int ny)
 {
-    return ny >= 8 && ny <= 4096 && (ny & (ny - 1)) == 0;
+    return ny >= 16 && ny <= 4096 && (ny & (ny - 1)) == 0;
 }
 
 // generic spectral multiplier on nv fields (drop-in derivY / derivYY / filteringY)
 template <typename T, int N>
-__global__ void __launch_bounds__(YShape<N>::THREADS) k_yapply(YApplyArgs<T> a)
+__global__ void __launch_bounds__((YShape<T, N>::THREADS)) k_yapply(YApplyArgs<T> a)
 {
-    constexpr int TPR = YShape<N>::TPR, ROWS = YShape<N>::ROWS;
-    __shared__ cx<T> lds_all[ROWS * N];
+    using Y = YShape<T, N>;
+    constexpr int TPR = Y::TPR;
+    constexpr int E = Y::E;
+    __shared__ T s_re[Y::ROWS * Y::LDSW];
+    __shared__ T s_im[Y::ROWS * Y::LDSW];
     const int t = threadIdx.x % TPR, rl = threadIdx.x / TPR;
-    const int i_raw = blockIdx.x * ROWS + rl;
+    const int i_raw = blockIdx.x * Y::ROWS + rl;
     const bool active = i_raw < a.nx;
     const size_t row = (size_t)(active ? i_raw : a.nx - 1) * N;
-    const size_t n = (size_t)a.nx * N;
-    cx<T>* lds = lds_all + rl * N;
+    T* lre = s_re + rl * Y::LDSW;
+    T* lim = s_im + rl * Y::LDSW;
     const double* __restrict__ mul =
         a.mode == Y_D1 ? a.tb.md : (a.mode == Y_D2 ? a.tb.mdd : a.tb.mf);
     for (int p = 0; p < a.nv; p += 2) {
         const bool two = (p + 1) < a.nv;
         const T* fa = a.in[p] + row;
-        const T* fb = two ? a.in[p + 1] + row : nullptr;
-        cx<T> v[8];
+        const T* fb = two ? a.in[p + 1] + row : fa;
+        cx<T> v[E];
 #pragma unroll
-        for (int m = 0; m < 8; m++) {
+        for (int m = 0; m < E; m++) {
             const int k = t + m * TPR;
             v[m] = cx<T>{fa[k], two ? fb[k] : (T)0};
         }
-        fft_fwd<T, N>(v, t, lds, a.tb.tw);
+        fft_fwd<T, N, E>(v, t, lre, lim, a.tb.tw);
 #pragma unroll
-        for (int m = 0; m < 8; m++) {
-            const int k = t + m * TPR;
-            const T s = (T)mul[k];
+        for (int m = 0; m < E; m++) {
+            const T s = (T)mul[t + m * TPR];
             v[m] = (a.mode == Y_D1) ? mul_i_scal(v[m], s) : mul_scal(v[m], s);
         }
-        fft_inv<T, N>(v, t, lds, a.tb.tw);
+        fft_inv<T, N, E>(v, t, lre, lim, a.tb.tw);
         if (active) {
             T* oa = a.out[p] + row;
-            T* ob = two ? a.out[p + 1] + row : nullptr;
 #pragma unroll
-            for (int m = 0; m < 8; m++) {
-                const int k = t + m * TPR;
-                oa[k] = v[m].re;
-                if (two) ob[k] = v[m].im;
+            for (int m = 0; m < E; m++) oa[t + m * TPR] = v[m].re;
+            if (two) {
+                T* ob = a.out[p + 1] + row;
+#pragma unroll
+                for (int m = 0; m < E; m++) ob[t + m * TPR] = v[m].im;
             }
         }
-        (void)n;
     }
 }
 
-// fused y-sweep of one RK stage (driver)
+// Fused y-sweep of one RK stage (driver).  Each thread owns the same 16
+// positions of its row in physical and spectral space, so the filtered state
+// it writes is re-read by the same thread only (E and the CFL limits).
 template <typename T, int N>
-__global__ void __launch_bounds__(YShape<N>::THREADS) k_ysweep(YSweepArgs<T> a)
+__global__ void __launch_bounds__((YShape<T, N>::THREADS)) k_ysweep(YSweepArgs<T> a)
 {
-    constexpr int TPR = YShape<N>::TPR, ROWS = YShape<N>::ROWS;
-    __shared__ cx<T> lds_all[ROWS * N];
+    using Y = YShape<T, N>;
+    constexpr int TPR = Y::TPR;
+    constexpr int E = Y::E;
+    __shared__ T s_re[Y::ROWS * Y::LDSW];
+    __shared__ T s_im[Y::ROWS * Y::LDSW];
     __shared__ double red[16];
     const int t = threadIdx.x % TPR, rl = threadIdx.x / TPR;
-    const int i_raw = blockIdx.x * ROWS + rl;
+    const int i_raw = blockIdx.x * Y::ROWS + rl;
     const bool active = i_raw < a.nx;
     const size_t row = (size_t)(active ? i_raw : a.nx - 1) * N;
     const size_t n = (size_t)a.nx * N;
-    cx<T>* lds = lds_all + rl * N;
+    T* lre = s_re + rl * Y::LDSW;
+    T* lim = s_im + rl * Y::LDSW;
     const double2* __restrict__ tw = a.tb.tw;
+    T* const u = a.u;
 
-    T keep[8][8];  // filtered state at the 8 positions this thread owns
     // field pairs packed as re/im of one complex line
     constexpr int PA[4] = {RHO, UY, BX, BZ};
     constexpr int PB[4] = {UX, UZ, BY, PR};
+    if (a.filter || a.derivs) {
+#pragma unroll 1
+        for (int p = 0; p < 4; p++) {
+            const int fa = PA[p], fb = PB[p];
+            T* ua = u + fa * n + row;
+            T* ub = u + fb * n + row;
+            // opaque copy of t: the multiplier loads stay inside the loop
+            // instead of being hoisted into 24 live registers
+            int tq = t;
+            asm volatile("" : "+v"(tq));
+            cx<T> v[E];
 #pragma unroll
-    for (int p = 0; p < 4; p++) {
-        const int fa = PA[p], fb = PB[p];
-        T* ua = a.u + fa * n + row;
-        T* ub = a.u + fb * n + row;
-        cx<T> v[8];
-#pragma unroll
-        for (int m = 0; m < 8; m++) {
-            const int k = t + m * TPR;
-            v[m] = cx<T>{ua[k], ub[k]};
-        }
-        if (!a.filter) {
-#pragma unroll
-            for (int m = 0; m < 8; m++) {
-                keep[fa][m] = v[m].re;
-                keep[fb][m] = v[m].im;
+            for (int m = 0; m < E; m++) {
+                const int k = t + m * TPR;
+                v[m] = cx<T>{ua[k], ub[k]};
             }
-        }
-        if (!a.filter && !a.derivs) continue;
-        fft_fwd<T, N>(v, t, lds, tw);
-        cx<T> Z[8];
-        if (a.filter) {
+            fft_fwd<T, N, E>(v, t, lre, lim, tw);
+            if (a.filter) {
+                cx<T> Z[E];
 #pragma unroll
-            for (int m = 0; m < 8; m++) {
-                Z[m] = v[m];
-                v[m] = mul_scal(v[m], (T)a.tb.mf[t + m * TPR]);
-            }
-            fft_inv<T, N>(v, t, lds, tw);
-#pragma unroll
-            for (int m = 0; m < 8; m++) {
-                keep[fa][m] = v[m].re;
-                keep[fb][m] = v[m].im;
-            }
-            if (active) {
-#pragma unroll
-                for (int m = 0; m < 8; m++) {
-                    const int k = t + m * TPR;
-                    ua[k] = v[m].re;
-                    ub[k] = v[m].im;
+                for (int m = 0; m < E; m++) {
+                    Z[m] = v[m];
+                    v[m] = mul_scal(v[m], (T)a.tb.mf[tq + m * TPR]);
                 }
+                fft_inv<T, N, E>(v, t, lre, lim, tw);
+                if (active) {
+#pragma unroll
+                    for (int m = 0; m < E; m++) {
+                        const int k = t + m * TPR;
+                        ua[k] = v[m].re;
+                        ub[k] = v[m].im;
+                    }
+                }
+                if (a.derivs) {
+#pragma unroll
+                    for (int m = 0; m < E; m++)
+                        v[m] = mul_i_scal(Z[m], (T)a.tb.mfd[tq + m * TPR]);
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < E; m++) v[m] = mul_i_scal(v[m], (T)a.tb.md[tq + m * TPR]);
             }
             if (a.derivs) {
+                fft_inv<T, N, E>(v, t, lre, lim, tw);
+                if (active) {
+                    T* da = a.dudy + fa * n + row;
+                    T* db = a.dudy + fb * n + row;
 #pragma unroll
-                for (int m = 0; m < 8; m++) v[m] = mul_i_scal(Z[m], (T)a.tb.mfd[t + m * TPR]);
-            }
-        } else {
-#pragma unroll
-            for (int m = 0; m < 8; m++) v[m] = mul_i_scal(v[m], (T)a.tb.md[t + m * TPR]);
-        }
-        if (a.derivs) {
-            fft_inv<T, N>(v, t, lds, tw);
-            if (active) {
-                T* da = a.dudy + fa * n + row;
-                T* db = a.dudy + fb * n + row;
-#pragma unroll
-                for (int m = 0; m < 8; m++) {
-                    const int k = t + m * TPR;
-                    da[k] = v[m].re;
-                    db[k] = v[m].im;
+                    for (int m = 0; m < E; m++) {
+                        const int k = t + m * TPR;
+                        da[k] = v[m].re;
+                        db[k] = v[m].im;
+                    }
                 }
             }
         }
     }
 
     if (a.derivs) {
-        // E = -u x B of the filtered state (calcRHS.c:24-43)
-        T ex[8], ey[8], ez[8];
+        // E = -u x B of the filtered state (calcRHS.c:24-43) and its d/dy
+        cx<T> v[E];
+        T ez[E];
 #pragma unroll
-        for (int m = 0; m < 8; m++) {
-            ex[m] = keep[UZ][m] * keep[BY][m] - keep[UY][m] * keep[BZ][m];
-            ey[m] = keep[UX][m] * keep[BZ][m] - keep[UZ][m] * keep[BX][m];
-            ez[m] = keep[UY][m] * keep[BX][m] - keep[UX][m] * keep[BY][m];
-        }
-        if (active) {
-#pragma unroll
-            for (int m = 0; m < 8; m++) {
-                const int k = t + m * TPR;
-                a.E[0 * n + row + k] = ex[m];
-                a.E[1 * n + row + k] = ey[m];
-                a.E[2 * n + row + k] = ez[m];
+        for (int m = 0; m < E; m++) {
+            const size_t k = row + t + m * TPR;
+            const T ux = u[UX * n + k], uy = u[UY * n + k], uz = u[UZ * n + k];
+            const T bx = u[BX * n + k], by = u[BY * n + k], bz = u[BZ * n + k];
+            const T ex = uz * by - uy * bz;
+            const T ey = ux * bz - uz * bx;
+            ez[m] = uy * bx - ux * by;
+            v[m] = cx<T>{ex, ey};
+            if (active) {
+                a.E[0 * n + k] = ex;
+                a.E[1 * n + k] = ey;
+                a.E[2 * n + k] = ez[m];
             }
         }
-        cx<T> v[8];
+        fft_fwd<T, N, E>(v, t, lre, lim, tw);
 #pragma unroll
-        for (int m = 0; m < 8; m++) v[m] = cx<T>{ex[m], ey[m]};
-        fft_fwd<T, N>(v, t, lds, tw);
-#pragma unroll
-        for (int m = 0; m < 8; m++) v[m] = mul_i_scal(v[m], (T)a.tb.md[t + m * TPR]);
-        fft_inv<T, N>(v, t, lds, tw);
+        for (int m = 0; m < E; m++) v[m] = mul_i_scal(v[m], (T)a.tb.md[t + m * TPR]);
+        fft_inv<T, N, E>(v, t, lre, lim, tw);
         if (active) {
 #pragma unroll
-            for (int m = 0; m < 8; m++) {
-                const int k = t + m * TPR;
-                a.dEdy[0 * n + row + k] = v[m].re;
-                a.dEdy[1 * n + row + k] = v[m].im;
+            for (int m = 0; m < E; m++) {
+                const size_t k = row + t + m * TPR;
+                a.dEdy[0 * n + k] = v[m].re;
+                a.dEdy[1 * n + k] = v[m].im;
             }
         }
 #pragma unroll
-        for (int m = 0; m < 8; m++) v[m] = cx<T>{ez[m], (T)0};
-        fft_fwd<T, N>(v, t, lds, tw);
+        for (int m = 0; m < E; m++) v[m] = cx<T>{ez[m], (T)0};
+        fft_fwd<T, N, E>(v, t, lre, lim, tw);
 #pragma unroll
-        for (int m = 0; m < 8; m++) v[m] = mul_i_scal(v[m], (T)a.tb.md[t + m * TPR]);
-        fft_inv<T, N>(v, t, lds, tw);
+        for (int m = 0; m < E; m++) v[m] = mul_i_scal(v[m], (T)a.tb.md[t + m * TPR]);
+        fft_inv<T, N, E>(v, t, lre, lim, tw);
         if (active) {
 #pragma unroll
-            for (int m = 0; m < 8; m++) a.dEdy[2 * n + row + t + m * TPR] = v[m].re;
+            for (int m = 0; m < E; m++) a.dEdy[2 * n + row + t + m * TPR] = v[m].re;
         }
     }
 
     if (a.dt_limits) {
         double mn = INFINITY;
         if (active) {
-#pragma unroll
-            for (int m = 0; m < 8; m++)
-                mn = fmin(mn, cell_dt(keep[RHO][m], keep[UX][m], keep[UY][m], keep[BX][m],
-                                      keep[BY][m], keep[BZ][m], keep[PR][m], a.dp));
+#pragma unroll 4
+            for (int m = 0; m < E; m++) {
+                const size_t k = row + t + m * TPR;
+                mn = fmin(mn, cell_dt(u[RHO * n + k], u[UX * n + k], u[UY * n + k],
+                                      u[BX * n + k], u[BY * n + k], u[BZ * n + k],
+                                      u[PR * n + k], a.dp));
+            }
         }
         block_min_dt(mn, a.rs, red);
     }
@@ -614,7 +460,8 @@ template <typename T, int N>
 static hipError_t launch_y_n(const YApplyArgs<T>* ya, const YSweepArgs<T>* ys, hipStream_t s)
 {
     const int nx = ya ? ya->nx : ys->nx;
-    dim3 grid((nx + YShape<N>::ROWS - 1) / YShape<N>::ROWS), block(YShape<N>::THREADS);
+    using Y = YShape<T, N>;
+    dim3 grid((nx + Y::ROWS - 1) / Y::ROWS), block(Y::THREADS);
     if (ya) hipLaunchKernelGGL((k_yapply<T, N>), grid, block, 0, s, *ya);
     else hipLaunchKernelGGL((k_ysweep<T, N>), grid, block, 0, s, *ys);
     return hipGetLastError();
@@ -624,7 +471,6 @@ template <typename T>
 static hipError_t launch_y(int ny, const YApplyArgs<T>* ya, const YSweepArgs<T>* ys, hipStream_t s)
 {
     switch (ny) {
-    case 8: return launch_y_n<T, 8>(ya, ys, s);
     case 16: return launch_y_n<T, 16>(ya, ys, s);
     case 32: return launch_y_n<T, 32>(ya, ys, s);
     case 64: return launch_y_n<T, 64>(ya, ys, s);
@@ -734,23 +580,78 @@ __global__ void k_rhs(RhsArgs<T> a)
     }
 }
 
+// 16-byte vector of consecutive cells (2 in FP64, 4 in FP32)
 template <typename T>
-__global__ void k_rhs_rk(RhsRkArgs<T> a)
+struct alignas(16) Vec {
+    static constexpr int W = 16 / sizeof(T);
+    T x[W];
+};
+
+template <typename T>
+__device__ __forceinline__ Vec<T> ldv(const T* __restrict__ p)
 {
+    return *reinterpret_cast<const Vec<T>*>(p);
+}
+
+// fused RHS + RK update; each thread handles one 16-byte vector of cells
+// per field (n is a multiple of the vector width: nx, ny are powers of two)
+template <typename T>
+__global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T> a)
+{
+    constexpr int W = Vec<T>::W;
     const size_t n = a.n;
     const T cc = (T)a.rs->cc[a.stage];
     const T dd = (T)a.rs->dd[a.stage];
+    const T g = (T)a.gamma;
     const bool use_prev = a.stage > 0;
-    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
-         p += (size_t)gridDim.x * blockDim.x) {
-        T R[8], rho;
-        rhs_cell<T>(a.u, a.dudx, a.dudy, a.dEdx, a.dEdy, n, p, (T)a.gamma, R, rho);
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n / W;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = q * W;
+        Vec<T> u[8], dx[8], dy[8], ex[3], ey[3];
+#pragma unroll
+        for (int f = 0; f < 8; f++) {
+            u[f] = ldv(a.u + f * n + p);
+            dx[f] = ldv(a.dudx + f * n + p);
+            dy[f] = ldv(a.dudy + f * n + p);
+        }
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            ex[f] = ldv(a.dEdx + f * n + p);
+            ey[f] = ldv(a.dEdy + f * n + p);
+        }
+        Vec<T> R[8];
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            // calcRHSInnerPoint, calcRHS.c:85-142
+            const T rho = u[RHO].x[w], ux = u[UX].x[w], uy = u[UY].x[w];
+            const T bx = u[BX].x[w], by = u[BY].x[w], bz = u[BZ].x[w], pr = u[PR].x[w];
+            const T divU = dx[UX].x[w] + dy[UY].x[w];
+            const T jx = dy[BZ].x[w];
+            const T jy = -dx[BZ].x[w];
+            const T jz = dx[BY].x[w] - dy[BX].x[w];
+            const T ir = (T)1 / rho;
+            R[RHO].x[w] = -rho * divU - ux * dx[RHO].x[w] - uy * dy[RHO].x[w];
+            R[UX].x[w] = -ux * dx[UX].x[w] - uy * dy[UX].x[w] - dx[PR].x[w] * ir +
+                         jy * bz * ir - jz * by * ir;
+            R[UY].x[w] = -ux * dx[UY].x[w] - uy * dy[UY].x[w] - dy[PR].x[w] * ir +
+                         jz * bx * ir - jx * bz * ir;
+            R[UZ].x[w] = -ux * dx[UZ].x[w] - uy * dy[UZ].x[w] + jx * by * ir - jy * bx * ir;
+            R[BX].x[w] = -ey[2].x[w];
+            R[BY].x[w] = ex[2].x[w];
+            R[BZ].x[w] = ey[0].x[w] - ex[1].x[w];
+            R[PR].x[w] = -ux * dx[PR].x[w] - uy * dy[PR].x[w] - g * pr * divU;
+        }
 #pragma unroll
         for (int f = 0; f < 8; f++) {
             // rkExecute, timeAdvance.c:31-35 (R_prev is zero in stage 0)
-            const T rp = use_prev ? a.rprev[f * n + p] : (T)0;
-            a.u[f * n + p] = a.u[f * n + p] + cc * R[f] + dd * rp;
-            if (a.write_rprev) a.rprev[f * n + p] = R[f];
+            Vec<T> rp;
+            if (use_prev) rp = ldv(a.rprev + f * n + p);
+            Vec<T> o;
+#pragma unroll
+            for (int w = 0; w < W; w++)
+                o.x[w] = u[f].x[w] + cc * R[f].x[w] + dd * (use_prev ? rp.x[w] : (T)0);
+            *reinterpret_cast<Vec<T>*>(a.u + f * n + p) = o;
+            if (a.write_rprev) *reinterpret_cast<Vec<T>*>(a.rprev + f * n + p) = R[f];
         }
     }
 }
@@ -816,10 +717,10 @@ __global__ void k_fill(T* p, T v, size_t n)
 }
 
 // per-block partial sums (deterministic order): [0] energy density,
-// [1] max |div B| (from the given derivative arrays; skipped if null)
+// [1] max |dBx/dx + dBy/dy| (from the two given derivative fields; skipped if null)
 template <typename T>
-__global__ void __launch_bounds__(256) k_diag(const T* __restrict__ u, const T* __restrict__ dudx,
-                                              const T* __restrict__ dudy, double* partial,
+__global__ void __launch_bounds__(256) k_diag(const T* __restrict__ u, const T* __restrict__ dbx,
+                                              const T* __restrict__ dby, double* partial,
                                               size_t n, double gamma)
 {
     __shared__ double se[256], sm[256];
@@ -831,7 +732,7 @@ __global__ void __launch_bounds__(256) k_diag(const T* __restrict__ u, const T* 
         const double bx = u[BX * n + p], by = u[BY * n + p], bz = u[BZ * n + p];
         e += 0.5 * rho * (ux * ux + uy * uy + uz * uz) + u[PR * n + p] / (gamma - 1.0) +
              0.5 * (bx * bx + by * by + bz * bz);
-        if (dudx) m = fmax(m, fabs((double)dudx[BX * n + p] + (double)dudy[BY * n + p]));
+        if (dbx) m = fmax(m, fabs((double)dbx[p] + (double)dby[p]));
     }
     se[threadIdx.x] = e;
     sm[threadIdx.x] = m;
@@ -847,6 +748,24 @@ __global__ void __launch_bounds__(256) k_diag(const T* __restrict__ u, const T* 
         partial[2 * blockIdx.x] = se[0];
         partial[2 * blockIdx.x + 1] = sm[0];
     }
+}
+
+// plain streaming copy of known size (bandwidth calibration of the profiler
+// counters and of the achievable HBM rate)
+__global__ void __launch_bounds__(256) k_copy(const double2* __restrict__ a,
+                                              double2* __restrict__ b, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s)
+{
+    const size_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy, pw_grid(n, 256), dim3(256), 0, s, (const double2*)src,
+                       (double2*)dst, n);
+    return hipGetLastError();
 }
 
 // ============================================================== launchers
@@ -874,7 +793,7 @@ hipError_t launch_rhs(const RhsArgs<T>& a, hipStream_t s)
 template <typename T>
 hipError_t launch_rhs_rk(const RhsRkArgs<T>& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_rhs_rk<T>), pw_grid(a.n, 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_rhs_rk<T>), pw_grid(a.n / Vec<T>::W, 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -919,10 +838,10 @@ hipError_t launch_fill(T* p, T v, size_t n, hipStream_t s)
 }
 
 template <typename T>
-hipError_t launch_diag(const T* u, const T* dudx, const T* dudy, double* partial, int nblocks,
+hipError_t launch_diag(const T* u, const T* dbx, const T* dby, double* partial, int nblocks,
                        size_t n, double gamma, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_diag<T>), dim3(nblocks), dim3(256), 0, s, u, dudx, dudy, partial, n,
+    hipLaunchKernelGGL((k_diag<T>), dim3(nblocks), dim3(256), 0, s, u, dbx, dby, partial, n,
                        gamma);
     return hipGetLastError();
 }

@@ -133,7 +133,7 @@ def test_diagnostics_match_oracle(m):
     e_ref = orc.energy(u, dx, dy)
     div = orc.derivX(u[4:5], dx)[0] + orc.derivY(u[5:6], 2 * np.pi)[0]
     assert abs(e - e_ref) <= 1e-12 * e_ref
-    assert abs(dmax - np.abs(div).max()) <= 1e-9 * max(np.abs(div).max(), 1e-300) + 1e-14
+    assert abs(dmax - np.abs(div).max()) <= 1e-9 * np.abs(div).max() + 1e-14
 
 
 def test_fp32_error_vs_fp64_oracle(m):
@@ -163,8 +163,10 @@ def test_full_size_properties_2048(m):
     e1, d1 = c.diagnostics()
     u = c.state()
     assert np.isfinite(u).all()
+    print(f"[ctx] 2048^2: energy {e0:.12e} -> {e1:.12e}, max|div B| {d0:.3e} -> {d1:.3e}")
     assert abs(e1 - e0) <= 1e-6 * e0          # ideal MHD conserves energy; filters dissipate little
-    assert d1 <= 1e-6 * np.abs(u[4:7]).max() * nx / (2 * np.pi)
+    # |dB/dx| ~ rms * kmax ~ 1.6: div B stays at the scheme's truncation level
+    assert d1 <= 1e-4 and d1 <= 100 * max(d0, 1e-9)
     c2 = m.Context(nx, ny, hall=1, di=0.02)
     c2.set_state(u0)
     c2.run(4, graph=True)
