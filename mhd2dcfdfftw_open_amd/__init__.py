@@ -15,7 +15,8 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libmhd2d.so")
+# MHD2D_LIB selects another build of the same library (tuning variants)
+LIB_PATH = os.environ.get("MHD2D_LIB", os.path.join(PKG, "libmhd2d.so"))
 
 FP64, FP32 = 0, 1
 RHO, UX, UY, UZ, BX, BY, BZ, PR = range(8)

@@ -19,13 +19,16 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
-BUILD = os.path.join(PKG, "_build")
-LIB = os.path.join(PKG, "libmhd2d.so")
-LIB_CORE = os.path.join(PKG, "libmhd2d_core.so")
+BUILD = os.environ.get("MHD2D_BUILD_DIR", os.path.join(PKG, "_build"))
+LIB = os.environ.get("MHD2D_LIB_OUT", os.path.join(PKG, "libmhd2d.so"))
+LIB_CORE = os.path.join(os.path.dirname(LIB), os.path.basename(LIB).replace(
+    "libmhd2d", "libmhd2d_core"))
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+# extra compile flags (e.g. tuning -D switches of the kernels) for variant builds
+EXTRA = os.environ.get("MHD2D_EXTRA_FLAGS", "").split()
+COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", *EXTRA,
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-result", "-I", os.path.join(ROOT, "include")]
 CORE_SOURCES = ["mhd2d_kernels.hip", "mhd2d_tables.cpp", "mhd2d_ctx.cpp",

@@ -35,6 +35,8 @@ struct XTables {
     const double* q;      // [nx]
     const double* pf;     // [G] product of -l over each chunk
     const double* qb;     // [G] product of q over each chunk
+    int kconv;            // l, invd, q equal (bitwise) their fixed point on rows kconv..nx-2
+    double linf, dinf, qinf;
 };
 
 template <typename T>

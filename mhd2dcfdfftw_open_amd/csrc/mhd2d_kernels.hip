@@ -80,8 +80,11 @@ __device__ __forceinline__ void block_min_dt(double v, RunScalars* rs, double* r
 }
 
 // ============================================================== x solves
+#ifndef MHD2D_X_MINB
+#define MHD2D_X_MINB 1
+#endif
 template <typename T, int C, int W, int KIND>
-__global__ void __launch_bounds__(W * 64) k_xsolve(XSolveArgs<T> a)
+__global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T> a)
 {
     const int G = blockDim.x / W;
     const int c = threadIdx.x % W;
@@ -144,10 +147,25 @@ __global__ void __launch_bounds__(W * 64) k_xsolve(XSolveArgs<T> a)
     const double* __restrict__ ID = a.tb.invd + s;
     const double* __restrict__ Q = a.tb.q + s;
 
+    // Rows from tb.kconv to N-2 carry the factors' exact FP64 fixed point,
+    // so a chunk inside that range uses the scalar constants (no loads).
+    // (FP32 only: in FP64 the second code path costs more registers than
+    // the table loads it saves)
+#ifndef MHD2D_X_CONST
+#define MHD2D_X_CONST (sizeof(T) == 4)
+#endif
+    const bool interior = MHD2D_X_CONST && (s >= a.tb.kconv) && !last;
+    const T linf = (T)a.tb.linf, dinf = (T)a.tb.dinf, qinf = (T)a.tb.qinf;
+
     // ---- forward recurrence y_i = r_i - l_i y_{i-1}
     T y = 0;
+    if (interior) {
 #pragma unroll
-    for (int k = 0; k < C; k++) y = w[k] - (T)L[k] * y;
+        for (int k = 0; k < C; k++) y = w[k] - linf * y;
+    } else {
+#pragma unroll
+        for (int k = 0; k < C; k++) y = w[k] - (T)L[k] * y;
+    }
     sh_in[threadIdx.x] = y;
     __syncthreads();
     {
@@ -166,16 +184,29 @@ __global__ void __launch_bounds__(W * 64) k_xsolve(XSolveArgs<T> a)
     }
     __syncthreads();
     y = (g > 0) ? sh_fw[(g - 1) * W + c] : (T)0;
+    if (interior) {
 #pragma unroll
-    for (int k = 0; k < C; k++) {
-        y = w[k] - (T)L[k] * y;
-        w[k] = y;
+        for (int k = 0; k < C; k++) {
+            y = w[k] - linf * y;
+            w[k] = y;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+            y = w[k] - (T)L[k] * y;
+            w[k] = y;
+        }
     }
 
     // ---- backward recurrence x_i = y_i invd_i + q_i x_{i+1}
     T x = 0;
+    if (interior) {
 #pragma unroll
-    for (int k = C - 1; k >= 0; k--) x = w[k] * (T)ID[k] + (T)Q[k] * x;
+        for (int k = C - 1; k >= 0; k--) x = w[k] * dinf + qinf * x;
+    } else {
+#pragma unroll
+        for (int k = C - 1; k >= 0; k--) x = w[k] * (T)ID[k] + (T)Q[k] * x;
+    }
     sh_in[threadIdx.x] = x;
     __syncthreads();
     {
@@ -195,22 +226,34 @@ __global__ void __launch_bounds__(W * 64) k_xsolve(XSolveArgs<T> a)
     }
     __syncthreads();
     x = (g < G - 1) ? sh_bw[(g + 1) * W + c] : (T)0;
+    if (interior) {
 #pragma unroll
-    for (int k = C - 1; k >= 0; k--) {
-        x = w[k] * (T)ID[k] + (T)Q[k] * x;
-        w[k] = x;
+        for (int k = C - 1; k >= 0; k--) {
+            x = w[k] * dinf + qinf * x;
+            w[k] = x;
+        }
+    } else {
+#pragma unroll
+        for (int k = C - 1; k >= 0; k--) {
+            x = w[k] * (T)ID[k] + (T)Q[k] * x;
+            w[k] = x;
+        }
     }
     T* __restrict__ po = o + (size_t)s * ny;
 #pragma unroll
     for (int k = 0; k < C; k++) po[k * ny] = w[k];
 }
 
+#ifndef MHD2D_X_W
+#define MHD2D_X_W 16
+#endif
+
 void xsolve_shape(int nx, int elem_bytes, int* chunk, int* width, int* groups)
 {
     int C = nx / 64;
     if (C < 8) C = 8;
-    int W = 16;
-    if (elem_bytes == 8 && C > 32) W = 8;
+    int W = MHD2D_X_W;
+    if (C > 32) W = 8;  // 64-row chunks: 8-column strips keep 512 threads per block
     *chunk = C;
     *width = W;
     *groups = nx / C;
@@ -235,13 +278,19 @@ hipError_t launch_xsolve(const XSolveArgs<T>& a, int nv, hipStream_t s)
     int C, W, G;
     xsolve_shape(a.nx, sizeof(T), &C, &W, &G);
     if (a.nx % C || G > 64 || G < 2 || a.ny % W || nv < 1 || nv > 16) return hipErrorInvalidValue;
+    if (W == 8) {
+        switch (C) {
+        case 8: return launch_xsolve_cw<T, 8, 8>(a, nv, s);
+        case 16: return launch_xsolve_cw<T, 16, 8>(a, nv, s);
+        case 32: return launch_xsolve_cw<T, 32, 8>(a, nv, s);
+        case 64: return launch_xsolve_cw<T, 64, 8>(a, nv, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (C) {
     case 8: return launch_xsolve_cw<T, 8, 16>(a, nv, s);
     case 16: return launch_xsolve_cw<T, 16, 16>(a, nv, s);
     case 32: return launch_xsolve_cw<T, 32, 16>(a, nv, s);
-    case 64:
-        if (W == 8) return launch_xsolve_cw<T, 64, 8>(a, nv, s);
-        return launch_xsolve_cw<T, 64, 16>(a, nv, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -249,10 +298,23 @@ hipError_t launch_xsolve(const XSolveArgs<T>& a, int nv, hipStream_t s)
 // ============================================================== y operators
 // elements per thread of the row FFTs: 8 keeps an FP64 line plus its
 // spectrum within ~100 VGPRs; FP32 lines take 16
+#ifndef MHD2D_Y_E64
+#define MHD2D_Y_E64 8
+#endif
 template <typename T>
 struct YElems {
-    static constexpr int E = sizeof(T) == 8 ? 8 : 16;
+    static constexpr int E = sizeof(T) == 8 ? MHD2D_Y_E64 : 16;
 };
+
+// optional occupancy request for the fused y-sweep (0: compiler's choice)
+#ifndef MHD2D_Y_WAVES
+#define MHD2D_Y_WAVES 0
+#endif
+#if MHD2D_Y_WAVES > 0
+#define MHD2D_Y_ATTR __attribute__((amdgpu_waves_per_eu(MHD2D_Y_WAVES)))
+#else
+#define MHD2D_Y_ATTR
+#endif
 
 template <typename T, int N>
 struct YShape {
@@ -319,7 +381,7 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) k_yapply(YApplyArgs<T
 // positions of its row in physical and spectral space, so the filtered state
 // it writes is re-read by the same thread only (E and the CFL limits).
 template <typename T, int N>
-__global__ void __launch_bounds__((YShape<T, N>::THREADS)) k_ysweep(YSweepArgs<T> a)
+__global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep(YSweepArgs<T> a)
 {
     using Y = YShape<T, N>;
     constexpr int TPR = Y::TPR;

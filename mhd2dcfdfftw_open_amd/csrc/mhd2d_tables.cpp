@@ -134,8 +134,20 @@ hipError_t GridTables::build(int nx_, int ny_, double Lx_, double Ly_, int elem_
     e = hipMemcpy(blob_, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
     const double* d = (const double*)blob_;
-    for (int k = 0; k < NXKIND; k++)
-        xt[k] = XTables{d + off_l[k], d + off_id[k], d + off_q[k], d + off_pf[k], d + off_qb[k]};
+    for (int k = 0; k < NXKIND; k++) {
+        // first row from which l, invd and q stay bitwise equal to their
+        // values at row nx-3 up to row nx-2 (the Thomas factors' FP64 fixed
+        // point: row 25 for the derivatives, 87 for the filter)
+        const double* L = p + off_l[k];
+        const double* ID = p + off_id[k];
+        const double* Q = p + off_q[k];
+        int kc = nx - 2;
+        while (kc > 0 && L[kc - 1] == L[nx - 3] && ID[kc - 1] == ID[nx - 3] && Q[kc - 1] == Q[nx - 3])
+            kc--;
+        xt[k] = XTables{d + off_l[k], d + off_id[k], d + off_q[k], d + off_pf[k], d + off_qb[k],
+                        kc, L[nx - 3], ID[nx - 3], Q[nx - 3]};
+        kconv[k] = kc;
+    }
     yt.md = d + off_md;
     yt.mdd = d + off_mdd;
     yt.mf = d + off_mf;

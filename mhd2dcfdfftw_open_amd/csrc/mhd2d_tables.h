@@ -31,6 +31,7 @@ public:
     int chunk = 0, width = 0, groups = 0;
     XStencil st[NXKIND];
     XTables xt[NXKIND];
+    int kconv[NXKIND] = {0, 0, 0};
     YTables yt{};
 
     // host copies (FP64) for inspection and tests
