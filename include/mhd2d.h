@@ -138,6 +138,12 @@ int mhd2d_get_scalars(mhd2d_ctx *c, double *dt, double *time, long long *steps);
 /* total energy sum(rho u^2/2 + p/(gamma-1) + B^2/2) dx dy and max |div B|
  * of the current state, FP64 reductions */
 int mhd2d_diagnostics(mhd2d_ctx *c, double *energy, double *max_divb);
+/* dBx/dx + dBy/dy of the current state, [nx][ny] FP64 host array */
+int mhd2d_get_divb(mhd2d_ctx *c, double *divb);
+/* 1 when the run has stopped because the new dt fell below 1e-8 (mhd.c:131):
+ * that step and all later ones are not taken; *dt_stop gets the small dt.
+ * 0 while running, or an error code > 1 */
+int mhd2d_stopped(mhd2d_ctx *c, double *dt_stop);
 /* wall time of nsteps on the context's stream (hipEvents), in ms */
 int mhd2d_time_run(mhd2d_ctx *c, int nsteps, int use_graph, double *ms);
 /* average device time per launch of each kernel class over nsteps, in ms:
@@ -151,6 +157,69 @@ size_t mhd2d_device_bytes(const mhd2d_ctx *c);
  * achievable copy rate beside the 8 TB/s spec peak. */
 int mhd2d_copy_probe(size_t bytes, int iters, double *gbps);
 const char *mhd2d_last_error(void);
+
+/* ===================================================================== */
+/* 3. the reference's run configuration, output files and run loop       */
+/* ===================================================================== */
+/* the 15 values of the reference's `input` file, in its order
+ * (initialize.c:456-525) */
+typedef struct mhd2d_input {
+    int boundaryTypeX;
+    double cfl, Tmax, dtOut;
+    int nx, ny;
+    double Lx, Ly, gamma;
+    int hall;
+    double di, resistivity, viscosity;
+    int divbclean, outdivb;
+} mhd2d_input;
+
+/* parse an `input` file: each value from the first 16 characters of its
+ * line with atoi/atof, as readLine/readInput do (initialize.c:442-525) */
+int mhd2d_read_input(const char *path, mhd2d_input *in);
+/* write an `input` file the reference's reader accepts */
+int mhd2d_write_input(const char *path, const mhd2d_input *in);
+/* xgrid.dat / ygrid.dat: [size_t n][n doubles] (writeOutput.c:5-45) */
+int mhd2d_write_grid(const char *dir, int nx, int ny, double Lx, double Ly);
+/* out%04d.dat: [double t][nvar*nx*ny doubles] (writeOutput.c:47-76) */
+int mhd2d_write_array(const char *dir, int iout, double t, const double *uu,
+                      int nvar, int nx, int ny);
+/* divB%04d.dat: [double t][nx*ny doubles] (writeOutput.c:78-107) */
+int mhd2d_write_divb(const char *dir, int iout, double t, const double *divb,
+                     int nx, int ny);
+/* read an out%04d.dat back (restart; the reference has no reader) */
+int mhd2d_read_array(const char *path, double *t, double *uu, size_t count);
+/* the `log` file of mhd.c:319-338 */
+int mhd2d_write_log(const char *dir, double t, double dt, double seconds,
+                    long long step);
+
+enum { MHD2D_DRIVE_AUTO = 0, MHD2D_DRIVE_FUSED = 1, MHD2D_DRIVE_DROPIN = 2 };
+
+typedef struct mhd2d_drive_opts {
+    const char *outdir;     /* directory of the output files ("" = cwd) */
+    int mode;               /* MHD2D_DRIVE_*: AUTO = FUSED for ideal MHD,
+                               DROPIN otherwise */
+    int precision;          /* MHD2D_FP64 / MHD2D_FP32 (FUSED only) */
+    int verbose;            /* print the reference's progress lines */
+    long long max_steps;    /* stop after this many steps (0: no cap) */
+} mhd2d_drive_opts;
+
+typedef struct mhd2d_drive_report {
+    double time, dt;        /* simulated time reached, last dt */
+    long long steps;        /* RK3 steps taken */
+    int outputs;            /* out%04d.dat files written */
+    int stopped;            /* 1: ended by the dt < 1e-8 stop (mhd.c:131) */
+    double seconds;         /* wall time of the loop */
+    double energy;          /* total energy at the end (FUSED only) */
+    double max_divb;        /* max |div B| at the last output */
+} mhd2d_drive_report;
+
+/* The time-driven loop of mhd.c:58-230 on the GPU: outputs every dtOut,
+ * log every dtOut/10, the dt < 1e-8 stop, final output after Tmax.  Writes
+ * xgrid/ygrid are the caller's (mhd2d_write_grid).  Single thread; the
+ * DROPIN engine uses the library's own storage of the reference globals
+ * (libmhd2d.so only). */
+int mhd2d_drive(const mhd2d_input *in, const double *uu0,
+                const mhd2d_drive_opts *o, mhd2d_drive_report *rep);
 
 /* ===================================================================== */
 /* initial states (host, FP64) shared by the device runs and the checker  */

@@ -30,6 +30,36 @@ class MHD2DError(RuntimeError):
     pass
 
 
+class Input(ctypes.Structure):
+    """the 15 values of the reference's `input` file (initialize.c:456-525)"""
+    _fields_ = [("boundaryTypeX", ctypes.c_int), ("cfl", ctypes.c_double),
+                ("Tmax", ctypes.c_double), ("dtOut", ctypes.c_double),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("Lx", ctypes.c_double), ("Ly", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("hall", ctypes.c_int), ("di", ctypes.c_double),
+                ("resistivity", ctypes.c_double), ("viscosity", ctypes.c_double),
+                ("divbclean", ctypes.c_int), ("outdivb", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class DriveOpts(ctypes.Structure):
+    _fields_ = [("outdir", ctypes.c_char_p), ("mode", ctypes.c_int),
+                ("precision", ctypes.c_int), ("verbose", ctypes.c_int),
+                ("max_steps", ctypes.c_longlong)]
+
+
+class DriveReport(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_double), ("dt", ctypes.c_double),
+                ("steps", ctypes.c_longlong), ("outputs", ctypes.c_int),
+                ("stopped", ctypes.c_int), ("seconds", ctypes.c_double),
+                ("energy", ctypes.c_double), ("max_divb", ctypes.c_double)]
+
+
+DRIVE_AUTO, DRIVE_FUSED, DRIVE_DROPIN = 0, 1, 2
+
+
 class Params(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int),
                 ("Lx", ctypes.c_double), ("Ly", ctypes.c_double),
@@ -90,6 +120,17 @@ def lib():
         "sendrecvHaloPointsdivBX": (None, []),
         "sendrecvHaloPointsEfieldX": (None, []),
         "calc_Cp_fix_Cr": (None, []),
+        "mhd2d_get_divb": (i, [vp, p]),
+        "mhd2d_stopped": (i, [vp, p]),
+        "mhd2d_read_input": (i, [ctypes.c_char_p, ctypes.POINTER(Input)]),
+        "mhd2d_write_input": (i, [ctypes.c_char_p, ctypes.POINTER(Input)]),
+        "mhd2d_write_grid": (i, [ctypes.c_char_p, i, i, d, d]),
+        "mhd2d_write_array": (i, [ctypes.c_char_p, i, d, p, i, i, i]),
+        "mhd2d_write_divb": (i, [ctypes.c_char_p, i, d, p, i, i]),
+        "mhd2d_read_array": (i, [ctypes.c_char_p, p, p, ctypes.c_size_t]),
+        "mhd2d_write_log": (i, [ctypes.c_char_p, d, d, d, ll]),
+        "mhd2d_drive": (i, [ctypes.POINTER(Input), p, ctypes.POINTER(DriveOpts),
+                            ctypes.POINTER(DriveReport)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -132,6 +173,30 @@ def linear_wave(nx, ny, seed=42, amp=1e-4, Lx=2 * np.pi, Ly=2 * np.pi):
     uu = np.zeros((8, nx, ny))
     lib().mhd2d_ic_linear_wave(nx, ny, Lx, Ly, seed, amp, _ptr(uu))
     return uu
+
+
+# ------------------------------------------------------------------ run configuration / files
+def read_input(path):
+    inp = Input()
+    _check(lib().mhd2d_read_input(os.fsencode(path), ctypes.byref(inp)))
+    return inp
+
+
+def write_input(path, **kw):
+    inp = Input(**kw)
+    _check(lib().mhd2d_write_input(os.fsencode(path), ctypes.byref(inp)))
+    return inp
+
+
+def drive(inp, uu0, outdir="", mode=DRIVE_AUTO, precision=FP64, verbose=False, max_steps=0):
+    """The time-driven loop of mhd.c on the GPU; returns the report dict."""
+    uu0 = np.ascontiguousarray(uu0, dtype=np.float64)
+    assert uu0.shape == (8, inp.nx, inp.ny)
+    o = DriveOpts(os.fsencode(outdir), mode, precision, int(verbose), max_steps)
+    rep = DriveReport()
+    _check(lib().mhd2d_write_grid(os.fsencode(outdir), inp.nx, inp.ny, inp.Lx, inp.Ly))
+    _check(lib().mhd2d_drive(ctypes.byref(inp), _ptr(uu0), ctypes.byref(o), ctypes.byref(rep)))
+    return {k: getattr(rep, k) for k, _ in rep._fields_}
 
 
 # ------------------------------------------------------------------ context API
@@ -196,6 +261,18 @@ class Context:
 
     def device_bytes(self):
         return lib().mhd2d_device_bytes(self.h)
+
+    def divb(self):
+        out = np.zeros((self.nx, self.ny))
+        _check(lib().mhd2d_get_divb(self.h, _ptr(out)))
+        return out
+
+    def stopped(self):
+        d = ctypes.c_double()
+        r = lib().mhd2d_stopped(self.h, ctypes.byref(d))
+        if r > 1:
+            _check(r)
+        return bool(r), d.value
 
 
 # ------------------------------------------------------------------ drop-in layer

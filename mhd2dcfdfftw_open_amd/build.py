@@ -32,8 +32,10 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", *EXTRA,
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-result", "-I", os.path.join(ROOT, "include")]
 CORE_SOURCES = ["mhd2d_kernels.hip", "mhd2d_tables.cpp", "mhd2d_ctx.cpp",
-                "mhd2d_dropin.cpp", "mhd2d_ic.cpp"]
-GLOBAL_SOURCES = ["mhd2d_globals.cpp"]
+                "mhd2d_dropin.cpp", "mhd2d_ic.cpp", "mhd2d_io.cpp"]
+# the storage of the reference globals and what needs it (standalone library)
+GLOBAL_SOURCES = ["mhd2d_globals.cpp", "mhd2d_driver.cpp"]
+RUN_EXE = os.path.join(os.path.dirname(LIB), "mhd2d_run")
 
 
 def _compile(src):
@@ -64,12 +66,21 @@ def build(verbose=False):
         if r.returncode != 0:
             raise RuntimeError(f"link failed for {out}:\n{r.stdout}{r.stderr}")
         os.replace(tmp, out)
+    # the reference-style executable, linked against the standalone library
+    tmp = RUN_EXE + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "mhd2d_run_main.cpp"), "-o", tmp, LIB,
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed for {RUN_EXE}:\n{r.stdout}{r.stderr}")
+    os.replace(tmp, RUN_EXE)
     return LIB
 
 
 def clean():
     shutil.rmtree(BUILD, ignore_errors=True)
-    for p in (LIB, LIB_CORE):
+    for p in (LIB, LIB_CORE, RUN_EXE):
         if os.path.exists(p):
             os.remove(p)
 

@@ -107,6 +107,7 @@ hipError_t xderiv(mhd2d_ctx* c)
     for (int v = 0; v < 8; v++) a.in[v] = u + v * c->n;
     for (int v = 0; v < 3; v++) a.in[8 + v] = E + v * c->n;
     for (int v = 0; v < 11; v++) a.out[v] = dx + v * c->n;
+    a.rs = c->rs;
     a.st = c->tb.st[XD1];
     a.tb = c->tb.xt[XD1];
     a.kind = XD1;
@@ -121,6 +122,7 @@ hipError_t xfilter(mhd2d_ctx* c)
     XSolveArgs<T> a;
     T* u = (T*)c->u;
     for (int v = 0; v < 8; v++) a.in[v] = a.out[v] = u + v * c->n;
+    a.rs = c->rs;
     a.st = c->tb.st[XFILT];
     a.tb = c->tb.xt[XFILT];
     a.kind = XFILT;
@@ -189,8 +191,26 @@ hipError_t flush(mhd2d_ctx* c)
     return e;
 }
 
+// After a stop (dt < 1e-8, mhd.c:131) the stopping step's y-sweep has
+// already applied the pending filter and every later launch was a no-op.
+hipError_t check_stop(mhd2d_ctx* c, int* stopped, double* dt_stop)
+{
+    RunScalars r;
+    hipError_t e = hipMemcpyAsync(&r, c->rs, sizeof(r), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return e;
+    if (r.stop) c->pending_filter = false;
+    if (stopped) *stopped = r.stop;
+    if (dt_stop) *dt_stop = r.dt_stop;
+    return hipSuccess;
+}
+
 hipError_t do_flush(mhd2d_ctx* c)
 {
+    if (c->pending_filter) {
+        hipError_t e = check_stop(c, nullptr, nullptr);
+        if (e != hipSuccess) return e;
+    }
     return c->elem == 8 ? flush<double>(c) : flush<float>(c);
 }
 
@@ -417,6 +437,74 @@ int mhd2d_get_scalars(mhd2d_ctx* c, double* dt, double* time, long long* steps)
     if (dt) *dt = r.dt;
     if (time) *time = r.step > 0 ? r.time + r.dt : 0.0;
     if (steps) *steps = r.step;
+    return MHD2D_OK;
+}
+
+int mhd2d_stopped(mhd2d_ctx* c, double* dt_stop)
+{
+    if (!c) return fail(MHD2D_EINVAL, "null argument");
+    int st = 0;
+    HIPCHK(check_stop(c, &st, dt_stop));
+    return st;
+}
+
+int mhd2d_get_divb(mhd2d_ctx* c, double* divb)
+{
+    if (!c || !divb) return fail(MHD2D_EINVAL, "null argument");
+    HIPCHK(do_flush(c));
+    const size_t n = c->n;
+    // dBx/dx + dBy/dy of the current state, through the E scratch buffer
+    if (c->elem == 8) {
+        double* u = (double*)c->u;
+        double* sc = (double*)c->E;
+        XSolveArgs<double> xa;
+        xa.in[0] = u + BX * n;
+        xa.out[0] = sc;
+        xa.st = c->tb.st[XD1];
+        xa.tb = c->tb.xt[XD1];
+        xa.kind = XD1;
+        xa.nx = c->p.nx;
+        xa.ny = c->p.ny;
+        HIPCHK(launch_xsolve<double>(xa, 1, c->stream));
+        YApplyArgs<double> ya;
+        ya.in[0] = u + BY * n;
+        ya.out[0] = sc + n;
+        ya.tb = c->tb.yt;
+        ya.nv = 1;
+        ya.mode = Y_D1;
+        ya.nx = c->p.nx;
+        ya.ny = c->p.ny;
+        HIPCHK(launch_yapply<double>(ya, c->stream));
+        HIPCHK(launch_add<double>(sc, sc + n, sc + 2 * n, n, c->stream));
+        HIPCHK(hipMemcpyAsync(divb, sc + 2 * n, n * 8, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        float* u = (float*)c->u;
+        float* sc = (float*)c->E;
+        XSolveArgs<float> xa;
+        xa.in[0] = u + BX * n;
+        xa.out[0] = sc;
+        xa.st = c->tb.st[XD1];
+        xa.tb = c->tb.xt[XD1];
+        xa.kind = XD1;
+        xa.nx = c->p.nx;
+        xa.ny = c->p.ny;
+        HIPCHK(launch_xsolve<float>(xa, 1, c->stream));
+        YApplyArgs<float> ya;
+        ya.in[0] = u + BY * n;
+        ya.out[0] = sc + n;
+        ya.tb = c->tb.yt;
+        ya.nv = 1;
+        ya.mode = Y_D1;
+        ya.nx = c->p.nx;
+        ya.ny = c->p.ny;
+        HIPCHK(launch_yapply<float>(ya, c->stream));
+        HIPCHK(launch_add<float>(sc, sc + n, sc + 2 * n, n, c->stream));
+        std::vector<float> tmp(n);
+        HIPCHK(hipMemcpyAsync(tmp.data(), sc + 2 * n, n * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < n; i++) divb[i] = tmp[i];
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
     return MHD2D_OK;
 }
 

@@ -11,6 +11,8 @@ namespace mhd2d {
 // field indices of the state, same order as the reference (macros.h:4-11)
 enum Field { RHO = 0, UX, UY, UZ, BX, BY, BZ, PR, NFIELD = 8 };
 
+struct RunScalars;
+
 // ---------------------------------------------------------------- x operators
 enum XKind { XD1 = 0, XD2 = 1, XFILT = 2, NXKIND = 3 };
 
@@ -45,6 +47,7 @@ struct XSolveArgs {
     T* out[16];
     XStencil st;
     XTables tb;
+    const RunScalars* rs = nullptr;  // optional: skip the launch once the run has stopped
     int kind;
     int nx, ny;
 };
@@ -86,12 +89,13 @@ struct DtParams {
 struct RunScalars {
     unsigned long long dtmin_bits;   // running min of the local dt limits
     int nonfinite;                   // set when a dt limit is not finite
-    int pad;
+    int stop;                        // set when the new dt is < 1e-8 (mhd.c:131)
     double dt;                       // dt of the current step
     double dt_prev;                  // dt of the previous step (0 at start)
     double time;                     // simulated time
     double cc[3], dd[3];             // RK3 coefficients times dt
     long long step;
+    double dt_stop;                  // the dt that triggered the stop
 };
 
 template <typename T>
@@ -173,6 +177,8 @@ hipError_t launch_diag(const T* u, const T* dbx, const T* dby, double* partial,
                        int nblocks, size_t n, double gamma, hipStream_t s);
 
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+template <typename T>
+hipError_t launch_add(const T* a, const T* b, T* out, size_t n, hipStream_t s);
 
 // chunk length (rows per thread) and strip width used by the x solves
 void xsolve_shape(int nx, int elem_bytes, int* chunk, int* width, int* groups);

@@ -86,6 +86,7 @@ __device__ __forceinline__ void block_min_dt(double v, RunScalars* rs, double* r
 template <typename T, int C, int W, int KIND>
 __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T> a)
 {
+    if (a.rs && a.rs->stop) return;  // the run stopped (dt < 1e-8): uniform per block
     const int G = blockDim.x / W;
     const int c = threadIdx.x % W;
     const int g = threadIdx.x / W;
@@ -389,6 +390,7 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
     __shared__ T s_re[Y::ROWS * Y::LDSW];
     __shared__ T s_im[Y::ROWS * Y::LDSW];
     __shared__ double red[16];
+    if (a.rs->stop) return;  // the run stopped (dt < 1e-8): uniform per block
     const int t = threadIdx.x % TPR, rl = threadIdx.x / TPR;
     const int i_raw = blockIdx.x * Y::ROWS + rl;
     const bool active = i_raw < a.nx;
@@ -661,6 +663,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T> a)
 {
     constexpr int W = Vec<T>::W;
+    if (a.rs->stop) return;  // the run stopped (dt < 1e-8)
     const size_t n = a.n;
     const T cc = (T)a.rs->cc[a.stage];
     const T dd = (T)a.rs->dd[a.stage];
@@ -747,12 +750,22 @@ __global__ void __launch_bounds__(256) k_dt_limits(const T* __restrict__ u, DtPa
 // step, mhd.c:58), then rkInit's coefficients (timeAdvance.c:20-27).
 __global__ void k_dt_finalize(RunScalars* rs, double cfl)
 {
+    if (rs->stop) return;
     const double m = __longlong_as_double((long long)rs->dtmin_bits) * cfl;
     const bool first = rs->step == 0;
     const double dt0 = first ? 0.0 : rs->dt;
     const double dt = (dt0 < 0.98 * m || dt0 > 1.02 * m) ? m : dt0;
     if (!first) rs->time += rs->dt;
     rs->dt_prev = first ? 0.0 : rs->dt;
+    if (dt < 1e-8) {
+        // mhd.c:131-151: the step is not taken; the time stays at the
+        // previous step's end (time + dt with dt = 0)
+        rs->stop = 1;
+        rs->dt_stop = dt;
+        rs->dt = 0.0;
+        rs->dtmin_bits = 0x7ff0000000000000ull;
+        return;
+    }
     rs->dt = dt;
     rs->cc[0] = 8.0 / 15.0 * dt;
     rs->dd[0] = 0.0;
@@ -768,6 +781,15 @@ __global__ void k_dt_reset(RunScalars* rs)
 {
     rs->dtmin_bits = 0x7ff0000000000000ull;
     rs->nonfinite = 0;
+}
+
+template <typename T>
+__global__ void k_add(const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ out,
+                      size_t n)
+{
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x)
+        out[p] = a[p] + b[p];
 }
 
 template <typename T>
@@ -893,6 +915,13 @@ hipError_t launch_dt_reset(RunScalars* rs, hipStream_t s)
 }
 
 template <typename T>
+hipError_t launch_add(const T* a, const T* b, T* out, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_add<T>), pw_grid(n, 256), dim3(256), 0, s, a, b, out, n);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_fill(T* p, T v, size_t n, hipStream_t s)
 {
     hipLaunchKernelGGL((k_fill<T>), pw_grid(n, 256), dim3(256), 0, s, p, v, n);
@@ -921,6 +950,7 @@ hipError_t launch_diag(const T* u, const T* dbx, const T* dby, double* partial, 
     template hipError_t launch_dt_limits<T>(const T*, const DtParams&, RunScalars*, int, int, \
                                             hipStream_t);                                    \
     template hipError_t launch_fill<T>(T*, T, size_t, hipStream_t);                          \
+    template hipError_t launch_add<T>(const T*, const T*, T*, size_t, hipStream_t);          \
     template hipError_t launch_diag<T>(const T*, const T*, const T*, double*, int, size_t,   \
                                        double, hipStream_t);
 

@@ -47,7 +47,8 @@ def test_library_exports_every_declared_symbol(libname):
     funcs, data = declared_symbols()
     if libname == "libmhd2d_core.so":
         # storage of the reference globals comes from the reference's initialize.o
-        funcs = funcs - {"mhd2d_host_initialize", "mhd2d_host_finalize"}
+        # (and the run loop that uses the library's own storage of them)
+        funcs = funcs - {"mhd2d_host_initialize", "mhd2d_host_finalize", "mhd2d_drive"}
     missing = (funcs | data) - exported
     assert not missing, f"{libname} lacks {sorted(missing)}"
 
