@@ -110,6 +110,11 @@ typedef struct mhd2d_params {
     double viscosity;       /* must be 0 in the context API */
     int divbclean;          /* must be 0 in the context API */
     int precision;          /* MHD2D_FP64 or MHD2D_FP32 (state and arithmetic) */
+    int xblocks;            /* 0 or 1: x is one block (the reference on one MPI
+                               rank); npe > 1: x split into npe blocks sized as
+                               decomposeGrid1D (initialize.c:547-569), each
+                               solved with its own end rows, so results match a
+                               reference run on npe ranks; nx / npe >= 4 */
 } mhd2d_params;
 
 /* status codes */
@@ -201,6 +206,8 @@ typedef struct mhd2d_drive_opts {
     int precision;          /* MHD2D_FP64 / MHD2D_FP32 (FUSED only) */
     int verbose;            /* print the reference's progress lines */
     long long max_steps;    /* stop after this many steps (0: no cap) */
+    int xblocks;            /* emulate a run on this many MPI ranks (FUSED
+                               only; 0 or 1: one rank) */
 } mhd2d_drive_opts;
 
 typedef struct mhd2d_drive_report {

@@ -47,7 +47,7 @@ class Input(ctypes.Structure):
 class DriveOpts(ctypes.Structure):
     _fields_ = [("outdir", ctypes.c_char_p), ("mode", ctypes.c_int),
                 ("precision", ctypes.c_int), ("verbose", ctypes.c_int),
-                ("max_steps", ctypes.c_longlong)]
+                ("max_steps", ctypes.c_longlong), ("xblocks", ctypes.c_int)]
 
 
 class DriveReport(ctypes.Structure):
@@ -66,7 +66,8 @@ class Params(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("cfl", ctypes.c_double),
                 ("hall", ctypes.c_int), ("di", ctypes.c_double),
                 ("resistivity", ctypes.c_double), ("viscosity", ctypes.c_double),
-                ("divbclean", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("divbclean", ctypes.c_int), ("precision", ctypes.c_int),
+                ("xblocks", ctypes.c_int)]
 
 
 def lib():
@@ -188,11 +189,13 @@ def write_input(path, **kw):
     return inp
 
 
-def drive(inp, uu0, outdir="", mode=DRIVE_AUTO, precision=FP64, verbose=False, max_steps=0):
-    """The time-driven loop of mhd.c on the GPU; returns the report dict."""
+def drive(inp, uu0, outdir="", mode=DRIVE_AUTO, precision=FP64, verbose=False, max_steps=0,
+          npe=1):
+    """The time-driven loop of mhd.c on the GPU; returns the report dict.
+    npe > 1 reproduces a run on npe MPI ranks (x split into npe blocks)."""
     uu0 = np.ascontiguousarray(uu0, dtype=np.float64)
     assert uu0.shape == (8, inp.nx, inp.ny)
-    o = DriveOpts(os.fsencode(outdir), mode, precision, int(verbose), max_steps)
+    o = DriveOpts(os.fsencode(outdir), mode, precision, int(verbose), max_steps, npe)
     rep = DriveReport()
     _check(lib().mhd2d_write_grid(os.fsencode(outdir), inp.nx, inp.ny, inp.Lx, inp.Ly))
     _check(lib().mhd2d_drive(ctypes.byref(inp), _ptr(uu0), ctypes.byref(o), ctypes.byref(rep)))
@@ -204,9 +207,11 @@ class Context:
     """Fused device stepping of one grid (mhd2d_create ... mhd2d_destroy)."""
 
     def __init__(self, nx, ny, Lx=2 * np.pi, Ly=2 * np.pi, gamma=5.0 / 3.0, cfl=0.3,
-                 hall=0, di=0.0, precision=FP64):
+                 hall=0, di=0.0, precision=FP64, npe=1):
+        """npe > 1 splits x into npe blocks as a reference run on npe MPI
+        ranks does (decomposeGrid1D, initialize.c:547-569)."""
         self.nx, self.ny = nx, ny
-        self.params = Params(nx, ny, Lx, Ly, gamma, cfl, hall, di, 0.0, 0.0, 0, precision)
+        self.params = Params(nx, ny, Lx, Ly, gamma, cfl, hall, di, 0.0, 0.0, 0, precision, npe)
         h = ctypes.c_void_p()
         _check(lib().mhd2d_create(ctypes.byref(self.params), ctypes.byref(h)))
         self.h = h

@@ -246,6 +246,8 @@ int check_params(const mhd2d_params* p)
                     "divbclean = 0); use the drop-in entry points for the dissipative terms");
     if (p->precision != MHD2D_FP64 && p->precision != MHD2D_FP32)
         return fail(MHD2D_EINVAL, "precision must be MHD2D_FP64 or MHD2D_FP32");
+    if (p->xblocks < 0 || p->xblocks > p->nx / 4)
+        return fail(MHD2D_EINVAL, "xblocks must be in [0, nx/4] (blocks of at least 4 rows)");
     return MHD2D_OK;
 }
 
@@ -265,7 +267,7 @@ int mhd2d_create(const mhd2d_params* p, mhd2d_ctx** out)
     c->p = *p;
     c->elem = p->precision == MHD2D_FP64 ? 8 : 4;
     c->n = (size_t)p->nx * p->ny;
-    hipError_t e = c->tb.build(p->nx, p->ny, p->Lx, p->Ly, c->elem);
+    hipError_t e = c->tb.build(p->nx, p->ny, p->Lx, p->Ly, c->elem, SchemeAlphas(), p->xblocks);
     if (e != hipSuccess) {
         delete c;
         return fail(MHD2D_EHIP, "table build", e);

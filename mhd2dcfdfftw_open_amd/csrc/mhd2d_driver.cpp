@@ -56,6 +56,7 @@ int drive_fused(const mhd2d_input* in, const double* uu0, const mhd2d_drive_opts
     p.hall = in->hall;
     p.di = in->di;
     p.precision = o->precision;
+    p.xblocks = o->xblocks;
     mhd2d_ctx* c = nullptr;
     int rc = mhd2d_create(&p, &c);
     if (rc) return rc;
@@ -248,6 +249,10 @@ extern "C" int mhd2d_drive(const mhd2d_input* in, const double* uu0, const mhd2d
     const bool ideal = in->resistivity == 0 && in->viscosity == 0 && in->divbclean == 0;
     int mode = o->mode;
     if (mode == MHD2D_DRIVE_AUTO) mode = ideal ? MHD2D_DRIVE_FUSED : MHD2D_DRIVE_DROPIN;
+    if (mode == MHD2D_DRIVE_DROPIN && o->xblocks > 1) {
+        std::fprintf(stderr, "mhd2d_drive: x blocks (npe > 1) need the fused engine\n");
+        return MHD2D_EINVAL;
+    }
     if (mode == MHD2D_DRIVE_FUSED) {
         if (!ideal) {
             std::fprintf(stderr, "mhd2d_drive: the fused engine runs ideal MHD only\n");

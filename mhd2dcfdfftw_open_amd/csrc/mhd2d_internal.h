@@ -39,6 +39,11 @@ struct XTables {
     const double* qb;     // [G] product of q over each chunk
     int kconv;            // l, invd, q equal (bitwise) their fixed point on rows kconv..nx-2
     double linf, dinf, qinf;
+    // x split into blocks solved separately (the reference's MPI ranks): per
+    // chunk, bit k set if row g*C+k opens (bstart) / closes (bend) a block.
+    // Null when x is one block (rows 0 and nx-1 are the only end rows).
+    const unsigned long long* bstart;
+    const unsigned long long* bend;
 };
 
 template <typename T>

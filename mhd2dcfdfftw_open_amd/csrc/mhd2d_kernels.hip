@@ -83,7 +83,9 @@ __device__ __forceinline__ void block_min_dt(double v, RunScalars* rs, double* r
 #ifndef MHD2D_X_MINB
 #define MHD2D_X_MINB 1
 #endif
-template <typename T, int C, int W, int KIND>
+// BLK: x is split into blocks (tb.bstart / tb.bend), each with the end rows
+// of its own non-cyclic solve; otherwise rows 0 and nx-1 are the only ends.
+template <typename T, int C, int W, int KIND, bool BLK>
 __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T> a)
 {
     if (a.rs && a.rs->stop) return;  // the run stopped (dt < 1e-8): uniform per block
@@ -118,7 +120,12 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T> a
 
     const XStencil& st = a.st;
     const T c0 = (T)st.c0, c1 = (T)st.c1, c2 = (T)st.c2, c3 = (T)st.c3;
-    const bool first = (g == 0), last = (g == G - 1);
+    const bool first = !BLK && (g == 0), last = !BLK && (g == G - 1);
+    unsigned long long bs = 0, be = 0;
+    if constexpr (BLK) {
+        bs = a.tb.bstart[g];
+        be = a.tb.bend[g];
+    }
     T rfirst = 0, rlast = 0;
     if (first)   // row 0 over f[-2..2]
         rfirst = (T)st.e[0] * w[1] + (T)st.e[1] * w[2] + (T)st.e[2] * w[3] +
@@ -139,6 +146,15 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T> a
         } else {
             r = c0 * f0 + c1 * (fp1 + fm1) + c2 * (fp2 + fm2) + c3 * (fp3 + fm3);
         }
+        if constexpr (BLK) {
+            // a block's first row over f[-2..2], its last over f[+2..-2]
+            if ((bs >> k) & 1ull)
+                r = (T)st.e[0] * fm2 + (T)st.e[1] * fm1 + (T)st.e[2] * f0 +
+                    (T)st.e[3] * fp1 + (T)st.e[4] * fp2;
+            else if ((be >> k) & 1ull)
+                r = (T)st.esign * ((T)st.e[0] * fp2 + (T)st.e[1] * fp1 + (T)st.e[2] * f0 +
+                                   (T)st.e[3] * fm1 + (T)st.e[4] * fm2);
+        }
         w[k] = r;  // w[k] is not read again by later rows
     }
     if (first) w[0] = rfirst;
@@ -155,7 +171,7 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T> a
 #ifndef MHD2D_X_CONST
 #define MHD2D_X_CONST (sizeof(T) == 4)
 #endif
-    const bool interior = MHD2D_X_CONST && (s >= a.tb.kconv) && !last;
+    const bool interior = MHD2D_X_CONST && !BLK && (s >= a.tb.kconv) && !last;
     const T linf = (T)a.tb.linf, dinf = (T)a.tb.dinf, qinf = (T)a.tb.qinf;
 
     // ---- forward recurrence y_i = r_i - l_i y_{i-1}
@@ -260,16 +276,25 @@ void xsolve_shape(int nx, int elem_bytes, int* chunk, int* width, int* groups)
     *groups = nx / C;
 }
 
+template <typename T, int C, int W, bool BLK>
+static void launch_xsolve_kind(const XSolveArgs<T>& a, dim3 grid, dim3 block, hipStream_t s)
+{
+    switch (a.kind) {
+    case XD1: hipLaunchKernelGGL((k_xsolve<T, C, W, XD1, BLK>), grid, block, 0, s, a); break;
+    case XD2: hipLaunchKernelGGL((k_xsolve<T, C, W, XD2, BLK>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((k_xsolve<T, C, W, XFILT, BLK>), grid, block, 0, s, a); break;
+    }
+}
+
 template <typename T, int C, int W>
 static hipError_t launch_xsolve_cw(const XSolveArgs<T>& a, int nv, hipStream_t s)
 {
     const int G = a.nx / C;
     dim3 grid(a.ny / W, nv), block(W * G);
-    switch (a.kind) {
-    case XD1: hipLaunchKernelGGL((k_xsolve<T, C, W, XD1>), grid, block, 0, s, a); break;
-    case XD2: hipLaunchKernelGGL((k_xsolve<T, C, W, XD2>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((k_xsolve<T, C, W, XFILT>), grid, block, 0, s, a); break;
-    }
+    if (a.tb.bstart && a.tb.bend)
+        launch_xsolve_kind<T, C, W, true>(a, grid, block, s);
+    else
+        launch_xsolve_kind<T, C, W, false>(a, grid, block, s);
     return hipGetLastError();
 }
 

@@ -6,7 +6,7 @@
 //
 //   mhd2d_run [--input FILE] [--out DIR] [--ic ot|turbulence|wave|FILE]
 //             [--seed N] [--rms R] [--kmax K] [--amp A]
-//             [--engine auto|fused|dropin] [--fp32] [--max-steps N] [--quiet]
+//             [--engine auto|fused|dropin] [--fp32] [--npe N] [--max-steps N] [--quiet]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -20,7 +20,7 @@ static void usage()
     std::fprintf(stderr,
                  "usage: mhd2d_run [--input FILE] [--out DIR] [--ic ot|turbulence|wave|FILE]\n"
                  "                 [--seed N] [--rms R] [--kmax K] [--amp A]\n"
-                 "                 [--engine auto|fused|dropin] [--fp32] [--max-steps N] [--quiet]\n");
+                 "                 [--engine auto|fused|dropin] [--fp32] [--npe N] [--max-steps N] [--quiet]\n");
 }
 
 int main(int argc, char** argv)
@@ -28,7 +28,7 @@ int main(int argc, char** argv)
     std::string input = "./input", out = "", ic = "turbulence", engine = "auto";
     unsigned long long seed = 42;
     double rms = 0.1, amp = 1e-4;
-    int kmax = 16, fp32 = 0, quiet = 0;
+    int kmax = 16, fp32 = 0, quiet = 0, npe = 1;
     long long max_steps = 0;
     for (int k = 1; k < argc; k++) {
         std::string a = argv[k];
@@ -48,6 +48,7 @@ int main(int argc, char** argv)
         else if (a == "--amp") amp = std::atof(next());
         else if (a == "--engine") engine = next();
         else if (a == "--fp32") fp32 = 1;
+        else if (a == "--npe") npe = std::atoi(next());  // results of an npe-rank run
         else if (a == "--max-steps") max_steps = std::atoll(next());
         else if (a == "--quiet") quiet = 1;
         else {
@@ -91,6 +92,7 @@ int main(int argc, char** argv)
     o.mode = engine == "fused" ? MHD2D_DRIVE_FUSED
                                : (engine == "dropin" ? MHD2D_DRIVE_DROPIN : MHD2D_DRIVE_AUTO);
     o.precision = fp32 ? MHD2D_FP32 : MHD2D_FP64;
+    o.xblocks = npe;
     o.verbose = !quiet;
     o.max_steps = max_steps;
     mhd2d_drive_report rep;

@@ -40,10 +40,18 @@ void GridTables::factor(int n, double alpha, double alphaSub, double* l, double*
     q[n - 1] = 0.0;
 }
 
+void GridTables::block_range(int n, int nb, int b, int* off, int* size)
+{
+    const int base = n / nb, rem = n % nb;
+    *off = b * base + (b < rem ? b : rem);
+    *size = base + (b < rem ? 1 : 0);
+}
+
 hipError_t GridTables::build(int nx_, int ny_, double Lx_, double Ly_, int elem_bytes_,
-                             const SchemeAlphas& al)
+                             const SchemeAlphas& al, int xblocks_)
 {
     release();
+    xblocks = xblocks_ < 1 ? 1 : xblocks_;
     nx = nx_;
     ny = ny_;
     Lx = Lx_;
@@ -83,7 +91,8 @@ hipError_t GridTables::build(int nx_, int ny_, double Lx_, double Ly_, int elem_
     const size_t nxs = (size_t)nx, gs = (size_t)G;
     const size_t x_doubles = NXKIND * (3 * nxs + 2 * gs);
     const size_t y_doubles = 4 * (size_t)N + 2 * (size_t)N;  // md, mdd, mf, mfd + tw
-    std::vector<double> h(x_doubles + y_doubles);
+    const size_t m_words = xblocks > 1 ? 2 * gs : 0;          // block-edge masks
+    std::vector<double> h(x_doubles + y_doubles + m_words);
     double* p = h.data();
     const double alph[NXKIND][2] = {{al.alpha1, al.alphaS1}, {al.alpha2, al.alphaS2},
                                     {al.alphaF, al.alphaSF}};
@@ -95,7 +104,15 @@ hipError_t GridTables::build(int nx_, int ny_, double Lx_, double Ly_, int elem_
         off_q[k] = o;  o += nxs;
         off_pf[k] = o; o += gs;
         off_qb[k] = o; o += gs;
-        factor(nx, alph[k][0], alph[k][1], p + off_l[k], p + off_id[k], p + off_q[k]);
+        // one factorisation per block: l = 0 on a block's first row and q = 0
+        // on its last, so both recurrences (and the chunk products) restart
+        // at every block edge by themselves
+        for (int b = 0; b < xblocks; b++) {
+            int bo, bn;
+            block_range(nx, xblocks, b, &bo, &bn);
+            factor(bn, alph[k][0], alph[k][1], p + off_l[k] + bo, p + off_id[k] + bo,
+                   p + off_q[k] + bo);
+        }
         for (int g = 0; g < G; g++) {
             double pf = 1.0, qb = 1.0;
             for (int i = g * C; i < (g + 1) * C; i++) {
@@ -128,6 +145,18 @@ hipError_t GridTables::build(int nx_, int ny_, double Lx_, double Ly_, int elem_
         p[off_tw + 2 * k] = (double)cosl(ang);
         p[off_tw + 2 * k + 1] = (double)(-sinl(ang));
     }
+    const size_t off_bs = off_tw + 2 * (size_t)N, off_be = off_bs + gs;
+    if (xblocks > 1) {
+        std::vector<unsigned long long> bs(G, 0ull), be(G, 0ull);
+        for (int b = 0; b < xblocks; b++) {
+            int bo, bn;
+            block_range(nx, xblocks, b, &bo, &bn);
+            bs[bo / C] |= 1ull << (bo % C);
+            be[(bo + bn - 1) / C] |= 1ull << ((bo + bn - 1) % C);
+        }
+        std::memcpy(p + off_bs, bs.data(), gs * sizeof(double));
+        std::memcpy(p + off_be, be.data(), gs * sizeof(double));
+    }
 
     hipError_t e = hipMalloc(&blob_, h.size() * sizeof(double));
     if (e != hipSuccess) return e;
@@ -144,8 +173,15 @@ hipError_t GridTables::build(int nx_, int ny_, double Lx_, double Ly_, int elem_
         int kc = nx - 2;
         while (kc > 0 && L[kc - 1] == L[nx - 3] && ID[kc - 1] == ID[nx - 3] && Q[kc - 1] == Q[nx - 3])
             kc--;
+        if (xblocks > 1) kc = nx;  // several blocks: no single fixed-point range
+        const unsigned long long* bsd = nullptr;
+        const unsigned long long* bed = nullptr;
+        if (xblocks > 1) {
+            bsd = (const unsigned long long*)(d + off_bs);
+            bed = (const unsigned long long*)(d + off_be);
+        }
         xt[k] = XTables{d + off_l[k], d + off_id[k], d + off_q[k], d + off_pf[k], d + off_qb[k],
-                        kc, L[nx - 3], ID[nx - 3], Q[nx - 3]};
+                        kc, L[nx - 3], ID[nx - 3], Q[nx - 3], bsd, bed};
         kconv[k] = kc;
     }
     yt.md = d + off_md;

@@ -21,14 +21,17 @@ public:
     GridTables(const GridTables&) = delete;
     GridTables& operator=(const GridTables&) = delete;
 
-    // nx, ny powers of two; dx = Lx/nx; elem_bytes selects the x-solve chunking
+    // nx, ny powers of two; dx = Lx/nx; elem_bytes selects the x-solve
+    // chunking.  xblocks > 1 splits x into that many blocks sized as the
+    // reference's decomposeGrid1D (initialize.c:547-569), each with its own
+    // end rows and Thomas factors, as a run on xblocks MPI ranks has them.
     hipError_t build(int nx, int ny, double Lx, double Ly, int elem_bytes,
-                     const SchemeAlphas& al = SchemeAlphas());
+                     const SchemeAlphas& al = SchemeAlphas(), int xblocks = 1);
     void release();
 
     int nx = 0, ny = 0, elem_bytes = 0;
     double Lx = 0, Ly = 0, dx = 0, dy = 0;
-    int chunk = 0, width = 0, groups = 0;
+    int chunk = 0, width = 0, groups = 0, xblocks = 1;
     XStencil st[NXKIND];
     XTables xt[NXKIND];
     int kconv[NXKIND] = {0, 0, 0};
@@ -37,6 +40,8 @@ public:
     // host copies (FP64) for inspection and tests
     static void factor(int n, double alpha, double alphaSub, double* l, double* invd,
                        double* q);
+    // offset and size of block b of n rows split over nb (initialize.c:547-569)
+    static void block_range(int n, int nb, int b, int* off, int* size);
 
 private:
     void* blob_ = nullptr;  // one allocation holding every table

@@ -218,29 +218,72 @@ void orc_line_filter(int N, const double *A, const double *H0,
 /* ------------------------------------------------------------------------ */
 typedef enum { OP_D1, OP_D2, OP_F } line_op;
 
-static void x_apply(line_op op, int nx, int ny, double dx, int nv,
-                    const double *in, double *out)
+/* decomposeGrid1D, initialize.c:547-569: block sizes n/npe, the first
+ * n%npe blocks one larger */
+static void block_range(int n, int npe, int b, int *off, int *size)
+{
+    int base = n / npe, rem = n % npe, o = 0;
+    for (int k = 0; k < b; k++) o += base + (k < rem ? 1 : 0);
+    *off = o;
+    *size = base + (b < rem ? 1 : 0);
+}
+
+/* x operator with the x range split into npe blocks as the reference's MPI
+ * ranks do: each block is solved on its own with a 2-point halo from its
+ * periodic neighbours (parallel.c:13-24), so rows at block edges are the
+ * explicit closure rows */
+static void x_apply_npe(line_op op, int nx, int ny, double dx, int nv, int npe,
+                        const double *in, double *out)
 {
     orc_coefs c;
     orc_coefs_default(&c);
-    double *line = (double *)malloc(sizeof(double) * nx);
+    /* whole column first: halos are the neighbours' unfiltered values, and
+     * `in` may alias `out` */
+    double *col = (double *)malloc(sizeof(double) * nx);
     double *sol = (double *)malloc(sizeof(double) * nx);
     double H0[2], H1[2];
     for (int v = 0; v < nv; v++) {
         const double *f = in + (size_t)v * nx * ny;
         double *g = out + (size_t)v * nx * ny;
         for (int j = 0; j < ny; j++) {
-            for (int i = 0; i < nx; i++) line[i] = f[(size_t)i * ny + j];
-            H0[0] = line[nx - 2]; H0[1] = line[nx - 1];
-            H1[0] = line[0];      H1[1] = line[1];
-            if (op == OP_D1) orc_line_deriv1_c(&c, nx, dx, line, H0, H1, sol);
-            else if (op == OP_D2) line_deriv2_c(&c, nx, dx, line, H0, H1, sol);
-            else orc_line_filter_c(&c, nx, line, H0, H1, sol);
+            for (int i = 0; i < nx; i++) col[i] = f[(size_t)i * ny + j];
+            for (int b = 0; b < npe; b++) {
+                int off, N;
+                block_range(nx, npe, b, &off, &N);
+                for (int h = 0; h < 2; h++) {
+                    H0[h] = col[(off - 2 + h + nx) % nx];
+                    H1[h] = col[(off + N + h) % nx];
+                }
+                if (op == OP_D1) orc_line_deriv1_c(&c, N, dx, col + off, H0, H1, sol + off);
+                else if (op == OP_D2) line_deriv2_c(&c, N, dx, col + off, H0, H1, sol + off);
+                else orc_line_filter_c(&c, N, col + off, H0, H1, sol + off);
+            }
             for (int i = 0; i < nx; i++) g[(size_t)i * ny + j] = sol[i];
         }
     }
-    free(line);
+    free(col);
     free(sol);
+}
+
+static void x_apply(line_op op, int nx, int ny, double dx, int nv,
+                    const double *in, double *out)
+{
+    x_apply_npe(op, nx, ny, dx, nv, 1, in, out);
+}
+
+void orc_derivX_npe(int nx, int ny, double dx, int nv, int npe, const double *in, double *out)
+{
+    x_apply_npe(OP_D1, nx, ny, dx, nv, npe, in, out);
+}
+
+void orc_derivXX_npe(int nx, int ny, double dx, int nv, int npe, const double *in, double *out)
+{
+    x_apply_npe(OP_D2, nx, ny, dx, nv, npe, in, out);
+}
+
+void orc_filterX_npe(int nx, int ny, int nv, int npe, double *inout)
+{
+    x_apply_npe(OP_F, nx, ny, 0.0, nv, npe, inout, inout);
 }
 
 void orc_derivX(int nx, int ny, double dx, int nv, const double *in, double *out)
@@ -555,7 +598,7 @@ double orc_energy(int nx, int ny, double dx, double dy, double gamma,
 /* run driver: the stage order of mhd.c:154-206                               */
 /* ------------------------------------------------------------------------ */
 struct orc_sim {
-    int nx, ny, nvar, hall, divbclean;
+    int nx, ny, nvar, hall, divbclean, npe;
     double Lx, Ly, dx, dy, gamma, cfl, di, resistivity, viscosity, constDiffDivB;
     double dt, time;
     long steps;
@@ -570,7 +613,7 @@ orc_sim *orc_sim_create(int nx, int ny, double Lx, double Ly, double gamma,
 {
     orc_sim *s = (orc_sim *)calloc(1, sizeof(orc_sim));
     size_t n = (size_t)nx * ny;
-    s->nx = nx; s->ny = ny; s->nvar = 8;
+    s->nx = nx; s->ny = ny; s->nvar = 8; s->npe = 1;
     s->Lx = Lx; s->Ly = Ly;
     s->dx = Lx / nx;             /* periodic x, initialize.c:125-128 */
     s->dy = Ly / ny;             /* initialize.c:135 */
@@ -648,32 +691,32 @@ void orc_sim_stage(orc_sim *s, int iRK)
 {
     int nx = s->nx, ny = s->ny;
     size_t n = (size_t)nx * ny;
-    orc_derivX(nx, ny, s->dx, 8, s->uu, s->dudx);            /* mhd.c:160 */
+    orc_derivX_npe(nx, ny, s->dx, 8, s->npe, s->uu, s->dudx);            /* mhd.c:160 */
     orc_derivY(nx, ny, s->Ly, 8, s->uu, s->dudy);            /* mhd.c:161 */
     if (s->viscosity > 0) {                                  /* mhd.c:163-167 */
-        orc_derivXX(nx, ny, s->dx, 3, s->uu + UX * n, s->d2udx2);
+        orc_derivXX_npe(nx, ny, s->dx, 3, s->npe, s->uu + UX * n, s->d2udx2);
         orc_derivYY(nx, ny, s->Ly, 3, s->uu + UX * n, s->d2udy2);
     }
     /* mhd.c:169-173 also runs these when Hall==1, but their output is read
      * only when resistivity > 0 (calcRHS.c:517), so they are skipped here. */
     if (s->resistivity > 0) {
-        orc_derivXX(nx, ny, s->dx, 3, s->uu + BX * n, s->d2Bdx2);
+        orc_derivXX_npe(nx, ny, s->dx, 3, s->npe, s->uu + BX * n, s->d2Bdx2);
         orc_derivYY(nx, ny, s->Ly, 3, s->uu + BX * n, s->d2Bdy2);
     }
     if (s->divbclean == 1) {                                 /* mhd.c:175-182 */
         orc_divb(nx, ny, s->dudx, s->dudy, s->divB);
-        orc_derivX(nx, ny, s->dx, 1, s->divB, s->ddivBdx);
+        orc_derivX_npe(nx, ny, s->dx, 1, s->npe, s->divB, s->ddivBdx);
         orc_derivY(nx, ny, s->Ly, 1, s->divB, s->ddivBdy);
     }
     orc_efield(nx, ny, s->uu, s->E);                         /* mhd.c:186 */
-    orc_derivX(nx, ny, s->dx, 3, s->E, s->dEdx);             /* mhd.c:188 */
+    orc_derivX_npe(nx, ny, s->dx, 3, s->npe, s->E, s->dEdx);             /* mhd.c:188 */
     orc_derivY(nx, ny, s->Ly, 3, s->E, s->dEdy);             /* mhd.c:189 */
     orc_rhs(nx, ny, s->gamma, s->resistivity, s->viscosity, s->divbclean,
             s->constDiffDivB, s->uu, s->dudx, s->dudy, s->dEdx, s->dEdy,
             s->d2udx2, s->d2udy2, s->d2Bdx2, s->d2Bdy2, s->ddivBdx,
             s->ddivBdy, s->dudt);                            /* mhd.c:192 */
     orc_rk_execute(8 * n, s->cc[iRK], s->dd[iRK], s->uu, s->dudt, s->dudtRK);
-    orc_filterX(nx, ny, 8, s->uu);                           /* filtering.c:100 */
+    orc_filterX_npe(nx, ny, 8, s->npe, s->uu);                           /* filtering.c:100 */
     orc_filterY(nx, ny, 8, s->uu);                           /* filtering.c:101 */
 }
 
@@ -696,6 +739,9 @@ void orc_sim_run_fixed_dt(orc_sim *s, int nsteps, const double *dts)
 {
     for (int k = 0; k < nsteps; k++) sim_step(s, &dts[k]);
 }
+
+/* emulate the reference run on npe MPI ranks along x (block closure) */
+void orc_sim_set_npe(orc_sim *s, int npe) { s->npe = npe < 1 ? 1 : npe; }
 
 double orc_sim_dt(const orc_sim *s) { return s->dt; }
 double orc_sim_time(const orc_sim *s) { return s->time; }

@@ -63,6 +63,11 @@ void orc_derivY(int nx, int ny, double Ly, int nv, const double *in, double *out
 void orc_derivYY(int nx, int ny, double Ly, int nv, const double *in, double *out);
 void orc_filterX(int nx, int ny, int nv, double *inout);
 void orc_filterY(int nx, int ny, int nv, double *inout);
+/* the same x operators with the x range split into npe blocks like the
+ * reference's MPI ranks (decomposeGrid1D, initialize.c:547-569) */
+void orc_derivX_npe(int nx, int ny, double dx, int nv, int npe, const double *in, double *out);
+void orc_derivXX_npe(int nx, int ny, double dx, int nv, int npe, const double *in, double *out);
+void orc_filterX_npe(int nx, int ny, int nv, int npe, double *inout);
 
 /* ---- pointwise physics (calcRHS.c, timeAdvance.c) ---- */
 void orc_efield(int nx, int ny, const double *uu, double *E);
@@ -94,6 +99,7 @@ void orc_sim_get_derivs(const orc_sim *s, double *dudx, double *dudy);
 void orc_sim_run(orc_sim *s, int nsteps);
 void orc_sim_run_fixed_dt(orc_sim *s, int nsteps, const double *dts);
 void orc_sim_stage(orc_sim *s, int iRK);                /* one RK stage */
+void orc_sim_set_npe(orc_sim *s, int npe);             /* x blocks (MPI ranks) */
 double orc_sim_dt(const orc_sim *s);
 double orc_sim_time(const orc_sim *s);
 long orc_sim_steps(const orc_sim *s);

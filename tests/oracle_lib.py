@@ -46,6 +46,10 @@ def lib():
             "orc_derivYY": (None, [i, i, d, i, p, p]),
             "orc_filterX": (None, [i, i, i, p]),
             "orc_filterY": (None, [i, i, i, p]),
+            "orc_derivX_npe": (None, [i, i, d, i, i, p, p]),
+            "orc_derivXX_npe": (None, [i, i, d, i, i, p, p]),
+            "orc_filterX_npe": (None, [i, i, i, i, p]),
+            "orc_sim_set_npe": (None, [ctypes.c_void_p, i]),
             "orc_efield": (None, [i, i, p, p]),
             "orc_divb": (None, [i, i, p, p, p]),
             "orc_rhs": (None, [i, i, d, d, d, i, d] + [p] * 12),
@@ -120,19 +124,25 @@ def filtY_table(ny, afy=0.45):
 
 
 # ---------------------------------------------------------------- fields
-def derivX(arr, dx):
+def derivX(arr, dx, npe=1):
     arr = _c(arr)
     nv, nx, ny = arr.shape
     out = np.zeros_like(arr)
-    lib().orc_derivX(nx, ny, dx, nv, _ptr(arr), _ptr(out))
+    if npe == 1:
+        lib().orc_derivX(nx, ny, dx, nv, _ptr(arr), _ptr(out))
+    else:
+        lib().orc_derivX_npe(nx, ny, dx, nv, npe, _ptr(arr), _ptr(out))
     return out
 
 
-def derivXX(arr, dx):
+def derivXX(arr, dx, npe=1):
     arr = _c(arr)
     nv, nx, ny = arr.shape
     out = np.zeros_like(arr)
-    lib().orc_derivXX(nx, ny, dx, nv, _ptr(arr), _ptr(out))
+    if npe == 1:
+        lib().orc_derivXX(nx, ny, dx, nv, _ptr(arr), _ptr(out))
+    else:
+        lib().orc_derivXX_npe(nx, ny, dx, nv, npe, _ptr(arr), _ptr(out))
     return out
 
 
@@ -152,10 +162,13 @@ def derivYY(arr, Ly):
     return out
 
 
-def filterX(arr):
+def filterX(arr, npe=1):
     out = _c(arr).copy()
     nv, nx, ny = out.shape
-    lib().orc_filterX(nx, ny, nv, _ptr(out))
+    if npe == 1:
+        lib().orc_filterX(nx, ny, nv, _ptr(out))
+    else:
+        lib().orc_filterX_npe(nx, ny, nv, npe, _ptr(out))
     return out
 
 
@@ -225,10 +238,11 @@ class Sim:
 
     def __init__(self, nx, ny, Lx=2 * np.pi, Ly=2 * np.pi, gamma=5.0 / 3.0,
                  cfl=0.3, hall=0, di=0.0, resistivity=0.0, viscosity=0.0,
-                 divbclean=0):
+                 divbclean=0, npe=1):
         self.nx, self.ny = nx, ny
         self.h = lib().orc_sim_create(nx, ny, Lx, Ly, gamma, cfl, hall, di,
                                       resistivity, viscosity, divbclean)
+        lib().orc_sim_set_npe(self.h, npe)
 
     def __del__(self):
         if getattr(self, "h", None):

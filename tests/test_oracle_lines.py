@@ -173,8 +173,16 @@ def _circulant_rows(N, stencil):
     return S
 
 
-def _dense_operator(kind, N, h):
-    """Assemble the periodic-x operator of the reference as M^-1 S."""
+def _dense_operator(kind, N, h, sizes=None):
+    """Assemble the periodic-x operator of the reference as M^-1 S.
+
+    `sizes` splits the line into consecutive blocks, one per MPI rank of the
+    reference (decomposeGrid1D, initialize.c:547-569): M becomes block
+    diagonal with the end-row couplings inside each block, and the end-row
+    stencils sit at every block edge while still reading the global periodic
+    neighbours (the halo exchange of parallel.c)."""
+    sizes = [N] if sizes is None else list(sizes)
+    assert sum(sizes) == N
     if kind == "d1":
         L = LELE_D1
         inner = {1: L["a"] / 2 / h, -1: -L["a"] / 2 / h, 2: L["b"] / 4 / h,
@@ -204,17 +212,20 @@ def _dense_operator(kind, N, h):
         a_in, a_end_lo, a_end_hi = G["alpha"], 0.95, 0.95
     S = _circulant_rows(N, inner)
     Send = _circulant_rows(N, end)
-    S[0] = Send[0]
-    if kind == "f":
-        # the last row mirrors the first: x_{N-1} + 0.95 x_{N-2}
-        Srev = _circulant_rows(N, {-k: c for k, c in end.items()})
-        S[N - 1] = Srev[N - 1]
-    else:
-        # derivative end rows are centred stencils: the same at both ends
-        S[N - 1] = Send[N - 1]
-    M = np.eye(N) + a_in * (np.eye(N, k=1) + np.eye(N, k=-1))
-    M[0, 1] = a_end_lo
-    M[N - 1, N - 2] = a_end_hi
+    # the last row mirrors the first (x_{N-1} + 0.95 x_{N-2} for the filter);
+    # derivative end rows are centred stencils, the same at both ends
+    Slast = _circulant_rows(N, {-k: c for k, c in end.items()}) if kind == "f" else Send
+    M = np.eye(N)
+    off = 0
+    for n in sizes:
+        a, b = off, off + n - 1
+        S[a] = Send[a]
+        S[b] = Slast[b]
+        for i in range(a, b):
+            M[i, i + 1] = M[i + 1, i] = a_in
+        M[a, a + 1] = a_end_lo
+        M[b, b - 1] = a_end_hi
+        off += n
     return np.linalg.solve(M, S)
 
 
@@ -228,3 +239,28 @@ def test_periodic_line_vs_dense_assembly(kind, N):
     D = _dense_operator(kind, N, h)
     scale = np.abs(D @ f).max()
     np.testing.assert_allclose(X, D @ f, rtol=0, atol=1e-13 * max(scale, 1.0))
+
+
+@pytest.mark.parametrize("kind", ["d1", "d2", "f"])
+@pytest.mark.parametrize("nx,npe", [(48, 2), (64, 4), (30, 4), (64, 8)])
+def test_block_closure_vs_dense_assembly(kind, nx, npe):
+    """x operators split over npe MPI ranks: one block per rank, the first
+    nx % npe ranks one row larger (initialize.c:547-569)."""
+    h = 2 * np.pi / nx
+    sizes = [nx // npe + (1 if r < nx % npe else 0) for r in range(npe)]
+    D = _dense_operator(kind, nx, h, sizes)
+    rng = np.random.default_rng(nx * 10 + npe)
+    arr = rng.standard_normal((2, nx, 3))
+    if kind == "d1":
+        out = orc.derivX(arr, h, npe=npe)
+    elif kind == "d2":
+        out = orc.derivXX(arr, h, npe=npe)
+    else:
+        out = orc.filterX(arr, npe=npe)
+    want = np.einsum("ik,vkj->vij", D, arr)
+    scale = np.abs(want).max()
+    np.testing.assert_allclose(out, want, rtol=0, atol=1e-13 * max(scale, 1.0))
+    # the closure is visible: the single-block operator differs at block edges
+    one = {"d1": orc.derivX, "d2": orc.derivXX}.get(kind)
+    single = one(arr, h) if one else orc.filterX(arr)
+    assert np.abs(single - out).max() > 1e-6 * scale
