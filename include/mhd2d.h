@@ -50,11 +50,17 @@ void derivYY(int nv, real *arrMain, real *arrDeriv, int iVarStartMain);
 /* replaces filtering.c:98-102 (filtering.h:6): compact x filter then
  * spectral y filter of all NVAR fields of uu, in place */
 void filtering(void);
-/* replace calcRHS.c:46-75, :13-22, :24-43 (calcRHS.h:4-6) */
+/* replace calcRHS.c:46-75, :13-22, :24-43 (calcRHS.h:4-6).  calcRHS covers
+ * the periodic branch with viscosity, resistivity and both div B cleanings:
+ * diffusive (divBcleaning 1) and hyperbolic GLM (2, NVAR = 9 with psi last,
+ * calcRHS.c:144-158, using the Ch / Cp the last calcDt set) */
 void calcRHS(void);
 void calcDivB(void);
 void calcEfield(void);
-/* replace timeAdvance.c:20-27, :29-36, :38-153 (timeAdvance.h:6-8) */
+/* replace timeAdvance.c:20-27, :29-36, :38-153 (timeAdvance.h:6-8).  With
+ * divBcleaning == 2, calcDt also sets Ch_divBcleaning to the fastest speed
+ * of the loop's last cell (nx-1, ny-1) and Cp via calc_Cp_fix_Cr, as the
+ * reference's loop leaves them (timeAdvance.c:107-112) */
 void rkInit(real dt);
 void rkExecute(int iRK);
 real calcDt(real dt0);
@@ -224,7 +230,8 @@ typedef struct mhd2d_drive_report {
  * log every dtOut/10, the dt < 1e-8 stop, final output after Tmax.  Writes
  * xgrid/ygrid are the caller's (mhd2d_write_grid).  Single thread; the
  * DROPIN engine uses the library's own storage of the reference globals
- * (libmhd2d.so only). */
+ * (libmhd2d.so only).  uu0 holds the 8 physical fields; with divbclean == 2
+ * psi starts at 0 and the output files hold NVAR = 9 fields. */
 int mhd2d_drive(const mhd2d_input *in, const double *uu0,
                 const mhd2d_drive_opts *o, mhd2d_drive_report *rep);
 

@@ -29,6 +29,7 @@ struct Out {
     const mhd2d_input* in;
     const mhd2d_drive_opts* o;
     int iout = 0;
+    int nvar = 8;   // fields per output file (NVAR: 9 with hyperbolic cleaning)
 };
 
 int emit(Out& w, double t, const double* uu, const double* divb, double maxdivb, double dt)
@@ -36,7 +37,7 @@ int emit(Out& w, double t, const double* uu, const double* divb, double maxdivb,
     if (w.o->verbose)
         std::printf("  Output at time = %10.4f, max(div B) = %12.4e, dt = %12.4e\n", t, maxdivb,
                     dt);
-    int rc = mhd2d_write_array(w.o->outdir, w.iout, t, uu, 8, w.in->nx, w.in->ny);
+    int rc = mhd2d_write_array(w.o->outdir, w.iout, t, uu, w.nvar, w.in->nx, w.in->ny);
     if (!rc && w.in->outdivb == 1 && divb)
         rc = mhd2d_write_divb(w.o->outdir, w.iout, t, divb, w.in->nx, w.in->ny);
     w.iout++;
@@ -153,8 +154,8 @@ int drive_dropin(const mhd2d_input* in, const double* uu0, const mhd2d_drive_opt
     if (rc) return rc;
     mhd2d_dropin_set_sync(MHD2D_SYNC_STRICT);
     const size_t n = (size_t)in->nx * in->ny;
-    std::memcpy(uu, uu0, 8 * n * sizeof(double));   // usrInitialize()
-    Out w{in, o, 0};
+    std::memcpy(uu, uu0, 8 * n * sizeof(double));   // usrInitialize(); psi starts at 0
+    Out w{in, o, 0, NVAR};
     const double t0 = now_s();
     double dt = calcDt(0.0);                          // mhd.c:58
     rkInit(dt);                                       // mhd.c:63
@@ -259,10 +260,6 @@ extern "C" int mhd2d_drive(const mhd2d_input* in, const double* uu0, const mhd2d
             return MHD2D_EINVAL;
         }
         return drive_fused(in, uu0, o, rep);
-    }
-    if (in->divbclean == 2) {
-        std::fprintf(stderr, "mhd2d_drive: divBcleaning == 2 is not supported\n");
-        return MHD2D_EINVAL;
     }
     return drive_dropin(in, uu0, o, rep);
 }

@@ -236,11 +236,13 @@ void calcDivB(void)
     CHK(launch_divb<double>(dx_, dy_, d, plane(), g_stream));
 }
 
-// calcRHS.c:46-75 for the periodic branch, with addSourceTerm (:498-544)
+// calcRHS.c:46-75 for the periodic branch, with addSourceTerm (:498-544) and,
+// for divBcleaning == 2, the hyperbolic cleaning terms with the Ch / Cp the
+// last calcDt left in the globals (calcRHS.c:144-158)
 void calcRHS(void)
 {
     setup();
-    if (divBcleaning == 2) DIE("hyperbolic div B cleaning (divBcleaning == 2) is not supported");
+    if (divBcleaning == 2 && NVAR != 9) DIE("divBcleaning == 2 needs NVAR == 9 (psi)");
     RhsArgs<double> a;
     a.u = dev_in(uu, (size_t)NVAR);
     a.dudx = dev_in(dudx, (size_t)NVAR);
@@ -258,6 +260,8 @@ void calcRHS(void)
     a.resistivity = resistivity;
     a.viscosity = viscosity;
     a.constDiffDivB = constDiffDivB;
+    a.ch = Ch_divBcleaning;
+    a.cp = Cp_divBcleaning;
     a.divbclean = divBcleaning;
     a.n = plane();
     CHK(launch_rhs<double>(a, g_stream));
@@ -288,11 +292,11 @@ void rkExecute(int iRK)
     CHK(launch_rk<double>(du, dt_, drk, g_cc[iRK], g_dd[iRK], arrSize, g_stream));
 }
 
-// timeAdvance.c:38-153 (without the divBcleaning == 2 branch)
+// timeAdvance.c:38-153; with divBcleaning == 2 it also sets Ch / Cp
+// (timeAdvance.c:107-112) from the last cell of the loop
 real calcDt(real dt0)
 {
     setup();
-    if (divBcleaning == 2) DIE("hyperbolic div B cleaning (divBcleaning == 2) is not supported");
     const double* du = dev_in(uu, (size_t)NVAR);
     DtParams dp;
     dp.dx = dx;
@@ -305,6 +309,7 @@ real calcDt(real dt0)
     dp.hall = Hall;
     CHK(launch_dt_reset(g_rs, g_stream));
     CHK(launch_dt_limits<double>(du, dp, g_rs, (int)nx, (int)ny, g_stream));
+    if (divBcleaning == 2) CHK(launch_glm_ch<double>(du, dp, g_rs, plane(), g_stream));
     RunScalars r;
     CHK(hipMemcpyAsync(&r, g_rs, sizeof(r), hipMemcpyDeviceToHost, g_stream));
     CHK(hipStreamSynchronize(g_stream));
@@ -312,6 +317,10 @@ real calcDt(real dt0)
     std::memcpy(&dtmin, &r.dtmin_bits, sizeof(double));
     dtmin *= cfl;
     const double dt = (dt0 < 0.98 * dtmin || dt0 > 1.02 * dtmin) ? dtmin : dt0;
+    if (divBcleaning == 2) {
+        Ch_divBcleaning = r.glm_ch;
+        calc_Cp_fix_Cr();
+    }
     if (g_sync == MHD2D_SYNC_STRICT) sync_all();
     return dt;
 }

@@ -101,6 +101,7 @@ struct RunScalars {
     double cc[3], dd[3];             // RK3 coefficients times dt
     long long step;
     double dt_stop;                  // the dt that triggered the stop
+    double glm_ch;                   // fastest speed of the last cell (GLM Ch)
 };
 
 template <typename T>
@@ -133,6 +134,7 @@ struct RhsArgs {
     const T* ddivBdy;
     T* dudt;
     double gamma, resistivity, viscosity, constDiffDivB;
+    double ch, cp;    // hyperbolic cleaning (divbclean == 2): field 8 is psi
     int divbclean;
     size_t n;
 };
@@ -174,6 +176,9 @@ template <typename T>
 hipError_t launch_dt_limits(const T* u, const DtParams& dp, RunScalars* rs, int nx,
                             int ny, hipStream_t s);
 hipError_t launch_dt_finalize(RunScalars* rs, double cfl, int first, hipStream_t s);
+template <typename T>
+hipError_t launch_glm_ch(const T* u, const DtParams& dp, RunScalars* rs, size_t n,
+                         hipStream_t s);
 hipError_t launch_dt_reset(RunScalars* rs, hipStream_t s);
 template <typename T>
 hipError_t launch_fill(T* p, T v, size_t n, hipStream_t s);

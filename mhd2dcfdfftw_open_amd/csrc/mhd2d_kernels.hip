@@ -22,9 +22,11 @@ namespace mhd2d {
 
 // ============================================================== dt limits
 // per-cell CFL limit of calcDt, timeAdvance.c:50-136 (divBcleaning==2 excluded)
-__device__ __forceinline__ double cell_dt(double rho, double ux, double uy, double bx,
-                                          double by, double bz, double p,
-                                          const DtParams& dp)
+// fastest signal speeds along x and y of one cell (timeAdvance.c:49-105)
+__device__ __forceinline__ void cell_speeds(double rho, double ux, double uy, double bx,
+                                            double by, double bz, double p,
+                                            const DtParams& dp, double& cmx_out,
+                                            double& cmy_out)
 {
     const double cs2 = dp.gamma * p / rho;
     const double sr = sqrt(rho);
@@ -45,6 +47,16 @@ __device__ __forceinline__ double cell_dt(double rho, double ux, double uy, doub
         cmx = fmax(cmx, ch);
         cmy = fmax(cmy, ch);
     }
+    cmx_out = cmx;
+    cmy_out = cmy;
+}
+
+__device__ __forceinline__ double cell_dt(double rho, double ux, double uy, double bx,
+                                          double by, double bz, double p,
+                                          const DtParams& dp)
+{
+    double cmx, cmy;
+    cell_speeds(rho, ux, uy, bx, by, bz, p, dp, cmx, cmy);
     double dtx = dp.dx / cmx, dty = dp.dy / cmy;
     if (dp.resistivity > 0) {
         dtx = fmin(dtx, dp.dx * dp.dx / dp.resistivity);
@@ -649,8 +661,19 @@ __global__ void k_rhs(RhsArgs<T> a)
     const size_t n = a.n;
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
          p += (size_t)gridDim.x * blockDim.x) {
-        T R[8], rho;
+        T R[8], rho, rpsi = 0;
         rhs_cell<T>(a.u, a.dudx, a.dudy, a.dEdx, a.dEdy, n, p, (T)a.gamma, R, rho);
+        if (a.divbclean == 2) {
+            // hyperbolic cleaning, Dedner et al. 2002 eq. 24 (calcRHS.c:144-158)
+            const T divB = a.dudx[BX * n + p] + a.dudy[BY * n + p];
+            const T ch = (T)a.ch, cp = (T)a.cp;
+            R[UX] = R[UX] - divB * a.u[BX * n + p];
+            R[UY] = R[UY] - divB * a.u[BY * n + p];
+            R[UZ] = R[UZ] - divB * a.u[BZ * n + p];
+            R[BX] = R[BX] - a.dudx[8 * n + p];
+            R[BY] = R[BY] - a.dudy[8 * n + p];
+            rpsi = -ch * ch * (divB + a.u[8 * n + p] / cp / cp);
+        }
         // addSourceTerm, calcRHS.c:498-544
         if (a.viscosity > 0) {
             for (int c = 0; c < 3; c++)
@@ -666,7 +689,21 @@ __global__ void k_rhs(RhsArgs<T> a)
         }
 #pragma unroll
         for (int f = 0; f < 8; f++) a.dudt[f * n + p] = R[f];
+        if (a.divbclean == 2) a.dudt[8 * n + p] = rpsi;
     }
+}
+
+// Ch of calcDt's divBcleaning == 2 branch (timeAdvance.c:107-112): the value
+// the cell loop leaves, i.e. max(cmaxx, cmaxy) of the last cell (n - 1)
+template <typename T>
+__global__ void k_glm_ch(const T* __restrict__ u, DtParams dp, RunScalars* rs, size_t n)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const size_t p = n - 1;
+    double cmx, cmy;
+    cell_speeds(u[RHO * n + p], u[UX * n + p], u[UY * n + p], u[BX * n + p], u[BY * n + p],
+                u[BZ * n + p], u[PR * n + p], dp, cmx, cmy);
+    rs->glm_ch = fmax(cmx, cmy);
 }
 
 // 16-byte vector of consecutive cells (2 in FP64, 4 in FP32)
@@ -925,6 +962,16 @@ hipError_t launch_dt_limits(const T* u, const DtParams& dp, RunScalars* rs, int 
     hipLaunchKernelGGL((k_dt_limits<T>), dim3((unsigned)b), dim3(256), 0, s, u, dp, rs, n);
     return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_glm_ch(const T* u, const DtParams& dp, RunScalars* rs, size_t n,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL((k_glm_ch<T>), dim3(1), dim3(64), 0, s, u, dp, rs, n);
+    return hipGetLastError();
+}
+template hipError_t launch_glm_ch<double>(const double*, const DtParams&, RunScalars*, size_t,
+                                          hipStream_t);
 
 hipError_t launch_dt_finalize(RunScalars* rs, double cfl, int first, hipStream_t s)
 {

@@ -18,8 +18,8 @@
 
 #define ORC_PI 3.14159265358979323846264338327950288
 
-/* field indices, macros.h:4-11 */
-enum { RHO = 0, UX, UY, UZ, BX, BY, BZ, PR };
+/* field indices, macros.h:4-12 (PSI only with divBcleaning == 2) */
+enum { RHO = 0, UX, UY, UZ, BX, BY, BZ, PR, PSI };
 
 /* ------------------------------------------------------------------------ */
 /* coefficients: initialize.c:572-631 with the alphas of parameter.h:24-34   */
@@ -454,14 +454,15 @@ void orc_divb(int nx, int ny, const double *dudx, const double *dudy, double *di
     for (size_t p = 0; p < n; p++) divB[p] = dudx[BX * n + p] + dudy[BY * n + p];
 }
 
-/* calcRHSInnerPoint calcRHS.c:77-142 + addSourceTerm calcRHS.c:498-544 */
-void orc_rhs(int nx, int ny, double gamma, double resistivity, double viscosity,
-             int divbclean, double constDiffDivB,
-             const double *uu, const double *dudx, const double *dudy,
-             const double *dEdx, const double *dEdy,
-             const double *d2udx2, const double *d2udy2,
-             const double *d2Bdx2, const double *d2Bdy2,
-             const double *ddivBdx, const double *ddivBdy, double *dudt)
+/* calcRHSInnerPoint calcRHS.c:77-159 + addSourceTerm calcRHS.c:498-544.
+ * ch/cp: per x row, the Ch and Cp of the rank owning that row (GLM only). */
+static void rhs_impl(int nx, int ny, double gamma, double resistivity, double viscosity,
+                     int divbclean, double constDiffDivB, const double *ch,
+                     const double *cp, const double *uu, const double *dudx,
+                     const double *dudy, const double *dEdx, const double *dEdy,
+                     const double *d2udx2, const double *d2udy2,
+                     const double *d2Bdx2, const double *d2Bdy2,
+                     const double *ddivBdx, const double *ddivBdy, double *dudt)
 {
     size_t n = (size_t)nx * ny;
     for (size_t p = 0; p < n; p++) {
@@ -486,6 +487,17 @@ void orc_rhs(int nx, int ny, double gamma, double resistivity, double viscosity,
         dudt[BZ * n + p] = dEdy[0 * n + p] - dEdx[1 * n + p];
         dudt[PR * n + p] = -ux * dudx[PR * n + p] - uy * dudy[PR * n + p] - gamma * pr * divU;
 
+        if (divbclean == 2) {  /* Dedner et al. 2002 eq. 24, calcRHS.c:144-158 */
+            size_t i = p / (size_t)ny;
+            double divB = dudx[BX * n + p] + dudy[BY * n + p];
+            double psi = uu[PSI * n + p];
+            dudt[UX * n + p] = dudt[UX * n + p] - divB * bx;
+            dudt[UY * n + p] = dudt[UY * n + p] - divB * by;
+            dudt[UZ * n + p] = dudt[UZ * n + p] - divB * bz;
+            dudt[BX * n + p] = dudt[BX * n + p] - dudx[PSI * n + p];
+            dudt[BY * n + p] = dudt[BY * n + p] - dudy[PSI * n + p];
+            dudt[PSI * n + p] = -ch[i] * ch[i] * (divB + psi / cp[i] / cp[i]);
+        }
         if (viscosity > 0) {  /* calcRHS.c:501-515 */
             for (int c = 0; c < 3; c++)
                 dudt[(UX + c) * n + p] += viscosity / rho *
@@ -503,6 +515,35 @@ void orc_rhs(int nx, int ny, double gamma, double resistivity, double viscosity,
     }
 }
 
+void orc_rhs(int nx, int ny, double gamma, double resistivity, double viscosity,
+             int divbclean, double constDiffDivB,
+             const double *uu, const double *dudx, const double *dudy,
+             const double *dEdx, const double *dEdy,
+             const double *d2udx2, const double *d2udy2,
+             const double *d2Bdx2, const double *d2Bdy2,
+             const double *ddivBdx, const double *ddivBdy, double *dudt)
+{
+    if (divbclean == 2) divbclean = 0;  /* GLM needs Ch, Cp: orc_rhs_glm */
+    rhs_impl(nx, ny, gamma, resistivity, viscosity, divbclean, constDiffDivB, NULL, NULL,
+             uu, dudx, dudy, dEdx, dEdy, d2udx2, d2udy2, d2Bdx2, d2Bdy2, ddivBdx,
+             ddivBdy, dudt);
+}
+
+void orc_rhs_glm(int nx, int ny, double gamma, double resistivity, double viscosity,
+                 double ch, double cp, const double *uu, const double *dudx,
+                 const double *dudy, const double *dEdx, const double *dEdy,
+                 const double *d2udx2, const double *d2udy2,
+                 const double *d2Bdx2, const double *d2Bdy2, double *dudt)
+{
+    double *chr = (double *)malloc(sizeof(double) * nx);
+    double *cpr = (double *)malloc(sizeof(double) * nx);
+    for (int i = 0; i < nx; i++) { chr[i] = ch; cpr[i] = cp; }
+    rhs_impl(nx, ny, gamma, resistivity, viscosity, 2, 0.0, chr, cpr, uu, dudx, dudy,
+             dEdx, dEdy, d2udx2, d2udy2, d2Bdx2, d2Bdy2, NULL, NULL, dudt);
+    free(chr);
+    free(cpr);
+}
+
 /* rkExecute, timeAdvance.c:29-36 */
 void orc_rk_execute(size_t n, double cc, double dd, double *uu,
                     const double *dudt, double *dudtRK)
@@ -516,7 +557,50 @@ void orc_rk_execute(size_t n, double cc, double dd, double *uu,
 static inline double dmax(double a, double b) { return a > b ? a : b; }
 static inline double dmin(double a, double b) { return a < b ? a : b; }
 
-/* calcDt, timeAdvance.c:38-153 (divBcleaning==2 branch excluded) */
+/* the largest signal speeds along x and y of cell p, timeAdvance.c:49-105 */
+static void cell_speeds(size_t n, size_t p, double dx, double dy, double gamma,
+                        int hall, double di, const double *uu, double *cmx_out,
+                        double *cmy_out)
+{
+    double rho = uu[RHO * n + p];
+    double cs2 = gamma * uu[PR * n + p] / rho;
+    double cax = uu[BX * n + p] / sqrt(rho);
+    double cay = uu[BY * n + p] / sqrt(rho);
+    double caz = uu[BZ * n + p] / sqrt(rho);
+    double ca2 = cax * cax + cay * cay + caz * caz;
+    double cms2 = cs2 + ca2;
+    double nsx = sqrt(dmax(cms2 * cms2 - 4 * cs2 * cax * cax, 0.));
+    double nsy = sqrt(dmax(cms2 * cms2 - 4 * cs2 * cay * cay, 0.));
+    double cfx = sqrt(cms2 + nsx) / sqrt(2.);
+    double cfy = sqrt(cms2 + nsy) / sqrt(2.);
+    double csx = sqrt(dmax(cms2 - nsx, 0.)) / sqrt(2.);
+    double csy = sqrt(dmax(cms2 - nsy, 0.)) / sqrt(2.);
+    double ux = uu[UX * n + p], uy = uu[UY * n + p];
+    double m;
+    m = dmax(fabs(ux), fabs(ux - cax));
+    m = dmax(fabs(ux - csx), m);
+    m = dmax(fabs(ux - cfx), m);
+    m = dmax(fabs(ux + cax), m);
+    m = dmax(fabs(ux + csx), m);
+    double cmx = dmax(fabs(ux + cfx), m);
+    m = dmax(fabs(uy), fabs(uy - cay));
+    m = dmax(fabs(uy - csy), m);
+    m = dmax(fabs(uy - cfy), m);
+    m = dmax(fabs(uy + cay), m);
+    m = dmax(fabs(uy + csy), m);
+    double cmy = dmax(fabs(uy + cfy), m);
+    if (hall == 1) {  /* whistler limit, timeAdvance.c:97-105 */
+        double ch = di / dmin(dx, dy) / rho *
+            dmax(fabs(uu[BX * n + p]),
+                 dmax(fabs(uu[BY * n + p]), fabs(uu[BZ * n + p])));
+        cmx = dmax(cmx, ch);
+        cmy = dmax(cmy, ch);
+    }
+    *cmx_out = cmx;
+    *cmy_out = cmy;
+}
+
+/* calcDt, timeAdvance.c:38-153 (the divBcleaning==2 branch sets Ch: orc_glm_ch) */
 double orc_calc_dt(int nx, int ny, double dx, double dy, double gamma,
                    double cfl, int hall, double di, double resistivity,
                    double viscosity, const double *uu, double dt0)
@@ -524,40 +608,8 @@ double orc_calc_dt(int nx, int ny, double dx, double dy, double gamma,
     size_t n = (size_t)nx * ny;
     double dtmin = -1;
     for (size_t p = 0; p < n; p++) {
-        double rho = uu[RHO * n + p];
-        double cs2 = gamma * uu[PR * n + p] / rho;
-        double cax = uu[BX * n + p] / sqrt(rho);
-        double cay = uu[BY * n + p] / sqrt(rho);
-        double caz = uu[BZ * n + p] / sqrt(rho);
-        double ca2 = cax * cax + cay * cay + caz * caz;
-        double cms2 = cs2 + ca2;
-        double nsx = sqrt(dmax(cms2 * cms2 - 4 * cs2 * cax * cax, 0.));
-        double nsy = sqrt(dmax(cms2 * cms2 - 4 * cs2 * cay * cay, 0.));
-        double cfx = sqrt(cms2 + nsx) / sqrt(2.);
-        double cfy = sqrt(cms2 + nsy) / sqrt(2.);
-        double csx = sqrt(dmax(cms2 - nsx, 0.)) / sqrt(2.);
-        double csy = sqrt(dmax(cms2 - nsy, 0.)) / sqrt(2.);
-        double ux = uu[UX * n + p], uy = uu[UY * n + p];
-        double m;
-        m = dmax(fabs(ux), fabs(ux - cax));
-        m = dmax(fabs(ux - csx), m);
-        m = dmax(fabs(ux - cfx), m);
-        m = dmax(fabs(ux + cax), m);
-        m = dmax(fabs(ux + csx), m);
-        double cmx = dmax(fabs(ux + cfx), m);
-        m = dmax(fabs(uy), fabs(uy - cay));
-        m = dmax(fabs(uy - csy), m);
-        m = dmax(fabs(uy - cfy), m);
-        m = dmax(fabs(uy + cay), m);
-        m = dmax(fabs(uy + csy), m);
-        double cmy = dmax(fabs(uy + cfy), m);
-        if (hall == 1) {  /* whistler limit, timeAdvance.c:97-105 */
-            double ch = di / dmin(dx, dy) / rho *
-                dmax(fabs(uu[BX * n + p]),
-                     dmax(fabs(uu[BY * n + p]), fabs(uu[BZ * n + p])));
-            cmx = dmax(cmx, ch);
-            cmy = dmax(cmy, ch);
-        }
+        double cmx, cmy;
+        cell_speeds(n, p, dx, dy, gamma, hall, di, uu, &cmx, &cmy);
         double dtx = dx / cmx, dty = dy / cmy;
         if (resistivity > 0) {
             dtx = dmin(dtx, dx * dx / resistivity);
@@ -574,6 +626,17 @@ double orc_calc_dt(int nx, int ny, double dx, double dy, double gamma,
     /* hysteresis, timeAdvance.c:144-151 */
     if (dt0 < 0.98 * dtmin || dt0 > 1.02 * dtmin) return dtmin;
     return dt0;
+}
+
+/* Ch of the divBcleaning==2 branch of calcDt (timeAdvance.c:107-112): it is
+ * assigned inside the cell loop, so the value left is that of the last cell
+ * the rank visits, (iSize-1, jSize-1) of its block, not a maximum. */
+double orc_glm_ch(int nx, int ny, double dx, double dy, double gamma, int hall,
+                  double di, const double *uu, int i, int j)
+{
+    double cmx, cmy;
+    cell_speeds((size_t)nx * ny, (size_t)i * ny + j, dx, dy, gamma, hall, di, uu, &cmx, &cmy);
+    return dmax(cmx, cmy);
 }
 
 /* total energy: sum of 1/2 rho u^2 + p/(gamma-1) + 1/2 B^2 times the cell
@@ -605,6 +668,7 @@ struct orc_sim {
     double cc[3], dd[3];
     double *uu, *dudx, *dudy, *E, *dEdx, *dEdy, *dudt, *dudtRK;
     double *d2udx2, *d2udy2, *d2Bdx2, *d2Bdy2, *divB, *ddivBdx, *ddivBdy;
+    double *ch, *cp;   /* per x row: Ch and Cp of the owning rank (GLM) */
 };
 
 orc_sim *orc_sim_create(int nx, int ny, double Lx, double Ly, double gamma,
@@ -613,7 +677,8 @@ orc_sim *orc_sim_create(int nx, int ny, double Lx, double Ly, double gamma,
 {
     orc_sim *s = (orc_sim *)calloc(1, sizeof(orc_sim));
     size_t n = (size_t)nx * ny;
-    s->nx = nx; s->ny = ny; s->nvar = 8; s->npe = 1;
+    s->nx = nx; s->ny = ny; s->npe = 1;
+    s->nvar = divbclean == 2 ? 9 : 8;   /* initialize.c:205-214 */
     s->Lx = Lx; s->Ly = Ly;
     s->dx = Lx / nx;             /* periodic x, initialize.c:125-128 */
     s->dy = Ly / ny;             /* initialize.c:135 */
@@ -622,11 +687,14 @@ orc_sim *orc_sim_create(int nx, int ny, double Lx, double Ly, double gamma,
     s->divbclean = divbclean;
     /* initialize.c:17 and :154 */
     s->constDiffDivB = 2.0 * dmax(s->dx, s->dy) * dmax(s->dx, s->dy);
-    s->uu = (double *)calloc(8 * n, sizeof(double));
-    s->dudx = (double *)calloc(8 * n, sizeof(double));
-    s->dudy = (double *)calloc(8 * n, sizeof(double));
-    s->dudt = (double *)calloc(8 * n, sizeof(double));
-    s->dudtRK = (double *)calloc(8 * n, sizeof(double));
+    size_t nv = (size_t)s->nvar;
+    s->uu = (double *)calloc(nv * n, sizeof(double));
+    s->dudx = (double *)calloc(nv * n, sizeof(double));
+    s->dudy = (double *)calloc(nv * n, sizeof(double));
+    s->dudt = (double *)calloc(nv * n, sizeof(double));
+    s->dudtRK = (double *)calloc(nv * n, sizeof(double));
+    s->ch = (double *)calloc(nx, sizeof(double));
+    s->cp = (double *)calloc(nx, sizeof(double));
     s->E = (double *)calloc(3 * n, sizeof(double));
     s->dEdx = (double *)calloc(3 * n, sizeof(double));
     s->dEdy = (double *)calloc(3 * n, sizeof(double));
@@ -647,6 +715,7 @@ void orc_sim_destroy(orc_sim *s)
     free(s->E); free(s->dEdx); free(s->dEdy);
     free(s->d2udx2); free(s->d2udy2); free(s->d2Bdx2); free(s->d2Bdy2);
     free(s->divB); free(s->ddivBdx); free(s->ddivBdy);
+    free(s->ch); free(s->cp);
     free(s);
 }
 
@@ -656,18 +725,34 @@ static void rk_init(orc_sim *s, double dt)
     s->cc[0] = 8.0 / 15.0 * dt; s->dd[0] = 0.0;
     s->cc[1] = 5.0 / 12.0 * dt; s->dd[1] = -17. / 60. * dt;
     s->cc[2] = 0.75 * dt;       s->dd[2] = -5. / 12. * dt;
-    memset(s->dudtRK, 0, sizeof(double) * 8 * (size_t)s->nx * s->ny);
+    memset(s->dudtRK, 0, sizeof(double) * s->nvar * (size_t)s->nx * s->ny);
+}
+
+/* the Ch / Cp side effect of calcDt (timeAdvance.c:107-112,
+ * divBcleaning.c:7-10), per rank */
+static void sim_glm_update(orc_sim *s)
+{
+    if (s->divbclean != 2) return;
+    for (int b = 0; b < s->npe; b++) {
+        int off, N;
+        block_range(s->nx, s->npe, b, &off, &N);
+        double ch = orc_glm_ch(s->nx, s->ny, s->dx, s->dy, s->gamma, s->hall, s->di,
+                               s->uu, off + N - 1, s->ny - 1);
+        double cp = sqrt(0.18 * ch);   /* Cr_divBCleaning, divBcleaning.c:4 */
+        for (int i = off; i < off + N; i++) { s->ch[i] = ch; s->cp[i] = cp; }
+    }
 }
 
 static double sim_calc_dt(orc_sim *s, double dt0)
 {
+    sim_glm_update(s);
     return orc_calc_dt(s->nx, s->ny, s->dx, s->dy, s->gamma, s->cfl, s->hall,
                        s->di, s->resistivity, s->viscosity, s->uu, dt0);
 }
 
 void orc_sim_set_state(orc_sim *s, const double *uu)
 {
-    memcpy(s->uu, uu, sizeof(double) * 8 * (size_t)s->nx * s->ny);
+    memcpy(s->uu, uu, sizeof(double) * s->nvar * (size_t)s->nx * s->ny);
     s->time = 0.0;
     s->steps = 0;
     s->dt = sim_calc_dt(s, 0.0);   /* mhd.c:58 */
@@ -676,12 +761,12 @@ void orc_sim_set_state(orc_sim *s, const double *uu)
 
 void orc_sim_get_state(const orc_sim *s, double *uu)
 {
-    memcpy(uu, s->uu, sizeof(double) * 8 * (size_t)s->nx * s->ny);
+    memcpy(uu, s->uu, sizeof(double) * s->nvar * (size_t)s->nx * s->ny);
 }
 
 void orc_sim_get_derivs(const orc_sim *s, double *dudx, double *dudy)
 {
-    size_t n = 8 * (size_t)s->nx * s->ny;
+    size_t n = s->nvar * (size_t)s->nx * s->ny;
     if (dudx) memcpy(dudx, s->dudx, sizeof(double) * n);
     if (dudy) memcpy(dudy, s->dudy, sizeof(double) * n);
 }
@@ -691,8 +776,9 @@ void orc_sim_stage(orc_sim *s, int iRK)
 {
     int nx = s->nx, ny = s->ny;
     size_t n = (size_t)nx * ny;
-    orc_derivX_npe(nx, ny, s->dx, 8, s->npe, s->uu, s->dudx);            /* mhd.c:160 */
-    orc_derivY(nx, ny, s->Ly, 8, s->uu, s->dudy);            /* mhd.c:161 */
+    int nv = s->nvar;
+    orc_derivX_npe(nx, ny, s->dx, nv, s->npe, s->uu, s->dudx);           /* mhd.c:160 */
+    orc_derivY(nx, ny, s->Ly, nv, s->uu, s->dudy);           /* mhd.c:161 */
     if (s->viscosity > 0) {                                  /* mhd.c:163-167 */
         orc_derivXX_npe(nx, ny, s->dx, 3, s->npe, s->uu + UX * n, s->d2udx2);
         orc_derivYY(nx, ny, s->Ly, 3, s->uu + UX * n, s->d2udy2);
@@ -711,13 +797,13 @@ void orc_sim_stage(orc_sim *s, int iRK)
     orc_efield(nx, ny, s->uu, s->E);                         /* mhd.c:186 */
     orc_derivX_npe(nx, ny, s->dx, 3, s->npe, s->E, s->dEdx);             /* mhd.c:188 */
     orc_derivY(nx, ny, s->Ly, 3, s->E, s->dEdy);             /* mhd.c:189 */
-    orc_rhs(nx, ny, s->gamma, s->resistivity, s->viscosity, s->divbclean,
-            s->constDiffDivB, s->uu, s->dudx, s->dudy, s->dEdx, s->dEdy,
-            s->d2udx2, s->d2udy2, s->d2Bdx2, s->d2Bdy2, s->ddivBdx,
-            s->ddivBdy, s->dudt);                            /* mhd.c:192 */
-    orc_rk_execute(8 * n, s->cc[iRK], s->dd[iRK], s->uu, s->dudt, s->dudtRK);
-    orc_filterX_npe(nx, ny, 8, s->npe, s->uu);                           /* filtering.c:100 */
-    orc_filterY(nx, ny, 8, s->uu);                           /* filtering.c:101 */
+    rhs_impl(nx, ny, s->gamma, s->resistivity, s->viscosity, s->divbclean,
+             s->constDiffDivB, s->ch, s->cp, s->uu, s->dudx, s->dudy, s->dEdx, s->dEdy,
+             s->d2udx2, s->d2udy2, s->d2Bdx2, s->d2Bdy2, s->ddivBdx,
+             s->ddivBdy, s->dudt);                           /* mhd.c:192 */
+    orc_rk_execute(nv * n, s->cc[iRK], s->dd[iRK], s->uu, s->dudt, s->dudtRK);
+    orc_filterX_npe(nx, ny, nv, s->npe, s->uu);              /* filtering.c:100 */
+    orc_filterY(nx, ny, nv, s->uu);                          /* filtering.c:101 */
 }
 
 static void sim_step(orc_sim *s, const double *forced_dt)
@@ -725,7 +811,12 @@ static void sim_step(orc_sim *s, const double *forced_dt)
     for (int iRK = 0; iRK < 3; iRK++) orc_sim_stage(s, iRK);
     s->time += s->dt;                                        /* mhd.c:201 */
     s->steps++;
-    s->dt = forced_dt ? *forced_dt : sim_calc_dt(s, s->dt);  /* mhd.c:205 */
+    if (forced_dt) {
+        sim_glm_update(s);   /* calcDt's Ch side effect still happens */
+        s->dt = *forced_dt;
+    } else {
+        s->dt = sim_calc_dt(s, s->dt);                       /* mhd.c:205 */
+    }
     rk_init(s, s->dt);                                       /* mhd.c:206 */
 }
 
@@ -742,6 +833,10 @@ void orc_sim_run_fixed_dt(orc_sim *s, int nsteps, const double *dts)
 
 /* emulate the reference run on npe MPI ranks along x (block closure) */
 void orc_sim_set_npe(orc_sim *s, int npe) { s->npe = npe < 1 ? 1 : npe; }
+
+int orc_sim_nvar(const orc_sim *s) { return s->nvar; }
+double orc_sim_glm_ch(const orc_sim *s, int i) { return s->ch[i]; }
+double orc_sim_glm_cp(const orc_sim *s, int i) { return s->cp[i]; }
 
 double orc_sim_dt(const orc_sim *s) { return s->dt; }
 double orc_sim_time(const orc_sim *s) { return s->time; }

@@ -79,6 +79,13 @@ void orc_rhs(int nx, int ny, double gamma, double resistivity, double viscosity,
              const double *d2udx2, const double *d2udy2,
              const double *d2Bdx2, const double *d2Bdy2,
              const double *ddivBdx, const double *ddivBdy, double *dudt);
+/* the same with divBcleaning == 2 (hyperbolic GLM cleaning): uu, dudx, dudy,
+ * dudt hold 9 fields, PSI last; ch, cp as calcDt / calc_Cp_fix_Cr leave them */
+void orc_rhs_glm(int nx, int ny, double gamma, double resistivity, double viscosity,
+                 double ch, double cp, const double *uu, const double *dudx,
+                 const double *dudy, const double *dEdx, const double *dEdy,
+                 const double *d2udx2, const double *d2udy2,
+                 const double *d2Bdx2, const double *d2Bdy2, double *dudt);
 void orc_rk_execute(size_t n, double cc, double dd, double *uu,
                     const double *dudt, double *dudtRK);
 double orc_calc_dt(int nx, int ny, double dx, double dy, double gamma,
@@ -86,6 +93,9 @@ double orc_calc_dt(int nx, int ny, double dx, double dy, double gamma,
                    double viscosity, const double *uu, double dt0);
 double orc_energy(int nx, int ny, double dx, double dy, double gamma,
                   const double *uu);
+/* max(cmaxx, cmaxy) of cell (i, j): the Ch calcDt leaves (timeAdvance.c:107-112) */
+double orc_glm_ch(int nx, int ny, double dx, double dy, double gamma, int hall,
+                  double di, const double *uu, int i, int j);
 
 /* ---- whole-run driver replaying mhd.c:154-206 ---- */
 typedef struct orc_sim orc_sim;
@@ -93,13 +103,16 @@ orc_sim *orc_sim_create(int nx, int ny, double Lx, double Ly, double gamma,
                         double cfl, int hall, double di, double resistivity,
                         double viscosity, int divbclean);
 void orc_sim_destroy(orc_sim *s);
-void orc_sim_set_state(orc_sim *s, const double *uu);   /* also dt=calcDt(0) */
+void orc_sim_set_state(orc_sim *s, const double *uu);   /* nvar fields; dt=calcDt(0) */
 void orc_sim_get_state(const orc_sim *s, double *uu);
 void orc_sim_get_derivs(const orc_sim *s, double *dudx, double *dudy);
 void orc_sim_run(orc_sim *s, int nsteps);
 void orc_sim_run_fixed_dt(orc_sim *s, int nsteps, const double *dts);
 void orc_sim_stage(orc_sim *s, int iRK);                /* one RK stage */
 void orc_sim_set_npe(orc_sim *s, int npe);             /* x blocks (MPI ranks) */
+int orc_sim_nvar(const orc_sim *s);                    /* 9 with divbclean == 2 */
+double orc_sim_glm_ch(const orc_sim *s, int i);        /* Ch, Cp of x row i */
+double orc_sim_glm_cp(const orc_sim *s, int i);
 double orc_sim_dt(const orc_sim *s);
 double orc_sim_time(const orc_sim *s);
 long orc_sim_steps(const orc_sim *s);

@@ -50,6 +50,11 @@ def lib():
             "orc_derivXX_npe": (None, [i, i, d, i, i, p, p]),
             "orc_filterX_npe": (None, [i, i, i, i, p]),
             "orc_sim_set_npe": (None, [ctypes.c_void_p, i]),
+            "orc_sim_nvar": (i, [ctypes.c_void_p]),
+            "orc_sim_glm_ch": (d, [ctypes.c_void_p, i]),
+            "orc_sim_glm_cp": (d, [ctypes.c_void_p, i]),
+            "orc_rhs_glm": (None, [i, i, d, d, d, d, d] + [p] * 10),
+            "orc_glm_ch": (d, [i, i, d, d, d, i, d, p, i, i]),
             "orc_efield": (None, [i, i, p, p]),
             "orc_divb": (None, [i, i, p, p, p]),
             "orc_rhs": (None, [i, i, d, d, d, i, d] + [p] * 12),
@@ -213,6 +218,28 @@ def rhs(uu, dudx, dudy, dEdx, dEdy, gamma=5.0 / 3.0, resistivity=0.0,
     return out
 
 
+def rhs_glm(uu, dudx, dudy, dEdx, dEdy, ch, cp, gamma=5.0 / 3.0, resistivity=0.0,
+            viscosity=0.0, d2udx2=None, d2udy2=None, d2Bdx2=None, d2Bdy2=None):
+    """calcRHS with divBcleaning == 2: 9 fields in uu, dudx, dudy and out."""
+    uu = _c(uu)
+    _, nx, ny = uu.shape
+    z3 = np.zeros((3, nx, ny))
+    args = [uu, dudx, dudy, dEdx, dEdy,
+            z3 if d2udx2 is None else d2udx2, z3 if d2udy2 is None else d2udy2,
+            z3 if d2Bdx2 is None else d2Bdx2, z3 if d2Bdy2 is None else d2Bdy2]
+    args = [_c(a) for a in args]
+    out = np.zeros((9, nx, ny))
+    lib().orc_rhs_glm(nx, ny, gamma, resistivity, viscosity, ch, cp,
+                      *[_ptr(a) for a in args], _ptr(out))
+    return out
+
+
+def glm_ch(uu, dx, dy, i, j, gamma=5.0 / 3.0, hall=0, di=0.0):
+    uu = _c(uu)
+    _, nx, ny = uu.shape
+    return lib().orc_glm_ch(nx, ny, dx, dy, gamma, hall, di, _ptr(uu), i, j)
+
+
 def rk_execute(uu, dudt, dudtRK, cc, dd):
     uu, dudt, dudtRK = _c(uu).copy(), _c(dudt), _c(dudtRK).copy()
     lib().orc_rk_execute(uu.size, cc, dd, _ptr(uu), _ptr(dudt), _ptr(dudtRK))
@@ -243,6 +270,7 @@ class Sim:
         self.h = lib().orc_sim_create(nx, ny, Lx, Ly, gamma, cfl, hall, di,
                                       resistivity, viscosity, divbclean)
         lib().orc_sim_set_npe(self.h, npe)
+        self.nvar = lib().orc_sim_nvar(self.h)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -251,17 +279,19 @@ class Sim:
 
     def set_state(self, uu):
         uu = _c(uu)
-        assert uu.shape == (8, self.nx, self.ny)
+        if uu.shape[0] == 8 and self.nvar == 9:   # psi starts at 0 (initialize.c:221)
+            uu = np.concatenate([uu, np.zeros((1, self.nx, self.ny))])
+        assert uu.shape == (self.nvar, self.nx, self.ny)
         lib().orc_sim_set_state(self.h, _ptr(uu))
 
     def state(self):
-        uu = np.zeros((8, self.nx, self.ny))
+        uu = np.zeros((self.nvar, self.nx, self.ny))
         lib().orc_sim_get_state(self.h, _ptr(uu))
         return uu
 
     def derivs(self):
-        a = np.zeros((8, self.nx, self.ny))
-        b = np.zeros((8, self.nx, self.ny))
+        a = np.zeros((self.nvar, self.nx, self.ny))
+        b = np.zeros((self.nvar, self.nx, self.ny))
         lib().orc_sim_get_derivs(self.h, _ptr(a), _ptr(b))
         return a, b
 
@@ -274,6 +304,10 @@ class Sim:
 
     def stage(self, irk):
         lib().orc_sim_stage(self.h, irk)
+
+    def glm(self, i=0):
+        """(Ch, Cp) in force for x row i (hyperbolic cleaning)."""
+        return lib().orc_sim_glm_ch(self.h, i), lib().orc_sim_glm_cp(self.h, i)
 
     @property
     def dt(self):

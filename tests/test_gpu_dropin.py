@@ -150,6 +150,56 @@ def test_calcRHS(m, variant):
         G.close()
 
 
+@pytest.mark.parametrize("kw", [dict(divbclean=2), dict(divbclean=2, viscosity=0.03,
+                                                         resistivity=0.02)])
+def test_calcRHS_hyperbolic_cleaning(m, kw):
+    nx, ny = 32, 64
+    G = _globals(m, nx, ny, gamma=1.4, **kw)
+    L = m.lib()
+    try:
+        assert G.nvar == 9
+        rng = np.random.default_rng(8)
+        arrs = {}
+        for name in ("uu", "dudx", "dudy", "dEdx", "dEdy", "d2udx2", "d2udy2", "d2Bdx2",
+                     "d2Bdy2"):
+            a = G.array(name)
+            a[:] = rng.uniform(-1, 1, a.shape)
+            arrs[name] = a
+        arrs["uu"][0] = 1.0 + rng.uniform(0, 1, (nx, ny))
+        G.scalar("Ch_divBcleaning").value = 1.7
+        L.calc_Cp_fix_Cr()
+        cp = G.scalar("Cp_divBcleaning").value
+        assert cp == np.sqrt(0.18 * 1.7)
+        L.calcRHS()
+        L.mhd2d_dropin_sync_host()
+        want = orc.rhs_glm(arrs["uu"], arrs["dudx"], arrs["dudy"], arrs["dEdx"], arrs["dEdy"],
+                           1.7, cp, gamma=1.4, resistivity=kw.get("resistivity", 0.0),
+                           viscosity=kw.get("viscosity", 0.0), d2udx2=arrs["d2udx2"],
+                           d2udy2=arrs["d2udy2"], d2Bdx2=arrs["d2Bdx2"], d2Bdy2=arrs["d2Bdy2"])
+        assert_fields_close(G.array("dudt"), want, what=f"calcRHS[glm {kw}]")
+    finally:
+        G.close()
+
+
+@pytest.mark.parametrize("hall", [0, 1])
+def test_calcDt_sets_glm_speeds_from_the_last_cell(m, hall):
+    nx, ny = 64, 32
+    G = _globals(m, nx, ny, Lx=2.0, Ly=3.0, cfl=0.4, hall=hall, di=0.3, divbclean=2)
+    L = m.lib()
+    try:
+        uu = G.array("uu")
+        uu[:8] = random_state(nx, ny, 12)
+        uu[8] = 0.1
+        want_dt = orc.calc_dt(uu[:8], 2.0 / nx, 3.0 / ny, cfl=0.4, hall=hall, di=0.3, dt0=0.0)
+        got = L.calcDt(0.0)
+        assert abs(got - want_dt) <= 1e-14 * want_dt
+        ch = orc.glm_ch(uu[:8], 2.0 / nx, 3.0 / ny, nx - 1, ny - 1, hall=hall, di=0.3)
+        assert abs(G.scalar("Ch_divBcleaning").value - ch) <= 1e-14 * ch
+        assert G.scalar("Cp_divBcleaning").value == np.sqrt(0.18 * G.scalar("Ch_divBcleaning").value)
+    finally:
+        G.close()
+
+
 def test_rkinit_rkexecute(m):
     nx, ny = 32, 32
     G = _globals(m, nx, ny)
@@ -242,24 +292,30 @@ def _mhd_c_loop(L, G, nsteps):
     return t, dt
 
 
-@pytest.mark.parametrize("case", ["ot_hall", "ot_visc_eta", "turb_divbclean"])
+@pytest.mark.parametrize("case", ["ot_hall", "ot_visc_eta", "turb_divbclean", "turb_glm",
+                                  "ot_glm_hall_visc"])
 def test_reference_stage_loop_matches_oracle(m, case):
     import mhd2dcfdfftw_open_amd as pkg
     nx, ny, steps = 64, 32, 8
     kw = dict(ot_hall=dict(hall=1, di=0.05),
               ot_visc_eta=dict(viscosity=2e-3, resistivity=1e-3),
-              turb_divbclean=dict(divbclean=1))[case]
+              turb_divbclean=dict(divbclean=1),
+              turb_glm=dict(divbclean=2),
+              ot_glm_hall_visc=dict(divbclean=2, hall=1, di=0.05, viscosity=2e-3))[case]
     u0 = pkg.orszag_tang(nx, ny) if case.startswith("ot") else pkg.turbulence(nx, ny, seed=3,
                                                                               kmax=6)
     G = _globals(m, nx, ny, **kw)
     L = m.lib()
     try:
-        G.array("uu")[:] = u0
+        G.array("uu")[:8] = u0             # psi (field 8) starts at 0
         t, dt = _mhd_c_loop(L, G, steps)   # strict mode: calcDt refreshed the host arrays
         S = orc.Sim(nx, ny, **kw)
         S.set_state(u0)
         S.run(steps)
         assert abs(dt - S.dt) <= 1e-12 * S.dt and abs(t - S.time) <= 1e-12 * S.time
         assert_fields_close(G.array("uu"), S.state(), what=f"stage loop [{case}]")
+        if G.nvar == 9:   # psi itself, at 10x its conditioning floor (tests/util.py)
+            psi_err = np.linalg.norm(G.array("uu")[8] - S.state()[8]) / np.linalg.norm(S.state()[8])
+            assert psi_err <= 1e-8, psi_err
     finally:
         G.close()

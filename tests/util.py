@@ -24,15 +24,25 @@ def rel_l2(a, b, floor=None):
 SMALL_FIELD = 1e-6
 
 
+# The GLM potential psi (field 8 with hyperbolic cleaning) is driven by
+# div B, a residual of O(1) derivatives that cancel to ~1e-6 of the state, so
+# its own relative error is set by that cancellation: perturbing the input
+# state by one ulp moves psi by ~1e-9 relative in the FP64 oracle itself
+# (64x32 turbulence, 8 steps).  It is measured like a small field, against
+# the rms of the physical fields, and separately (tests/test_gpu_dropin.py)
+# relatively at 1e-8.
+ABS_FIELDS = (8,)
+
+
 def field_errors(a, b):
     """per-field relative L2 of two [nv, nx, ny] arrays"""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
-    scale = np.sqrt(np.mean(b ** 2)) or 1.0
+    scale = np.sqrt(np.mean(b[:8] ** 2)) or 1.0
     out = []
     for v in range(b.shape[0]):
         rms_v = np.sqrt(np.mean(b[v] ** 2))
-        if rms_v < SMALL_FIELD * scale:
+        if rms_v < SMALL_FIELD * scale or v in ABS_FIELDS:
             out.append(np.sqrt(np.mean((a[v] - b[v]) ** 2)) / scale)
         else:
             out.append(rel_l2(a[v], b[v]))
