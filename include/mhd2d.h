@@ -121,6 +121,12 @@ typedef struct mhd2d_params {
                                decomposeGrid1D (initialize.c:547-569), each
                                solved with its own end rows, so results match a
                                reference run on npe ranks; nx / npe >= 4 */
+    int batch;              /* 0 or 1: one run; B > 1: B independent runs of
+                               this grid advanced together, one launch per
+                               kernel and stage for all of them; each has its
+                               own state, d_i (mhd2d_set_di), dt, time and
+                               stop.  Instance 0 is the one the calls without
+                               an _at suffix address */
 } mhd2d_params;
 
 /* status codes */
@@ -155,6 +161,18 @@ int mhd2d_get_divb(mhd2d_ctx *c, double *divb);
  * that step and all later ones are not taken; *dt_stop gets the small dt.
  * 0 while running, or an error code > 1 */
 int mhd2d_stopped(mhd2d_ctx *c, double *dt_stop);
+/* per-instance forms for batch > 1 (inst in [0, batch)); every instance
+ * needs a state before mhd2d_run.  Setting a state completes the pending
+ * filter of the others and restarts only that instance's time and dt. */
+int mhd2d_set_state_at(mhd2d_ctx *c, int inst, const double *uu);
+int mhd2d_get_state_at(mhd2d_ctx *c, int inst, double *uu);
+/* ion inertial length of one instance (default: params.di), used by its
+ * Hall CFL limit; takes effect from the instance's next dt */
+int mhd2d_set_di(mhd2d_ctx *c, int inst, double di);
+int mhd2d_get_scalars_at(mhd2d_ctx *c, int inst, double *dt, double *time,
+                         long long *steps);
+int mhd2d_diagnostics_at(mhd2d_ctx *c, int inst, double *energy, double *max_divb);
+int mhd2d_stopped_at(mhd2d_ctx *c, int inst, double *dt_stop);
 /* wall time of nsteps on the context's stream (hipEvents), in ms */
 int mhd2d_time_run(mhd2d_ctx *c, int nsteps, int use_graph, double *ms);
 /* average device time per launch of each kernel class over nsteps, in ms:

@@ -67,7 +67,7 @@ class Params(ctypes.Structure):
                 ("hall", ctypes.c_int), ("di", ctypes.c_double),
                 ("resistivity", ctypes.c_double), ("viscosity", ctypes.c_double),
                 ("divbclean", ctypes.c_int), ("precision", ctypes.c_int),
-                ("xblocks", ctypes.c_int)]
+                ("xblocks", ctypes.c_int), ("batch", ctypes.c_int)]
 
 
 def lib():
@@ -123,6 +123,12 @@ def lib():
         "calc_Cp_fix_Cr": (None, []),
         "mhd2d_get_divb": (i, [vp, p]),
         "mhd2d_stopped": (i, [vp, p]),
+        "mhd2d_set_state_at": (i, [vp, i, p]),
+        "mhd2d_get_state_at": (i, [vp, i, p]),
+        "mhd2d_set_di": (i, [vp, i, d]),
+        "mhd2d_get_scalars_at": (i, [vp, i, p, p, ctypes.POINTER(ll)]),
+        "mhd2d_diagnostics_at": (i, [vp, i, p, p]),
+        "mhd2d_stopped_at": (i, [vp, i, p]),
         "mhd2d_read_input": (i, [ctypes.c_char_p, ctypes.POINTER(Input)]),
         "mhd2d_write_input": (i, [ctypes.c_char_p, ctypes.POINTER(Input)]),
         "mhd2d_write_grid": (i, [ctypes.c_char_p, i, i, d, d]),
@@ -204,14 +210,16 @@ def drive(inp, uu0, outdir="", mode=DRIVE_AUTO, precision=FP64, verbose=False, m
 
 # ------------------------------------------------------------------ context API
 class Context:
-    """Fused device stepping of one grid (mhd2d_create ... mhd2d_destroy)."""
+    """Fused device stepping of one grid (mhd2d_create ... mhd2d_destroy), or
+    of `batch` independent runs of that grid advanced together."""
 
     def __init__(self, nx, ny, Lx=2 * np.pi, Ly=2 * np.pi, gamma=5.0 / 3.0, cfl=0.3,
-                 hall=0, di=0.0, precision=FP64, npe=1):
+                 hall=0, di=0.0, precision=FP64, npe=1, batch=1):
         """npe > 1 splits x into npe blocks as a reference run on npe MPI
         ranks does (decomposeGrid1D, initialize.c:547-569)."""
-        self.nx, self.ny = nx, ny
-        self.params = Params(nx, ny, Lx, Ly, gamma, cfl, hall, di, 0.0, 0.0, 0, precision, npe)
+        self.nx, self.ny, self.batch = nx, ny, batch
+        self.params = Params(nx, ny, Lx, Ly, gamma, cfl, hall, di, 0.0, 0.0, 0, precision, npe,
+                             batch)
         h = ctypes.c_void_p()
         _check(lib().mhd2d_create(ctypes.byref(self.params), ctypes.byref(h)))
         self.h = h
@@ -224,15 +232,18 @@ class Context:
     def __del__(self):
         self.close()
 
-    def set_state(self, uu):
+    def set_state(self, uu, inst=0):
         uu = np.ascontiguousarray(uu, dtype=np.float64)
         assert uu.shape == (8, self.nx, self.ny)
-        _check(lib().mhd2d_set_state(self.h, _ptr(uu)))
+        _check(lib().mhd2d_set_state_at(self.h, inst, _ptr(uu)))
 
-    def state(self):
+    def state(self, inst=0):
         uu = np.zeros((8, self.nx, self.ny))
-        _check(lib().mhd2d_get_state(self.h, _ptr(uu)))
+        _check(lib().mhd2d_get_state_at(self.h, inst, _ptr(uu)))
         return uu
+
+    def set_di(self, inst, di):
+        _check(lib().mhd2d_set_di(self.h, inst, di))
 
     def run(self, nsteps, graph=False):
         f = lib().mhd2d_run_graph if graph else lib().mhd2d_run
@@ -241,15 +252,14 @@ class Context:
     def sync(self):
         _check(lib().mhd2d_sync(self.h))
 
-    def scalars(self):
+    def scalars(self, inst=0):
         dt, t, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
-        _check(lib().mhd2d_get_scalars(self.h, ctypes.byref(dt), ctypes.byref(t),
-                                       ctypes.byref(n)))
+        _check(lib().mhd2d_get_scalars_at(self.h, inst, _ptr1(dt), _ptr1(t), ctypes.byref(n)))
         return dt.value, t.value, n.value
 
-    def diagnostics(self):
+    def diagnostics(self, inst=0):
         e, m = ctypes.c_double(), ctypes.c_double()
-        _check(lib().mhd2d_diagnostics(self.h, ctypes.byref(e), ctypes.byref(m)))
+        _check(lib().mhd2d_diagnostics_at(self.h, inst, _ptr1(e), _ptr1(m)))
         return e.value, m.value
 
     def time_run(self, nsteps, graph=False):
@@ -272,12 +282,16 @@ class Context:
         _check(lib().mhd2d_get_divb(self.h, _ptr(out)))
         return out
 
-    def stopped(self):
+    def stopped(self, inst=0):
         d = ctypes.c_double()
-        r = lib().mhd2d_stopped(self.h, ctypes.byref(d))
+        r = lib().mhd2d_stopped_at(self.h, inst, _ptr1(d))
         if r > 1:
             _check(r)
         return bool(r), d.value
+
+
+def _ptr1(c_double):
+    return ctypes.cast(ctypes.byref(c_double), _dp)
 
 
 # ------------------------------------------------------------------ drop-in layer

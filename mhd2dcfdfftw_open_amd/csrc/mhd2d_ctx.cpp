@@ -6,7 +6,9 @@
 //             RHS + low-storage RK update    (pointwise)
 //             x-filter of u                  (8 compact line solves)
 // which is the reference order of mhd.c:154-206 with the y-filter of each
-// stage carried into the next stage's y-sweep.
+// stage carried into the next stage's y-sweep.  A context holds `batch`
+// independent runs of one grid; every launch covers all of them (the
+// instance index is a grid dimension) and each has its own RunScalars.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -44,21 +46,25 @@ int fail(int code, const char* what, hipError_t e = hipSuccess)
 struct mhd2d_ctx {
     mhd2d_params p{};
     int elem = 8;
+    int B = 1;              // instances
     size_t n = 0;
     GridTables tb;
     hipStream_t stream = nullptr;
-    void* u = nullptr;      // [8][nx][ny]
-    void* dudx = nullptr;   // [11][nx][ny]: 8 state derivatives + 3 of E
-    void* dudy = nullptr;   // [11][nx][ny]
-    void* E = nullptr;      // [3][nx][ny]
-    void* rprev = nullptr;  // [8][nx][ny]
-    RunScalars* rs = nullptr;
+    void* u = nullptr;      // [B][11][nx][ny]: 8 state fields, then E (3)
+    void* dudx = nullptr;   // [B][11][nx][ny]: 8 state derivatives + 3 of E
+    void* dudy = nullptr;   // [B][11][nx][ny]
+    void* rprev = nullptr;  // [B][8][nx][ny]
+    RunScalars* rs = nullptr;  // [B]
     double* partial = nullptr;
     int npartial = 0;
     bool pending_filter = false;
     size_t bytes = 0;
     hipGraphExec_t graph = nullptr;
-    bool have_state = false;
+    std::vector<char> have_state;
+    std::vector<double> di;
+    size_t su() const { return 11 * n; }  // per-instance strides (elements)
+    size_t sd() const { return 11 * n; }
+    size_t sr() const { return 8 * n; }
 };
 
 namespace {
@@ -84,7 +90,7 @@ hipError_t ysweep(mhd2d_ctx* c, int filter, int derivs, int dt_limits)
     T* dudy = (T*)c->dudy;
     a.u = (T*)c->u;
     a.dudy = dudy;
-    a.E = (T*)c->E;
+    a.E = (T*)c->u + 8 * c->n;
     a.dEdy = dudy + 8 * c->n;
     a.tb = c->tb.yt;
     a.dp = dt_params(c);
@@ -94,18 +100,18 @@ hipError_t ysweep(mhd2d_ctx* c, int filter, int derivs, int dt_limits)
     a.filter = filter;
     a.derivs = derivs;
     a.dt_limits = dt_limits;
-    return launch_ysweep<T>(a, c->stream);
+    a.bs_u = c->su();
+    a.bs_d = c->sd();
+    return launch_ysweep<T>(a, c->stream, c->B);
 }
 
 template <typename T>
 hipError_t xderiv(mhd2d_ctx* c)
 {
     XSolveArgs<T> a;
-    T* u = (T*)c->u;
-    T* E = (T*)c->E;
+    T* u = (T*)c->u;  // 8 state fields and E
     T* dx = (T*)c->dudx;
-    for (int v = 0; v < 8; v++) a.in[v] = u + v * c->n;
-    for (int v = 0; v < 3; v++) a.in[8 + v] = E + v * c->n;
+    for (int v = 0; v < 11; v++) a.in[v] = u + v * c->n;
     for (int v = 0; v < 11; v++) a.out[v] = dx + v * c->n;
     a.rs = c->rs;
     a.st = c->tb.st[XD1];
@@ -113,7 +119,9 @@ hipError_t xderiv(mhd2d_ctx* c)
     a.kind = XD1;
     a.nx = c->p.nx;
     a.ny = c->p.ny;
-    return launch_xsolve<T>(a, 11, c->stream);
+    a.bs_in = c->su();
+    a.bs_out = c->sd();
+    return launch_xsolve<T>(a, 11, c->stream, c->B);
 }
 
 template <typename T>
@@ -128,7 +136,8 @@ hipError_t xfilter(mhd2d_ctx* c)
     a.kind = XFILT;
     a.nx = c->p.nx;
     a.ny = c->p.ny;
-    return launch_xsolve<T>(a, 8, c->stream);
+    a.bs_in = a.bs_out = c->su();
+    return launch_xsolve<T>(a, 8, c->stream, c->B);
 }
 
 template <typename T>
@@ -148,7 +157,10 @@ hipError_t rhs_rk(mhd2d_ctx* c, int stage)
     a.stage = stage;
     a.write_rprev = stage < 2;  // rkInit zeroes R_prev before it is read again
     a.n = c->n;
-    return launch_rhs_rk<T>(a, c->stream);
+    a.bs_u = c->su();
+    a.bs_d = c->sd();
+    a.bs_r = c->sr();
+    return launch_rhs_rk<T>(a, c->stream, c->B);
 }
 
 // one RK3 step; when ev is given, an event is recorded after every launch
@@ -164,7 +176,7 @@ hipError_t step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
         if (e != hipSuccess) return e;
         mark(0);
         if (s == 0) {
-            e = launch_dt_finalize(c->rs, c->p.cfl, 0, c->stream);
+            e = launch_dt_finalize(c->rs, c->p.cfl, 0, c->stream, c->B);
             if (e != hipSuccess) return e;
             mark(0);
         }
@@ -191,24 +203,35 @@ hipError_t flush(mhd2d_ctx* c)
     return e;
 }
 
-// After a stop (dt < 1e-8, mhd.c:131) the stopping step's y-sweep has
-// already applied the pending filter and every later launch was a no-op.
-hipError_t check_stop(mhd2d_ctx* c, int* stopped, double* dt_stop)
+hipError_t read_scalars(mhd2d_ctx* c, std::vector<RunScalars>& r)
 {
-    RunScalars r;
-    hipError_t e = hipMemcpyAsync(&r, c->rs, sizeof(r), hipMemcpyDeviceToHost, c->stream);
+    r.resize(c->B);
+    hipError_t e = hipMemcpyAsync(r.data(), c->rs, sizeof(RunScalars) * c->B,
+                                  hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e;
+}
+
+// After a stop (dt < 1e-8, mhd.c:131) the stopping step's y-sweep has
+// already applied the pending filter and every later launch was a no-op;
+// the flush y-sweep skips stopped instances by itself.
+hipError_t check_stop(mhd2d_ctx* c, int inst, int* stopped, double* dt_stop)
+{
+    std::vector<RunScalars> r;
+    hipError_t e = read_scalars(c, r);
     if (e != hipSuccess) return e;
-    if (r.stop) c->pending_filter = false;
-    if (stopped) *stopped = r.stop;
-    if (dt_stop) *dt_stop = r.dt_stop;
+    bool all = true;
+    for (auto& q : r) all = all && q.stop;
+    if (all) c->pending_filter = false;
+    if (stopped) *stopped = r[inst].stop;
+    if (dt_stop) *dt_stop = r[inst].dt_stop;
     return hipSuccess;
 }
 
 hipError_t do_flush(mhd2d_ctx* c)
 {
     if (c->pending_filter) {
-        hipError_t e = check_stop(c, nullptr, nullptr);
+        hipError_t e = check_stop(c, 0, nullptr, nullptr);
         if (e != hipSuccess) return e;
     }
     return c->elem == 8 ? flush<double>(c) : flush<float>(c);
@@ -248,6 +271,8 @@ int check_params(const mhd2d_params* p)
         return fail(MHD2D_EINVAL, "precision must be MHD2D_FP64 or MHD2D_FP32");
     if (p->xblocks < 0 || p->xblocks > p->nx / 4)
         return fail(MHD2D_EINVAL, "xblocks must be in [0, nx/4] (blocks of at least 4 rows)");
+    if (p->batch < 0 || p->batch > 65535)
+        return fail(MHD2D_EINVAL, "batch must be in [0, 65535]");
     return MHD2D_OK;
 }
 
@@ -266,17 +291,20 @@ int mhd2d_create(const mhd2d_params* p, mhd2d_ctx** out)
     mhd2d_ctx* c = new mhd2d_ctx();
     c->p = *p;
     c->elem = p->precision == MHD2D_FP64 ? 8 : 4;
+    c->B = p->batch > 1 ? p->batch : 1;
     c->n = (size_t)p->nx * p->ny;
+    c->have_state.assign(c->B, 0);
+    c->di.assign(c->B, p->di);
     hipError_t e = c->tb.build(p->nx, p->ny, p->Lx, p->Ly, c->elem, SchemeAlphas(), p->xblocks);
     if (e != hipSuccess) {
         delete c;
         return fail(MHD2D_EHIP, "table build", e);
     }
-    const size_t fb = c->n * c->elem;
+    const size_t fb = c->n * c->elem * c->B;
     c->npartial = 1024;
-    size_t sizes[] = {8 * fb, 11 * fb, 11 * fb, 3 * fb, 8 * fb};
-    void** ptrs[] = {&c->u, &c->dudx, &c->dudy, &c->E, &c->rprev};
-    for (int k = 0; k < 5; k++) {
+    size_t sizes[] = {11 * fb, 11 * fb, 11 * fb, 8 * fb};
+    void** ptrs[] = {&c->u, &c->dudx, &c->dudy, &c->rprev};
+    for (int k = 0; k < 4; k++) {
         e = hipMalloc(ptrs[k], sizes[k]);
         if (e != hipSuccess) {
             mhd2d_destroy(c);
@@ -284,7 +312,7 @@ int mhd2d_create(const mhd2d_params* p, mhd2d_ctx** out)
         }
         c->bytes += sizes[k];
     }
-    e = hipMalloc(&c->rs, sizeof(RunScalars));
+    e = hipMalloc(&c->rs, sizeof(RunScalars) * c->B);
     if (e == hipSuccess) e = hipMalloc(&c->partial, 2 * c->npartial * sizeof(double));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -300,7 +328,7 @@ void mhd2d_destroy(mhd2d_ctx* c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
-    void* ptrs[] = {c->u, c->dudx, c->dudy, c->E, c->rprev, c->rs, c->partial};
+    void* ptrs[] = {c->u, c->dudx, c->dudy, c->rprev, c->rs, c->partial};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -339,18 +367,22 @@ int mhd2d_copy_probe(size_t bytes, int iters, double* gbps)
     return MHD2D_OK;
 }
 
-int mhd2d_set_state(mhd2d_ctx* c, const double* uu)
+int mhd2d_set_state_at(mhd2d_ctx* c, int inst, const double* uu)
 {
     if (!c || !uu) return fail(MHD2D_EINVAL, "null argument");
+    if (inst < 0 || inst >= c->B) return fail(MHD2D_EINVAL, "instance out of range");
+    // the other instances keep their pending y filter: complete it first
+    HIPCHK(do_flush(c));
     const size_t cnt = 8 * c->n;
     if (c->elem == 8) {
-        HIPCHK(hipMemcpyAsync(c->u, uu, cnt * 8, hipMemcpyHostToDevice, c->stream));
+        double* dst = (double*)c->u + inst * c->su();
+        HIPCHK(hipMemcpyAsync(dst, uu, cnt * 8, hipMemcpyHostToDevice, c->stream));
     } else {
-        // stage through the R_prev buffer (8 fields of FP32 = 4 of FP64)
         double* tmp = nullptr;
         HIPCHK(hipMalloc(&tmp, cnt * 8));
         HIPCHK(hipMemcpyAsync(tmp, uu, cnt * 8, hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_to_float, dim3(2048), dim3(256), 0, c->stream, tmp, (float*)c->u, cnt);
+        hipLaunchKernelGGL(k_to_float, dim3(2048), dim3(256), 0, c->stream, tmp,
+                           (float*)c->u + inst * c->su(), cnt);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipFree(tmp));
@@ -358,25 +390,41 @@ int mhd2d_set_state(mhd2d_ctx* c, const double* uu)
     RunScalars z;
     std::memset(&z, 0, sizeof(z));
     z.dtmin_bits = 0x7ff0000000000000ull;
-    HIPCHK(hipMemcpyAsync(c->rs, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+    z.di = c->di[inst];
+    HIPCHK(hipMemcpyAsync(c->rs + inst, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->pending_filter = false;
-    c->have_state = true;
+    c->have_state[inst] = 1;
     return MHD2D_OK;
 }
 
-int mhd2d_get_state(mhd2d_ctx* c, double* uu)
+int mhd2d_set_state(mhd2d_ctx* c, const double* uu) { return mhd2d_set_state_at(c, 0, uu); }
+
+int mhd2d_set_di(mhd2d_ctx* c, int inst, double di)
+{
+    if (!c) return fail(MHD2D_EINVAL, "null argument");
+    if (inst < 0 || inst >= c->B) return fail(MHD2D_EINVAL, "instance out of range");
+    if (!(di >= 0)) return fail(MHD2D_EINVAL, "di must be >= 0");
+    c->di[inst] = di;
+    HIPCHK(hipMemcpyAsync(&c->rs[inst].di, &c->di[inst], sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MHD2D_OK;
+}
+
+int mhd2d_get_state_at(mhd2d_ctx* c, int inst, double* uu)
 {
     if (!c || !uu) return fail(MHD2D_EINVAL, "null argument");
+    if (inst < 0 || inst >= c->B) return fail(MHD2D_EINVAL, "instance out of range");
     HIPCHK(do_flush(c));
     const size_t cnt = 8 * c->n;
     if (c->elem == 8) {
-        HIPCHK(hipMemcpyAsync(uu, c->u, cnt * 8, hipMemcpyDeviceToHost, c->stream));
+        const double* src = (const double*)c->u + inst * c->su();
+        HIPCHK(hipMemcpyAsync(uu, src, cnt * 8, hipMemcpyDeviceToHost, c->stream));
     } else {
         double* tmp = nullptr;
         HIPCHK(hipMalloc(&tmp, cnt * 8));
-        hipLaunchKernelGGL(k_to_double, dim3(2048), dim3(256), 0, c->stream, (const float*)c->u,
-                           tmp, cnt);
+        hipLaunchKernelGGL(k_to_double, dim3(2048), dim3(256), 0, c->stream,
+                           (const float*)c->u + inst * c->su(), tmp, cnt);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(uu, tmp, cnt * 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -386,10 +434,21 @@ int mhd2d_get_state(mhd2d_ctx* c, double* uu)
     return MHD2D_OK;
 }
 
+int mhd2d_get_state(mhd2d_ctx* c, double* uu) { return mhd2d_get_state_at(c, 0, uu); }
+
+namespace {
+int check_ready(const mhd2d_ctx* c)
+{
+    for (char h : c->have_state)
+        if (!h) return fail(MHD2D_EINVAL, "every instance needs mhd2d_set_state first");
+    return MHD2D_OK;
+}
+}  // namespace
+
 int mhd2d_run(mhd2d_ctx* c, int nsteps)
 {
     if (!c || nsteps < 0) return fail(MHD2D_EINVAL, "bad argument");
-    if (!c->have_state) return fail(MHD2D_EINVAL, "mhd2d_set_state must come first");
+    if (int rc = check_ready(c)) return rc;
     for (int k = 0; k < nsteps; k++) HIPCHK(do_step(c, nullptr, nullptr));
     return MHD2D_OK;
 }
@@ -397,7 +456,7 @@ int mhd2d_run(mhd2d_ctx* c, int nsteps)
 int mhd2d_run_graph(mhd2d_ctx* c, int nsteps)
 {
     if (!c || nsteps < 0) return fail(MHD2D_EINVAL, "bad argument");
-    if (!c->have_state) return fail(MHD2D_EINVAL, "mhd2d_set_state must come first");
+    if (int rc = check_ready(c)) return rc;
     if (nsteps == 0) return MHD2D_OK;
     int k = 0;
     if (!c->pending_filter) {  // the first step differs (no pending y filter)
@@ -422,19 +481,26 @@ int mhd2d_run_graph(mhd2d_ctx* c, int nsteps)
 int mhd2d_sync(mhd2d_ctx* c)
 {
     if (!c) return fail(MHD2D_EINVAL, "null argument");
-    HIPCHK(hipStreamSynchronize(c->stream));
-    int nf = 0;
-    HIPCHK(hipMemcpy(&nf, &c->rs->nonfinite, sizeof(int), hipMemcpyDeviceToHost));
-    if (nf) return fail(MHD2D_ENONFINITE, "a CFL limit was not finite (state blew up)");
+    std::vector<RunScalars> r;
+    HIPCHK(read_scalars(c, r));
+    for (int i = 0; i < c->B; i++)
+        if (r[i].nonfinite)
+            return fail(MHD2D_ENONFINITE, "a CFL limit was not finite (state blew up)");
     return MHD2D_OK;
 }
 
 int mhd2d_get_scalars(mhd2d_ctx* c, double* dt, double* time, long long* steps)
 {
+    return mhd2d_get_scalars_at(c, 0, dt, time, steps);
+}
+
+int mhd2d_get_scalars_at(mhd2d_ctx* c, int inst, double* dt, double* time, long long* steps)
+{
     if (!c) return fail(MHD2D_EINVAL, "null argument");
+    if (inst < 0 || inst >= c->B) return fail(MHD2D_EINVAL, "instance out of range");
     RunScalars r;
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(&r, c->rs, sizeof(r), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&r, c->rs + inst, sizeof(r), hipMemcpyDeviceToHost));
     // rs->time holds the sum of the dts of all steps but the last
     if (dt) *dt = r.dt;
     if (time) *time = r.step > 0 ? r.time + r.dt : 0.0;
@@ -442,11 +508,14 @@ int mhd2d_get_scalars(mhd2d_ctx* c, double* dt, double* time, long long* steps)
     return MHD2D_OK;
 }
 
-int mhd2d_stopped(mhd2d_ctx* c, double* dt_stop)
+int mhd2d_stopped(mhd2d_ctx* c, double* dt_stop) { return mhd2d_stopped_at(c, 0, dt_stop); }
+
+int mhd2d_stopped_at(mhd2d_ctx* c, int inst, double* dt_stop)
 {
     if (!c) return fail(MHD2D_EINVAL, "null argument");
+    if (inst < 0 || inst >= c->B) return fail(MHD2D_EINVAL, "instance out of range");
     int st = 0;
-    HIPCHK(check_stop(c, &st, dt_stop));
+    HIPCHK(check_stop(c, inst, &st, dt_stop));
     return st;
 }
 
@@ -455,10 +524,11 @@ int mhd2d_get_divb(mhd2d_ctx* c, double* divb)
     if (!c || !divb) return fail(MHD2D_EINVAL, "null argument");
     HIPCHK(do_flush(c));
     const size_t n = c->n;
-    // dBx/dx + dBy/dy of the current state, through the E scratch buffer
+    // dBx/dx + dBy/dy of the current state of instance 0, through its E
+    // fields (scratch: the next y-sweep recomputes E)
     if (c->elem == 8) {
         double* u = (double*)c->u;
-        double* sc = (double*)c->E;
+        double* sc = u + 8 * n;
         XSolveArgs<double> xa;
         xa.in[0] = u + BX * n;
         xa.out[0] = sc;
@@ -481,7 +551,7 @@ int mhd2d_get_divb(mhd2d_ctx* c, double* divb)
         HIPCHK(hipMemcpyAsync(divb, sc + 2 * n, n * 8, hipMemcpyDeviceToHost, c->stream));
     } else {
         float* u = (float*)c->u;
-        float* sc = (float*)c->E;
+        float* sc = u + 8 * n;
         XSolveArgs<float> xa;
         xa.in[0] = u + BX * n;
         xa.out[0] = sc;
@@ -512,14 +582,21 @@ int mhd2d_get_divb(mhd2d_ctx* c, double* divb)
 
 int mhd2d_diagnostics(mhd2d_ctx* c, double* energy, double* max_divb)
 {
+    return mhd2d_diagnostics_at(c, 0, energy, max_divb);
+}
+
+int mhd2d_diagnostics_at(mhd2d_ctx* c, int inst, double* energy, double* max_divb)
+{
     if (!c) return fail(MHD2D_EINVAL, "null argument");
+    if (inst < 0 || inst >= c->B) return fail(MHD2D_EINVAL, "instance out of range");
     HIPCHK(do_flush(c));
-    // d(Bx)/dx and d(By)/dy of the current state into the E scratch buffer
+    // d(Bx)/dx and d(By)/dy of the instance's current state into its E
+    // fields (scratch: the next y-sweep recomputes E)
     const size_t n = c->n;
     int nb = c->npartial;
     if (c->elem == 8) {
-        double* u = (double*)c->u;
-        double* scratch = (double*)c->E;
+        double* u = (double*)c->u + inst * c->su();
+        double* scratch = u + 8 * n;
         XSolveArgs<double> xa;
         xa.in[0] = u + BX * n;
         xa.out[0] = scratch;
@@ -541,8 +618,8 @@ int mhd2d_diagnostics(mhd2d_ctx* c, double* energy, double* max_divb)
         HIPCHK(launch_diag<double>(u, scratch, scratch + n, c->partial, nb, n, c->p.gamma,
                                    c->stream));
     } else {
-        float* u = (float*)c->u;
-        float* scratch = (float*)c->E;
+        float* u = (float*)c->u + inst * c->su();
+        float* scratch = u + 8 * n;
         XSolveArgs<float> xa;
         xa.in[0] = u + BX * n;
         xa.out[0] = scratch;

@@ -55,6 +55,9 @@ struct XSolveArgs {
     const RunScalars* rs = nullptr;  // optional: skip the launch once the run has stopped
     int kind;
     int nx, ny;
+    // instance z of a batch (blockIdx.z) reads in[v] + z * bs_in, writes
+    // out[v] + z * bs_out and checks rs[z]
+    size_t bs_in = 0, bs_out = 0;
 };
 
 // ---------------------------------------------------------------- y operators
@@ -102,6 +105,7 @@ struct RunScalars {
     long long step;
     double dt_stop;                  // the dt that triggered the stop
     double glm_ch;                   // fastest speed of the last cell (GLM Ch)
+    double di;                       // ion inertial length of this instance (Hall CFL)
 };
 
 template <typename T>
@@ -117,6 +121,9 @@ struct YSweepArgs {          // fused y-sweep of the driver
     int filter;              // apply the pending y filter first
     int dt_limits;           // reduce the local dt limits into rs
     int derivs;              // compute the y derivatives and E
+    // instance z of a batch (blockIdx.y): u, E + z * bs_u; dudy, dEdy +
+    // z * bs_d; rs + z (whose di replaces dp.di)
+    size_t bs_u = 0, bs_d = 0;
 };
 
 template <typename T>
@@ -152,19 +159,22 @@ struct RhsRkArgs {           // fused RHS + low-storage RK update (driver)
     int stage;
     int write_rprev;
     size_t n;
+    // instance z of a batch (blockIdx.y): u + z * bs_u, dudx/dudy/dEdx/dEdy
+    // + z * bs_d, rprev + z * bs_r, rs + z
+    size_t bs_u = 0, bs_d = 0, bs_r = 0;
 };
 
 // ---------------------------------------------------------------- launchers
 template <typename T>
-hipError_t launch_xsolve(const XSolveArgs<T>& a, int nv, hipStream_t s);
+hipError_t launch_xsolve(const XSolveArgs<T>& a, int nv, hipStream_t s, int nb = 1);
 template <typename T>
 hipError_t launch_yapply(const YApplyArgs<T>& a, hipStream_t s);
 template <typename T>
-hipError_t launch_ysweep(const YSweepArgs<T>& a, hipStream_t s);
+hipError_t launch_ysweep(const YSweepArgs<T>& a, hipStream_t s, int nb = 1);
 template <typename T>
 hipError_t launch_rhs(const RhsArgs<T>& a, hipStream_t s);
 template <typename T>
-hipError_t launch_rhs_rk(const RhsRkArgs<T>& a, hipStream_t s);
+hipError_t launch_rhs_rk(const RhsRkArgs<T>& a, hipStream_t s, int nb = 1);
 template <typename T>
 hipError_t launch_efield(const T* u, T* E, size_t n, hipStream_t s);
 template <typename T>
@@ -175,7 +185,8 @@ hipError_t launch_rk(T* u, const T* dudt, T* dudtRK, double cc, double dd, size_
 template <typename T>
 hipError_t launch_dt_limits(const T* u, const DtParams& dp, RunScalars* rs, int nx,
                             int ny, hipStream_t s);
-hipError_t launch_dt_finalize(RunScalars* rs, double cfl, int first, hipStream_t s);
+hipError_t launch_dt_finalize(RunScalars* rs, double cfl, int first, hipStream_t s,
+                              int nb = 1);
 template <typename T>
 hipError_t launch_glm_ch(const T* u, const DtParams& dp, RunScalars* rs, size_t n,
                          hipStream_t s);

@@ -100,7 +100,8 @@ __device__ __forceinline__ void block_min_dt(double v, RunScalars* rs, double* r
 template <typename T, int C, int W, int KIND, bool BLK>
 __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T> a)
 {
-    if (a.rs && a.rs->stop) return;  // the run stopped (dt < 1e-8): uniform per block
+    const int z = blockIdx.z;  // instance of a batch
+    if (a.rs && a.rs[z].stop) return;  // the run stopped (dt < 1e-8): uniform per block
     const int G = blockDim.x / W;
     const int c = threadIdx.x % W;
     const int g = threadIdx.x / W;
@@ -108,8 +109,8 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T> a
     const int nx = a.nx;
     const size_t ny = (size_t)a.ny;
     const size_t j = (size_t)blockIdx.x * W + c;
-    const T* __restrict__ f = a.in[v] + j;
-    T* __restrict__ o = a.out[v] + j;
+    const T* __restrict__ f = a.in[v] + z * a.bs_in + j;
+    T* __restrict__ o = a.out[v] + z * a.bs_out + j;
     const int s = g * C;
 
     __shared__ T sh_in[W * 64];
@@ -299,10 +300,10 @@ static void launch_xsolve_kind(const XSolveArgs<T>& a, dim3 grid, dim3 block, hi
 }
 
 template <typename T, int C, int W>
-static hipError_t launch_xsolve_cw(const XSolveArgs<T>& a, int nv, hipStream_t s)
+static hipError_t launch_xsolve_cw(const XSolveArgs<T>& a, int nv, hipStream_t s, int nb)
 {
     const int G = a.nx / C;
-    dim3 grid(a.ny / W, nv), block(W * G);
+    dim3 grid(a.ny / W, nv, nb), block(W * G);
     if (a.tb.bstart && a.tb.bend)
         launch_xsolve_kind<T, C, W, true>(a, grid, block, s);
     else
@@ -311,24 +312,25 @@ static hipError_t launch_xsolve_cw(const XSolveArgs<T>& a, int nv, hipStream_t s
 }
 
 template <typename T>
-hipError_t launch_xsolve(const XSolveArgs<T>& a, int nv, hipStream_t s)
+hipError_t launch_xsolve(const XSolveArgs<T>& a, int nv, hipStream_t s, int nb)
 {
+    if (nb < 1 || nb > 65535) return hipErrorInvalidValue;
     int C, W, G;
     xsolve_shape(a.nx, sizeof(T), &C, &W, &G);
     if (a.nx % C || G > 64 || G < 2 || a.ny % W || nv < 1 || nv > 16) return hipErrorInvalidValue;
     if (W == 8) {
         switch (C) {
-        case 8: return launch_xsolve_cw<T, 8, 8>(a, nv, s);
-        case 16: return launch_xsolve_cw<T, 16, 8>(a, nv, s);
-        case 32: return launch_xsolve_cw<T, 32, 8>(a, nv, s);
-        case 64: return launch_xsolve_cw<T, 64, 8>(a, nv, s);
+        case 8: return launch_xsolve_cw<T, 8, 8>(a, nv, s, nb);
+        case 16: return launch_xsolve_cw<T, 16, 8>(a, nv, s, nb);
+        case 32: return launch_xsolve_cw<T, 32, 8>(a, nv, s, nb);
+        case 64: return launch_xsolve_cw<T, 64, 8>(a, nv, s, nb);
         default: return hipErrorInvalidValue;
         }
     }
     switch (C) {
-    case 8: return launch_xsolve_cw<T, 8, 16>(a, nv, s);
-    case 16: return launch_xsolve_cw<T, 16, 16>(a, nv, s);
-    case 32: return launch_xsolve_cw<T, 32, 16>(a, nv, s);
+    case 8: return launch_xsolve_cw<T, 8, 16>(a, nv, s, nb);
+    case 16: return launch_xsolve_cw<T, 16, 16>(a, nv, s, nb);
+    case 32: return launch_xsolve_cw<T, 32, 16>(a, nv, s, nb);
     default: return hipErrorInvalidValue;
     }
 }
@@ -427,7 +429,9 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
     __shared__ T s_re[Y::ROWS * Y::LDSW];
     __shared__ T s_im[Y::ROWS * Y::LDSW];
     __shared__ double red[16];
-    if (a.rs->stop) return;  // the run stopped (dt < 1e-8): uniform per block
+    const int z = blockIdx.y;  // instance of a batch
+    RunScalars* const rs = a.rs + z;
+    if (rs->stop) return;  // the run stopped (dt < 1e-8): uniform per block
     const int t = threadIdx.x % TPR, rl = threadIdx.x / TPR;
     const int i_raw = blockIdx.x * Y::ROWS + rl;
     const bool active = i_raw < a.nx;
@@ -436,7 +440,10 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
     T* lre = s_re + rl * Y::LDSW;
     T* lim = s_im + rl * Y::LDSW;
     const double2* __restrict__ tw = a.tb.tw;
-    T* const u = a.u;
+    T* const u = a.u + z * a.bs_u;
+    T* const Ef = a.E + z * a.bs_u;
+    T* const dudy = a.dudy + z * a.bs_d;
+    T* const dEdy = a.dEdy + z * a.bs_d;
 
     // field pairs packed as re/im of one complex line
     constexpr int PA[4] = {RHO, UY, BX, BZ};
@@ -486,8 +493,8 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
             if (a.derivs) {
                 fft_inv<T, N, E>(v, t, lre, lim, tw);
                 if (active) {
-                    T* da = a.dudy + fa * n + row;
-                    T* db = a.dudy + fb * n + row;
+                    T* da = dudy + fa * n + row;
+                    T* db = dudy + fb * n + row;
 #pragma unroll
                     for (int m = 0; m < E; m++) {
                         const int k = t + m * TPR;
@@ -513,9 +520,9 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
             ez[m] = uy * bx - ux * by;
             v[m] = cx<T>{ex, ey};
             if (active) {
-                a.E[0 * n + k] = ex;
-                a.E[1 * n + k] = ey;
-                a.E[2 * n + k] = ez[m];
+                Ef[0 * n + k] = ex;
+                Ef[1 * n + k] = ey;
+                Ef[2 * n + k] = ez[m];
             }
         }
         fft_fwd<T, N, E>(v, t, lre, lim, tw);
@@ -526,8 +533,8 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
 #pragma unroll
             for (int m = 0; m < E; m++) {
                 const size_t k = row + t + m * TPR;
-                a.dEdy[0 * n + k] = v[m].re;
-                a.dEdy[1 * n + k] = v[m].im;
+                dEdy[0 * n + k] = v[m].re;
+                dEdy[1 * n + k] = v[m].im;
             }
         }
 #pragma unroll
@@ -538,11 +545,13 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
         fft_inv<T, N, E>(v, t, lre, lim, tw);
         if (active) {
 #pragma unroll
-            for (int m = 0; m < E; m++) a.dEdy[2 * n + row + t + m * TPR] = v[m].re;
+            for (int m = 0; m < E; m++) dEdy[2 * n + row + t + m * TPR] = v[m].re;
         }
     }
 
     if (a.dt_limits) {
+        DtParams dp = a.dp;
+        dp.di = rs->di;
         double mn = INFINITY;
         if (active) {
 #pragma unroll 4
@@ -550,37 +559,40 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
                 const size_t k = row + t + m * TPR;
                 mn = fmin(mn, cell_dt(u[RHO * n + k], u[UX * n + k], u[UY * n + k],
                                       u[BX * n + k], u[BY * n + k], u[BZ * n + k],
-                                      u[PR * n + k], a.dp));
+                                      u[PR * n + k], dp));
             }
         }
-        block_min_dt(mn, a.rs, red);
+        block_min_dt(mn, rs, red);
     }
 }
 
 template <typename T, int N>
-static hipError_t launch_y_n(const YApplyArgs<T>* ya, const YSweepArgs<T>* ys, hipStream_t s)
+static hipError_t launch_y_n(const YApplyArgs<T>* ya, const YSweepArgs<T>* ys, hipStream_t s,
+                             int nb)
 {
     const int nx = ya ? ya->nx : ys->nx;
     using Y = YShape<T, N>;
-    dim3 grid((nx + Y::ROWS - 1) / Y::ROWS), block(Y::THREADS);
+    dim3 grid((nx + Y::ROWS - 1) / Y::ROWS, ya ? 1 : nb), block(Y::THREADS);
     if (ya) hipLaunchKernelGGL((k_yapply<T, N>), grid, block, 0, s, *ya);
     else hipLaunchKernelGGL((k_ysweep<T, N>), grid, block, 0, s, *ys);
     return hipGetLastError();
 }
 
 template <typename T>
-static hipError_t launch_y(int ny, const YApplyArgs<T>* ya, const YSweepArgs<T>* ys, hipStream_t s)
+static hipError_t launch_y(int ny, const YApplyArgs<T>* ya, const YSweepArgs<T>* ys, hipStream_t s,
+                           int nb = 1)
 {
+    if (nb < 1 || nb > 65535) return hipErrorInvalidValue;
     switch (ny) {
-    case 16: return launch_y_n<T, 16>(ya, ys, s);
-    case 32: return launch_y_n<T, 32>(ya, ys, s);
-    case 64: return launch_y_n<T, 64>(ya, ys, s);
-    case 128: return launch_y_n<T, 128>(ya, ys, s);
-    case 256: return launch_y_n<T, 256>(ya, ys, s);
-    case 512: return launch_y_n<T, 512>(ya, ys, s);
-    case 1024: return launch_y_n<T, 1024>(ya, ys, s);
-    case 2048: return launch_y_n<T, 2048>(ya, ys, s);
-    case 4096: return launch_y_n<T, 4096>(ya, ys, s);
+    case 16: return launch_y_n<T, 16>(ya, ys, s, nb);
+    case 32: return launch_y_n<T, 32>(ya, ys, s, nb);
+    case 64: return launch_y_n<T, 64>(ya, ys, s, nb);
+    case 128: return launch_y_n<T, 128>(ya, ys, s, nb);
+    case 256: return launch_y_n<T, 256>(ya, ys, s, nb);
+    case 512: return launch_y_n<T, 512>(ya, ys, s, nb);
+    case 1024: return launch_y_n<T, 1024>(ya, ys, s, nb);
+    case 2048: return launch_y_n<T, 2048>(ya, ys, s, nb);
+    case 4096: return launch_y_n<T, 4096>(ya, ys, s, nb);
     default: return hipErrorInvalidValue;
     }
 }
@@ -593,9 +605,9 @@ hipError_t launch_yapply(const YApplyArgs<T>& a, hipStream_t s)
 }
 
 template <typename T>
-hipError_t launch_ysweep(const YSweepArgs<T>& a, hipStream_t s)
+hipError_t launch_ysweep(const YSweepArgs<T>& a, hipStream_t s, int nb)
 {
-    return launch_y<T>(a.ny, nullptr, &a, s);
+    return launch_y<T>(a.ny, nullptr, &a, s, nb);
 }
 
 // ============================================================== pointwise
@@ -725,10 +737,18 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T> a)
 {
     constexpr int W = Vec<T>::W;
-    if (a.rs->stop) return;  // the run stopped (dt < 1e-8)
+    const int z = blockIdx.y;  // instance of a batch
+    const RunScalars* const rs = a.rs + z;
+    if (rs->stop) return;  // the run stopped (dt < 1e-8)
     const size_t n = a.n;
-    const T cc = (T)a.rs->cc[a.stage];
-    const T dd = (T)a.rs->dd[a.stage];
+    const T cc = (T)rs->cc[a.stage];
+    const T dd = (T)rs->dd[a.stage];
+    T* const uz = a.u + z * a.bs_u;
+    T* const rpz = a.rprev + z * a.bs_r;
+    const T* const dxz = a.dudx + z * a.bs_d;
+    const T* const dyz = a.dudy + z * a.bs_d;
+    const T* const exz = a.dEdx + z * a.bs_d;
+    const T* const eyz = a.dEdy + z * a.bs_d;
     const T g = (T)a.gamma;
     const bool use_prev = a.stage > 0;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n / W;
@@ -737,14 +757,14 @@ __global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T> a)
         Vec<T> u[8], dx[8], dy[8], ex[3], ey[3];
 #pragma unroll
         for (int f = 0; f < 8; f++) {
-            u[f] = ldv(a.u + f * n + p);
-            dx[f] = ldv(a.dudx + f * n + p);
-            dy[f] = ldv(a.dudy + f * n + p);
+            u[f] = ldv(uz + f * n + p);
+            dx[f] = ldv(dxz + f * n + p);
+            dy[f] = ldv(dyz + f * n + p);
         }
 #pragma unroll
         for (int f = 0; f < 3; f++) {
-            ex[f] = ldv(a.dEdx + f * n + p);
-            ey[f] = ldv(a.dEdy + f * n + p);
+            ex[f] = ldv(exz + f * n + p);
+            ey[f] = ldv(eyz + f * n + p);
         }
         Vec<T> R[8];
 #pragma unroll
@@ -772,13 +792,13 @@ __global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T> a)
         for (int f = 0; f < 8; f++) {
             // rkExecute, timeAdvance.c:31-35 (R_prev is zero in stage 0)
             Vec<T> rp;
-            if (use_prev) rp = ldv(a.rprev + f * n + p);
+            if (use_prev) rp = ldv(rpz + f * n + p);
             Vec<T> o;
 #pragma unroll
             for (int w = 0; w < W; w++)
                 o.x[w] = u[f].x[w] + cc * R[f].x[w] + dd * (use_prev ? rp.x[w] : (T)0);
-            *reinterpret_cast<Vec<T>*>(a.u + f * n + p) = o;
-            if (a.write_rprev) *reinterpret_cast<Vec<T>*>(a.rprev + f * n + p) = R[f];
+            *reinterpret_cast<Vec<T>*>(uz + f * n + p) = o;
+            if (a.write_rprev) *reinterpret_cast<Vec<T>*>(rpz + f * n + p) = R[f];
         }
     }
 }
@@ -810,8 +830,9 @@ __global__ void __launch_bounds__(256) k_dt_limits(const T* __restrict__ u, DtPa
 // dt for the coming step: cfl * global min, with the +-2 % hysteresis of
 // timeAdvance.c:144-151 against the previous step's dt (0 before the first
 // step, mhd.c:58), then rkInit's coefficients (timeAdvance.c:20-27).
-__global__ void k_dt_finalize(RunScalars* rs, double cfl)
+__global__ void k_dt_finalize(RunScalars* rs0, double cfl)
 {
+    RunScalars* rs = rs0 + blockIdx.x;  // one block per instance
     if (rs->stop) return;
     const double m = __longlong_as_double((long long)rs->dtmin_bits) * cfl;
     const bool first = rs->step == 0;
@@ -937,9 +958,12 @@ hipError_t launch_rhs(const RhsArgs<T>& a, hipStream_t s)
 }
 
 template <typename T>
-hipError_t launch_rhs_rk(const RhsRkArgs<T>& a, hipStream_t s)
+hipError_t launch_rhs_rk(const RhsRkArgs<T>& a, hipStream_t s, int nb)
 {
-    hipLaunchKernelGGL((k_rhs_rk<T>), pw_grid(a.n / Vec<T>::W, 256), dim3(256), 0, s, a);
+    if (nb < 1 || nb > 65535) return hipErrorInvalidValue;
+    dim3 grid = pw_grid(a.n / Vec<T>::W, 256);
+    grid.y = nb;
+    hipLaunchKernelGGL((k_rhs_rk<T>), grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -973,10 +997,11 @@ hipError_t launch_glm_ch(const T* u, const DtParams& dp, RunScalars* rs, size_t 
 template hipError_t launch_glm_ch<double>(const double*, const DtParams&, RunScalars*, size_t,
                                           hipStream_t);
 
-hipError_t launch_dt_finalize(RunScalars* rs, double cfl, int first, hipStream_t s)
+hipError_t launch_dt_finalize(RunScalars* rs, double cfl, int first, hipStream_t s, int nb)
 {
     (void)first;
-    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, s, rs, cfl);
+    if (nb < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_dt_finalize, dim3(nb), dim3(1), 0, s, rs, cfl);
     return hipGetLastError();
 }
 
@@ -1011,11 +1036,11 @@ hipError_t launch_diag(const T* u, const T* dbx, const T* dby, double* partial, 
 
 // explicit instantiations
 #define MHD2D_INST(T)                                                                        \
-    template hipError_t launch_xsolve<T>(const XSolveArgs<T>&, int, hipStream_t);           \
+    template hipError_t launch_xsolve<T>(const XSolveArgs<T>&, int, hipStream_t, int);      \
     template hipError_t launch_yapply<T>(const YApplyArgs<T>&, hipStream_t);                 \
-    template hipError_t launch_ysweep<T>(const YSweepArgs<T>&, hipStream_t);                 \
+    template hipError_t launch_ysweep<T>(const YSweepArgs<T>&, hipStream_t, int);            \
     template hipError_t launch_rhs<T>(const RhsArgs<T>&, hipStream_t);                       \
-    template hipError_t launch_rhs_rk<T>(const RhsRkArgs<T>&, hipStream_t);                  \
+    template hipError_t launch_rhs_rk<T>(const RhsRkArgs<T>&, hipStream_t, int);             \
     template hipError_t launch_efield<T>(const T*, T*, size_t, hipStream_t);                 \
     template hipError_t launch_divb<T>(const T*, const T*, T*, size_t, hipStream_t);         \
     template hipError_t launch_rk<T>(T*, const T*, T*, double, double, size_t, hipStream_t); \
