@@ -103,7 +103,10 @@ void mhd2d_host_finalize(void);
 /* ===================================================================== */
 typedef struct mhd2d_ctx mhd2d_ctx;
 
-enum { MHD2D_FP64 = 0, MHD2D_FP32 = 1 };
+/* precision: FP64 everywhere; FP32 everywhere; MIXED = FP64 state, filters
+ * and RK accumulation with FP32 derivative solves, derivative arrays and
+ * R_prev (the form that meets 1e-6 against FP64 over 200 steps) */
+enum { MHD2D_FP64 = 0, MHD2D_FP32 = 1, MHD2D_MIXED = 2 };
 
 typedef struct mhd2d_params {
     int nx, ny;             /* grid, powers of two in [16, 4096] */
@@ -115,7 +118,7 @@ typedef struct mhd2d_params {
     double resistivity;     /* must be 0 in the context API */
     double viscosity;       /* must be 0 in the context API */
     int divbclean;          /* must be 0 in the context API */
-    int precision;          /* MHD2D_FP64 or MHD2D_FP32 (state and arithmetic) */
+    int precision;          /* MHD2D_FP64, MHD2D_FP32 or MHD2D_MIXED */
     int xblocks;            /* 0 or 1: x is one block (the reference on one MPI
                                rank); npe > 1: x split into npe blocks sized as
                                decomposeGrid1D (initialize.c:547-569), each

@@ -18,7 +18,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 # MHD2D_LIB selects another build of the same library (tuning variants)
 LIB_PATH = os.environ.get("MHD2D_LIB", os.path.join(PKG, "libmhd2d.so"))
 
-FP64, FP32 = 0, 1
+FP64, FP32, MIXED = 0, 1, 2
 RHO, UX, UY, UZ, BX, BY, BZ, PR = range(8)
 FIELDS = ("rho", "ux", "uy", "uz", "Bx", "By", "Bz", "p")
 

@@ -45,7 +45,8 @@ int fail(int code, const char* what, hipError_t e = hipSuccess)
 
 struct mhd2d_ctx {
     mhd2d_params p{};
-    int elem = 8;
+    int elem = 8;           // bytes per state value
+    int delem = 8;          // bytes per derivative / R_prev value
     int B = 1;              // instances
     size_t n = 0;
     GridTables tb;
@@ -83,11 +84,11 @@ DtParams dt_params(const mhd2d_ctx* c)
     return d;
 }
 
-template <typename T>
+template <typename T, typename TD>
 hipError_t ysweep(mhd2d_ctx* c, int filter, int derivs, int dt_limits)
 {
-    YSweepArgs<T> a;
-    T* dudy = (T*)c->dudy;
+    YSweepArgs<T, TD> a;
+    TD* dudy = (TD*)c->dudy;
     a.u = (T*)c->u;
     a.dudy = dudy;
     a.E = (T*)c->u + 8 * c->n;
@@ -102,15 +103,15 @@ hipError_t ysweep(mhd2d_ctx* c, int filter, int derivs, int dt_limits)
     a.dt_limits = dt_limits;
     a.bs_u = c->su();
     a.bs_d = c->sd();
-    return launch_ysweep<T>(a, c->stream, c->B);
+    return launch_ysweep<T, TD>(a, c->stream, c->B);
 }
 
-template <typename T>
+template <typename T, typename TD>
 hipError_t xderiv(mhd2d_ctx* c)
 {
-    XSolveArgs<T> a;
-    T* u = (T*)c->u;  // 8 state fields and E
-    T* dx = (T*)c->dudx;
+    XSolveArgs<TD, T> a;  // solved in the derivative type
+    T* u = (T*)c->u;      // 8 state fields and E
+    TD* dx = (TD*)c->dudx;
     for (int v = 0; v < 11; v++) a.in[v] = u + v * c->n;
     for (int v = 0; v < 11; v++) a.out[v] = dx + v * c->n;
     a.rs = c->rs;
@@ -121,7 +122,7 @@ hipError_t xderiv(mhd2d_ctx* c)
     a.ny = c->p.ny;
     a.bs_in = c->su();
     a.bs_out = c->sd();
-    return launch_xsolve<T>(a, 11, c->stream, c->B);
+    return launch_xsolve<TD, T>(a, 11, c->stream, c->B);
 }
 
 template <typename T>
@@ -140,18 +141,18 @@ hipError_t xfilter(mhd2d_ctx* c)
     return launch_xsolve<T>(a, 8, c->stream, c->B);
 }
 
-template <typename T>
+template <typename T, typename TD>
 hipError_t rhs_rk(mhd2d_ctx* c, int stage)
 {
-    RhsRkArgs<T> a;
-    T* dx = (T*)c->dudx;
-    T* dy = (T*)c->dudy;
+    RhsRkArgs<T, TD> a;
+    TD* dx = (TD*)c->dudx;
+    TD* dy = (TD*)c->dudy;
     a.u = (T*)c->u;
     a.dudx = dx;
     a.dudy = dy;
     a.dEdx = dx + 8 * c->n;
     a.dEdy = dy + 8 * c->n;
-    a.rprev = (T*)c->rprev;
+    a.rprev = (TD*)c->rprev;
     a.rs = c->rs;
     a.gamma = c->p.gamma;
     a.stage = stage;
@@ -160,11 +161,11 @@ hipError_t rhs_rk(mhd2d_ctx* c, int stage)
     a.bs_u = c->su();
     a.bs_d = c->sd();
     a.bs_r = c->sr();
-    return launch_rhs_rk<T>(a, c->stream, c->B);
+    return launch_rhs_rk<T, TD>(a, c->stream, c->B);
 }
 
 // one RK3 step; when ev is given, an event is recorded after every launch
-template <typename T>
+template <typename T, typename TD>
 hipError_t step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
 {
     hipError_t e;
@@ -172,7 +173,7 @@ hipError_t step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
         if (ev) (void)hipEventRecord(ev[(*nev)++], c->stream);
     };
     for (int s = 0; s < 3; s++) {
-        e = ysweep<T>(c, (s > 0 || c->pending_filter) ? 1 : 0, 1, s == 0 ? 1 : 0);
+        e = ysweep<T, TD>(c, (s > 0 || c->pending_filter) ? 1 : 0, 1, s == 0 ? 1 : 0);
         if (e != hipSuccess) return e;
         mark(0);
         if (s == 0) {
@@ -180,10 +181,10 @@ hipError_t step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
             if (e != hipSuccess) return e;
             mark(0);
         }
-        e = xderiv<T>(c);
+        e = xderiv<T, TD>(c);
         if (e != hipSuccess) return e;
         mark(0);
-        e = rhs_rk<T>(c, s);
+        e = rhs_rk<T, TD>(c, s);
         if (e != hipSuccess) return e;
         mark(0);
         e = xfilter<T>(c);
@@ -194,11 +195,11 @@ hipError_t step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
     return hipSuccess;
 }
 
-template <typename T>
+template <typename T, typename TD>
 hipError_t flush(mhd2d_ctx* c)
 {
     if (!c->pending_filter) return hipSuccess;
-    hipError_t e = ysweep<T>(c, 1, 0, 1);
+    hipError_t e = ysweep<T, TD>(c, 1, 0, 1);
     c->pending_filter = false;
     return e;
 }
@@ -234,12 +235,14 @@ hipError_t do_flush(mhd2d_ctx* c)
         hipError_t e = check_stop(c, 0, nullptr, nullptr);
         if (e != hipSuccess) return e;
     }
-    return c->elem == 8 ? flush<double>(c) : flush<float>(c);
+    if (c->elem == 4) return flush<float, float>(c);
+    return c->delem == 8 ? flush<double, double>(c) : flush<double, float>(c);
 }
 
 hipError_t do_step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
 {
-    return c->elem == 8 ? step<double>(c, ev, nev) : step<float>(c, ev, nev);
+    if (c->elem == 4) return step<float, float>(c, ev, nev);
+    return c->delem == 8 ? step<double, double>(c, ev, nev) : step<double, float>(c, ev, nev);
 }
 
 __global__ void k_to_float(const double* __restrict__ a, float* __restrict__ b, size_t n)
@@ -267,8 +270,9 @@ int check_params(const mhd2d_params* p)
         return fail(MHD2D_EINVAL,
                     "the context API runs the ideal path only (resistivity = viscosity = "
                     "divbclean = 0); use the drop-in entry points for the dissipative terms");
-    if (p->precision != MHD2D_FP64 && p->precision != MHD2D_FP32)
-        return fail(MHD2D_EINVAL, "precision must be MHD2D_FP64 or MHD2D_FP32");
+    if (p->precision != MHD2D_FP64 && p->precision != MHD2D_FP32 &&
+        p->precision != MHD2D_MIXED)
+        return fail(MHD2D_EINVAL, "precision must be MHD2D_FP64, MHD2D_FP32 or MHD2D_MIXED");
     if (p->xblocks < 0 || p->xblocks > p->nx / 4)
         return fail(MHD2D_EINVAL, "xblocks must be in [0, nx/4] (blocks of at least 4 rows)");
     if (p->batch < 0 || p->batch > 65535)
@@ -290,19 +294,20 @@ int mhd2d_create(const mhd2d_params* p, mhd2d_ctx** out)
     if (rc) return rc;
     mhd2d_ctx* c = new mhd2d_ctx();
     c->p = *p;
-    c->elem = p->precision == MHD2D_FP64 ? 8 : 4;
+    c->elem = p->precision == MHD2D_FP32 ? 4 : 8;
+    c->delem = p->precision == MHD2D_FP64 ? 8 : 4;
     c->B = p->batch > 1 ? p->batch : 1;
     c->n = (size_t)p->nx * p->ny;
     c->have_state.assign(c->B, 0);
     c->di.assign(c->B, p->di);
-    hipError_t e = c->tb.build(p->nx, p->ny, p->Lx, p->Ly, c->elem, SchemeAlphas(), p->xblocks);
+    hipError_t e = c->tb.build(p->nx, p->ny, p->Lx, p->Ly, c->delem, SchemeAlphas(), p->xblocks);
     if (e != hipSuccess) {
         delete c;
         return fail(MHD2D_EHIP, "table build", e);
     }
-    const size_t fb = c->n * c->elem * c->B;
+    const size_t fb = c->n * c->elem * c->B, db = c->n * c->delem * c->B;
     c->npartial = 1024;
-    size_t sizes[] = {11 * fb, 11 * fb, 11 * fb, 8 * fb};
+    size_t sizes[] = {11 * fb, 11 * db, 11 * db, 8 * db};
     void** ptrs[] = {&c->u, &c->dudx, &c->dudy, &c->rprev};
     for (int k = 0; k < 4; k++) {
         e = hipMalloc(ptrs[k], sizes[k]);

@@ -46,9 +46,11 @@ struct XTables {
     const unsigned long long* bend;
 };
 
-template <typename T>
+// T: arithmetic and output type; TI: input type (a FP64 state feeding FP32
+// derivative solves in the mixed form)
+template <typename T, typename TI = T>
 struct XSolveArgs {
-    const T* in[16];
+    const TI* in[16];
     T* out[16];
     XStencil st;
     XTables tb;
@@ -108,12 +110,13 @@ struct RunScalars {
     double di;                       // ion inertial length of this instance (Hall CFL)
 };
 
-template <typename T>
+// T: state and arithmetic type; TD: type of the stored y derivatives
+template <typename T, typename TD = T>
 struct YSweepArgs {          // fused y-sweep of the driver
     T* u;                    // [8][nx][ny], in place (filtered)
-    T* dudy;                 // [8][nx][ny]
+    TD* dudy;                // [8][nx][ny]
     T* E;                    // [3][nx][ny]
-    T* dEdy;                 // [3][nx][ny]
+    TD* dEdy;                // [3][nx][ny]
     YTables tb;
     DtParams dp;
     RunScalars* rs;
@@ -146,14 +149,15 @@ struct RhsArgs {
     size_t n;
 };
 
-template <typename T>
+// T: state and arithmetic type; TD: type of the derivatives and of R_prev
+template <typename T, typename TD = T>
 struct RhsRkArgs {           // fused RHS + low-storage RK update (driver)
     T* u;
-    const T* dudx;
-    const T* dudy;
-    const T* dEdx;
-    const T* dEdy;
-    T* rprev;
+    const TD* dudx;
+    const TD* dudy;
+    const TD* dEdx;
+    const TD* dEdy;
+    TD* rprev;
     const RunScalars* rs;
     double gamma;
     int stage;
@@ -165,16 +169,16 @@ struct RhsRkArgs {           // fused RHS + low-storage RK update (driver)
 };
 
 // ---------------------------------------------------------------- launchers
-template <typename T>
-hipError_t launch_xsolve(const XSolveArgs<T>& a, int nv, hipStream_t s, int nb = 1);
+template <typename T, typename TI = T>
+hipError_t launch_xsolve(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int nb = 1);
 template <typename T>
 hipError_t launch_yapply(const YApplyArgs<T>& a, hipStream_t s);
-template <typename T>
-hipError_t launch_ysweep(const YSweepArgs<T>& a, hipStream_t s, int nb = 1);
+template <typename T, typename TD = T>
+hipError_t launch_ysweep(const YSweepArgs<T, TD>& a, hipStream_t s, int nb = 1);
 template <typename T>
 hipError_t launch_rhs(const RhsArgs<T>& a, hipStream_t s);
-template <typename T>
-hipError_t launch_rhs_rk(const RhsRkArgs<T>& a, hipStream_t s, int nb = 1);
+template <typename T, typename TD = T>
+hipError_t launch_rhs_rk(const RhsRkArgs<T, TD>& a, hipStream_t s, int nb = 1);
 template <typename T>
 hipError_t launch_efield(const T* u, T* E, size_t n, hipStream_t s);
 template <typename T>

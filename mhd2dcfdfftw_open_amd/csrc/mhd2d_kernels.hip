@@ -17,6 +17,7 @@
 #include "mhd2d_internal.h"
 
 #include <cmath>
+#include <type_traits>
 
 namespace mhd2d {
 
@@ -97,8 +98,8 @@ __device__ __forceinline__ void block_min_dt(double v, RunScalars* rs, double* r
 #endif
 // BLK: x is split into blocks (tb.bstart / tb.bend), each with the end rows
 // of its own non-cyclic solve; otherwise rows 0 and nx-1 are the only ends.
-template <typename T, int C, int W, int KIND, bool BLK>
-__global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T> a)
+template <typename T, typename TI, int C, int W, int KIND, bool BLK>
+__global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, TI> a)
 {
     const int z = blockIdx.z;  // instance of a batch
     if (a.rs && a.rs[z].stop) return;  // the run stopped (dt < 1e-8): uniform per block
@@ -109,7 +110,7 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T> a
     const int nx = a.nx;
     const size_t ny = (size_t)a.ny;
     const size_t j = (size_t)blockIdx.x * W + c;
-    const T* __restrict__ f = a.in[v] + z * a.bs_in + j;
+    const TI* __restrict__ f = a.in[v] + z * a.bs_in + j;
     T* __restrict__ o = a.out[v] + z * a.bs_out + j;
     const int s = g * C;
 
@@ -120,15 +121,15 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T> a
     // rows s-3 .. s+C+2 of this column; only the 3-row halos can wrap in x
     T w[C + 6];
     {
-        const T* __restrict__ lo = f + (size_t)(s > 0 ? s - 3 : nx - 3) * ny;
-        const T* __restrict__ hi = f + (size_t)(s + C < nx ? s + C : 0) * ny;
-        const T* __restrict__ mid = f + (size_t)s * ny;
+        const TI* __restrict__ lo = f + (size_t)(s > 0 ? s - 3 : nx - 3) * ny;
+        const TI* __restrict__ hi = f + (size_t)(s + C < nx ? s + C : 0) * ny;
+        const TI* __restrict__ mid = f + (size_t)s * ny;
 #pragma unroll
-        for (int k = 0; k < 3; k++) w[k] = lo[k * ny];
+        for (int k = 0; k < 3; k++) w[k] = (T)lo[k * ny];
 #pragma unroll
-        for (int k = 0; k < C; k++) w[k + 3] = mid[k * ny];
+        for (int k = 0; k < C; k++) w[k + 3] = (T)mid[k * ny];
 #pragma unroll
-        for (int k = 0; k < 3; k++) w[C + 3 + k] = hi[k * ny];
+        for (int k = 0; k < 3; k++) w[C + 3 + k] = (T)hi[k * ny];
     }
 
     const XStencil& st = a.st;
@@ -289,48 +290,54 @@ void xsolve_shape(int nx, int elem_bytes, int* chunk, int* width, int* groups)
     *groups = nx / C;
 }
 
-template <typename T, int C, int W, bool BLK>
-static void launch_xsolve_kind(const XSolveArgs<T>& a, dim3 grid, dim3 block, hipStream_t s)
+template <typename T, typename TI, int C, int W, bool BLK>
+static void launch_xsolve_kind(const XSolveArgs<T, TI>& a, dim3 grid, dim3 block, hipStream_t s)
 {
-    switch (a.kind) {
-    case XD1: hipLaunchKernelGGL((k_xsolve<T, C, W, XD1, BLK>), grid, block, 0, s, a); break;
-    case XD2: hipLaunchKernelGGL((k_xsolve<T, C, W, XD2, BLK>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((k_xsolve<T, C, W, XFILT, BLK>), grid, block, 0, s, a); break;
+    if constexpr (std::is_same<T, TI>::value) {
+        switch (a.kind) {
+        case XD1: hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD1, BLK>), grid, block, 0, s, a); break;
+        case XD2: hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD2, BLK>), grid, block, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XFILT, BLK>), grid, block, 0, s, a); break;
+        }
+    } else {
+        // converting inputs: first derivatives of a FP64 state only
+        hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD1, BLK>), grid, block, 0, s, a);
     }
 }
 
-template <typename T, int C, int W>
-static hipError_t launch_xsolve_cw(const XSolveArgs<T>& a, int nv, hipStream_t s, int nb)
+template <typename T, typename TI, int C, int W>
+static hipError_t launch_xsolve_cw(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int nb)
 {
     const int G = a.nx / C;
     dim3 grid(a.ny / W, nv, nb), block(W * G);
     if (a.tb.bstart && a.tb.bend)
-        launch_xsolve_kind<T, C, W, true>(a, grid, block, s);
+        launch_xsolve_kind<T, TI, C, W, true>(a, grid, block, s);
     else
-        launch_xsolve_kind<T, C, W, false>(a, grid, block, s);
+        launch_xsolve_kind<T, TI, C, W, false>(a, grid, block, s);
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_xsolve(const XSolveArgs<T>& a, int nv, hipStream_t s, int nb)
+template <typename T, typename TI>
+hipError_t launch_xsolve(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int nb)
 {
     if (nb < 1 || nb > 65535) return hipErrorInvalidValue;
+    if (!std::is_same<T, TI>::value && a.kind != XD1) return hipErrorInvalidValue;
     int C, W, G;
     xsolve_shape(a.nx, sizeof(T), &C, &W, &G);
     if (a.nx % C || G > 64 || G < 2 || a.ny % W || nv < 1 || nv > 16) return hipErrorInvalidValue;
     if (W == 8) {
         switch (C) {
-        case 8: return launch_xsolve_cw<T, 8, 8>(a, nv, s, nb);
-        case 16: return launch_xsolve_cw<T, 16, 8>(a, nv, s, nb);
-        case 32: return launch_xsolve_cw<T, 32, 8>(a, nv, s, nb);
-        case 64: return launch_xsolve_cw<T, 64, 8>(a, nv, s, nb);
+        case 8: return launch_xsolve_cw<T, TI, 8, 8>(a, nv, s, nb);
+        case 16: return launch_xsolve_cw<T, TI, 16, 8>(a, nv, s, nb);
+        case 32: return launch_xsolve_cw<T, TI, 32, 8>(a, nv, s, nb);
+        case 64: return launch_xsolve_cw<T, TI, 64, 8>(a, nv, s, nb);
         default: return hipErrorInvalidValue;
         }
     }
     switch (C) {
-    case 8: return launch_xsolve_cw<T, 8, 16>(a, nv, s, nb);
-    case 16: return launch_xsolve_cw<T, 16, 16>(a, nv, s, nb);
-    case 32: return launch_xsolve_cw<T, 32, 16>(a, nv, s, nb);
+    case 8: return launch_xsolve_cw<T, TI, 8, 16>(a, nv, s, nb);
+    case 16: return launch_xsolve_cw<T, TI, 16, 16>(a, nv, s, nb);
+    case 32: return launch_xsolve_cw<T, TI, 32, 16>(a, nv, s, nb);
     default: return hipErrorInvalidValue;
     }
 }
@@ -420,8 +427,8 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) k_yapply(YApplyArgs<T
 // Fused y-sweep of one RK stage (driver).  Each thread owns the same 16
 // positions of its row in physical and spectral space, so the filtered state
 // it writes is re-read by the same thread only (E and the CFL limits).
-template <typename T, int N>
-__global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep(YSweepArgs<T> a)
+template <typename T, typename TD, int N>
+__global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep(YSweepArgs<T, TD> a)
 {
     using Y = YShape<T, N>;
     constexpr int TPR = Y::TPR;
@@ -442,8 +449,8 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
     const double2* __restrict__ tw = a.tb.tw;
     T* const u = a.u + z * a.bs_u;
     T* const Ef = a.E + z * a.bs_u;
-    T* const dudy = a.dudy + z * a.bs_d;
-    T* const dEdy = a.dEdy + z * a.bs_d;
+    TD* const dudy = a.dudy + z * a.bs_d;
+    TD* const dEdy = a.dEdy + z * a.bs_d;
 
     // field pairs packed as re/im of one complex line
     constexpr int PA[4] = {RHO, UY, BX, BZ};
@@ -493,13 +500,13 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
             if (a.derivs) {
                 fft_inv<T, N, E>(v, t, lre, lim, tw);
                 if (active) {
-                    T* da = dudy + fa * n + row;
-                    T* db = dudy + fb * n + row;
+                    TD* da = dudy + fa * n + row;
+                    TD* db = dudy + fb * n + row;
 #pragma unroll
                     for (int m = 0; m < E; m++) {
                         const int k = t + m * TPR;
-                        da[k] = v[m].re;
-                        db[k] = v[m].im;
+                        da[k] = (TD)v[m].re;
+                        db[k] = (TD)v[m].im;
                     }
                 }
             }
@@ -533,8 +540,8 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
 #pragma unroll
             for (int m = 0; m < E; m++) {
                 const size_t k = row + t + m * TPR;
-                dEdy[0 * n + k] = v[m].re;
-                dEdy[1 * n + k] = v[m].im;
+                dEdy[0 * n + k] = (TD)v[m].re;
+                dEdy[1 * n + k] = (TD)v[m].im;
             }
         }
 #pragma unroll
@@ -545,7 +552,7 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
         fft_inv<T, N, E>(v, t, lre, lim, tw);
         if (active) {
 #pragma unroll
-            for (int m = 0; m < E; m++) dEdy[2 * n + row + t + m * TPR] = v[m].re;
+            for (int m = 0; m < E; m++) dEdy[2 * n + row + t + m * TPR] = (TD)v[m].re;
         }
     }
 
@@ -566,33 +573,38 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
     }
 }
 
-template <typename T, int N>
-static hipError_t launch_y_n(const YApplyArgs<T>* ya, const YSweepArgs<T>* ys, hipStream_t s,
+template <typename T, typename TD, int N>
+static hipError_t launch_y_n(const YApplyArgs<T>* ya, const YSweepArgs<T, TD>* ys, hipStream_t s,
                              int nb)
 {
     const int nx = ya ? ya->nx : ys->nx;
     using Y = YShape<T, N>;
     dim3 grid((nx + Y::ROWS - 1) / Y::ROWS, ya ? 1 : nb), block(Y::THREADS);
-    if (ya) hipLaunchKernelGGL((k_yapply<T, N>), grid, block, 0, s, *ya);
-    else hipLaunchKernelGGL((k_ysweep<T, N>), grid, block, 0, s, *ys);
+    if constexpr (std::is_same<T, TD>::value) {
+        if (ya) {
+            hipLaunchKernelGGL((k_yapply<T, N>), grid, block, 0, s, *ya);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((k_ysweep<T, TD, N>), grid, block, 0, s, *ys);
     return hipGetLastError();
 }
 
-template <typename T>
-static hipError_t launch_y(int ny, const YApplyArgs<T>* ya, const YSweepArgs<T>* ys, hipStream_t s,
-                           int nb = 1)
+template <typename T, typename TD = T>
+static hipError_t launch_y(int ny, const YApplyArgs<T>* ya, const YSweepArgs<T, TD>* ys,
+                           hipStream_t s, int nb = 1)
 {
     if (nb < 1 || nb > 65535) return hipErrorInvalidValue;
     switch (ny) {
-    case 16: return launch_y_n<T, 16>(ya, ys, s, nb);
-    case 32: return launch_y_n<T, 32>(ya, ys, s, nb);
-    case 64: return launch_y_n<T, 64>(ya, ys, s, nb);
-    case 128: return launch_y_n<T, 128>(ya, ys, s, nb);
-    case 256: return launch_y_n<T, 256>(ya, ys, s, nb);
-    case 512: return launch_y_n<T, 512>(ya, ys, s, nb);
-    case 1024: return launch_y_n<T, 1024>(ya, ys, s, nb);
-    case 2048: return launch_y_n<T, 2048>(ya, ys, s, nb);
-    case 4096: return launch_y_n<T, 4096>(ya, ys, s, nb);
+    case 16: return launch_y_n<T, TD, 16>(ya, ys, s, nb);
+    case 32: return launch_y_n<T, TD, 32>(ya, ys, s, nb);
+    case 64: return launch_y_n<T, TD, 64>(ya, ys, s, nb);
+    case 128: return launch_y_n<T, TD, 128>(ya, ys, s, nb);
+    case 256: return launch_y_n<T, TD, 256>(ya, ys, s, nb);
+    case 512: return launch_y_n<T, TD, 512>(ya, ys, s, nb);
+    case 1024: return launch_y_n<T, TD, 1024>(ya, ys, s, nb);
+    case 2048: return launch_y_n<T, TD, 2048>(ya, ys, s, nb);
+    case 4096: return launch_y_n<T, TD, 4096>(ya, ys, s, nb);
     default: return hipErrorInvalidValue;
     }
 }
@@ -601,13 +613,13 @@ template <typename T>
 hipError_t launch_yapply(const YApplyArgs<T>& a, hipStream_t s)
 {
     if (a.nv < 1 || a.nv > 16) return hipErrorInvalidValue;
-    return launch_y<T>(a.ny, &a, nullptr, s);
+    return launch_y<T, T>(a.ny, &a, nullptr, s);
 }
 
-template <typename T>
-hipError_t launch_ysweep(const YSweepArgs<T>& a, hipStream_t s, int nb)
+template <typename T, typename TD>
+hipError_t launch_ysweep(const YSweepArgs<T, TD>& a, hipStream_t s, int nb)
 {
-    return launch_y<T>(a.ny, nullptr, &a, s, nb);
+    return launch_y<T, TD>(a.ny, nullptr, &a, s, nb);
 }
 
 // ============================================================== pointwise
@@ -718,25 +730,26 @@ __global__ void k_glm_ch(const T* __restrict__ u, DtParams dp, RunScalars* rs, s
     rs->glm_ch = fmax(cmx, cmy);
 }
 
-// 16-byte vector of consecutive cells (2 in FP64, 4 in FP32)
-template <typename T>
-struct alignas(16) Vec {
-    static constexpr int W = 16 / sizeof(T);
+// W consecutive cells of one field, loaded and stored as one vector
+template <typename T, int W>
+struct alignas(sizeof(T) * W) Pack {
     T x[W];
 };
 
-template <typename T>
-__device__ __forceinline__ Vec<T> ldv(const T* __restrict__ p)
+template <int W, typename T>
+__device__ __forceinline__ Pack<T, W> ldp(const T* __restrict__ p)
 {
-    return *reinterpret_cast<const Vec<T>*>(p);
+    return *reinterpret_cast<const Pack<T, W>*>(p);
 }
 
-// fused RHS + RK update; each thread handles one 16-byte vector of cells
-// per field (n is a multiple of the vector width: nx, ny are powers of two)
-template <typename T>
-__global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T> a)
+// fused RHS + RK update; each thread handles W = 16 / sizeof(T) consecutive
+// cells (one 16-byte vector of each state field) per field (n is a multiple
+// of W: nx, ny are powers of two).  Derivatives and R_prev are read and
+// written as TD, the arithmetic and the state update run in T.
+template <typename T, typename TD>
+__global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T, TD> a)
 {
-    constexpr int W = Vec<T>::W;
+    constexpr int W = 16 / sizeof(T);
     const int z = blockIdx.y;  // instance of a batch
     const RunScalars* const rs = a.rs + z;
     if (rs->stop) return;  // the run stopped (dt < 1e-8)
@@ -744,61 +757,69 @@ __global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T> a)
     const T cc = (T)rs->cc[a.stage];
     const T dd = (T)rs->dd[a.stage];
     T* const uz = a.u + z * a.bs_u;
-    T* const rpz = a.rprev + z * a.bs_r;
-    const T* const dxz = a.dudx + z * a.bs_d;
-    const T* const dyz = a.dudy + z * a.bs_d;
-    const T* const exz = a.dEdx + z * a.bs_d;
-    const T* const eyz = a.dEdy + z * a.bs_d;
+    TD* const rpz = a.rprev + z * a.bs_r;
+    const TD* const dxz = a.dudx + z * a.bs_d;
+    const TD* const dyz = a.dudy + z * a.bs_d;
+    const TD* const exz = a.dEdx + z * a.bs_d;
+    const TD* const eyz = a.dEdy + z * a.bs_d;
     const T g = (T)a.gamma;
     const bool use_prev = a.stage > 0;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n / W;
          q += (size_t)gridDim.x * blockDim.x) {
         const size_t p = q * W;
-        Vec<T> u[8], dx[8], dy[8], ex[3], ey[3];
+        Pack<T, W> u[8];
+        Pack<TD, W> dx[8], dy[8], ex[3], ey[3];
 #pragma unroll
         for (int f = 0; f < 8; f++) {
-            u[f] = ldv(uz + f * n + p);
-            dx[f] = ldv(dxz + f * n + p);
-            dy[f] = ldv(dyz + f * n + p);
+            u[f] = ldp<W>(uz + f * n + p);
+            dx[f] = ldp<W>(dxz + f * n + p);
+            dy[f] = ldp<W>(dyz + f * n + p);
         }
 #pragma unroll
         for (int f = 0; f < 3; f++) {
-            ex[f] = ldv(exz + f * n + p);
-            ey[f] = ldv(eyz + f * n + p);
+            ex[f] = ldp<W>(exz + f * n + p);
+            ey[f] = ldp<W>(eyz + f * n + p);
         }
-        Vec<T> R[8];
+        Pack<T, W> R[8];
 #pragma unroll
         for (int w = 0; w < W; w++) {
             // calcRHSInnerPoint, calcRHS.c:85-142
             const T rho = u[RHO].x[w], ux = u[UX].x[w], uy = u[UY].x[w];
             const T bx = u[BX].x[w], by = u[BY].x[w], bz = u[BZ].x[w], pr = u[PR].x[w];
-            const T divU = dx[UX].x[w] + dy[UY].x[w];
-            const T jx = dy[BZ].x[w];
-            const T jy = -dx[BZ].x[w];
-            const T jz = dx[BY].x[w] - dy[BX].x[w];
+            T d_x[8], d_y[8];
+#pragma unroll
+            for (int f = 0; f < 8; f++) {
+                d_x[f] = (T)dx[f].x[w];
+                d_y[f] = (T)dy[f].x[w];
+            }
+            const T divU = d_x[UX] + d_y[UY];
+            const T jx = d_y[BZ];
+            const T jy = -d_x[BZ];
+            const T jz = d_x[BY] - d_y[BX];
             const T ir = (T)1 / rho;
-            R[RHO].x[w] = -rho * divU - ux * dx[RHO].x[w] - uy * dy[RHO].x[w];
-            R[UX].x[w] = -ux * dx[UX].x[w] - uy * dy[UX].x[w] - dx[PR].x[w] * ir +
-                         jy * bz * ir - jz * by * ir;
-            R[UY].x[w] = -ux * dx[UY].x[w] - uy * dy[UY].x[w] - dy[PR].x[w] * ir +
-                         jz * bx * ir - jx * bz * ir;
-            R[UZ].x[w] = -ux * dx[UZ].x[w] - uy * dy[UZ].x[w] + jx * by * ir - jy * bx * ir;
-            R[BX].x[w] = -ey[2].x[w];
-            R[BY].x[w] = ex[2].x[w];
-            R[BZ].x[w] = ey[0].x[w] - ex[1].x[w];
-            R[PR].x[w] = -ux * dx[PR].x[w] - uy * dy[PR].x[w] - g * pr * divU;
+            R[RHO].x[w] = -rho * divU - ux * d_x[RHO] - uy * d_y[RHO];
+            R[UX].x[w] = -ux * d_x[UX] - uy * d_y[UX] - d_x[PR] * ir + jy * bz * ir - jz * by * ir;
+            R[UY].x[w] = -ux * d_x[UY] - uy * d_y[UY] - d_y[PR] * ir + jz * bx * ir - jx * bz * ir;
+            R[UZ].x[w] = -ux * d_x[UZ] - uy * d_y[UZ] + jx * by * ir - jy * bx * ir;
+            R[BX].x[w] = -(T)ey[2].x[w];
+            R[BY].x[w] = (T)ex[2].x[w];
+            R[BZ].x[w] = (T)ey[0].x[w] - (T)ex[1].x[w];
+            R[PR].x[w] = -ux * d_x[PR] - uy * d_y[PR] - g * pr * divU;
         }
 #pragma unroll
         for (int f = 0; f < 8; f++) {
             // rkExecute, timeAdvance.c:31-35 (R_prev is zero in stage 0)
-            Vec<T> rp;
-            if (use_prev) rp = ldv(rpz + f * n + p);
-            Vec<T> o;
+            Pack<TD, W> rp;
+            if (use_prev) rp = ldp<W>(rpz + f * n + p);
+            Pack<T, W> o;
+            Pack<TD, W> rs_out;
 #pragma unroll
-            for (int w = 0; w < W; w++)
-                o.x[w] = u[f].x[w] + cc * R[f].x[w] + dd * (use_prev ? rp.x[w] : (T)0);
-            *reinterpret_cast<Vec<T>*>(uz + f * n + p) = o;
-            if (a.write_rprev) *reinterpret_cast<Vec<T>*>(rpz + f * n + p) = R[f];
+            for (int w = 0; w < W; w++) {
+                o.x[w] = u[f].x[w] + cc * R[f].x[w] + dd * (use_prev ? (T)rp.x[w] : (T)0);
+                rs_out.x[w] = (TD)R[f].x[w];
+            }
+            *reinterpret_cast<Pack<T, W>*>(uz + f * n + p) = o;
+            if (a.write_rprev) *reinterpret_cast<Pack<TD, W>*>(rpz + f * n + p) = rs_out;
         }
     }
 }
@@ -957,13 +978,13 @@ hipError_t launch_rhs(const RhsArgs<T>& a, hipStream_t s)
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_rhs_rk(const RhsRkArgs<T>& a, hipStream_t s, int nb)
+template <typename T, typename TD>
+hipError_t launch_rhs_rk(const RhsRkArgs<T, TD>& a, hipStream_t s, int nb)
 {
     if (nb < 1 || nb > 65535) return hipErrorInvalidValue;
-    dim3 grid = pw_grid(a.n / Vec<T>::W, 256);
+    dim3 grid = pw_grid(a.n / (16 / sizeof(T)), 256);
     grid.y = nb;
-    hipLaunchKernelGGL((k_rhs_rk<T>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_rhs_rk<T, TD>), grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1036,11 +1057,11 @@ hipError_t launch_diag(const T* u, const T* dbx, const T* dby, double* partial, 
 
 // explicit instantiations
 #define MHD2D_INST(T)                                                                        \
-    template hipError_t launch_xsolve<T>(const XSolveArgs<T>&, int, hipStream_t, int);      \
+    template hipError_t launch_xsolve<T, T>(const XSolveArgs<T, T>&, int, hipStream_t, int); \
     template hipError_t launch_yapply<T>(const YApplyArgs<T>&, hipStream_t);                 \
-    template hipError_t launch_ysweep<T>(const YSweepArgs<T>&, hipStream_t, int);            \
+    template hipError_t launch_ysweep<T, T>(const YSweepArgs<T, T>&, hipStream_t, int);     \
     template hipError_t launch_rhs<T>(const RhsArgs<T>&, hipStream_t);                       \
-    template hipError_t launch_rhs_rk<T>(const RhsRkArgs<T>&, hipStream_t, int);             \
+    template hipError_t launch_rhs_rk<T, T>(const RhsRkArgs<T, T>&, hipStream_t, int);      \
     template hipError_t launch_efield<T>(const T*, T*, size_t, hipStream_t);                 \
     template hipError_t launch_divb<T>(const T*, const T*, T*, size_t, hipStream_t);         \
     template hipError_t launch_rk<T>(T*, const T*, T*, double, double, size_t, hipStream_t); \
@@ -1053,5 +1074,13 @@ hipError_t launch_diag(const T* u, const T* dbx, const T* dby, double* partial, 
 
 MHD2D_INST(double)
 MHD2D_INST(float)
+
+// mixed form: FP64 state, FP32 derivatives and R_prev
+template hipError_t launch_xsolve<float, double>(const XSolveArgs<float, double>&, int,
+                                                 hipStream_t, int);
+template hipError_t launch_ysweep<double, float>(const YSweepArgs<double, float>&, hipStream_t,
+                                                 int);
+template hipError_t launch_rhs_rk<double, float>(const RhsRkArgs<double, float>&, hipStream_t,
+                                                 int);
 
 }  // namespace mhd2d
