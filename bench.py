@@ -1,13 +1,19 @@
 #!/usr/bin/env python3
-"""RK3 cell-updates/s of the fused FP64 device path on one MI355X.
+"""RK3 cell-updates/s on one MI355X: plain FP64 against the mixed-precision
+and batched forms, with their rel-L2 error against FP64.
 
-Workload (default): 2048 x 2048 periodic box [0, 2pi)^2, 8 fields, FP64,
-seeded broadband Alfvenic turbulence on rho = p = 1, B0 = (1, 0, 0)
+Workload of `value` (default): 2048 x 2048 periodic box [0, 2pi)^2, 8 fields,
+FP64, seeded broadband Alfvenic turbulence on rho = p = 1, B0 = (1, 0, 0)
 (1 <= |k| <= 16, |k|^(-5/3), rms 0.1, dB from A_z), gamma = 5/3, cfl = 0.3,
 Hall CFL term with d_i = 0.02, 200 RK3 steps.  A step is one full RK3 step
 (3 stages with derivatives, RHS, update, x and y filters, and the CFL dt).
 
-    python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu]
+Beside it (object "forms"): the mixed form on the same workload, and the
+batched ensemble of 64 linear waves on 512 x 512 grids with d_i drawn per
+instance, in mixed precision and in FP32, each with its speed-up over FP64
+and its per-field rel-L2 error against an FP64 run of the same steps.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu] [--no-forms]
 
 Prints ONE JSON line.  The state is resident in HBM before the timed region;
 time is taken with hipEvents on the stream the kernels run on, bracketed by
@@ -25,24 +31,28 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
-B_ALG_FP64 = 1680.0        # algorithmic bytes per cell-update, 2-sweep schedule, FP64
-WORDS = 8                  # bytes per FP64 word
+# algorithmic bytes per cell-update of the canonical 2-sweep schedule
+# (70 words per stage, 3 stages): all FP64, all FP32, and the mixed form
+# (32 state words per stage at 8 B, 38 derivative / R_prev words at 4 B)
+B_ALG = {"fp64": 1680.0, "fp32": 840.0, "mixed": 3 * (32 * 8 + 38 * 4)}
 
 
-def kernel_alg_bytes(nx, ny, elem=8):
+def kernel_alg_bytes(nx, ny, elem=8, delem=None, batch=1):
     """Algorithmic HBM bytes per launch of each kernel class (inputs read once,
-    outputs written once), per cell times the cells of one launch."""
-    n = nx * ny
+    outputs written once), per cell times the cells of one launch.  elem:
+    bytes of a state (and E) value, delem: of a derivative / R_prev value."""
+    n = nx * ny * batch
+    s, d = elem, (elem if delem is None else delem)
     return {
-        # y-sweep: read u (8), write filtered u (8), dudy (8), E (3), dEdy (3)
-        "ysweep": 30 * n * elem,
+        # y-sweep: read u (8), write filtered u (8) and E (3), dudy (8), dEdy (3)
+        "ysweep": n * (19 * s + 11 * d),
         # x-derivative: read u (8) and E (3), write d/dx of both (11)
-        "xderiv": 22 * n * elem,
-        # RHS + RK, average of the three stages: read u, dudx, dudy (30) plus
-        # R_prev in stages 2-3 (8); write u (8) and R_prev in stages 1-2 (8)
-        "rhs_rk": (46 + 54 + 46) / 3.0 * n * elem,
+        "xderiv": n * (11 * s + 11 * d),
+        # RHS + RK, average of the three stages: read u (8) and dudx, dudy
+        # (22), R_prev in stages 2-3 (8); write u (8), R_prev in stages 1-2 (8)
+        "rhs_rk": n * (16 * s + 22 * d + (16.0 / 3.0) * 2 * d),
         # x-filter in place: read 8, write 8
-        "xfilter": 16 * n * elem,
+        "xfilter": n * 16 * s,
     }
 
 
@@ -137,6 +147,79 @@ def load_traffic(kernel):
         return None
 
 
+def _precision(m, name):
+    return {"fp64": m.FP64, "fp32": m.FP32, "mixed": m.MIXED}[name]
+
+
+def _bytes(name):
+    return {"fp64": (8, 8), "fp32": (4, 4), "mixed": (8, 4)}[name]
+
+
+def run_single(m, u0, nx, ny, prec, steps, warmup, graph, hall, di):
+    """(cell-updates/s, ms, context) of one run; the context holds the state
+    after warmup + steps"""
+    ctx = m.Context(nx, ny, hall=hall, di=di, precision=_precision(m, prec))
+    ctx.set_state(u0)
+    ctx.run(warmup, graph=graph)
+    ctx.sync()
+    ms = ctx.time_run(steps, graph=graph)
+    ctx.sync()
+    return nx * ny * steps / (ms * 1e-3), ms, ctx
+
+
+def ensemble_states(m, nx, ny, count, seed):
+    """64-instance ensemble: rho = p = 1, B0 = (1, 0, 0) with one seeded
+    linear wave (integer kx, ky in [1, 8], amplitude 1e-4, random phase) per
+    instance; d_i uniform in [0, 0.2]"""
+    rng = np.random.default_rng(seed)
+    dis = rng.uniform(0.0, 0.2, count)
+    return [m.linear_wave(nx, ny, seed=seed + b, amp=1e-4) for b in range(count)], dis
+
+
+def run_batch(m, states, dis, nx, ny, prec, steps, warmup, hall=1):
+    B = len(states)
+    ctx = m.Context(nx, ny, hall=hall, precision=_precision(m, prec), batch=B)
+    for b in range(B):
+        ctx.set_di(b, float(dis[b]))
+        ctx.set_state(states[b], inst=b)
+    ctx.run(warmup, graph=True)
+    ctx.sync()
+    ms = ctx.time_run(steps, graph=True)
+    ctx.sync()
+    return B * nx * ny * steps / (ms * 1e-3), ms, ctx
+
+
+def field_errors(a, b):
+    """per-field rel-L2 of a against b (fields of b below 1e-6 of its rms
+    measured in absolute L2 over that rms)"""
+    scale = np.sqrt(np.mean(b ** 2)) or 1.0
+    out = []
+    for v in range(b.shape[0]):
+        nb = np.sqrt(np.mean(b[v] ** 2))
+        d = np.sqrt(np.mean((a[v] - b[v]) ** 2))
+        out.append(float(d / scale if nb < 1e-6 * scale else d / nb))
+    return out
+
+
+def roofline_of(ctx, nx, ny, elem, delem, batch, profile_steps, copy_gbps):
+    prof = ctx.profile_run(profile_steps)
+    ctx.sync()
+    alg = kernel_alg_bytes(nx, ny, elem, delem, batch)
+    share = {k: prof[k][0] * prof[k][1] for k in alg}
+    dom = max(share, key=share.get)
+    dom_ms = prof[dom][0]
+    achieved = alg[dom] / (dom_ms * 1e-3)
+    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK, "avg_launch_ms": dom_ms,
+            "alg_bytes_per_launch": alg[dom],
+            "frac_vs_copy_probe": achieved / (copy_gbps * 1e9)}
+    kernels = {k: {"avg_ms": prof[k][0], "launches_per_step": prof[k][1] / profile_steps,
+                   "GB/s": (alg[k] / (prof[k][0] * 1e-3) / 1e9) if k in alg and prof[k][0] > 0
+                   else None}
+               for k in prof}
+    return roof, kernels
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,8 +227,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--ny", type=int, default=2048)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "mixed"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the mixed and batched measurements")
+    ap.add_argument("--ens-steps", type=int, default=500,
+                    help="steps of the 64 x 512^2 ensemble")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--copy-probe", action="store_true",
@@ -173,42 +260,21 @@ def main():
 
     nx, ny = args.nx, args.ny
     hall, di = 1, 0.02
-    prec = m.FP64 if args.precision == "fp64" else m.FP32
-    elem = 8 if prec == m.FP64 else 4
+    graph = bool(args.graph)
+    elem, delem = _bytes(args.precision)
     u0 = m.turbulence(nx, ny, seed=42, rms=0.1, kmin=1, kmax=16)
-
-    ctx = m.Context(nx, ny, hall=hall, di=di, precision=prec)
-    ctx.set_state(u0)
-    ctx.run(args.warmup, graph=bool(args.graph))
-    ctx.sync()
-    ms = ctx.time_run(args.steps, graph=bool(args.graph))
-    ctx.sync()
-    cells = nx * ny * args.steps
-    value = cells / (ms * 1e-3)
-
-    # per-kernel device time on the kernels' own stream (hipEvents)
-    prof = ctx.profile_run(args.profile_steps)
-    ctx.sync()
-    alg = kernel_alg_bytes(nx, ny, elem)
-    share = {k: prof[k][0] * prof[k][1] for k in alg}
-    dom = max(share, key=share.get)
-    dom_ms = prof[dom][0]
-    achieved = alg[dom] / (dom_ms * 1e-3)
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                "traffic": load_traffic(dom), "avg_launch_ms": dom_ms,
-                "alg_bytes_per_launch": alg[dom]}
-    kernels = {k: {"avg_ms": prof[k][0], "launches_per_step": prof[k][1] / args.profile_steps,
-                   "GB/s": (alg[k] / (prof[k][0] * 1e-3) / 1e9) if k in alg and prof[k][0] > 0
-                   else None}
-               for k in prof}
-    energy, divb = ctx.diagnostics()
     copy_gbps = m.copy_probe(COPY_BYTES, 10)
-    roofline["frac_vs_copy_probe"] = achieved / (copy_gbps * 1e9)
+
+    value, ms, ctx = run_single(m, u0, nx, ny, args.precision, args.steps, args.warmup, graph,
+                                hall, di)
+    roofline, kernels = roofline_of(ctx, nx, ny, elem, delem, 1, args.profile_steps, copy_gbps)
+    roofline["traffic"] = load_traffic(roofline["kernel"]) if args.precision == "fp64" else None
+    energy, divb = ctx.diagnostics()
+    # the state after warmup + steps (profile_run advanced it further: rerun)
+    ref_state = None
 
     out = {
-        "metric": "RK3 cell-updates/s (plain FP64 path)" if prec == m.FP64 else
-                  "RK3 cell-updates/s (FP32 path)",
+        "metric": "RK3 cell-updates/s: plain FP64 vs mixed/batched at rel-L2 <=1e-6 vs FP64",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -218,20 +284,72 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == m.FP64 else "f32",
+        "dtype": {"fp64": "f64", "fp32": "f32", "mixed": "f64 state, f32 derivatives"}[
+            args.precision],
         "data": "synthetic (seeded turbulence initial state, seed 42)",
         "config": {"workload": f"hall-mhd2d turbulence {nx}x{ny}, 8 fields, "
                                f"{args.precision}, d_i={di}, 200-step run",
-                   "nx": nx, "ny": ny, "graph": bool(args.graph)},
+                   "nx": nx, "ny": ny, "graph": graph, "precision": args.precision},
         "roofline": roofline,
-        "step_roofline_frac": value * (B_ALG_FP64 * elem / 8) / HBM_PEAK,
+        "step_roofline_frac": value * B_ALG[args.precision] / HBM_PEAK,
         "kernels": kernels,
         "copy_probe_GBps": copy_gbps,
         "diagnostics": {"energy": energy, "max_divb": divb},
     }
+    ctx.close()
+
+    if not args.no_forms and args.precision == "fp64":
+        forms = {}
+        nsteps = args.warmup + args.steps
+        # FP64 reference state after warmup + steps, for the error columns
+        _, _, ref = run_single(m, u0, nx, ny, "fp64", args.steps, args.warmup, graph, hall, di)
+        ref_state = ref.state()
+        ref.close()
+        for prec in ("mixed", "fp32"):
+            v, ms_p, c = run_single(m, u0, nx, ny, prec, args.steps, args.warmup, graph, hall,
+                                    di)
+            e = field_errors(c.state(), ref_state)
+            pe, pd = _bytes(prec)
+            roof, kern = roofline_of(c, nx, ny, pe, pd, 1, args.profile_steps, copy_gbps)
+            forms[f"{prec}_{nx}"] = {
+                "workload": f"turbulence {nx}x{ny}, {prec}", "cell_updates_per_s": v,
+                "ms_per_step": ms_p / args.steps, "speedup_vs_fp64": v / value,
+                "rel_l2_vs_fp64_max": max(e), "rel_l2_vs_fp64": e,
+                "after_steps": nsteps, "meets_1e-6": bool(max(e) <= 1e-6),
+                "step_roofline_frac": v * B_ALG[prec] / HBM_PEAK, "roofline": roof,
+                "kernels": kern}
+            c.close()
+        # batched ensemble: 64 x 512^2 linear waves, d_i per instance
+        enx = eny = 512
+        states, dis = ensemble_states(m, enx, eny, 64, 42)
+        res = {}
+        for prec in ("fp64", "mixed", "fp32"):
+            v, ms_p, c = run_batch(m, states, dis, enx, eny, prec, args.ens_steps, args.warmup)
+            res[prec] = (v, ms_p, c)
+        vref, _, cref = res["fp64"]
+        checked = (0, 21, 42, 63)
+        # errors first: profiling advances the runs
+        ref_states = {b: cref.state(b) for b in checked}
+        errs_of = {prec: [max(field_errors(res[prec][2].state(b), ref_states[b]))
+                          for b in checked] for prec in res}
+        for prec in ("fp64", "mixed", "fp32"):
+            v, ms_p, c = res[prec]
+            errs = errs_of[prec]
+            pe, pd = _bytes(prec)
+            roof, kern = roofline_of(c, enx, eny, pe, pd, 64, args.profile_steps, copy_gbps)
+            forms[f"batched64x512_{prec}"] = {
+                "workload": f"64 x {enx}x{eny} linear waves, d_i ~ U[0, 0.2], {prec}, "
+                            "one launch per kernel and stage, hipGraph step",
+                "cell_updates_per_s": v, "ms_per_step": ms_p / args.ens_steps,
+                "speedup_vs_fp64_2048": v / value, "speedup_vs_fp64_batched": v / vref,
+                "rel_l2_vs_fp64_max": max(errs), "after_steps": args.warmup + args.ens_steps,
+                "instances_checked": list(checked), "meets_1e-6": bool(max(errs) <= 1e-6),
+                "step_roofline_frac": v * B_ALG[prec] / HBM_PEAK, "roofline": roof}
+        for prec in res:
+            res[prec][2].close()
+        out["forms"] = forms
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nx, ny, u0, hall, di)
-    ctx.close()
     print(json.dumps(out))
 
 

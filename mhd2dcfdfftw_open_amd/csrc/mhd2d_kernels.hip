@@ -124,12 +124,18 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
         const TI* __restrict__ lo = f + (size_t)(s > 0 ? s - 3 : nx - 3) * ny;
         const TI* __restrict__ hi = f + (size_t)(s + C < nx ? s + C : 0) * ny;
         const TI* __restrict__ mid = f + (size_t)s * ny;
+        // A narrowing load (FP64 state into an FP32 derivative solve) first
+        // subtracts the column's row-0 value in TI: the operator annihilates
+        // constants, and the rounding then scales with the variation along
+        // the line instead of with a large uniform background.
+        TI c0 = 0;
+        if constexpr (!std::is_same<T, TI>::value) c0 = f[0];
 #pragma unroll
-        for (int k = 0; k < 3; k++) w[k] = (T)lo[k * ny];
+        for (int k = 0; k < 3; k++) w[k] = (T)(lo[k * ny] - c0);
 #pragma unroll
-        for (int k = 0; k < C; k++) w[k + 3] = (T)mid[k * ny];
+        for (int k = 0; k < C; k++) w[k + 3] = (T)(mid[k * ny] - c0);
 #pragma unroll
-        for (int k = 0; k < 3; k++) w[C + 3 + k] = (T)hi[k * ny];
+        for (int k = 0; k < 3; k++) w[C + 3 + k] = (T)(hi[k * ny] - c0);
     }
 
     const XStencil& st = a.st;
@@ -321,6 +327,8 @@ template <typename T, typename TI>
 hipError_t launch_xsolve(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int nb)
 {
     if (nb < 1 || nb > 65535) return hipErrorInvalidValue;
+    // converting inputs: first derivatives only (they annihilate the
+    // subtracted constant; the filter does not)
     if (!std::is_same<T, TI>::value && a.kind != XD1) return hipErrorInvalidValue;
     int C, W, G;
     xsolve_shape(a.nx, sizeof(T), &C, &W, &G);

@@ -85,3 +85,18 @@ def test_mixed_diagnostics_in_fp64(m):
     e, _ = ctx.diagnostics()
     dx = dy = 2 * np.pi / nx
     assert e == pytest.approx(orc.energy(u0, dx, dy), rel=1e-13)
+
+
+@pytest.mark.parametrize("seed", [42, 43])
+def test_mixed_small_amplitude_wave_within_1e6(m, seed):
+    # a 1e-4 linear wave on rho = p = 1, B0 = x: FP32 rounding of the
+    # background would swamp the wave's x-derivatives without the constant
+    # subtraction of the narrowing x-solve loads
+    nx = ny = 256
+    print(f"[mixed] linear wave seed {seed}", flush=True)
+    u0 = m.linear_wave(nx, ny, seed=seed, amp=1e-4)
+    ref = _run(m, u0, nx, ny, m.FP64, 200, hall=1, di=0.1)
+    mix = _run(m, u0, nx, ny, m.MIXED, 200, hall=1, di=0.1)
+    e = field_errors(mix.state(), ref.state())
+    print(f"[mixed] wave: max rel-L2 {e.max():.2e}")
+    assert e.max() <= MIXED_TOL, e
