@@ -8,10 +8,11 @@ FP64, seeded broadband Alfvenic turbulence on rho = p = 1, B0 = (1, 0, 0)
 Hall CFL term with d_i = 0.02, 200 RK3 steps.  A step is one full RK3 step
 (3 stages with derivatives, RHS, update, x and y filters, and the CFL dt).
 
-Beside it (object "forms"): the mixed form on the same workload, and the
-batched ensemble of 64 linear waves on 512 x 512 grids with d_i drawn per
-instance, in mixed precision and in FP32, each with its speed-up over FP64
-and its per-field rel-L2 error against an FP64 run of the same steps.
+Beside it (object "forms"): the mixed form and FP32 on the same workload;
+the batched ensemble of 64 linear waves on 512 x 512 grids with d_i drawn per
+instance; and the uneven ensemble of 96 turbulence runs on four grid sizes;
+each in FP64, mixed and FP32, with its speed-up over FP64 and its per-field
+rel-L2 error against an FP64 run of the same steps.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu] [--no-forms]
 
@@ -174,6 +175,20 @@ def ensemble_states(m, nx, ny, count, seed):
     rng = np.random.default_rng(seed)
     dis = rng.uniform(0.0, 0.2, count)
     return [m.linear_wave(nx, ny, seed=seed + b, amp=1e-4) for b in range(count)], dis
+
+
+def uneven_runs(m, count, seed):
+    """uneven ensemble: grid sizes drawn from {256x256, 512x256, 1024x1024,
+    2048x512}, turbulence with rms ~ U[0.05, 0.2], d_i ~ U[0, 0.1]"""
+    rng = np.random.default_rng(seed)
+    sizes = [(256, 256), (512, 256), (1024, 1024), (2048, 512)]
+    runs, states = [], []
+    for k in range(count):
+        nx, ny = sizes[rng.integers(0, 4)]
+        runs.append(dict(nx=nx, ny=ny, hall=1, di=float(rng.uniform(0, 0.1))))
+        states.append(m.turbulence(nx, ny, seed=seed + k, rms=float(rng.uniform(0.05, 0.2)),
+                                   kmax=16))
+    return runs, states
 
 
 def run_batch(m, states, dis, nx, ny, prec, steps, warmup, hall=1):
@@ -347,6 +362,37 @@ def main():
                 "step_roofline_frac": v * B_ALG[prec] / HBM_PEAK, "roofline": roof}
         for prec in res:
             res[prec][2].close()
+        # uneven ensemble: 96 runs, sizes drawn from four shapes, per-run rms, d_i
+        runs, ustates = uneven_runs(m, 96, 42)
+        ures = {}
+        for prec in ("fp64", "mixed", "fp32"):
+            E = m.Ensemble(runs, precision=_precision(m, prec))
+            for k in range(len(runs)):
+                E.set_state(k, ustates[k])
+            E.run(args.warmup, graph=True)
+            E.sync()
+            ms_p = E.time_run(args.steps, graph=True)
+            E.sync()
+            cells = sum(r["nx"] * r["ny"] for r in runs)
+            ures[prec] = (cells * args.steps / (ms_p * 1e-3), ms_p, E)
+        checked = (0, 31, 64, 95)
+        refs = {k: ures["fp64"][2].state(k) for k in checked}
+        for prec in ("fp64", "mixed", "fp32"):
+            v, ms_p, E = ures[prec]
+            errs = [max(field_errors(E.state(k), refs[k])) for k in checked]
+            forms[f"uneven96_{prec}"] = {
+                "workload": "96 turbulence runs, sizes drawn from 256x256, 512x256, "
+                            "1024x1024, 2048x512, rms ~ U[0.05, 0.2], d_i ~ U[0, 0.1], "
+                            f"{prec}; one batched context per size, one hipGraph per step "
+                            "forking into the four size groups",
+                "cells": sum(r["nx"] * r["ny"] for r in runs), "groups": E.groups,
+                "cell_updates_per_s": v, "ms_per_step": ms_p / args.steps,
+                "speedup_vs_fp64_2048": v / value, "speedup_vs_fp64_uneven": v / ures["fp64"][0],
+                "rel_l2_vs_fp64_max": max(errs), "after_steps": args.warmup + args.steps,
+                "instances_checked": list(checked), "meets_1e-6": bool(max(errs) <= 1e-6),
+                "step_roofline_frac": v * B_ALG[prec] / HBM_PEAK}
+        for prec in ures:
+            ures[prec][2].close()
         out["forms"] = forms
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nx, ny, u0, hall, di)

@@ -190,6 +190,26 @@ size_t mhd2d_device_bytes(const mhd2d_ctx *c);
 int mhd2d_copy_probe(size_t bytes, int iters, double *gbps);
 const char *mhd2d_last_error(void);
 
+/* Ensembles of runs on grids of different sizes.  p[0..count) gives each
+ * run's parameters (the batch field is ignored); runs whose parameters agree
+ * except for di are grouped into one batched context per grid shape, and a
+ * step advances every group: eagerly on one stream per group, or as one
+ * hipGraph that forks into the groups' streams and joins them again.  All
+ * entries must share one precision.  inst indexes p. */
+typedef struct mhd2d_ens mhd2d_ens;
+int mhd2d_ens_create(const mhd2d_params *p, int count, mhd2d_ens **out);
+void mhd2d_ens_destroy(mhd2d_ens *e);
+int mhd2d_ens_groups(const mhd2d_ens *e);
+int mhd2d_ens_set_state(mhd2d_ens *e, int inst, const double *uu);
+int mhd2d_ens_get_state(mhd2d_ens *e, int inst, double *uu);
+int mhd2d_ens_run(mhd2d_ens *e, int nsteps, int use_graph);
+int mhd2d_ens_sync(mhd2d_ens *e);
+int mhd2d_ens_get_scalars(mhd2d_ens *e, int inst, double *dt, double *time,
+                          long long *steps);
+int mhd2d_ens_diagnostics(mhd2d_ens *e, int inst, double *energy, double *max_divb);
+/* wall time of nsteps of every group (hipEvents around the fork / join) */
+int mhd2d_ens_time_run(mhd2d_ens *e, int nsteps, int use_graph, double *ms);
+
 /* ===================================================================== */
 /* 3. the reference's run configuration, output files and run loop       */
 /* ===================================================================== */

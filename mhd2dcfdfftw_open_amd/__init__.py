@@ -129,6 +129,16 @@ def lib():
         "mhd2d_get_scalars_at": (i, [vp, i, p, p, ctypes.POINTER(ll)]),
         "mhd2d_diagnostics_at": (i, [vp, i, p, p]),
         "mhd2d_stopped_at": (i, [vp, i, p]),
+        "mhd2d_ens_create": (i, [ctypes.POINTER(Params), i, ctypes.POINTER(vp)]),
+        "mhd2d_ens_destroy": (None, [vp]),
+        "mhd2d_ens_groups": (i, [vp]),
+        "mhd2d_ens_set_state": (i, [vp, i, p]),
+        "mhd2d_ens_get_state": (i, [vp, i, p]),
+        "mhd2d_ens_run": (i, [vp, i, i]),
+        "mhd2d_ens_sync": (i, [vp]),
+        "mhd2d_ens_get_scalars": (i, [vp, i, p, p, ctypes.POINTER(ll)]),
+        "mhd2d_ens_diagnostics": (i, [vp, i, p, p]),
+        "mhd2d_ens_time_run": (i, [vp, i, i, p]),
         "mhd2d_read_input": (i, [ctypes.c_char_p, ctypes.POINTER(Input)]),
         "mhd2d_write_input": (i, [ctypes.c_char_p, ctypes.POINTER(Input)]),
         "mhd2d_write_grid": (i, [ctypes.c_char_p, i, i, d, d]),
@@ -288,6 +298,66 @@ class Context:
         if r > 1:
             _check(r)
         return bool(r), d.value
+
+
+class Ensemble:
+    """Runs on grids of different sizes advanced together
+    (mhd2d_ens_create ... mhd2d_ens_destroy).  `runs` is a list of dicts with
+    nx, ny and optionally Lx, Ly, gamma, cfl, hall, di, npe."""
+
+    def __init__(self, runs, precision=FP64):
+        arr = (Params * len(runs))()
+        for k, r in enumerate(runs):
+            arr[k] = Params(r["nx"], r["ny"], r.get("Lx", 2 * np.pi), r.get("Ly", 2 * np.pi),
+                            r.get("gamma", 5.0 / 3.0), r.get("cfl", 0.3), r.get("hall", 0),
+                            r.get("di", 0.0), 0.0, 0.0, 0, precision, r.get("npe", 1), 0)
+        self.shapes = [(r["nx"], r["ny"]) for r in runs]
+        h = ctypes.c_void_p()
+        _check(lib().mhd2d_ens_create(arr, len(runs), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mhd2d_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def groups(self):
+        return lib().mhd2d_ens_groups(self.h)
+
+    def set_state(self, inst, uu):
+        uu = np.ascontiguousarray(uu, dtype=np.float64)
+        assert uu.shape == (8,) + self.shapes[inst]
+        _check(lib().mhd2d_ens_set_state(self.h, inst, _ptr(uu)))
+
+    def state(self, inst):
+        uu = np.zeros((8,) + self.shapes[inst])
+        _check(lib().mhd2d_ens_get_state(self.h, inst, _ptr(uu)))
+        return uu
+
+    def run(self, nsteps, graph=False):
+        _check(lib().mhd2d_ens_run(self.h, nsteps, int(graph)))
+
+    def sync(self):
+        _check(lib().mhd2d_ens_sync(self.h))
+
+    def scalars(self, inst):
+        dt, t, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        _check(lib().mhd2d_ens_get_scalars(self.h, inst, _ptr1(dt), _ptr1(t), ctypes.byref(n)))
+        return dt.value, t.value, n.value
+
+    def diagnostics(self, inst):
+        e, m = ctypes.c_double(), ctypes.c_double()
+        _check(lib().mhd2d_ens_diagnostics(self.h, inst, _ptr1(e), _ptr1(m)))
+        return e.value, m.value
+
+    def time_run(self, nsteps, graph=False):
+        ms = ctypes.c_double()
+        _check(lib().mhd2d_ens_time_run(self.h, nsteps, int(graph), _ptr1(ms)))
+        return ms.value
 
 
 def _ptr1(c_double):

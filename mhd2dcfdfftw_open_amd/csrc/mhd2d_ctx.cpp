@@ -719,3 +719,224 @@ int mhd2d_profile_run(mhd2d_ctx* c, int nsteps, double* out_ms, int* out_launche
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ ensembles
+struct mhd2d_ens {
+    std::vector<mhd2d_ctx*> groups;
+    std::vector<std::pair<int, int>> where;  // instance -> (group, index in group)
+    hipStream_t master = nullptr;
+    std::vector<hipEvent_t> ev;             // [0] fork, [1 + g] join of group g
+    hipGraphExec_t graph = nullptr;
+};
+
+namespace {
+
+bool same_group(const mhd2d_params& a, const mhd2d_params& b)
+{
+    return a.nx == b.nx && a.ny == b.ny && a.Lx == b.Lx && a.Ly == b.Ly && a.gamma == b.gamma &&
+           a.cfl == b.cfl && a.hall == b.hall && a.resistivity == b.resistivity &&
+           a.viscosity == b.viscosity && a.divbclean == b.divbclean &&
+           a.precision == b.precision && a.xblocks == b.xblocks;
+}
+
+// the master stream waits for every group's queued work
+hipError_t join_into_master(mhd2d_ens* e)
+{
+    for (size_t g = 0; g < e->groups.size(); g++) {
+        hipError_t r = hipEventRecord(e->ev[1 + g], e->groups[g]->stream);
+        if (r == hipSuccess) r = hipStreamWaitEvent(e->master, e->ev[1 + g], 0);
+        if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+}
+
+// every group stream waits for the master stream's queued work
+hipError_t fork_from_master(mhd2d_ens* e)
+{
+    hipError_t r = hipEventRecord(e->ev[0], e->master);
+    for (auto* c : e->groups)
+        if (r == hipSuccess) r = hipStreamWaitEvent(c->stream, e->ev[0], 0);
+    return r;
+}
+
+hipError_t ens_step_all(mhd2d_ens* e)
+{
+    for (auto* c : e->groups) {
+        hipError_t r = do_step(c, nullptr, nullptr);
+        if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhd2d_ens_create(const mhd2d_params* p, int count, mhd2d_ens** out)
+{
+    if (!p || !out || count < 1) return fail(MHD2D_EINVAL, "bad argument");
+    *out = nullptr;
+    for (int i = 0; i < count; i++) {
+        if (int rc = check_params(&p[i])) return rc;
+        if (p[i].precision != p[0].precision)
+            return fail(MHD2D_EINVAL, "all runs of an ensemble share one precision");
+    }
+    // group the runs: first member's parameters, di per run
+    std::vector<std::vector<int>> members;
+    for (int i = 0; i < count; i++) {
+        size_t g = 0;
+        for (; g < members.size(); g++)
+            if (same_group(p[members[g][0]], p[i])) break;
+        if (g == members.size()) members.push_back({});
+        members[g].push_back(i);
+    }
+    mhd2d_ens* e = new mhd2d_ens();
+    e->where.resize(count);
+    for (size_t g = 0; g < members.size(); g++) {
+        mhd2d_params q = p[members[g][0]];
+        q.batch = (int)members[g].size();
+        mhd2d_ctx* c = nullptr;
+        int rc = mhd2d_create(&q, &c);
+        if (rc) {
+            mhd2d_ens_destroy(e);
+            return rc;
+        }
+        e->groups.push_back(c);
+        for (size_t k = 0; k < members[g].size(); k++) {
+            const int i = members[g][k];
+            e->where[i] = {(int)g, (int)k};
+            rc = mhd2d_set_di(c, (int)k, p[i].di);
+            if (rc) {
+                mhd2d_ens_destroy(e);
+                return rc;
+            }
+        }
+    }
+    hipError_t r = hipStreamCreateWithFlags(&e->master, hipStreamNonBlocking);
+    e->ev.resize(1 + e->groups.size(), nullptr);
+    for (auto& v : e->ev)
+        if (r == hipSuccess) r = hipEventCreateWithFlags(&v, hipEventDisableTiming);
+    if (r != hipSuccess) {
+        mhd2d_ens_destroy(e);
+        return fail(MHD2D_EHIP, "ensemble setup", r);
+    }
+    *out = e;
+    return MHD2D_OK;
+}
+
+void mhd2d_ens_destroy(mhd2d_ens* e)
+{
+    if (!e) return;
+    if (e->master) (void)hipStreamSynchronize(e->master);
+    if (e->graph) (void)hipGraphExecDestroy(e->graph);
+    for (auto v : e->ev)
+        if (v) (void)hipEventDestroy(v);
+    for (auto* c : e->groups) mhd2d_destroy(c);
+    if (e->master) (void)hipStreamDestroy(e->master);
+    delete e;
+}
+
+int mhd2d_ens_groups(const mhd2d_ens* e) { return e ? (int)e->groups.size() : 0; }
+
+int mhd2d_ens_set_state(mhd2d_ens* e, int inst, const double* uu)
+{
+    if (!e || inst < 0 || inst >= (int)e->where.size()) return fail(MHD2D_EINVAL, "bad argument");
+    HIPCHK(hipStreamSynchronize(e->master));
+    auto w = e->where[inst];
+    return mhd2d_set_state_at(e->groups[w.first], w.second, uu);
+}
+
+int mhd2d_ens_get_state(mhd2d_ens* e, int inst, double* uu)
+{
+    if (!e || inst < 0 || inst >= (int)e->where.size()) return fail(MHD2D_EINVAL, "bad argument");
+    HIPCHK(hipStreamSynchronize(e->master));
+    auto w = e->where[inst];
+    return mhd2d_get_state_at(e->groups[w.first], w.second, uu);
+}
+
+int mhd2d_ens_get_scalars(mhd2d_ens* e, int inst, double* dt, double* time, long long* steps)
+{
+    if (!e || inst < 0 || inst >= (int)e->where.size()) return fail(MHD2D_EINVAL, "bad argument");
+    HIPCHK(hipStreamSynchronize(e->master));
+    auto w = e->where[inst];
+    return mhd2d_get_scalars_at(e->groups[w.first], w.second, dt, time, steps);
+}
+
+int mhd2d_ens_diagnostics(mhd2d_ens* e, int inst, double* energy, double* max_divb)
+{
+    if (!e || inst < 0 || inst >= (int)e->where.size()) return fail(MHD2D_EINVAL, "bad argument");
+    HIPCHK(hipStreamSynchronize(e->master));
+    auto w = e->where[inst];
+    return mhd2d_diagnostics_at(e->groups[w.first], w.second, energy, max_divb);
+}
+
+int mhd2d_ens_sync(mhd2d_ens* e)
+{
+    if (!e) return fail(MHD2D_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(e->master));
+    for (auto* c : e->groups)
+        if (int rc = mhd2d_sync(c)) return rc;
+    return MHD2D_OK;
+}
+
+// Steps are queued on the master stream's timeline: the groups fork from it
+// and join back, so events on the master stream bracket all of them.
+int mhd2d_ens_run(mhd2d_ens* e, int nsteps, int use_graph)
+{
+    if (!e || nsteps < 0) return fail(MHD2D_EINVAL, "bad argument");
+    for (auto* c : e->groups)
+        if (int rc = check_ready(c)) return rc;
+    if (nsteps == 0) return MHD2D_OK;
+    HIPCHK(join_into_master(e));  // earlier work queued on the group streams
+    int k = 0;
+    bool fresh = false;
+    for (auto* c : e->groups) fresh = fresh || !c->pending_filter;
+    if (!use_graph || fresh) {
+        // eager steps (and the first step, which has no pending y filter)
+        const int ne = use_graph ? 1 : nsteps;
+        HIPCHK(fork_from_master(e));
+        for (; k < ne; k++) HIPCHK(ens_step_all(e));
+        HIPCHK(join_into_master(e));
+    }
+    if (k < nsteps) {
+        if (!e->graph) {
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(e->master, hipStreamCaptureModeThreadLocal));
+            hipError_t r = fork_from_master(e);
+            if (r == hipSuccess) r = ens_step_all(e);
+            if (r == hipSuccess) r = join_into_master(e);
+            hipError_t r2 = hipStreamEndCapture(e->master, &g);
+            if (r != hipSuccess) return fail(MHD2D_EHIP, "ensemble capture", r);
+            HIPCHK(r2);
+            HIPCHK(hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0));
+            HIPCHK(hipGraphDestroy(g));
+        }
+        for (; k < nsteps; k++) HIPCHK(hipGraphLaunch(e->graph, e->master));
+    }
+    // later work on the group streams follows the steps
+    HIPCHK(fork_from_master(e));
+    return MHD2D_OK;
+}
+
+int mhd2d_ens_time_run(mhd2d_ens* e, int nsteps, int use_graph, double* ms)
+{
+    if (!e || !ms) return fail(MHD2D_EINVAL, "null argument");
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(join_into_master(e));
+    HIPCHK(hipStreamSynchronize(e->master));
+    HIPCHK(hipEventRecord(a, e->master));
+    int rc = mhd2d_ens_run(e, nsteps, use_graph);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(b, e->master));
+    HIPCHK(hipEventSynchronize(b));
+    float f = 0;
+    HIPCHK(hipEventElapsedTime(&f, a, b));
+    *ms = f;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return MHD2D_OK;
+}
+
+}  // extern "C"
