@@ -7,8 +7,10 @@
 // elements at positions t + m*N/E, m = 0..E-1 ("strided" layout), so loads
 // and stores of whole lines are coalesced and pointwise work between a
 // forward and an inverse transform needs no data exchange.  Between passes
-// the line goes through LDS as separate real and imaginary arrays, padded by
-// one word every E so that the E-strided writes are free of bank conflicts.
+// the line goes through LDS as separate real and imaginary arrays whose
+// word index is XOR-swizzled (fft_swz) so that both the E-strided writes and
+// the unit-stride reads of every pass are free of bank conflicts, with no
+// padding.
 //
 // Twiddles come from a table of exp(-2 pi i t/N) (FP64, host-computed in
 // long double): per butterfly four entries are read and the rest formed by
@@ -149,13 +151,23 @@ struct FFTShape {
     static constexpr int PE = M / LE, REM = M % LE;
     static constexpr int NP = PE + (REM ? 1 : 0);
     static constexpr int TPR = N / E;          // threads per line
-    static constexpr int LDS = N + N / E;      // padded words per array
+    static constexpr int LDS = N;              // words per array
     static constexpr int radix(int p) { return p < PE ? E : (1 << REM); }
 };
 
-// one pad word every E words keeps the E-strided writes conflict-free
+// LDS word of line position p: a bijection of [0, N) that keeps the bits
+// above the low five (E = 8, FP64) or six (E = 16, FP32) and mixes higher
+// bits into them.  Checked exhaustively for every exchange of N = 16..4096:
+// each group of lanes served together by the LDS (16 / 32 lanes of 8-byte
+// words, 32 / 64 lanes of 4-byte words) hits distinct banks, for the
+// E-strided writes (position stride E within a group) as for the
+// unit-stride reads.
 template <int E>
-__device__ __forceinline__ int fft_pad(int p) { return p + p / E; }
+__device__ __forceinline__ int fft_swz(int p)
+{
+    if constexpr (E == 8) return p ^ ((p >> 3) & 31);
+    else return p ^ (((p >> 3) ^ (p >> 4)) & 63);
+}
 
 // v[r] *= W^(r k STEP), r = 1..R-1, from four table reads (idx1 = k STEP)
 // and at most three complex products per factor
@@ -226,7 +238,7 @@ __device__ __forceinline__ void fft_pass(cx<T> (&v)[E], int t, T* lre, T* lim,
         const int base = (t / NS) * NS * R + (t & (NS - 1));
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int q = fft_pad<E>(base + r * NS);
+            const int q = fft_swz<E>(base + r * NS);
             lre[q] = v[r].re;
             lim[q] = v[r].im;
         }
@@ -236,7 +248,7 @@ __device__ __forceinline__ void fft_pass(cx<T> (&v)[E], int t, T* lre, T* lim,
             const int j = t + b * TPR;
 #pragma unroll
             for (int r = 0; r < R2; r++) {
-                const int q = fft_pad<E>(j + r * (N / R2));
+                const int q = fft_swz<E>(j + r * (N / R2));
                 v[b * R2 + r] = cx<T>{lre[q], lim[q]};
             }
         }

@@ -114,9 +114,12 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
     T* __restrict__ o = a.out[v] + z * a.bs_out + j;
     const int s = g * C;
 
-    __shared__ T sh_in[W * 64];
-    __shared__ T sh_fw[W * 64];
-    __shared__ T sh_bw[W * 64];
+    // per-chunk carries, row g of W (+1 pad: the scans read them with
+    // consecutive lanes one row apart, which the pad spreads over the banks)
+    constexpr int WP = W + 1;
+    __shared__ T sh_in[WP * 64];
+    __shared__ T sh_fw[WP * 64];
+    __shared__ T sh_bw[WP * 64];
 
     // rows s-3 .. s+C+2 of this column; only the 3-row halos can wrap in x
     T w[C + 6];
@@ -203,11 +206,11 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
 #pragma unroll
         for (int k = 0; k < C; k++) y = w[k] - (T)L[k] * y;
     }
-    sh_in[threadIdx.x] = y;
+    sh_in[g * WP + c] = y;
     __syncthreads();
     {
         const int c2i = threadIdx.x / G, g2 = threadIdx.x % G;
-        T B = sh_in[g2 * W + c2i];
+        T B = sh_in[g2 * WP + c2i];
         T A = (T)a.tb.pf[g2];
         for (int d = 1; d < G; d <<= 1) {
             const T Ap = __shfl_up(A, d, G);
@@ -217,10 +220,10 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
                 A = A * Ap;
             }
         }
-        sh_fw[g2 * W + c2i] = B;  // y at the last row of chunk g2
+        sh_fw[g2 * WP + c2i] = B;  // y at the last row of chunk g2
     }
     __syncthreads();
-    y = (g > 0) ? sh_fw[(g - 1) * W + c] : (T)0;
+    y = (g > 0) ? sh_fw[(g - 1) * WP + c] : (T)0;
     if (interior) {
 #pragma unroll
         for (int k = 0; k < C; k++) {
@@ -244,12 +247,12 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
 #pragma unroll
         for (int k = C - 1; k >= 0; k--) x = w[k] * (T)ID[k] + (T)Q[k] * x;
     }
-    sh_in[threadIdx.x] = x;
+    sh_in[g * WP + c] = x;
     __syncthreads();
     {
         const int c2i = threadIdx.x / G, g2 = G - 1 - (threadIdx.x % G);
         const int gr = threadIdx.x % G;  // position in the reversed scan
-        T B = sh_in[g2 * W + c2i];
+        T B = sh_in[g2 * WP + c2i];
         T A = (T)a.tb.qb[g2];
         for (int d = 1; d < G; d <<= 1) {
             const T Ap = __shfl_up(A, d, G);
@@ -259,10 +262,10 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
                 A = A * Ap;
             }
         }
-        sh_bw[g2 * W + c2i] = B;  // x at the first row of chunk g2
+        sh_bw[g2 * WP + c2i] = B;  // x at the first row of chunk g2
     }
     __syncthreads();
-    x = (g < G - 1) ? sh_bw[(g + 1) * W + c] : (T)0;
+    x = (g < G - 1) ? sh_bw[(g + 1) * WP + c] : (T)0;
     if (interior) {
 #pragma unroll
         for (int k = C - 1; k >= 0; k--) {
