@@ -235,6 +235,58 @@ def roofline_of(ctx, nx, ny, elem, delem, batch, profile_steps, copy_gbps):
     return roof, kernels
 
 
+def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50, chunks=4):
+    """4096^2: mixed precision with the x-derivative as the tridiagonal solve
+    and as the dense A^-1 B through split-BF16 MFMA GEMMs (3 and 6
+    products), and plain FP32; per-field rel-L2 against FP64 every 50 steps
+    for 200 steps.  Each variant is timed over its 4 x 50 steps (hipEvents)."""
+    u0 = m.turbulence(n, n, seed=42, rms=0.1, kmin=1, kmax=16)
+    variants = [("fp64", m.FP64, 0), ("mixed", m.MIXED, 0), ("mixed_gemm6", m.MIXED, 6),
+                ("mixed_gemm3", m.MIXED, 3), ("fp32", m.FP32, 0)]
+    ref_states, out = [], {}
+    ref_value = None
+    for name, prec, xg in variants:
+        ctx = m.Context(n, n, hall=hall, di=di, precision=prec, xgemm=xg)
+        ctx.set_state(u0)
+        ctx.run(1, graph=True)   # the first step (no pending filter) and the capture
+        ctx.sync()
+        ctx.set_state(u0)
+        ms, errs = 0.0, []
+        for k in range(chunks):
+            ms += ctx.time_run(chunk, graph=True)
+            ctx.sync()
+            st = ctx.state()
+            if name == "fp64":
+                ref_states.append(st)
+            else:
+                errs.append(max(field_errors(st, ref_states[k])))
+        v = n * n * chunk * chunks / (ms * 1e-3)
+        if name == "fp64":
+            ref_value = v
+        pe, pd = (8, 8) if prec == m.FP64 else ((4, 4) if prec == m.FP32 else (8, 4))
+        roof, kern = roofline_of(ctx, n, n, pe, pd, 1, profile_steps, copy_gbps)
+        rec = {"workload": f"turbulence {n}x{n}, {name}", "cell_updates_per_s": v,
+               "ms_per_step": ms / (chunk * chunks), "speedup_vs_fp64_2048": v / value2048,
+               "speedup_vs_fp64_4096": v / ref_value, "kernels": kern, "roofline": roof,
+               "step_roofline_frac": v * B_ALG["fp64" if name == "fp64" else
+                                                ("fp32" if name == "fp32" else "mixed")] / HBM_PEAK}
+        if errs:
+            rec["rel_l2_vs_fp64_every_50"] = errs
+            rec["rel_l2_vs_fp64_max"] = max(errs)
+            rec["meets_1e-6"] = bool(max(errs) <= 1e-6)
+        if xg:
+            # MFMA work of one x-derivative launch (11 fields) over its time
+            flops = ctx.xgemm_panels() * 64 * 16 * 2 * n * 11 * xg
+            t = kern["xderiv"]["avg_ms"] * 1e-3
+            rec["xderiv_mfma"] = {"bf16_flops_per_launch": flops, "TFLOP/s": flops / t / 1e12,
+                                  "frac_of_2500_TFLOPs": flops / t / 2.5e15,
+                                  "panels": ctx.xgemm_panels(),
+                                  "dense_fraction": ctx.xgemm_panels() / ((n / 64) * (n / 16))}
+        out[f"{name}_{n}"] = rec
+        ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -393,6 +445,7 @@ def main():
                 "step_roofline_frac": v * B_ALG[prec] / HBM_PEAK}
         for prec in ures:
             ures[prec][2].close()
+        forms.update(c3_forms(m, value, args.profile_steps, copy_gbps, hall, di))
         out["forms"] = forms
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nx, ny, u0, hall, di)

@@ -130,6 +130,12 @@ typedef struct mhd2d_params {
                                own state, d_i (mhd2d_set_di), dt, time and
                                stop.  Instance 0 is the one the calls without
                                an _at suffix address */
+    int xgemm;              /* 0: x-derivatives by the compact tridiagonal
+                               solve; 3 or 6: by the dense operator A^-1 B
+                               split into BF16 planes and applied as MFMA GEMMs
+                               with FP32 accumulation, 3 or 6 split products
+                               (precision FP32 or MIXED, xblocks <= 1, nx and
+                               ny >= 64) */
 } mhd2d_params;
 
 /* status codes */
@@ -176,6 +182,14 @@ int mhd2d_get_scalars_at(mhd2d_ctx *c, int inst, double *dt, double *time,
                          long long *steps);
 int mhd2d_diagnostics_at(mhd2d_ctx *c, int inst, double *energy, double *max_divb);
 int mhd2d_stopped_at(mhd2d_ctx *c, int inst, double *dt_stop);
+/* d/dx of nv host fields [nv][nx][ny] (nv <= 11) with the context's own
+ * x-derivative (tridiagonal or GEMM, in its precision), into FP64 host out;
+ * the run's state is untouched (temporary device buffers) */
+int mhd2d_xderiv(mhd2d_ctx *c, int nv, const double *in, double *out);
+/* GEMM form: number of non-zero 64 x 16 panels of the split operator over
+ * all 64-row tiles (the MFMA work is panels * 64 * 16 * 2 * ny flops per
+ * field and split product); 0 for the tridiagonal form */
+long long mhd2d_xgemm_panels(const mhd2d_ctx *c);
 /* wall time of nsteps on the context's stream (hipEvents), in ms */
 int mhd2d_time_run(mhd2d_ctx *c, int nsteps, int use_graph, double *ms);
 /* average device time per launch of each kernel class over nsteps, in ms:

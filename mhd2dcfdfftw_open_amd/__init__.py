@@ -67,7 +67,7 @@ class Params(ctypes.Structure):
                 ("hall", ctypes.c_int), ("di", ctypes.c_double),
                 ("resistivity", ctypes.c_double), ("viscosity", ctypes.c_double),
                 ("divbclean", ctypes.c_int), ("precision", ctypes.c_int),
-                ("xblocks", ctypes.c_int), ("batch", ctypes.c_int)]
+                ("xblocks", ctypes.c_int), ("batch", ctypes.c_int), ("xgemm", ctypes.c_int)]
 
 
 def lib():
@@ -129,6 +129,8 @@ def lib():
         "mhd2d_get_scalars_at": (i, [vp, i, p, p, ctypes.POINTER(ll)]),
         "mhd2d_diagnostics_at": (i, [vp, i, p, p]),
         "mhd2d_stopped_at": (i, [vp, i, p]),
+        "mhd2d_xderiv": (i, [vp, i, p, p]),
+        "mhd2d_xgemm_panels": (ctypes.c_longlong, [vp]),
         "mhd2d_ens_create": (i, [ctypes.POINTER(Params), i, ctypes.POINTER(vp)]),
         "mhd2d_ens_destroy": (None, [vp]),
         "mhd2d_ens_groups": (i, [vp]),
@@ -224,12 +226,13 @@ class Context:
     of `batch` independent runs of that grid advanced together."""
 
     def __init__(self, nx, ny, Lx=2 * np.pi, Ly=2 * np.pi, gamma=5.0 / 3.0, cfl=0.3,
-                 hall=0, di=0.0, precision=FP64, npe=1, batch=1):
+                 hall=0, di=0.0, precision=FP64, npe=1, batch=1, xgemm=0):
         """npe > 1 splits x into npe blocks as a reference run on npe MPI
-        ranks does (decomposeGrid1D, initialize.c:547-569)."""
+        ranks does (decomposeGrid1D, initialize.c:547-569); xgemm 3 or 6
+        takes x-derivatives as split-BF16 MFMA GEMMs with the dense A^-1 B."""
         self.nx, self.ny, self.batch = nx, ny, batch
         self.params = Params(nx, ny, Lx, Ly, gamma, cfl, hall, di, 0.0, 0.0, 0, precision, npe,
-                             batch)
+                             batch, xgemm)
         h = ctypes.c_void_p()
         _check(lib().mhd2d_create(ctypes.byref(self.params), ctypes.byref(h)))
         self.h = h
@@ -251,6 +254,17 @@ class Context:
         uu = np.zeros((8, self.nx, self.ny))
         _check(lib().mhd2d_get_state_at(self.h, inst, _ptr(uu)))
         return uu
+
+    def xderiv(self, fields):
+        """d/dx of host fields [nv][nx][ny] with this context's x-derivative"""
+        f = np.ascontiguousarray(fields, dtype=np.float64)
+        assert f.shape[1:] == (self.nx, self.ny)
+        out = np.zeros_like(f)
+        _check(lib().mhd2d_xderiv(self.h, f.shape[0], _ptr(f), _ptr(out)))
+        return out
+
+    def xgemm_panels(self):
+        return lib().mhd2d_xgemm_panels(self.h)
 
     def set_di(self, inst, di):
         _check(lib().mhd2d_set_di(self.h, inst, di))
@@ -310,7 +324,8 @@ class Ensemble:
         for k, r in enumerate(runs):
             arr[k] = Params(r["nx"], r["ny"], r.get("Lx", 2 * np.pi), r.get("Ly", 2 * np.pi),
                             r.get("gamma", 5.0 / 3.0), r.get("cfl", 0.3), r.get("hall", 0),
-                            r.get("di", 0.0), 0.0, 0.0, 0, precision, r.get("npe", 1), 0)
+                            r.get("di", 0.0), 0.0, 0.0, 0, precision, r.get("npe", 1), 0,
+                            r.get("xgemm", 0))
         self.shapes = [(r["nx"], r["ny"]) for r in runs]
         h = ctypes.c_void_p()
         _check(lib().mhd2d_ens_create(arr, len(runs), ctypes.byref(h)))

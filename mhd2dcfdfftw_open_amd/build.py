@@ -31,7 +31,7 @@ EXTRA = os.environ.get("MHD2D_EXTRA_FLAGS", "").split()
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", *EXTRA,
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-result", "-I", os.path.join(ROOT, "include")]
-CORE_SOURCES = ["mhd2d_kernels.hip", "mhd2d_tables.cpp", "mhd2d_ctx.cpp",
+CORE_SOURCES = ["mhd2d_kernels.hip", "mhd2d_xgemm.hip", "mhd2d_tables.cpp", "mhd2d_ctx.cpp",
                 "mhd2d_dropin.cpp", "mhd2d_ic.cpp", "mhd2d_io.cpp"]
 # the storage of the reference globals and what needs it (standalone library)
 GLOBAL_SOURCES = ["mhd2d_globals.cpp", "mhd2d_driver.cpp"]

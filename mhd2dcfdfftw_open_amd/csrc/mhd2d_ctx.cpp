@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mhd2d.h"
@@ -50,6 +51,7 @@ struct mhd2d_ctx {
     int B = 1;              // instances
     size_t n = 0;
     GridTables tb;
+    XGemmOp xg;             // built when params.xgemm != 0
     hipStream_t stream = nullptr;
     void* u = nullptr;      // [B][11][nx][ny]: 8 state fields, then E (3)
     void* dudx = nullptr;   // [B][11][nx][ny]: 8 state derivatives + 3 of E
@@ -106,23 +108,56 @@ hipError_t ysweep(mhd2d_ctx* c, int filter, int derivs, int dt_limits)
     return launch_ysweep<T, TD>(a, c->stream, c->B);
 }
 
+// x-derivative of nv fields: in (state type, stride bs_in per instance) ->
+// out (derivative type); the tridiagonal solve or the MFMA GEMM form
 template <typename T, typename TD>
-hipError_t xderiv(mhd2d_ctx* c)
+hipError_t xderiv_fields(mhd2d_ctx* c, const T* in, TD* out, int nv, size_t bs_in,
+                         size_t bs_out, int nb, const RunScalars* rs)
 {
-    XSolveArgs<TD, T> a;  // solved in the derivative type
-    T* u = (T*)c->u;      // 8 state fields and E
-    TD* dx = (TD*)c->dudx;
-    for (int v = 0; v < 11; v++) a.in[v] = u + v * c->n;
-    for (int v = 0; v < 11; v++) a.out[v] = dx + v * c->n;
-    a.rs = c->rs;
+    if constexpr (std::is_same<TD, float>::value) {
+        if (c->p.xgemm) {
+            XGemmArgs<T> g;
+            for (int v = 0; v < nv; v++) {
+                g.in[v] = in + v * c->n;
+                g.out[v] = out + v * c->n;
+            }
+            g.mh = c->xg.mh;
+            g.mm = c->xg.mm;
+            g.ml = c->xg.ml;
+            g.klist = c->xg.klist_d;
+            g.kcount = c->xg.kcount_d;
+            g.kmax = c->xg.kmax;
+            g.nsplit = c->xg.nsplit;
+            g.rs = rs;
+            g.nx = c->p.nx;
+            g.ny = c->p.ny;
+            g.bs_in = bs_in;
+            g.bs_out = bs_out;
+            return launch_xgemm<T>(g, nv, c->stream, nb);
+        }
+    }
+    XSolveArgs<TD, T> a;
+    for (int v = 0; v < nv; v++) {
+        a.in[v] = in + v * c->n;
+        a.out[v] = out + v * c->n;
+    }
+    a.rs = rs;
     a.st = c->tb.st[XD1];
     a.tb = c->tb.xt[XD1];
     a.kind = XD1;
     a.nx = c->p.nx;
     a.ny = c->p.ny;
-    a.bs_in = c->su();
-    a.bs_out = c->sd();
-    return launch_xsolve<TD, T>(a, 11, c->stream, c->B);
+    a.bs_in = bs_in;
+    a.bs_out = bs_out;
+    return launch_xsolve<TD, T>(a, nv, c->stream, nb);
+}
+
+template <typename T, typename TD>
+hipError_t xderiv(mhd2d_ctx* c)
+{
+    // 8 state fields and E, solved in the derivative type
+    return xderiv_fields<T, TD>(c, (const T*)c->u, (TD*)c->dudx, 11, c->su(), c->sd(), c->B,
+                                c->rs);
 }
 
 template <typename T>
@@ -277,6 +312,12 @@ int check_params(const mhd2d_params* p)
         return fail(MHD2D_EINVAL, "xblocks must be in [0, nx/4] (blocks of at least 4 rows)");
     if (p->batch < 0 || p->batch > 65535)
         return fail(MHD2D_EINVAL, "batch must be in [0, 65535]");
+    if (p->xgemm != 0 && p->xgemm != 3 && p->xgemm != 6)
+        return fail(MHD2D_EINVAL, "xgemm must be 0, 3 or 6");
+    if (p->xgemm && (p->precision == MHD2D_FP64 || p->xblocks > 1 || p->nx < 64 || p->ny < 64))
+        return fail(MHD2D_EINVAL,
+                    "the GEMM x-derivative needs precision FP32 or MIXED, one x block and "
+                    "nx, ny >= 64");
     return MHD2D_OK;
 }
 
@@ -301,10 +342,12 @@ int mhd2d_create(const mhd2d_params* p, mhd2d_ctx** out)
     c->have_state.assign(c->B, 0);
     c->di.assign(c->B, p->di);
     hipError_t e = c->tb.build(p->nx, p->ny, p->Lx, p->Ly, c->delem, SchemeAlphas(), p->xblocks);
+    if (e == hipSuccess && p->xgemm) e = c->xg.build(c->tb, p->xgemm);
     if (e != hipSuccess) {
         delete c;
         return fail(MHD2D_EHIP, "table build", e);
     }
+    c->bytes += c->xg.bytes;
     const size_t fb = c->n * c->elem * c->B, db = c->n * c->delem * c->B;
     c->npartial = 1024;
     size_t sizes[] = {11 * fb, 11 * db, 11 * db, 8 * db};
@@ -658,6 +701,59 @@ int mhd2d_diagnostics_at(mhd2d_ctx* c, int inst, double* energy, double* max_div
     if (energy) *energy = e * c->tb.dx * c->tb.dy;
     if (max_divb) *max_divb = m;
     return MHD2D_OK;
+}
+
+}  // extern "C"
+
+namespace {
+template <typename T, typename TD>
+int xderiv_host(mhd2d_ctx* c, int nv, const double* in, double* out)
+{
+    const size_t cnt = (size_t)nv * c->n;
+    double* dd = nullptr;
+    T* tin = nullptr;
+    TD* tout = nullptr;
+    hipError_t e = hipMalloc(&dd, cnt * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&tin, cnt * sizeof(T));
+    if (e == hipSuccess) e = hipMalloc(&tout, cnt * sizeof(TD));
+    if (e == hipSuccess) e = hipMemcpyAsync(dd, in, cnt * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        if constexpr (std::is_same<T, double>::value)
+            e = hipMemcpyAsync(tin, dd, cnt * 8, hipMemcpyDeviceToDevice, c->stream);
+        else {
+            hipLaunchKernelGGL(k_to_float, dim3(2048), dim3(256), 0, c->stream, dd, tin, cnt);
+            e = hipGetLastError();
+        }
+    }
+    if (e == hipSuccess) e = xderiv_fields<T, TD>(c, tin, tout, nv, 0, 0, 1, nullptr);
+    if (e == hipSuccess) {
+        if constexpr (std::is_same<TD, double>::value)
+            e = hipMemcpyAsync(dd, tout, cnt * 8, hipMemcpyDeviceToDevice, c->stream);
+        else {
+            hipLaunchKernelGGL(k_to_double, dim3(2048), dim3(256), 0, c->stream, tout, dd, cnt);
+            e = hipGetLastError();
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dd, cnt * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dd);
+    (void)hipFree(tin);
+    (void)hipFree(tout);
+    if (e != hipSuccess) return fail(MHD2D_EHIP, "x-derivative", e);
+    return MHD2D_OK;
+}
+}  // namespace
+
+extern "C" {
+
+long long mhd2d_xgemm_panels(const mhd2d_ctx* c) { return c && c->p.xgemm ? c->xg.panels_total : 0; }
+
+int mhd2d_xderiv(mhd2d_ctx* c, int nv, const double* in, double* out)
+{
+    if (!c || !in || !out || nv < 1 || nv > 11) return fail(MHD2D_EINVAL, "bad argument");
+    if (c->elem == 4) return xderiv_host<float, float>(c, nv, in, out);
+    if (c->delem == 8) return xderiv_host<double, double>(c, nv, in, out);
+    return xderiv_host<double, float>(c, nv, in, out);
 }
 
 int mhd2d_time_run(mhd2d_ctx* c, int nsteps, int use_graph, double* ms)

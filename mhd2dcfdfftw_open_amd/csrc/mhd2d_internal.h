@@ -62,6 +62,26 @@ struct XSolveArgs {
     size_t bs_in = 0, bs_out = 0;
 };
 
+// x-derivative through the dense operator M = A^-1 B split into BF16 planes
+// (mh, mm, ml: row-major nx x nx), applied as an MFMA GEMM over the y-lines;
+// klist[t * kmax ...] lists the 16-column panels of row tile t (64 rows)
+// that hold non-zeros, kcount[t] how many
+template <typename TI>
+struct XGemmArgs {
+    const TI* in[16];
+    float* out[16];
+    const unsigned short* mh = nullptr;
+    const unsigned short* mm = nullptr;
+    const unsigned short* ml = nullptr;
+    const int* klist = nullptr;
+    const int* kcount = nullptr;
+    int kmax = 0;
+    int nsplit = 3;
+    const RunScalars* rs = nullptr;
+    int nx = 0, ny = 0;
+    size_t bs_in = 0, bs_out = 0;
+};
+
 // ---------------------------------------------------------------- y operators
 // Spectral multipliers, precomputed in FP64 for k = 0..ny-1 (full complex
 // spectrum, signed wavenumbers):
@@ -175,6 +195,8 @@ template <typename T>
 hipError_t launch_yapply(const YApplyArgs<T>& a, hipStream_t s);
 template <typename T, typename TD = T>
 hipError_t launch_ysweep(const YSweepArgs<T, TD>& a, hipStream_t s, int nb = 1);
+template <typename TI>
+hipError_t launch_xgemm(const XGemmArgs<TI>& a, int nv, hipStream_t s, int nb = 1);
 template <typename T>
 hipError_t launch_rhs(const RhsArgs<T>& a, hipStream_t s);
 template <typename T, typename TD = T>

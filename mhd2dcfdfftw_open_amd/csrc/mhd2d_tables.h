@@ -47,4 +47,26 @@ private:
     void* blob_ = nullptr;  // one allocation holding every table
 };
 
+// The first-derivative operator M = A^-1 B of one grid (one x block), formed
+// in FP64 and stored as BF16 planes with the non-zero panel lists, for the
+// MFMA form of the x-derivative.  nsplit 3 or 6 products.
+class XGemmOp {
+public:
+    XGemmOp() = default;
+    ~XGemmOp();
+    XGemmOp(const XGemmOp&) = delete;
+    XGemmOp& operator=(const XGemmOp&) = delete;
+    hipError_t build(const GridTables& tb, int nsplit);
+    void release();
+
+    int nx = 0, ny = 0, nsplit = 0, kmax = 0;
+    long long panels_total = 0;  // non-zero 64 x 16 panels over all row tiles
+    size_t bytes = 0;
+    const unsigned short *mh = nullptr, *mm = nullptr, *ml = nullptr;
+    const int *klist_d = nullptr, *kcount_d = nullptr;
+
+private:
+    void* blob_ = nullptr;
+};
+
 }  // namespace mhd2d

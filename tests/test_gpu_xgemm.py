@@ -1,0 +1,117 @@
+"""The x-derivative as the dense operator A^-1 B applied with MFMA
+(split BF16, FP32 accumulation) against the compact tridiagonal solve and
+the FP64 oracle.
+
+* mhd2d_xderiv with the tridiagonal solve reproduces the oracle's derivX
+  (FP64 at 1e-12; FP32 and mixed within their rounding);
+* the GEMM form with 6 split products matches the oracle within FP32-level
+  error, with 3 split products within BF16-pair error, on random and smooth
+  lines of 64…1024 rows (checks the MFMA operand layout, the panel lists and
+  the periodic corners of the operator);
+* whole runs with the GEMM form against FP64.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from util import field_errors
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def m():
+    import mhd2dcfdfftw_open_amd as m
+    m.lib()
+    return m
+
+
+def _fields(nx, ny, kind, nv=11, seed=0):
+    rng = np.random.default_rng(seed)
+    if kind == "random":
+        return rng.standard_normal((nv, nx, ny)) + 2.0
+    x = np.arange(nx)[:, None] * 2 * np.pi / nx
+    y = np.arange(ny)[None, :] * 2 * np.pi / ny
+    out = np.zeros((nv, nx, ny))
+    for v in range(nv):
+        a, b = rng.integers(1, 9, 2)
+        out[v] = 1.0 + np.sin(a * x + b * y + rng.uniform()) + 0.1 * np.cos(3 * a * x)
+    return out
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 3e-6), ("mixed", 3e-6)])
+def test_tridiagonal_xderiv_matches_oracle(m, prec, tol):
+    nx, ny = 256, 64
+    P = {"fp64": m.FP64, "fp32": m.FP32, "mixed": m.MIXED}[prec]
+    f = _fields(nx, ny, "smooth")
+    ctx = m.Context(nx, ny, precision=P)
+    d = ctx.xderiv(f)
+    want = orc.derivX(f, 2 * np.pi / nx)
+    e = field_errors(d, want)
+    print(f"[xgemm] tridiagonal {prec}: {e.max():.2e}")
+    assert e.max() <= tol, e
+
+
+@pytest.mark.parametrize("nx,ny", [(64, 64), (256, 128), (1024, 64)])
+@pytest.mark.parametrize("kind", ["smooth", "random"])
+@pytest.mark.parametrize("ns", [3, 6])
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+def test_gemm_xderiv_matches_oracle(m, nx, ny, kind, ns, prec):
+    P = m.MIXED if prec == "mixed" else m.FP32
+    f = _fields(nx, ny, kind, seed=nx + ny)
+    ctx = m.Context(nx, ny, precision=P, xgemm=ns)
+    d = ctx.xderiv(f)
+    want = orc.derivX(f, 2 * np.pi / nx)
+    e = field_errors(d, want)
+    tri = m.Context(nx, ny, precision=P).xderiv(f)
+    et = field_errors(tri, want)
+    print(f"[xgemm] {prec} ns={ns} {nx}x{ny} {kind}: gemm {e.max():.2e}, tridiagonal {et.max():.2e}")
+    tol = {3: 2e-3, 6: 2e-5}[ns]
+    assert e.max() <= tol, e
+
+
+@pytest.mark.parametrize("ns", [3, 6])
+def test_gemm_run_against_fp64(m, ns):
+    nx = ny = 256
+    u0 = m.turbulence(nx, ny, seed=42, kmax=16)
+    ref = m.Context(nx, ny, hall=1, di=0.02)
+    ref.set_state(u0)
+    g = m.Context(nx, ny, hall=1, di=0.02, precision=m.MIXED, xgemm=ns)
+    g.set_state(u0)
+    t = m.Context(nx, ny, hall=1, di=0.02, precision=m.MIXED)
+    t.set_state(u0)
+    for k in range(4):
+        ref.run(50, graph=True)
+        g.run(50, graph=True)
+        t.run(50, graph=True)
+        eg = field_errors(g.state(), ref.state()).max()
+        et = field_errors(t.state(), ref.state()).max()
+        print(f"[xgemm] run ns={ns} step {50 * (k + 1)}: gemm {eg:.2e}, tridiagonal {et:.2e}")
+    assert np.isfinite(g.state()).all()
+    # 6 products: within the mixed-form tolerance after 200 steps; 3 products
+    # (BF16-pair products, ~2^-16) are not, by an order of magnitude
+    assert eg <= {3: 5e-5, 6: 1e-6}[ns]
+
+
+def test_gemm_batch_equals_single(m):
+    nx = ny = 128
+    B = 3
+    ctx = m.Context(nx, ny, precision=m.MIXED, xgemm=6, batch=B, hall=1)
+    states = [m.turbulence(nx, ny, seed=5 + b, kmax=8) for b in range(B)]
+    for b in range(B):
+        ctx.set_state(states[b], inst=b)
+    ctx.run(4, graph=True)
+    for b in range(B):
+        one = m.Context(nx, ny, precision=m.MIXED, xgemm=6, hall=1)
+        one.set_state(states[b])
+        one.run(4)
+        np.testing.assert_array_equal(ctx.state(b), one.state())
+
+
+def test_gemm_rejected_where_undefined(m):
+    with pytest.raises(Exception):
+        m.Context(64, 64, precision=m.FP64, xgemm=6)
+    with pytest.raises(Exception):
+        m.Context(64, 64, precision=m.MIXED, xgemm=4)
+    with pytest.raises(Exception):
+        m.Context(64, 64, precision=m.MIXED, xgemm=6, npe=2)
