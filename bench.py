@@ -117,10 +117,11 @@ def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, nx, ny):
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
     for cls, key in classes.items():
         names = [k for k in fetch if key in k]
+        # k_xsolve<T, TI, C, W, KIND, BLK>: KIND 0 derivative, 2 filter
         if cls == "xderiv":
-            names = [k for k in names if ", 0>(" in k]
+            names = [k for k in names if ", 0, false>(" in k]
         if cls == "xfilter":
-            names = [k for k in names if ", 2>(" in k]
+            names = [k for k in names if ", 2, false>(" in k]
         if not names:
             continue
         k = names[0]
@@ -302,6 +303,8 @@ def main():
                     help="steps of the 64 x 512^2 ensemble")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--c3-only", action="store_true",
+                    help="only the 4096^2 precision / x-derivative comparison (prints its JSON)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the streaming-copy calibration kernel and print its rate")
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV",
@@ -323,6 +326,11 @@ def main():
     if args.copy_probe:
         print(json.dumps({"copy_probe_bytes": COPY_BYTES,
                           "copy_gbps": m.copy_probe(COPY_BYTES, 5)}))
+        return
+
+    if args.c3_only:
+        print(json.dumps(c3_forms(m, float("nan"), args.profile_steps, m.copy_probe(COPY_BYTES, 5),
+                                  1, 0.02)))
         return
 
     nx, ny = args.nx, args.ny
