@@ -18,6 +18,9 @@
 
 #define ORC_PI 3.14159265358979323846264338327950288
 
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+
 /* field indices, macros.h:4-12 (PSI only with divBcleaning == 2) */
 enum { RHO = 0, UX, UY, UZ, BX, BY, BZ, PR, PSI };
 
@@ -228,12 +231,156 @@ static void block_range(int n, int npe, int b, int *off, int *size)
     *size = base + (b < rem ? 1 : 0);
 }
 
+/* ---- open x-boundary (boundaryTypeX == 1) on the first / last rank ---- */
+/* boundary alphas, parameter.h:25-37 and initialize.c:27-40 */
+#define OB_ALPHA_B1 3.0
+#define OB_ALPHA_B2 5.0
+#define OB_ALPHA_F0 1.05
+#define OB_ALPHA_F1 0.49
+#define OB_ALPHA_F2 0.49
+/* near-boundary points, parameter.h:39-56 */
+#define OB_ALPHA1P2 (1. / 3.)
+#define OB_A1P2 (14. / 9.)
+#define OB_B1P2 (1. / 9.)
+#define OB_ALPHA2P2 (2. / 11.)
+#define OB_A2P2 (12. / 11.)
+#define OB_B2P2 (3. / 11.)
+#define OB_ALPHA1P1 (1. / 4.)
+#define OB_A1P1 1.5
+#define OB_ALPHA2P1 0.1
+#define OB_A2P1 1.2
+
+/* One line of the first (side 0) or last (side 1) rank with an open x
+ * boundary: derivCFD1st mathlib.c:87-159, derivCFD2nd mathlib.c:219-294,
+ * filterCFD mathlib.c:355-440, matrices initialize.c:655-708, :732-786,
+ * :809-861, coefficients initialize.c:581-629.  The open end reads no halo;
+ * the other end is the ordinary closure with the neighbour's halo. */
+static void line_open(line_op op, int side, const orc_coefs *c, int N, double DX,
+                      const double *A, const double *H0, const double *H1, double *X)
+{
+    double *B = (double *)malloc(sizeof(double) * N);
+    double *lo = (double *)malloc(sizeof(double) * N);
+    double *dg = (double *)malloc(sizeof(double) * N);
+    double *up = (double *)malloc(sizeof(double) * N);
+    double ai, as;             /* interior and sub alphas of the operator */
+    double ab, ap1, ap2;       /* boundary, P1 and P2 alphas (filter: F0, F1, F2) */
+    if (op == OP_D1) { ai = c->alpha1; as = c->alphaS1; ab = OB_ALPHA_B1; ap1 = OB_ALPHA1P1; ap2 = OB_ALPHA1P2; }
+    else if (op == OP_D2) { ai = c->alpha2; as = c->alphaS2; ab = OB_ALPHA_B2; ap1 = OB_ALPHA2P1; ap2 = OB_ALPHA2P2; }
+    else { ai = c->alphaF; as = c->alphaSF; ab = OB_ALPHA_F0; ap1 = OB_ALPHA_F1; ap2 = OB_ALPHA_F2; }
+    /* the ordinary rows (interior stencil, halo end), as in the periodic
+     * branch; the open end's rows are replaced below */
+    {
+        for (int i = 0; i < N; i++) {
+            double v;
+            if (op == OP_D1) {
+                if (i == 0) v = 1. / DX * (c->aS1 * H0[0] + c->bS1 * H0[1] + c->cS1 * A[0] + c->dS1 * A[1] + c->eS1 * A[2]);
+                else if (i == N - 1) v = -1. / DX * (c->aS1 * H1[1] + c->bS1 * H1[0] + c->cS1 * A[N - 1] + c->dS1 * A[N - 2] + c->eS1 * A[N - 3]);
+                else v = 1. / DX * (c->a1 / 2 * (at(A, H0, H1, N, i + 1) - at(A, H0, H1, N, i - 1)) +
+                                    c->b1 / 4 * (at(A, H0, H1, N, i + 2) - at(A, H0, H1, N, i - 2)) +
+                                    c->c1 / 6 * (at(A, H0, H1, N, i + 3) - at(A, H0, H1, N, i - 3)));
+            } else if (op == OP_D2) {
+                double DX2 = DX * DX;
+                if (i == 0) v = 1. / DX2 * (c->aS2 * H0[0] + c->bS2 * H0[1] + c->cS2 * A[0] + c->dS2 * A[1] + c->eS2 * A[2]);
+                else if (i == N - 1) v = 1. / DX2 * (c->aS2 * H1[1] + c->bS2 * H1[0] + c->cS2 * A[N - 1] + c->dS2 * A[N - 2] + c->eS2 * A[N - 3]);
+                else {
+                    double fi = A[i];
+                    v = 1. / DX2 * (c->a2 * (at(A, H0, H1, N, i + 1) - 2 * fi + at(A, H0, H1, N, i - 1)) +
+                                    c->b2 / 4 * (at(A, H0, H1, N, i + 2) - 2 * fi + at(A, H0, H1, N, i - 2)) +
+                                    c->c2 / 9 * (at(A, H0, H1, N, i + 3) - 2 * fi + at(A, H0, H1, N, i - 3)));
+                }
+            } else {
+                if (i == 0) v = c->aSF * H0[0] + c->bSF * H0[1] + c->cSF * A[0] + c->dSF * A[1] + c->eSF * A[2];
+                else if (i == N - 1) v = c->aSF * H1[1] + c->bSF * H1[0] + c->cSF * A[N - 1] + c->dSF * A[N - 2] + c->eSF * A[N - 3];
+                else v = c->aF * A[i] + c->bF / 2 * (at(A, H0, H1, N, i + 1) + at(A, H0, H1, N, i - 1)) +
+                         c->cF / 2 * (at(A, H0, H1, N, i + 2) + at(A, H0, H1, N, i - 2)) +
+                         c->dF / 2 * (at(A, H0, H1, N, i + 3) + at(A, H0, H1, N, i - 3));
+            }
+            B[i] = v;
+        }
+    }
+    /* ordinary matrix */
+    for (int i = 0; i < N; i++) dg[i] = 1.0;
+    for (int i = 0; i < N - 2; i++) lo[i] = ai;
+    lo[N - 2] = as;
+    up[0] = as;
+    for (int i = 1; i < N - 1; i++) up[i] = ai;
+    /* the open end: three rows of the boundary scheme */
+    double DX2 = DX * DX;
+    if (side == 0) {
+        if (op == OP_D1) {
+            double aB = -(11. + 2. * ab) / 6., bB = (6. - ab) / 2., cB = (2. * ab - 3.) / 2., dB = (2. - ab) / 6.;
+            B[0] = 1 / DX * (aB * A[0] + bB * A[1] + cB * A[2] + dB * A[3]);
+            B[1] = 1 / DX * (OB_A1P1 / 2 * (A[2] - A[0]));
+            B[2] = 1 / DX * (OB_A1P2 / 2 * (A[3] - A[1]) + OB_B1P2 / 4 * (A[4] - A[0]));
+        } else if (op == OP_D2) {
+            double aB = (11. * ab + 35.) / 12., bB = -(5. * ab + 26.) / 3., cB = (ab + 19.) / 2.,
+                   dB = (ab - 14.) / 3., eB = (11. - ab) / 12.;
+            B[0] = 1 / DX2 * (aB * A[0] + bB * A[1] + cB * A[2] + dB * A[3] + eB * A[4]);
+            B[1] = 1 / DX2 * (OB_A2P1 * (A[2] - 2 * A[1] + A[0]));
+            B[2] = 1 / DX2 * (OB_A2P2 * (A[3] - 2 * A[2] + A[1]) + OB_B2P2 / 4 * (A[4] - 2 * A[2] + A[0]));
+        } else {
+            double a0 = ab, a1 = ap1, a2 = ap2;
+            B[0] = (15. + a0) / 16. * A[0] + (1. + 3. * a0) / 4. * A[1] + 3. * (a0 - 1) / 8. * A[2] +
+                   (1. - a0) / 4. * A[3] + (a0 - 1.) / 16. * A[4];
+            B[1] = (1. + 14. * a1) / 16. * A[0] + (3. + 2. * a1) / 4. * A[1] + (3. + 2. * a1) / 8. * A[2] +
+                   (-1. + 2. * a1) / 4. * A[3] + (1. - 2. * a1) / 16. * A[4];
+            B[2] = (5. + 6. * a2) / 8. * A[2] + (1. + 2. * a2) / 2. / 2 * (A[3] + A[1]) +
+                   -(1. - 2. * a2) / 8. / 2 * (A[4] + A[0]);
+        }
+        up[0] = ab;
+        lo[0] = ap1;
+        up[1] = ap1;
+        lo[1] = ap2;
+        up[2] = ap2;
+    } else {
+        if (op == OP_D1) {
+            double aB = -(11. + 2. * ab) / 6., bB = (6. - ab) / 2., cB = (2. * ab - 3.) / 2., dB = (2. - ab) / 6.;
+            B[N - 3] = 1 / DX * (OB_A1P2 / 2 * (A[N - 2] - A[N - 4]) + OB_B1P2 / 4 * (A[N - 1] - A[N - 5]));
+            B[N - 2] = 1 / DX * (OB_A1P1 / 2 * (A[N - 1] - A[N - 3]));
+            B[N - 1] = -1 / DX * (aB * A[N - 1] + bB * A[N - 2] + cB * A[N - 3] + dB * A[N - 4]);
+        } else if (op == OP_D2) {
+            double aB = (11. * ab + 35.) / 12., bB = -(5. * ab + 26.) / 3., cB = (ab + 19.) / 2.,
+                   dB = (ab - 14.) / 3., eB = (11. - ab) / 12.;
+            B[N - 3] = 1 / DX2 * (OB_A2P2 * (A[N - 2] - 2 * A[N - 3] + A[N - 4]) +
+                                  OB_B2P2 / 4 * (A[N - 1] - 2 * A[N - 3] + A[N - 5]));
+            B[N - 2] = 1 / DX2 * (OB_A2P1 * (A[N - 1] - 2 * A[N - 2] + A[N - 3]));
+            B[N - 1] = 1 / DX2 * (aB * A[N - 1] + bB * A[N - 2] + cB * A[N - 3] + dB * A[N - 4] + eB * A[N - 5]);
+        } else {
+            double a0 = ab, a1 = ap1, a2 = ap2;
+            B[N - 3] = (5. + 6. * a2) / 8. * A[N - 3] + (1. + 2. * a2) / 2. / 2 * (A[N - 4] + A[N - 2]) +
+                       -(1. - 2. * a2) / 8. / 2 * (A[N - 5] + A[N - 1]);
+            B[N - 2] = (1. + 14. * a1) / 16. * A[N - 1] + (3. + 2. * a1) / 4. * A[N - 2] +
+                       (3. + 2. * a1) / 8. * A[N - 3] + (-1. + 2. * a1) / 4. * A[N - 4] +
+                       (1. - 2. * a1) / 16. * A[N - 5];
+            B[N - 1] = (15. + a0) / 16. * A[N - 1] + (1. + 3. * a0) / 4. * A[N - 2] +
+                       3. * (a0 - 1) / 8. * A[N - 3] + (1. - a0) / 4. * A[N - 4] + (a0 - 1.) / 16. * A[N - 5];
+        }
+        lo[N - 2] = ab;
+        up[N - 2] = ap1;
+        lo[N - 3] = ap1;
+        up[N - 3] = ap2;
+        lo[N - 4] = ap2;
+    }
+    orc_thomas(N, lo, dg, up, B, X);
+    free(B); free(lo); free(dg); free(up);
+}
+
 /* x operator with the x range split into npe blocks as the reference's MPI
  * ranks do: each block is solved on its own with a 2-point halo from its
  * periodic neighbours (parallel.c:13-24), so rows at block edges are the
- * explicit closure rows */
+ * explicit closure rows.  open != 0: the first and last blocks close the
+ * line with the open-boundary scheme instead (boundaryTypeX == 1, npe >= 2) */
+static void x_apply_npe_bc(line_op op, int nx, int ny, double dx, int nv, int npe, int open,
+                           const double *in, double *out);
+
 static void x_apply_npe(line_op op, int nx, int ny, double dx, int nv, int npe,
                         const double *in, double *out)
+{
+    x_apply_npe_bc(op, nx, ny, dx, nv, npe, 0, in, out);
+}
+
+static void x_apply_npe_bc(line_op op, int nx, int ny, double dx, int nv, int npe, int open,
+                           const double *in, double *out)
 {
     orc_coefs c;
     orc_coefs_default(&c);
@@ -254,7 +401,9 @@ static void x_apply_npe(line_op op, int nx, int ny, double dx, int nv, int npe,
                     H0[h] = col[(off - 2 + h + nx) % nx];
                     H1[h] = col[(off + N + h) % nx];
                 }
-                if (op == OP_D1) orc_line_deriv1_c(&c, N, dx, col + off, H0, H1, sol + off);
+                if (open && npe >= 2 && (b == 0 || b == npe - 1))
+                    line_open(op, b == 0 ? 0 : 1, &c, N, dx, col + off, H0, H1, sol + off);
+                else if (op == OP_D1) orc_line_deriv1_c(&c, N, dx, col + off, H0, H1, sol + off);
                 else if (op == OP_D2) line_deriv2_c(&c, N, dx, col + off, H0, H1, sol + off);
                 else orc_line_filter_c(&c, N, col + off, H0, H1, sol + off);
             }
@@ -284,6 +433,21 @@ void orc_derivXX_npe(int nx, int ny, double dx, int nv, int npe, const double *i
 void orc_filterX_npe(int nx, int ny, int nv, int npe, double *inout)
 {
     x_apply_npe(OP_F, nx, ny, 0.0, nv, npe, inout, inout);
+}
+
+void orc_derivX_open(int nx, int ny, double dx, int nv, int npe, const double *in, double *out)
+{
+    x_apply_npe_bc(OP_D1, nx, ny, dx, nv, npe, 1, in, out);
+}
+
+void orc_derivXX_open(int nx, int ny, double dx, int nv, int npe, const double *in, double *out)
+{
+    x_apply_npe_bc(OP_D2, nx, ny, dx, nv, npe, 1, in, out);
+}
+
+void orc_filterX_open(int nx, int ny, int nv, int npe, double *inout)
+{
+    x_apply_npe_bc(OP_F, nx, ny, 0.0, nv, npe, 1, inout, inout);
 }
 
 void orc_derivX(int nx, int ny, double dx, int nv, const double *in, double *out)
@@ -456,6 +620,138 @@ void orc_divb(int nx, int ny, const double *dudx, const double *dudy, double *di
 
 /* calcRHSInnerPoint calcRHS.c:77-159 + addSourceTerm calcRHS.c:498-544.
  * ch/cp: per x row, the Ch and Cp of the rank owning that row (GLM only). */
+/* calcCharacteristicsX, calcRHS.c:253-495: the x-derivative part of the
+ * primitive equations at an open boundary cell, split into the 7 waves
+ * (fast, Alfven, slow in both directions, entropy) with the waves entering
+ * the domain removed; boundary 0 = left (x = 0), 1 = right.  out[0..6] =
+ * d/dt of rho, ux, uy, uz, By, Bz, p. */
+static void characteristics_x(size_t n, size_t p, int boundary, double gamma,
+                              const double *uu, const double *dudx, double *out)
+{
+    double ux = uu[UX * n + p], uy = uu[UY * n + p], uz = uu[UZ * n + p];
+    double bx = uu[BX * n + p], by = uu[BY * n + p], bz = uu[BZ * n + p];
+    double rho = uu[RHO * n + p], pr = uu[PR * n + p];
+    double dux = dudx[UX * n + p], duy = dudx[UY * n + p], duz = dudx[UZ * n + p];
+    double dby = dudx[BY * n + p], dbz = dudx[BZ * n + p];
+    double drho = dudx[RHO * n + p], dp = dudx[PR * n + p];
+    (void)uy;
+    (void)uz;
+    double cFlow = ux;
+    double cAlfx = bx / sqrt(rho), cAlfy = by / sqrt(rho), cAlfz = bz / sqrt(rho);
+    double cSound2 = gamma * pr / rho, cSound = sqrt(cSound2);
+    double cAlfx2 = cAlfx * cAlfx;
+    double sgn = cAlfx >= 0 ? 1.0 : -1.0;
+    double cAlf2 = cAlfx2 + cAlfy * cAlfy + cAlfz * cAlfz;
+    double cAlfperp = sqrt(cAlfy * cAlfy + cAlfz * cAlfz);
+    double angY, angZ;
+    if (cAlfperp < 1e-7) {   /* calcRHS.c:307-315: degenerate direction */
+        angY = 0.0;
+        angZ = 0.0;
+    } else {
+        angY = cAlfy / cAlfperp;
+        angZ = cAlfz / cAlfperp;
+    }
+    double cM2 = cSound2 + cAlf2;
+    double cN2 = sqrt(dmax(cM2 * cM2 - 4 * cSound2 * cAlfx2, 0.));
+    double cFast = sqrt(dmax(cM2 + cN2, 0.)) / sqrt(2.0);
+    double cSlow = sqrt(dmax(cM2 - cN2, 0.)) / sqrt(2.0);
+    double al1 = sqrt(fabs((cFast * cFast - cAlfx2) / (cFast * cFast - cSlow * cSlow)));
+    double al2 = sqrt(fabs((cFast * cFast - cSound2) / (cFast * cFast - cSlow * cSlow)));
+    double byzP = angZ * dbz + angY * dby, uyzP = angZ * duz + angY * duy;
+    double byzM = angY * dbz - angZ * dby, uyzM = angY * duz - angZ * duy;
+    double sr = sqrt(rho);
+    double fP = (cFlow + cFast) * (al1 * (dp / rho / cFast + dux) + al2 * (byzP / sr - cAlfx / cFast * uyzP));
+    double aP = (cFlow + sgn * cAlfx) * (uyzM - sgn / sr * byzM);
+    double sP = (cFlow + cSlow) * (-al2 * (dp / rho / cSound + sgn * cAlfx / cFast * dux) +
+                                   al1 * (cSound / cFast / sr * byzP - sgn * uyzP));
+    double en = ux * cSound * (dp / gamma / pr - drho / rho);
+    double sM = (cFlow - cSlow) * (-al2 * (dp / rho / cSound - sgn * cAlfx / cFast * dux) +
+                                   al1 * (cSound / cFast / sr * byzP + sgn * uyzP));
+    double aM = (cFlow - sgn * cAlfx) * (uyzM + sgn / sr * byzM);
+    double fM = (cFlow - cFast) * (al1 * (dp / rho / cFast - dux) + al2 * (byzP / sr + cAlfx / cFast * uyzP));
+    if (boundary == 0) {     /* left: waves moving in +x enter (calcRHS.c:383-419) */
+        if (cFlow + cFast >= 0) fP = 0;
+        if (cFlow + sgn * cAlfx >= 0) aP = 0;
+        if (cFlow + cSlow >= 0) sP = 0;
+        if (cFlow >= 0) en = 0;
+        if (cFlow - cSlow >= 0) sM = 0;
+        if (cFlow - sgn * cAlfx >= 0) aM = 0;
+        if (cFlow - cFast >= 0) fM = 0;
+    } else {                 /* right: waves moving in -x enter (calcRHS.c:424-457) */
+        if (cFlow + cFast <= 0) fP = 0;
+        if (cFlow + sgn * cAlfx <= 0) aP = 0;
+        if (cFlow + cSlow <= 0) sP = 0;
+        if (cFlow <= 0) en = 0;
+        if (cFlow - cSlow <= 0) sM = 0;
+        if (cFlow - sgn * cAlfx <= 0) aM = 0;
+        if (cFlow - cFast <= 0) fM = 0;
+    }
+    /* calcRHS.c:461-494 */
+    out[0] = rho / cSound * en + rho * (al2 / 2 / cSound * (sP + sM) - al1 / 2 / cFast * (fP + fM));
+    out[1] = cSlow * al2 / 2 / cSound * (sP - sM) - al1 / 2 * (fP - fM);
+    out[2] = angZ / 2 * (aP + aM) + angY * (sgn * al1 / 2 * (sP - sM) + cAlfx * al2 / 2 / cFast * (fP - fM));
+    out[3] = -angY / 2 * (aP + aM) + angZ * (sgn * al1 / 2 * (sP - sM) + cAlfx * al2 / 2 / cFast * (fP - fM));
+    out[4] = -sgn * sr * angZ / 2 * (aP - aM) - sr * angY * (cSound * al1 / 2 / cFast * (sP + sM) + al2 / 2 * (fP + fM));
+    out[5] = sgn * sr * angY / 2 * (aP - aM) - sr * angZ * (cSound * al1 / 2 / cFast * (sP + sM) + al2 / 2 * (fP + fM));
+    out[6] = gamma * pr * (al2 / 2 / cSound * (sP + sM) - al1 / 2 / cFast * (fP + fM));
+}
+
+void orc_characteristics_x(int nx, int ny, int i, int j, int boundary, double gamma,
+                           const double *uu, const double *dudx, double *out)
+{
+    characteristics_x((size_t)nx * ny, (size_t)i * ny + j, boundary, gamma, uu, dudx, out);
+}
+
+/* calcRHSOpenBoundaryX, calcRHS.c:163-251: a cell of an open x boundary line
+ * (x = 0 on the first rank, x = Lx on the last) */
+static void open_boundary_cell(size_t n, size_t p, int boundary, double gamma, int divbclean,
+                               double ch, double cp, const double *uu, const double *dudx,
+                               const double *dudy, double *dudt)
+{
+    double rho = uu[RHO * n + p], ux = uu[UX * n + p], uy = uu[UY * n + p];
+    double bx = uu[BX * n + p], by = uu[BY * n + p], bz = uu[BZ * n + p];
+    double pr = uu[PR * n + p];
+    /* Bx keeps its d/dx terms in the u.grad form (calcRHS.c:172-179) */
+    double divU = dudx[UX * n + p] + dudy[UY * n + p];
+    dudt[BX * n + p] = -ux * dudx[BX * n + p] - uy * dudy[BX * n + p] + bx * dudx[UX * n + p] +
+                       by * dudy[UX * n + p] - divU * bx;
+    /* everything else without d/dx terms (calcRHS.c:182-214) */
+    double jx = dudy[BZ * n + p], jy = 0, jz = -dudy[BX * n + p];
+    divU = dudy[UY * n + p];
+    dudt[PR * n + p] = -uy * dudy[PR * n + p] - gamma * pr * divU;
+    dudt[RHO * n + p] = -rho * divU - uy * dudy[RHO * n + p];
+    dudt[UX * n + p] = -uy * dudy[UX * n + p] + jy * bz / rho - jz * by / rho;
+    dudt[UY * n + p] = -uy * dudy[UY * n + p] - dudy[PR * n + p] / rho + jz * bx / rho - jx * bz / rho;
+    dudt[UZ * n + p] = -uy * dudy[UZ * n + p] + jx * by / rho - jy * bx / rho;
+    dudt[BY * n + p] = -uy * dudy[BY * n + p] + by * dudy[UY * n + p] - divU * by;
+    dudt[BZ * n + p] = -uy * dudy[BZ * n + p] + by * dudy[UZ * n + p] - divU * bz;
+    /* the outgoing waves' x-derivative terms (calcRHS.c:216-239) */
+    double c[7];
+    characteristics_x(n, p, boundary, gamma, uu, dudx, c);
+    dudt[RHO * n + p] += c[0];
+    dudt[UX * n + p] += c[1];
+    dudt[UY * n + p] += c[2];
+    dudt[UZ * n + p] += c[3];
+    dudt[BY * n + p] += c[4];
+    dudt[BZ * n + p] += c[5];
+    dudt[PR * n + p] += c[6];
+    if (divbclean == 2) {    /* calcRHS.c:241-249: psi terms only */
+        double divB = dudx[BX * n + p] + dudy[BY * n + p];
+        dudt[BX * n + p] = dudt[BX * n + p] - dudx[PSI * n + p];
+        dudt[BY * n + p] = dudt[BY * n + p] - dudy[PSI * n + p];
+        dudt[PSI * n + p] = -ch * ch * (divB + uu[PSI * n + p] / cp / cp);
+    }
+}
+
+/* open != 0: rows 0 and nx-1 are open x boundaries (calcRHS.c:56-72) */
+static void rhs_impl_bc(int nx, int ny, double gamma, double resistivity, double viscosity,
+                        int divbclean, double constDiffDivB, const double *ch,
+                        const double *cp, const double *uu, const double *dudx,
+                        const double *dudy, const double *dEdx, const double *dEdy,
+                        const double *d2udx2, const double *d2udy2,
+                        const double *d2Bdx2, const double *d2Bdy2,
+                        const double *ddivBdx, const double *ddivBdy, double *dudt, int open);
+
 static void rhs_impl(int nx, int ny, double gamma, double resistivity, double viscosity,
                      int divbclean, double constDiffDivB, const double *ch,
                      const double *cp, const double *uu, const double *dudx,
@@ -464,8 +760,26 @@ static void rhs_impl(int nx, int ny, double gamma, double resistivity, double vi
                      const double *d2Bdx2, const double *d2Bdy2,
                      const double *ddivBdx, const double *ddivBdy, double *dudt)
 {
+    rhs_impl_bc(nx, ny, gamma, resistivity, viscosity, divbclean, constDiffDivB, ch, cp, uu,
+                dudx, dudy, dEdx, dEdy, d2udx2, d2udy2, d2Bdx2, d2Bdy2, ddivBdx, ddivBdy, dudt,
+                0);
+}
+
+static void rhs_impl_bc(int nx, int ny, double gamma, double resistivity, double viscosity,
+                        int divbclean, double constDiffDivB, const double *ch,
+                        const double *cp, const double *uu, const double *dudx,
+                        const double *dudy, const double *dEdx, const double *dEdy,
+                        const double *d2udx2, const double *d2udy2,
+                        const double *d2Bdx2, const double *d2Bdy2,
+                        const double *ddivBdx, const double *ddivBdy, double *dudt, int open)
+{
     size_t n = (size_t)nx * ny;
     for (size_t p = 0; p < n; p++) {
+        size_t row = p / (size_t)ny;
+        if (open && (row == 0 || row == (size_t)nx - 1)) {
+            open_boundary_cell(n, p, row == 0 ? 0 : 1, gamma, divbclean, ch ? ch[row] : 0,
+                               cp ? cp[row] : 0, uu, dudx, dudy, dudt);
+        } else {
         double rho = uu[RHO * n + p], ux = uu[UX * n + p], uy = uu[UY * n + p];
         double bx = uu[BX * n + p], by = uu[BY * n + p], bz = uu[BZ * n + p];
         double pr = uu[PR * n + p];
@@ -498,6 +812,9 @@ static void rhs_impl(int nx, int ny, double gamma, double resistivity, double vi
             dudt[BY * n + p] = dudt[BY * n + p] - dudy[PSI * n + p];
             dudt[PSI * n + p] = -ch[i] * ch[i] * (divB + psi / cp[i] / cp[i]);
         }
+        }
+        /* addSourceTerm, calcRHS.c:498-544, on every cell */
+        const double rho = uu[RHO * n + p];
         if (viscosity > 0) {  /* calcRHS.c:501-515 */
             for (int c = 0; c < 3; c++)
                 dudt[(UX + c) * n + p] += viscosity / rho *
@@ -554,8 +871,6 @@ void orc_rk_execute(size_t n, double cc, double dd, double *uu,
     }
 }
 
-static inline double dmax(double a, double b) { return a > b ? a : b; }
-static inline double dmin(double a, double b) { return a < b ? a : b; }
 
 /* the largest signal speeds along x and y of cell p, timeAdvance.c:49-105 */
 static void cell_speeds(size_t n, size_t p, double dx, double dy, double gamma,
@@ -661,7 +976,7 @@ double orc_energy(int nx, int ny, double dx, double dy, double gamma,
 /* run driver: the stage order of mhd.c:154-206                               */
 /* ------------------------------------------------------------------------ */
 struct orc_sim {
-    int nx, ny, nvar, hall, divbclean, npe;
+    int nx, ny, nvar, hall, divbclean, npe, open;
     double Lx, Ly, dx, dy, gamma, cfl, di, resistivity, viscosity, constDiffDivB;
     double dt, time;
     long steps;
@@ -777,32 +1092,32 @@ void orc_sim_stage(orc_sim *s, int iRK)
     int nx = s->nx, ny = s->ny;
     size_t n = (size_t)nx * ny;
     int nv = s->nvar;
-    orc_derivX_npe(nx, ny, s->dx, nv, s->npe, s->uu, s->dudx);           /* mhd.c:160 */
+    x_apply_npe_bc(OP_D1, nx, ny, s->dx, nv, s->npe, s->open, s->uu, s->dudx);           /* mhd.c:160 */
     orc_derivY(nx, ny, s->Ly, nv, s->uu, s->dudy);           /* mhd.c:161 */
     if (s->viscosity > 0) {                                  /* mhd.c:163-167 */
-        orc_derivXX_npe(nx, ny, s->dx, 3, s->npe, s->uu + UX * n, s->d2udx2);
+        x_apply_npe_bc(OP_D2, nx, ny, s->dx, 3, s->npe, s->open, s->uu + UX * n, s->d2udx2);
         orc_derivYY(nx, ny, s->Ly, 3, s->uu + UX * n, s->d2udy2);
     }
     /* mhd.c:169-173 also runs these when Hall==1, but their output is read
      * only when resistivity > 0 (calcRHS.c:517), so they are skipped here. */
     if (s->resistivity > 0) {
-        orc_derivXX_npe(nx, ny, s->dx, 3, s->npe, s->uu + BX * n, s->d2Bdx2);
+        x_apply_npe_bc(OP_D2, nx, ny, s->dx, 3, s->npe, s->open, s->uu + BX * n, s->d2Bdx2);
         orc_derivYY(nx, ny, s->Ly, 3, s->uu + BX * n, s->d2Bdy2);
     }
     if (s->divbclean == 1) {                                 /* mhd.c:175-182 */
         orc_divb(nx, ny, s->dudx, s->dudy, s->divB);
-        orc_derivX_npe(nx, ny, s->dx, 1, s->npe, s->divB, s->ddivBdx);
+        x_apply_npe_bc(OP_D1, nx, ny, s->dx, 1, s->npe, s->open, s->divB, s->ddivBdx);
         orc_derivY(nx, ny, s->Ly, 1, s->divB, s->ddivBdy);
     }
     orc_efield(nx, ny, s->uu, s->E);                         /* mhd.c:186 */
-    orc_derivX_npe(nx, ny, s->dx, 3, s->npe, s->E, s->dEdx);             /* mhd.c:188 */
+    x_apply_npe_bc(OP_D1, nx, ny, s->dx, 3, s->npe, s->open, s->E, s->dEdx);             /* mhd.c:188 */
     orc_derivY(nx, ny, s->Ly, 3, s->E, s->dEdy);             /* mhd.c:189 */
-    rhs_impl(nx, ny, s->gamma, s->resistivity, s->viscosity, s->divbclean,
-             s->constDiffDivB, s->ch, s->cp, s->uu, s->dudx, s->dudy, s->dEdx, s->dEdy,
-             s->d2udx2, s->d2udy2, s->d2Bdx2, s->d2Bdy2, s->ddivBdx,
-             s->ddivBdy, s->dudt);                           /* mhd.c:192 */
+    rhs_impl_bc(nx, ny, s->gamma, s->resistivity, s->viscosity, s->divbclean,
+                s->constDiffDivB, s->ch, s->cp, s->uu, s->dudx, s->dudy, s->dEdx, s->dEdy,
+                s->d2udx2, s->d2udy2, s->d2Bdx2, s->d2Bdy2, s->ddivBdx,
+                s->ddivBdy, s->dudt, s->open);               /* mhd.c:192 */
     orc_rk_execute(nv * n, s->cc[iRK], s->dd[iRK], s->uu, s->dudt, s->dudtRK);
-    orc_filterX_npe(nx, ny, nv, s->npe, s->uu);              /* filtering.c:100 */
+    x_apply_npe_bc(OP_F, nx, ny, 0.0, nv, s->npe, s->open, s->uu, s->uu);              /* filtering.c:100 */
     orc_filterY(nx, ny, nv, s->uu);                          /* filtering.c:101 */
 }
 
@@ -833,6 +1148,16 @@ void orc_sim_run_fixed_dt(orc_sim *s, int nsteps, const double *dts)
 
 /* emulate the reference run on npe MPI ranks along x (block closure) */
 void orc_sim_set_npe(orc_sim *s, int npe) { s->npe = npe < 1 ? 1 : npe; }
+
+/* boundaryTypeX == 1 (open x boundary, meaningful with npe >= 2): dx =
+ * Lx/(nx-1) (initialize.c:125-133) and constDiffDivB from it (:154); call
+ * before orc_sim_set_state */
+void orc_sim_set_open(orc_sim *s, int open)
+{
+    s->open = open ? 1 : 0;
+    s->dx = s->open ? s->Lx / (s->nx - 1) : s->Lx / s->nx;
+    s->constDiffDivB = 2.0 * dmax(s->dx, s->dy) * dmax(s->dx, s->dy);
+}
 
 int orc_sim_nvar(const orc_sim *s) { return s->nvar; }
 double orc_sim_glm_ch(const orc_sim *s, int i) { return s->ch[i]; }

@@ -68,6 +68,17 @@ void orc_filterY(int nx, int ny, int nv, double *inout);
 void orc_derivX_npe(int nx, int ny, double dx, int nv, int npe, const double *in, double *out);
 void orc_derivXX_npe(int nx, int ny, double dx, int nv, int npe, const double *in, double *out);
 void orc_filterX_npe(int nx, int ny, int nv, int npe, double *inout);
+/* the same with an open x boundary (boundaryTypeX == 1): the first and last
+ * blocks close with the one-sided schemes of mathlib.c:87-159, :219-294,
+ * :355-440 (npe >= 2) */
+void orc_derivX_open(int nx, int ny, double dx, int nv, int npe, const double *in, double *out);
+void orc_derivXX_open(int nx, int ny, double dx, int nv, int npe, const double *in, double *out);
+void orc_filterX_open(int nx, int ny, int nv, int npe, double *inout);
+/* characteristic x-derivative terms of cell (i, j) at an open boundary
+ * (0 left, 1 right), calcRHS.c:253-495: out[7] = d/dt of rho, ux, uy, uz,
+ * By, Bz, p */
+void orc_characteristics_x(int nx, int ny, int i, int j, int boundary, double gamma,
+                           const double *uu, const double *dudx, double *out);
 
 /* ---- pointwise physics (calcRHS.c, timeAdvance.c) ---- */
 void orc_efield(int nx, int ny, const double *uu, double *E);
@@ -111,6 +122,7 @@ void orc_sim_run_fixed_dt(orc_sim *s, int nsteps, const double *dts);
 void orc_sim_stage(orc_sim *s, int iRK);                /* one RK stage */
 void orc_sim_set_npe(orc_sim *s, int npe);             /* x blocks (MPI ranks) */
 int orc_sim_nvar(const orc_sim *s);                    /* 9 with divbclean == 2 */
+void orc_sim_set_open(orc_sim *s, int open);           /* open x boundary */
 double orc_sim_glm_ch(const orc_sim *s, int i);        /* Ch, Cp of x row i */
 double orc_sim_glm_cp(const orc_sim *s, int i);
 double orc_sim_dt(const orc_sim *s);

@@ -50,6 +50,11 @@ def lib():
             "orc_derivXX_npe": (None, [i, i, d, i, i, p, p]),
             "orc_filterX_npe": (None, [i, i, i, i, p]),
             "orc_sim_set_npe": (None, [ctypes.c_void_p, i]),
+            "orc_sim_set_open": (None, [ctypes.c_void_p, i]),
+            "orc_derivX_open": (None, [i, i, d, i, i, p, p]),
+            "orc_derivXX_open": (None, [i, i, d, i, i, p, p]),
+            "orc_filterX_open": (None, [i, i, i, i, p]),
+            "orc_characteristics_x": (None, [i, i, i, i, i, d, p, p, p]),
             "orc_sim_nvar": (i, [ctypes.c_void_p]),
             "orc_sim_glm_ch": (d, [ctypes.c_void_p, i]),
             "orc_sim_glm_cp": (d, [ctypes.c_void_p, i]),
@@ -148,6 +153,37 @@ def derivXX(arr, dx, npe=1):
         lib().orc_derivXX(nx, ny, dx, nv, _ptr(arr), _ptr(out))
     else:
         lib().orc_derivXX_npe(nx, ny, dx, nv, npe, _ptr(arr), _ptr(out))
+    return out
+
+
+def derivX_open(arr, dx, npe):
+    arr = _c(arr)
+    nv, nx, ny = arr.shape
+    out = np.zeros_like(arr)
+    lib().orc_derivX_open(nx, ny, dx, nv, npe, _ptr(arr), _ptr(out))
+    return out
+
+
+def derivXX_open(arr, dx, npe):
+    arr = _c(arr)
+    nv, nx, ny = arr.shape
+    out = np.zeros_like(arr)
+    lib().orc_derivXX_open(nx, ny, dx, nv, npe, _ptr(arr), _ptr(out))
+    return out
+
+
+def filterX_open(arr, npe):
+    out = _c(arr).copy()
+    nv, nx, ny = out.shape
+    lib().orc_filterX_open(nx, ny, nv, npe, _ptr(out))
+    return out
+
+
+def characteristics_x(uu, dudx, i, j, boundary, gamma=5.0 / 3.0):
+    uu, dudx = _c(uu), _c(dudx)
+    _, nx, ny = uu.shape
+    out = np.zeros(7)
+    lib().orc_characteristics_x(nx, ny, i, j, boundary, gamma, _ptr(uu), _ptr(dudx), _ptr(out))
     return out
 
 
@@ -265,11 +301,12 @@ class Sim:
 
     def __init__(self, nx, ny, Lx=2 * np.pi, Ly=2 * np.pi, gamma=5.0 / 3.0,
                  cfl=0.3, hall=0, di=0.0, resistivity=0.0, viscosity=0.0,
-                 divbclean=0, npe=1):
+                 divbclean=0, npe=1, open_x=False):
         self.nx, self.ny = nx, ny
         self.h = lib().orc_sim_create(nx, ny, Lx, Ly, gamma, cfl, hall, di,
                                       resistivity, viscosity, divbclean)
         lib().orc_sim_set_npe(self.h, npe)
+        lib().orc_sim_set_open(self.h, int(open_x))
         self.nvar = lib().orc_sim_nvar(self.h)
 
     def __del__(self):
