@@ -136,6 +136,13 @@ typedef struct mhd2d_params {
                                with FP32 accumulation, 3 or 6 split products
                                (precision FP32 or MIXED, xblocks <= 1, nx and
                                ny >= 64) */
+    int open_x;             /* 0: periodic x (boundaryTypeX 0); 1: open x
+                               boundary (boundaryTypeX 1): one-sided closures
+                               on the first and last x block and the
+                               characteristic outflow condition on the lines
+                               x = 0 and x = Lx (calcRHS.c:163-495), dx =
+                               Lx/(nx-1); needs xblocks >= 2 (the reference
+                               needs npe >= 2) and nx/xblocks >= 8 */
 } mhd2d_params;
 
 /* status codes */
@@ -246,6 +253,8 @@ int mhd2d_read_input(const char *path, mhd2d_input *in);
 int mhd2d_write_input(const char *path, const mhd2d_input *in);
 /* xgrid.dat / ygrid.dat: [size_t n][n doubles] (writeOutput.c:5-45) */
 int mhd2d_write_grid(const char *dir, int nx, int ny, double Lx, double Ly);
+/* the same with the open-x grid spacing Lx/(nx-1) when open_x != 0 */
+int mhd2d_write_grid_bc(const char *dir, int nx, int ny, double Lx, double Ly, int open_x);
 /* out%04d.dat: [double t][nvar*nx*ny doubles] (writeOutput.c:47-76) */
 int mhd2d_write_array(const char *dir, int iout, double t, const double *uu,
                       int nvar, int nx, int ny);

@@ -67,7 +67,8 @@ class Params(ctypes.Structure):
                 ("hall", ctypes.c_int), ("di", ctypes.c_double),
                 ("resistivity", ctypes.c_double), ("viscosity", ctypes.c_double),
                 ("divbclean", ctypes.c_int), ("precision", ctypes.c_int),
-                ("xblocks", ctypes.c_int), ("batch", ctypes.c_int), ("xgemm", ctypes.c_int)]
+                ("xblocks", ctypes.c_int), ("batch", ctypes.c_int), ("xgemm", ctypes.c_int),
+                ("open_x", ctypes.c_int)]
 
 
 def lib():
@@ -144,6 +145,7 @@ def lib():
         "mhd2d_read_input": (i, [ctypes.c_char_p, ctypes.POINTER(Input)]),
         "mhd2d_write_input": (i, [ctypes.c_char_p, ctypes.POINTER(Input)]),
         "mhd2d_write_grid": (i, [ctypes.c_char_p, i, i, d, d]),
+        "mhd2d_write_grid_bc": (i, [ctypes.c_char_p, i, i, d, d, i]),
         "mhd2d_write_array": (i, [ctypes.c_char_p, i, d, p, i, i, i]),
         "mhd2d_write_divb": (i, [ctypes.c_char_p, i, d, p, i, i]),
         "mhd2d_read_array": (i, [ctypes.c_char_p, p, p, ctypes.c_size_t]),
@@ -215,7 +217,8 @@ def drive(inp, uu0, outdir="", mode=DRIVE_AUTO, precision=FP64, verbose=False, m
     assert uu0.shape == (8, inp.nx, inp.ny)
     o = DriveOpts(os.fsencode(outdir), mode, precision, int(verbose), max_steps, npe)
     rep = DriveReport()
-    _check(lib().mhd2d_write_grid(os.fsencode(outdir), inp.nx, inp.ny, inp.Lx, inp.Ly))
+    _check(lib().mhd2d_write_grid_bc(os.fsencode(outdir), inp.nx, inp.ny, inp.Lx, inp.Ly,
+                                     int(inp.boundaryTypeX == 1)))
     _check(lib().mhd2d_drive(ctypes.byref(inp), _ptr(uu0), ctypes.byref(o), ctypes.byref(rep)))
     return {k: getattr(rep, k) for k, _ in rep._fields_}
 
@@ -226,13 +229,13 @@ class Context:
     of `batch` independent runs of that grid advanced together."""
 
     def __init__(self, nx, ny, Lx=2 * np.pi, Ly=2 * np.pi, gamma=5.0 / 3.0, cfl=0.3,
-                 hall=0, di=0.0, precision=FP64, npe=1, batch=1, xgemm=0):
+                 hall=0, di=0.0, precision=FP64, npe=1, batch=1, xgemm=0, open_x=False):
         """npe > 1 splits x into npe blocks as a reference run on npe MPI
         ranks does (decomposeGrid1D, initialize.c:547-569); xgemm 3 or 6
         takes x-derivatives as split-BF16 MFMA GEMMs with the dense A^-1 B."""
         self.nx, self.ny, self.batch = nx, ny, batch
         self.params = Params(nx, ny, Lx, Ly, gamma, cfl, hall, di, 0.0, 0.0, 0, precision, npe,
-                             batch, xgemm)
+                             batch, xgemm, int(open_x))
         h = ctypes.c_void_p()
         _check(lib().mhd2d_create(ctypes.byref(self.params), ctypes.byref(h)))
         self.h = h
@@ -325,7 +328,7 @@ class Ensemble:
             arr[k] = Params(r["nx"], r["ny"], r.get("Lx", 2 * np.pi), r.get("Ly", 2 * np.pi),
                             r.get("gamma", 5.0 / 3.0), r.get("cfl", 0.3), r.get("hall", 0),
                             r.get("di", 0.0), 0.0, 0.0, 0, precision, r.get("npe", 1), 0,
-                            r.get("xgemm", 0))
+                            r.get("xgemm", 0), int(r.get("open_x", False)))
         self.shapes = [(r["nx"], r["ny"]) for r in runs]
         h = ctypes.c_void_p()
         _check(lib().mhd2d_ens_create(arr, len(runs), ctypes.byref(h)))

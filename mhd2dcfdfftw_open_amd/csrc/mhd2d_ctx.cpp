@@ -196,7 +196,12 @@ hipError_t rhs_rk(mhd2d_ctx* c, int stage)
     a.bs_u = c->su();
     a.bs_d = c->sd();
     a.bs_r = c->sr();
-    return launch_rhs_rk<T, TD>(a, c->stream, c->B);
+    a.open_x = c->p.open_x;
+    a.nx = c->p.nx;
+    a.ny = c->p.ny;
+    hipError_t e = launch_rhs_rk<T, TD>(a, c->stream, c->B);
+    if (e == hipSuccess && c->p.open_x) e = launch_rhs_rk_open<T, TD>(a, c->stream, c->B);
+    return e;
 }
 
 // one RK3 step; when ev is given, an event is recorded after every launch
@@ -312,6 +317,11 @@ int check_params(const mhd2d_params* p)
         return fail(MHD2D_EINVAL, "xblocks must be in [0, nx/4] (blocks of at least 4 rows)");
     if (p->batch < 0 || p->batch > 65535)
         return fail(MHD2D_EINVAL, "batch must be in [0, 65535]");
+    if (p->open_x != 0 && p->open_x != 1) return fail(MHD2D_EINVAL, "open_x must be 0 or 1");
+    if (p->open_x && (p->xblocks < 2 || p->nx / p->xblocks < 8 || p->xgemm))
+        return fail(MHD2D_EINVAL,
+                    "an open x boundary needs xblocks >= 2 (the reference's npe >= 2), blocks "
+                    "of at least 8 rows and the tridiagonal x-derivative");
     if (p->xgemm != 0 && p->xgemm != 3 && p->xgemm != 6)
         return fail(MHD2D_EINVAL, "xgemm must be 0, 3 or 6");
     if (p->xgemm && (p->precision == MHD2D_FP64 || p->xblocks > 1 || p->nx < 64 || p->ny < 64))
@@ -341,7 +351,8 @@ int mhd2d_create(const mhd2d_params* p, mhd2d_ctx** out)
     c->n = (size_t)p->nx * p->ny;
     c->have_state.assign(c->B, 0);
     c->di.assign(c->B, p->di);
-    hipError_t e = c->tb.build(p->nx, p->ny, p->Lx, p->Ly, c->delem, SchemeAlphas(), p->xblocks);
+    hipError_t e = c->tb.build(p->nx, p->ny, p->Lx, p->Ly, c->delem, SchemeAlphas(), p->xblocks,
+                               p->open_x);
     if (e == hipSuccess && p->xgemm) e = c->xg.build(c->tb, p->xgemm);
     if (e != hipSuccess) {
         delete c;

@@ -58,6 +58,7 @@ int drive_fused(const mhd2d_input* in, const double* uu0, const mhd2d_drive_opts
     p.di = in->di;
     p.precision = o->precision;
     p.xblocks = o->xblocks;
+    p.open_x = in->boundaryTypeX == 1;
     mhd2d_ctx* c = nullptr;
     int rc = mhd2d_create(&p, &c);
     if (rc) return rc;
@@ -243,8 +244,14 @@ extern "C" int mhd2d_drive(const mhd2d_input* in, const double* uu0, const mhd2d
                            mhd2d_drive_report* rep)
 {
     if (!in || !uu0 || !o) return MHD2D_EINVAL;
-    if (in->boundaryTypeX != 0) {
-        std::fprintf(stderr, "mhd2d_drive: only the periodic x boundary is supported\n");
+    if (in->boundaryTypeX != 0 && in->boundaryTypeX != 1) {
+        std::fprintf(stderr, "mhd2d_drive: boundaryTypeX must be 0 (periodic) or 1 (open)\n");
+        return MHD2D_EINVAL;
+    }
+    if (in->boundaryTypeX == 1 && (o->xblocks < 2 || o->mode == MHD2D_DRIVE_DROPIN)) {
+        // the reference's open boundary needs npe >= 2 (parallel.c:25-61)
+        std::fprintf(stderr, "mhd2d_drive: the open x boundary needs the fused engine and "
+                             "npe >= 2 x blocks\n");
         return MHD2D_EINVAL;
     }
     const bool ideal = in->resistivity == 0 && in->viscosity == 0 && in->divbclean == 0;

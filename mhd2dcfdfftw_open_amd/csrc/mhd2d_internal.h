@@ -39,11 +39,16 @@ struct XTables {
     const double* qb;     // [G] product of q over each chunk
     int kconv;            // l, invd, q equal (bitwise) their fixed point on rows kconv..nx-2
     double linf, dinf, qinf;
-    // x split into blocks solved separately (the reference's MPI ranks): per
-    // chunk, bit k set if row g*C+k opens (bstart) / closes (bend) a block.
-    // Null when x is one block (rows 0 and nx-1 are the only end rows).
-    const unsigned long long* bstart;
-    const unsigned long long* bend;
+    // x split into blocks solved separately (the reference's MPI ranks),
+    // possibly with open x ends: rows whose right-hand side is not the
+    // interior stencil (block edges, the rows of an open end) take an
+    // explicit stencil over f[i-4 .. i+4].  Bit k of omask[g] marks row
+    // g*C + k; the marked rows of chunk g use ocoef[9 * (ofirst[g] + m)],
+    // m = 0, 1, ... in row order.  Null when x is one periodic block (rows 0
+    // and nx-1 are the only end rows and use XStencil.e).
+    const unsigned long long* omask;
+    const int* ofirst;
+    const double* ocoef;
 };
 
 // T: arithmetic and output type; TI: input type (a FP64 state feeding FP32
@@ -186,6 +191,9 @@ struct RhsRkArgs {           // fused RHS + low-storage RK update (driver)
     // instance z of a batch (blockIdx.y): u + z * bs_u, dudx/dudy/dEdx/dEdy
     // + z * bs_d, rprev + z * bs_r, rs + z
     size_t bs_u = 0, bs_d = 0, bs_r = 0;
+    // open x boundary: rows 0 and nx-1 are left to launch_rhs_rk_open
+    int open_x = 0;
+    int nx = 0, ny = 0;
 };
 
 // ---------------------------------------------------------------- launchers
@@ -201,6 +209,10 @@ template <typename T>
 hipError_t launch_rhs(const RhsArgs<T>& a, hipStream_t s);
 template <typename T, typename TD = T>
 hipError_t launch_rhs_rk(const RhsRkArgs<T, TD>& a, hipStream_t s, int nb = 1);
+// the two open boundary lines (x = 0, x = Lx): calcRHSOpenBoundaryX with the
+// characteristic terms, then the same RK update
+template <typename T, typename TD = T>
+hipError_t launch_rhs_rk_open(const RhsRkArgs<T, TD>& a, hipStream_t s, int nb = 1);
 template <typename T>
 hipError_t launch_efield(const T* u, T* E, size_t n, hipStream_t s);
 template <typename T>

@@ -105,8 +105,14 @@ int mhd2d_write_input(const char* path, const mhd2d_input* in)
 
 int mhd2d_write_grid(const char* dir, int nx, int ny, double Lx, double Ly)
 {
-    // periodic x and one rank: xgrid[i] = i * Lx / nx (initialize.c:127, :145-148)
-    const double dx = Lx / nx, dy = Ly / ny;
+    return mhd2d_write_grid_bc(dir, nx, ny, Lx, Ly, 0);
+}
+
+int mhd2d_write_grid_bc(const char* dir, int nx, int ny, double Lx, double Ly, int open_x)
+{
+    // xgrid[i] = i * dx with dx = Lx / nx (periodic) or Lx / (nx - 1) (open),
+    // initialize.c:125-133, :145-148; ygrid[j] = j * Ly / ny
+    const double dx = open_x ? Lx / (nx - 1) : Lx / nx, dy = Ly / ny;
     struct {
         const char* name;
         size_t n;

@@ -96,8 +96,9 @@ __device__ __forceinline__ void block_min_dt(double v, RunScalars* rs, double* r
 #ifndef MHD2D_X_MINB
 #define MHD2D_X_MINB 1
 #endif
-// BLK: x is split into blocks (tb.bstart / tb.bend), each with the end rows
-// of its own non-cyclic solve; otherwise rows 0 and nx-1 are the only ends.
+// BLK: x is split into blocks (and may have open ends): the rows marked in
+// tb.omask take explicit stencils over f[i-4 .. i+4] (tb.ocoef); otherwise
+// rows 0 and nx-1 are the only end rows.
 template <typename T, typename TI, int C, int W, int KIND, bool BLK>
 __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, TI> a)
 {
@@ -144,11 +145,13 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
     const XStencil& st = a.st;
     const T c0 = (T)st.c0, c1 = (T)st.c1, c2 = (T)st.c2, c3 = (T)st.c3;
     const bool first = !BLK && (g == 0), last = !BLK && (g == G - 1);
-    unsigned long long bs = 0, be = 0;
+    unsigned long long om = 0;
+    int oi = 0;
     if constexpr (BLK) {
-        bs = a.tb.bstart[g];
-        be = a.tb.bend[g];
+        om = a.tb.omask[g];
+        oi = a.tb.ofirst[g];
     }
+    T saved = 0;  // f at row - 4 of the current row (w[k - 1] before it is overwritten)
     T rfirst = 0, rlast = 0;
     if (first)   // row 0 over f[-2..2]
         rfirst = (T)st.e[0] * w[1] + (T)st.e[1] * w[2] + (T)st.e[2] * w[3] +
@@ -170,13 +173,19 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
             r = c0 * f0 + c1 * (fp1 + fm1) + c2 * (fp2 + fm2) + c3 * (fp3 + fm3);
         }
         if constexpr (BLK) {
-            // a block's first row over f[-2..2], its last over f[+2..-2]
-            if ((bs >> k) & 1ull)
-                r = (T)st.e[0] * fm2 + (T)st.e[1] * fm1 + (T)st.e[2] * f0 +
-                    (T)st.e[3] * fp1 + (T)st.e[4] * fp2;
-            else if ((be >> k) & 1ull)
-                r = (T)st.esign * ((T)st.e[0] * fp2 + (T)st.e[1] * fp1 + (T)st.e[2] * f0 +
-                                   (T)st.e[3] * fm1 + (T)st.e[4] * fm2);
+            // block edges and open-end rows: explicit stencil over f[-4 .. 4]
+            // (f[-4] exists from the second row of the chunk on, f[+4] up to
+            // the one before last; the tables put no weight where it does not)
+            if ((om >> k) & 1ull) {
+                const double* __restrict__ cf = a.tb.ocoef + 9 * oi;
+                T acc = (T)cf[1] * fm3 + (T)cf[2] * fm2 + (T)cf[3] * fm1 + (T)cf[4] * f0 +
+                        (T)cf[5] * fp1 + (T)cf[6] * fp2 + (T)cf[7] * fp3;
+                if (k >= 1) acc += (T)cf[0] * saved;
+                if (k + 7 <= C + 5) acc += (T)cf[8] * w[(k + 7 <= C + 5) ? k + 7 : 0];
+                r = acc;
+                oi++;
+            }
+            saved = fm3;
         }
         w[k] = r;  // w[k] is not read again by later rows
     }
@@ -319,7 +328,7 @@ static hipError_t launch_xsolve_cw(const XSolveArgs<T, TI>& a, int nv, hipStream
 {
     const int G = a.nx / C;
     dim3 grid(a.ny / W, nv, nb), block(W * G);
-    if (a.tb.bstart && a.tb.bend)
+    if (a.tb.omask)
         launch_xsolve_kind<T, TI, C, W, true>(a, grid, block, s);
     else
         launch_xsolve_kind<T, TI, C, W, false>(a, grid, block, s);
@@ -775,7 +784,10 @@ __global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T, TD> a)
     const TD* const eyz = a.dEdy + z * a.bs_d;
     const T g = (T)a.gamma;
     const bool use_prev = a.stage > 0;
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n / W;
+    // with open x ends the first and last rows belong to the boundary kernel
+    const size_t q0 = a.open_x ? (size_t)a.ny / W : 0;
+    const size_t q1 = a.open_x ? (n - (size_t)a.ny) / W : n / W;
+    for (size_t q = q0 + blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < q1;
          q += (size_t)gridDim.x * blockDim.x) {
         const size_t p = q * W;
         Pack<T, W> u[8];
@@ -832,6 +844,129 @@ __global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T, TD> a)
             *reinterpret_cast<Pack<T, W>*>(uz + f * n + p) = o;
             if (a.write_rprev) *reinterpret_cast<Pack<TD, W>*>(rpz + f * n + p) = rs_out;
         }
+    }
+}
+
+// calcCharacteristicsX (calcRHS.c:253-495) at one open boundary cell:
+// c[0..6] = outgoing-wave parts of d/dt of rho, ux, uy, uz, By, Bz, p
+template <typename T>
+__device__ __forceinline__ void characteristics_x(const T (&w)[8], const T (&d)[8], T gamma,
+                                                  int boundary, T (&c)[7])
+{
+    const T rho = w[RHO], ux = w[UX], bx = w[BX], by = w[BY], bz = w[BZ], pr = w[PR];
+    const T dux = d[UX], duy = d[UY], duz = d[UZ], dby = d[BY], dbz = d[BZ];
+    const T drho = d[RHO], dp = d[PR];
+    const T sr = sqrt(rho);
+    const T cFlow = ux;
+    const T cAlfx = bx / sr, cAlfy = by / sr, cAlfz = bz / sr;
+    const T cs2 = gamma * pr / rho, cs = sqrt(cs2);
+    const T cAlfx2 = cAlfx * cAlfx;
+    const T sgn = cAlfx >= 0 ? (T)1 : (T)-1;
+    const T cAlf2 = cAlfx2 + cAlfy * cAlfy + cAlfz * cAlfz;
+    const T cperp = sqrt(cAlfy * cAlfy + cAlfz * cAlfz);
+    T angY = 0, angZ = 0;
+    if (!(cperp < (T)1e-7)) {
+        angY = cAlfy / cperp;
+        angZ = cAlfz / cperp;
+    }
+    const T cM2 = cs2 + cAlf2;
+    const T cN2 = sqrt(fmax(cM2 * cM2 - 4 * cs2 * cAlfx2, (T)0));
+    const T r2 = sqrt((T)2);
+    const T cf = sqrt(fmax(cM2 + cN2, (T)0)) / r2, csl = sqrt(fmax(cM2 - cN2, (T)0)) / r2;
+    const T al1 = sqrt(fabs((cf * cf - cAlfx2) / (cf * cf - csl * csl)));
+    const T al2 = sqrt(fabs((cf * cf - cs2) / (cf * cf - csl * csl)));
+    const T byzP = angZ * dbz + angY * dby, uyzP = angZ * duz + angY * duy;
+    const T byzM = angY * dbz - angZ * dby, uyzM = angY * duz - angZ * duy;
+    T fP = (cFlow + cf) * (al1 * (dp / rho / cf + dux) + al2 * (byzP / sr - cAlfx / cf * uyzP));
+    T aP = (cFlow + sgn * cAlfx) * (uyzM - sgn / sr * byzM);
+    T sP = (cFlow + csl) * (-al2 * (dp / rho / cs + sgn * cAlfx / cf * dux) +
+                            al1 * (cs / cf / sr * byzP - sgn * uyzP));
+    T en = ux * cs * (dp / gamma / pr - drho / rho);
+    T sM = (cFlow - csl) * (-al2 * (dp / rho / cs - sgn * cAlfx / cf * dux) +
+                            al1 * (cs / cf / sr * byzP + sgn * uyzP));
+    T aM = (cFlow - sgn * cAlfx) * (uyzM + sgn / sr * byzM);
+    T fM = (cFlow - cf) * (al1 * (dp / rho / cf - dux) + al2 * (byzP / sr + cAlfx / cf * uyzP));
+    // drop the waves that enter the domain
+    if (boundary == 0) {
+        if (cFlow + cf >= 0) fP = 0;
+        if (cFlow + sgn * cAlfx >= 0) aP = 0;
+        if (cFlow + csl >= 0) sP = 0;
+        if (cFlow >= 0) en = 0;
+        if (cFlow - csl >= 0) sM = 0;
+        if (cFlow - sgn * cAlfx >= 0) aM = 0;
+        if (cFlow - cf >= 0) fM = 0;
+    } else {
+        if (cFlow + cf <= 0) fP = 0;
+        if (cFlow + sgn * cAlfx <= 0) aP = 0;
+        if (cFlow + csl <= 0) sP = 0;
+        if (cFlow <= 0) en = 0;
+        if (cFlow - csl <= 0) sM = 0;
+        if (cFlow - sgn * cAlfx <= 0) aM = 0;
+        if (cFlow - cf <= 0) fM = 0;
+    }
+    c[0] = rho / cs * en + rho * (al2 / 2 / cs * (sP + sM) - al1 / 2 / cf * (fP + fM));
+    c[1] = csl * al2 / 2 / cs * (sP - sM) - al1 / 2 * (fP - fM);
+    c[2] = angZ / 2 * (aP + aM) + angY * (sgn * al1 / 2 * (sP - sM) + cAlfx * al2 / 2 / cf * (fP - fM));
+    c[3] = -angY / 2 * (aP + aM) + angZ * (sgn * al1 / 2 * (sP - sM) + cAlfx * al2 / 2 / cf * (fP - fM));
+    c[4] = -sgn * sr * angZ / 2 * (aP - aM) - sr * angY * (cs * al1 / 2 / cf * (sP + sM) + al2 / 2 * (fP + fM));
+    c[5] = sgn * sr * angY / 2 * (aP - aM) - sr * angZ * (cs * al1 / 2 / cf * (sP + sM) + al2 / 2 * (fP + fM));
+    c[6] = gamma * pr * (al2 / 2 / cs * (sP + sM) - al1 / 2 / cf * (fP + fM));
+}
+
+// the two open boundary lines: calcRHSOpenBoundaryX (calcRHS.c:163-239) and
+// the RK update of k_rhs_rk; blockIdx.y = 0 the line x = 0, 1 the line x = Lx
+template <typename T, typename TD>
+__global__ void __launch_bounds__(256) k_rhs_rk_open(RhsRkArgs<T, TD> a)
+{
+    const int z = blockIdx.z;
+    const RunScalars* const rs = a.rs + z;
+    if (rs->stop) return;
+    const size_t n = a.n, ny = (size_t)a.ny;
+    const int boundary = blockIdx.y;
+    const size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (j >= ny) return;
+    const size_t p = (boundary == 0 ? 0 : (size_t)(a.nx - 1) * ny) + j;
+    T* const uz = a.u + z * a.bs_u;
+    TD* const rpz = a.rprev + z * a.bs_r;
+    const TD* const dxz = a.dudx + z * a.bs_d;
+    const TD* const dyz = a.dudy + z * a.bs_d;
+    const T cc = (T)rs->cc[a.stage], dd = (T)rs->dd[a.stage];
+    const T g = (T)a.gamma;
+    T w[8], d[8], e[8];
+#pragma unroll
+    for (int f = 0; f < 8; f++) {
+        w[f] = uz[f * n + p];
+        d[f] = (T)dxz[f * n + p];
+        e[f] = (T)dyz[f * n + p];
+    }
+    const T rho = w[RHO], ux = w[UX], uy = w[UY], bx = w[BX], by = w[BY], bz = w[BZ], pr = w[PR];
+    T R[8];
+    // Bx keeps its d/dx terms in the u.grad form
+    R[BX] = -ux * d[BX] - uy * e[BX] + bx * d[UX] + by * e[UX] - (d[UX] + e[UY]) * bx;
+    const T jx = e[BZ], jy = 0, jz = -e[BX];
+    const T divU = e[UY];
+    R[PR] = -uy * e[PR] - g * pr * divU;
+    R[RHO] = -rho * divU - uy * e[RHO];
+    R[UX] = -uy * e[UX] + jy * bz / rho - jz * by / rho;
+    R[UY] = -uy * e[UY] - e[PR] / rho + jz * bx / rho - jx * bz / rho;
+    R[UZ] = -uy * e[UZ] + jx * by / rho - jy * bx / rho;
+    R[BY] = -uy * e[BY] + by * e[UY] - divU * by;
+    R[BZ] = -uy * e[BZ] + by * e[UZ] - divU * bz;
+    T c[7];
+    characteristics_x<T>(w, d, g, boundary, c);
+    R[RHO] += c[0];
+    R[UX] += c[1];
+    R[UY] += c[2];
+    R[UZ] += c[3];
+    R[BY] += c[4];
+    R[BZ] += c[5];
+    R[PR] += c[6];
+    const bool use_prev = a.stage > 0;
+#pragma unroll
+    for (int f = 0; f < 8; f++) {
+        const T rp = use_prev ? (T)rpz[f * n + p] : (T)0;
+        uz[f * n + p] = w[f] + cc * R[f] + dd * rp;
+        if (a.write_rprev) rpz[f * n + p] = (TD)R[f];
     }
 }
 
@@ -990,6 +1125,15 @@ hipError_t launch_rhs(const RhsArgs<T>& a, hipStream_t s)
 }
 
 template <typename T, typename TD>
+hipError_t launch_rhs_rk_open(const RhsRkArgs<T, TD>& a, hipStream_t s, int nb)
+{
+    if (nb < 1 || nb > 65535 || a.nx < 2 || a.ny < 1) return hipErrorInvalidValue;
+    dim3 grid((a.ny + 255) / 256, 2, nb);
+    hipLaunchKernelGGL((k_rhs_rk_open<T, TD>), grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename T, typename TD>
 hipError_t launch_rhs_rk(const RhsRkArgs<T, TD>& a, hipStream_t s, int nb)
 {
     if (nb < 1 || nb > 65535) return hipErrorInvalidValue;
@@ -1073,6 +1217,7 @@ hipError_t launch_diag(const T* u, const T* dbx, const T* dby, double* partial, 
     template hipError_t launch_ysweep<T, T>(const YSweepArgs<T, T>&, hipStream_t, int);     \
     template hipError_t launch_rhs<T>(const RhsArgs<T>&, hipStream_t);                       \
     template hipError_t launch_rhs_rk<T, T>(const RhsRkArgs<T, T>&, hipStream_t, int);      \
+    template hipError_t launch_rhs_rk_open<T, T>(const RhsRkArgs<T, T>&, hipStream_t, int); \
     template hipError_t launch_efield<T>(const T*, T*, size_t, hipStream_t);                 \
     template hipError_t launch_divb<T>(const T*, const T*, T*, size_t, hipStream_t);         \
     template hipError_t launch_rk<T>(T*, const T*, T*, double, double, size_t, hipStream_t); \
@@ -1093,5 +1238,7 @@ template hipError_t launch_ysweep<double, float>(const YSweepArgs<double, float>
                                                  int);
 template hipError_t launch_rhs_rk<double, float>(const RhsRkArgs<double, float>&, hipStream_t,
                                                  int);
+template hipError_t launch_rhs_rk_open<double, float>(const RhsRkArgs<double, float>&,
+                                                      hipStream_t, int);
 
 }  // namespace mhd2d

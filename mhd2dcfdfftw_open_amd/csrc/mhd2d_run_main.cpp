@@ -83,7 +83,7 @@ int main(int argc, char** argv)
                     in.hall, in.di, in.resistivity, in.viscosity, in.divbclean);
         std::printf("****************************************************\n");
     }
-    if (mhd2d_write_grid(out.c_str(), in.nx, in.ny, in.Lx, in.Ly)) {
+    if (mhd2d_write_grid_bc(out.c_str(), in.nx, in.ny, in.Lx, in.Ly, in.boundaryTypeX == 1)) {
         std::fprintf(stderr, "mhd2d_run: cannot write the grid files\n");
         return 1;
     }

@@ -25,13 +25,17 @@ public:
     // chunking.  xblocks > 1 splits x into that many blocks sized as the
     // reference's decomposeGrid1D (initialize.c:547-569), each with its own
     // end rows and Thomas factors, as a run on xblocks MPI ranks has them.
+    // open_x: the first and last blocks end in the reference's open-boundary
+    // scheme (boundaryTypeX == 1, mathlib.c:87-159, :219-294, :355-440), and
+    // dx = Lx / (nx - 1); needs xblocks >= 2
     hipError_t build(int nx, int ny, double Lx, double Ly, int elem_bytes,
-                     const SchemeAlphas& al = SchemeAlphas(), int xblocks = 1);
+                     const SchemeAlphas& al = SchemeAlphas(), int xblocks = 1,
+                     int open_x = 0);
     void release();
 
     int nx = 0, ny = 0, elem_bytes = 0;
     double Lx = 0, Ly = 0, dx = 0, dy = 0;
-    int chunk = 0, width = 0, groups = 0, xblocks = 1;
+    int chunk = 0, width = 0, groups = 0, xblocks = 1, open_x = 0;
     XStencil st[NXKIND];
     XTables xt[NXKIND];
     int kconv[NXKIND] = {0, 0, 0};
@@ -40,6 +44,9 @@ public:
     // host copies (FP64) for inspection and tests
     static void factor(int n, double alpha, double alphaSub, double* l, double* invd,
                        double* q);
+    // the same for given couplings: lo[i] of x[i] in row i+1, up[i] of x[i+1] in row i
+    static void factor_rows(int n, const double* lo, const double* up, double* l, double* invd,
+                            double* q);
     // offset and size of block b of n rows split over nb (initialize.c:547-569)
     static void block_range(int n, int nb, int b, int* off, int* size);
 
