@@ -45,13 +45,14 @@ def kernel_alg_bytes(nx, ny, elem=8, delem=None, batch=1):
     n = nx * ny * batch
     s, d = elem, (elem if delem is None else delem)
     return {
-        # y-sweep: read u (8), write filtered u (8) and E (3), dudy (8), dEdy (3)
-        "ysweep": n * (19 * s + 11 * d),
-        # x-derivative: read u (8) and E (3), write d/dx of both (11)
-        "xderiv": n * (11 * s + 11 * d),
+        # y-sweep: read u (8), write filtered u (8), Ey and Ez (2), dudy (8),
+        # dEx/dy and dEz/dy (2)
+        "ysweep": n * (18 * s + 10 * d),
+        # x-derivative: read u (8), Ey and Ez (2), write d/dx of all (10)
+        "xderiv": n * (10 * s + 10 * d),
         # RHS + RK, average of the three stages: read u (8) and dudx, dudy
-        # (22), R_prev in stages 2-3 (8); write u (8), R_prev in stages 1-2 (8)
-        "rhs_rk": n * (16 * s + 22 * d + (16.0 / 3.0) * 2 * d),
+        # (20), R_prev in stages 2-3 (8); write u (8), R_prev in stages 1-2 (8)
+        "rhs_rk": n * (16 * s + 20 * d + (16.0 / 3.0) * 2 * d),
         # x-filter in place: read 8, write 8
         "xfilter": n * 16 * s,
     }
@@ -276,8 +277,8 @@ def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50,
             rec["rel_l2_vs_fp64_max"] = max(errs)
             rec["meets_1e-6"] = bool(max(errs) <= 1e-6)
         if xg:
-            # MFMA work of one x-derivative launch (11 fields) over its time
-            flops = ctx.xgemm_panels() * 64 * 16 * 2 * n * 11 * xg
+            # MFMA work of one x-derivative launch (10 fields) over its time
+            flops = ctx.xgemm_panels() * 64 * 16 * 2 * n * 10 * xg
             t = kern["xderiv"]["avg_ms"] * 1e-3
             rec["xderiv_mfma"] = {"bf16_flops_per_launch": flops, "TFLOP/s": flops / t / 1e12,
                                   "frac_of_2500_TFLOPs": flops / t / 2.5e15,

@@ -1,8 +1,8 @@
 // Context API: whole RK3 steps on the device with the fused schedule
-//   stage s:  y-sweep  (pending y-filter of stage s-1, d/dy of u, E, d/dy of E,
+//   stage s:  y-sweep  (pending y-filter of stage s-1, d/dy of u, E, dEx/dy, dEz/dy,
 //                       CFL limits when s == 0)
 //             [s == 0: dt finalize, the reference's calcDt + rkInit]
-//             x-derivative of u and E        (11 compact line solves)
+//             x-derivative of u, Ey and Ez   (10 compact line solves)
 //             RHS + low-storage RK update    (pointwise)
 //             x-filter of u                  (8 compact line solves)
 // which is the reference order of mhd.c:154-206 with the y-filter of each
@@ -53,9 +53,12 @@ struct mhd2d_ctx {
     GridTables tb;
     XGemmOp xg;             // built when params.xgemm != 0
     hipStream_t stream = nullptr;
-    void* u = nullptr;      // [B][11][nx][ny]: 8 state fields, then E (3)
-    void* dudx = nullptr;   // [B][11][nx][ny]: 8 state derivatives + 3 of E
-    void* dudy = nullptr;   // [B][11][nx][ny]
+    // u: [B][11][nx][ny]: 8 state fields, Ey, Ez, one scratch field (divB)
+    // dudx: [B][11][nx][ny]: d/dx of the 8 state fields, dEy/dx, dEz/dx
+    // dudy: [B][11][nx][ny]: d/dy of the 8 state fields, dEx/dy, dEz/dy
+    void* u = nullptr;
+    void* dudx = nullptr;
+    void* dudy = nullptr;
     void* rprev = nullptr;  // [B][8][nx][ny]
     RunScalars* rs = nullptr;  // [B]
     double* partial = nullptr;
@@ -155,8 +158,8 @@ hipError_t xderiv_fields(mhd2d_ctx* c, const T* in, TD* out, int nv, size_t bs_i
 template <typename T, typename TD>
 hipError_t xderiv(mhd2d_ctx* c)
 {
-    // 8 state fields and E, solved in the derivative type
-    return xderiv_fields<T, TD>(c, (const T*)c->u, (TD*)c->dudx, 11, c->su(), c->sd(), c->B,
+    // 8 state fields, Ey and Ez, solved in the derivative type
+    return xderiv_fields<T, TD>(c, (const T*)c->u, (TD*)c->dudx, 10, c->su(), c->sd(), c->B,
                                 c->rs);
 }
 

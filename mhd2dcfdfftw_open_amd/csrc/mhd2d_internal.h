@@ -140,8 +140,8 @@ template <typename T, typename TD = T>
 struct YSweepArgs {          // fused y-sweep of the driver
     T* u;                    // [8][nx][ny], in place (filtered)
     TD* dudy;                // [8][nx][ny]
-    T* E;                    // [3][nx][ny]
-    TD* dEdy;                // [3][nx][ny]
+    T* E;                    // [2][nx][ny]: Ey, Ez (for the x-solves)
+    TD* dEdy;                // [2][nx][ny]: dEx/dy, dEz/dy
     YTables tb;
     DtParams dp;
     RunScalars* rs;
@@ -180,8 +180,8 @@ struct RhsRkArgs {           // fused RHS + low-storage RK update (driver)
     T* u;
     const TD* dudx;
     const TD* dudy;
-    const TD* dEdx;
-    const TD* dEdy;
+    const TD* dEdx;          // [2][nx][ny]: dEy/dx, dEz/dx
+    const TD* dEdy;          // [2][nx][ny]: dEx/dy, dEz/dy
     TD* rprev;
     const RunScalars* rs;
     double gamma;

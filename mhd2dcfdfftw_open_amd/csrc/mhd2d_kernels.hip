@@ -96,21 +96,46 @@ __device__ __forceinline__ void block_min_dt(double v, RunScalars* rs, double* r
 #ifndef MHD2D_X_MINB
 #define MHD2D_X_MINB 1
 #endif
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  For 8-column strips (two strips per 128-byte FP64 line segment) the
+// linear workgroup id is remapped so that every XCD takes one contiguous
+// range of (strip, field, instance) tiles and neighbouring strips meet in
+// one L2 (measured -12 % x-derivative, -17 % x-filter time at nx = 4096).
+// 16-column FP64 strips fill whole lines; there the remap measured slower.
+template <int W>
+__device__ __forceinline__ void xcd_tile(int& x, int& y, int& z)
+{
+    x = blockIdx.x;
+    y = blockIdx.y;
+    z = blockIdx.z;
+    if constexpr (W == 8) {
+        const unsigned S = gridDim.x, V = gridDim.y;
+        const unsigned T = S * V * gridDim.z;
+        if ((T & 7u) == 0) {
+            const unsigned L = blockIdx.x + S * (blockIdx.y + V * blockIdx.z);
+            const unsigned Lp = (L & 7u) * (T >> 3) + (L >> 3);
+            x = (int)(Lp % S);
+            y = (int)((Lp / S) % V);
+            z = (int)(Lp / (S * V));
+        }
+    }
+}
+
 // BLK: x is split into blocks (and may have open ends): the rows marked in
 // tb.omask take explicit stencils over f[i-4 .. i+4] (tb.ocoef); otherwise
 // rows 0 and nx-1 are the only end rows.
 template <typename T, typename TI, int C, int W, int KIND, bool BLK>
 __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, TI> a)
 {
-    const int z = blockIdx.z;  // instance of a batch
+    int bx, v, z;  // column strip, field, instance of a batch
+    xcd_tile<W>(bx, v, z);
     if (a.rs && a.rs[z].stop) return;  // the run stopped (dt < 1e-8): uniform per block
     const int G = blockDim.x / W;
     const int c = threadIdx.x % W;
     const int g = threadIdx.x / W;
-    const int v = blockIdx.y;
     const int nx = a.nx;
     const size_t ny = (size_t)a.ny;
-    const size_t j = (size_t)blockIdx.x * W + c;
+    const size_t j = (size_t)bx * W + c;
     const TI* __restrict__ f = a.in[v] + z * a.bs_in + j;
     T* __restrict__ o = a.out[v] + z * a.bs_out + j;
     const int s = g * C;
@@ -302,7 +327,9 @@ void xsolve_shape(int nx, int elem_bytes, int* chunk, int* width, int* groups)
     int C = nx / 64;
     if (C < 8) C = 8;
     int W = MHD2D_X_W;
-    if (C > 32) W = 8;  // 64-row chunks: 8-column strips keep 512 threads per block
+    // 64-row chunks: 8-column strips keep 512 threads per block; lines of
+    // <= 512 rows (64-thread columns) measured faster in 8-column strips too
+    if (C > 32 || nx <= 512) W = 8;
     *chunk = C;
     *width = W;
     *groups = nx / C;
@@ -534,9 +561,11 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
     }
 
     if (a.derivs) {
-        // E = -u x B of the filtered state (calcRHS.c:24-43) and its d/dy
+        // E = -u x B of the filtered state (calcRHS.c:24-43).  The induction
+        // terms take dEz/dy, dEz/dx, dEx/dy and dEy/dx (calcRHS.c:135-137):
+        // Ey and Ez are stored for the x-solves, and (Ex, Ez) share one
+        // complex line for d/dy; Ex itself and dEy/dy are never needed.
         cx<T> v[E];
-        T ez[E];
 #pragma unroll
         for (int m = 0; m < E; m++) {
             const size_t k = row + t + m * TPR;
@@ -544,12 +573,11 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
             const T bx = u[BX * n + k], by = u[BY * n + k], bz = u[BZ * n + k];
             const T ex = uz * by - uy * bz;
             const T ey = ux * bz - uz * bx;
-            ez[m] = uy * bx - ux * by;
-            v[m] = cx<T>{ex, ey};
+            const T ez = uy * bx - ux * by;
+            v[m] = cx<T>{ex, ez};
             if (active) {
-                Ef[0 * n + k] = ex;
-                Ef[1 * n + k] = ey;
-                Ef[2 * n + k] = ez[m];
+                Ef[0 * n + k] = ey;
+                Ef[1 * n + k] = ez;
             }
         }
         fft_fwd<T, N, E>(v, t, lre, lim, tw);
@@ -563,16 +591,6 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
                 dEdy[0 * n + k] = (TD)v[m].re;
                 dEdy[1 * n + k] = (TD)v[m].im;
             }
-        }
-#pragma unroll
-        for (int m = 0; m < E; m++) v[m] = cx<T>{ez[m], (T)0};
-        fft_fwd<T, N, E>(v, t, lre, lim, tw);
-#pragma unroll
-        for (int m = 0; m < E; m++) v[m] = mul_i_scal(v[m], (T)a.tb.md[t + m * TPR]);
-        fft_inv<T, N, E>(v, t, lre, lim, tw);
-        if (active) {
-#pragma unroll
-            for (int m = 0; m < E; m++) dEdy[2 * n + row + t + m * TPR] = (TD)v[m].re;
         }
     }
 
@@ -791,7 +809,7 @@ __global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T, TD> a)
          q += (size_t)gridDim.x * blockDim.x) {
         const size_t p = q * W;
         Pack<T, W> u[8];
-        Pack<TD, W> dx[8], dy[8], ex[3], ey[3];
+        Pack<TD, W> dx[8], dy[8], ex[2], ey[2];
 #pragma unroll
         for (int f = 0; f < 8; f++) {
             u[f] = ldp<W>(uz + f * n + p);
@@ -799,7 +817,7 @@ __global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T, TD> a)
             dy[f] = ldp<W>(dyz + f * n + p);
         }
 #pragma unroll
-        for (int f = 0; f < 3; f++) {
+        for (int f = 0; f < 2; f++) {
             ex[f] = ldp<W>(exz + f * n + p);
             ey[f] = ldp<W>(eyz + f * n + p);
         }
@@ -824,9 +842,10 @@ __global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T, TD> a)
             R[UX].x[w] = -ux * d_x[UX] - uy * d_y[UX] - d_x[PR] * ir + jy * bz * ir - jz * by * ir;
             R[UY].x[w] = -ux * d_x[UY] - uy * d_y[UY] - d_y[PR] * ir + jz * bx * ir - jx * bz * ir;
             R[UZ].x[w] = -ux * d_x[UZ] - uy * d_y[UZ] + jx * by * ir - jy * bx * ir;
-            R[BX].x[w] = -(T)ey[2].x[w];
-            R[BY].x[w] = (T)ex[2].x[w];
-            R[BZ].x[w] = (T)ey[0].x[w] - (T)ex[1].x[w];
+            // calcRHS.c:135-137; ex = (dEy/dx, dEz/dx), ey = (dEx/dy, dEz/dy)
+            R[BX].x[w] = -(T)ey[1].x[w];
+            R[BY].x[w] = (T)ex[1].x[w];
+            R[BZ].x[w] = (T)ey[0].x[w] - (T)ex[0].x[w];
             R[PR].x[w] = -ux * d_x[PR] - uy * d_y[PR] - g * pr * divU;
         }
 #pragma unroll
