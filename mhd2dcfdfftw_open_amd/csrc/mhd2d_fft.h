@@ -7,10 +7,10 @@
 // elements at positions t + m*N/E, m = 0..E-1 ("strided" layout), so loads
 // and stores of whole lines are coalesced and pointwise work between a
 // forward and an inverse transform needs no data exchange.  Between passes
-// the line goes through LDS as separate real and imaginary arrays whose
-// word index is XOR-swizzled (fft_swz) so that both the E-strided writes and
-// the unit-stride reads of every pass are free of bank conflicts, with no
-// padding.
+// the line goes through LDS as separate real and imaginary arrays, padded
+// (fft_lds) so that both the E-strided writes and the unit-stride reads of
+// every exchange are free of bank conflicts and every thread addresses its
+// E words from one base register with immediate offsets.
 //
 // Twiddles come from a table of exp(-2 pi i t/N) (FP64, host-computed in
 // long double): per butterfly four entries are read and the rest formed by
@@ -151,22 +151,33 @@ struct FFTShape {
     static constexpr int PE = M / LE, REM = M % LE;
     static constexpr int NP = PE + (REM ? 1 : 0);
     static constexpr int TPR = N / E;          // threads per line
-    static constexpr int LDS = N;              // words per array
+    static constexpr int LDS = N + N / E;      // words per array (padded)
     static constexpr int radix(int p) { return p < PE ? E : (1 << REM); }
 };
 
-// LDS word of line position p: a bijection of [0, N) that keeps the bits
-// above the low five (E = 8, FP64) or six (E = 16, FP32) and mixes higher
-// bits into them.  Checked exhaustively for every exchange of N = 16..4096:
-// each group of lanes served together by the LDS (16 / 32 lanes of 8-byte
-// words, 32 / 64 lanes of 4-byte words) hits distinct banks, for the
-// E-strided writes (position stride E within a group) as for the
-// unit-stride reads.
-template <int E>
-__device__ __forceinline__ int fft_swz(int p)
+// LDS word of line position p in the exchange that follows a pass with
+// sub-transform length NS (1, E, E^2, ...).  The LDS serves G lanes per
+// cycle: 16 lanes of 8-byte words (E = 8, FP64) or 32 lanes of 4-byte words
+// (E = 16, FP32), one word per slot of G.  Such a group writes positions
+// (t / NS) NS E + t % NS + r NS: runs of NS consecutive positions E NS
+// apart (stride E for NS == 1), and reads G consecutive positions.
+//   NS == 1:  p + p / G          stride-E writes land on distinct slots
+//   NS <  G:  p + NS p / (E NS)  the G / NS runs of a group shift apart
+//   NS >= G:  p                  writes are already consecutive
+// Every term is additive in the per-thread base and the compile-time
+// offsets r NS (writes) and b N/E + r N/R (reads), so each thread forms one
+// address per exchange side.  Injective, below N + N/E, conflict-free and
+// additive for every exchange of N = 16..4096 (checked exhaustively by the
+// CPU tests).
+template <int E, int NS>
+__host__ __device__ constexpr int fft_lds(int p)
 {
-    if constexpr (E == 8) return p ^ ((p >> 3) & 31);
-    else return p ^ (((p >> 3) ^ (p >> 4)) & 63);
+    constexpr int G = E == 8 ? 16 : 32;
+    constexpr int LG = E == 8 ? 4 : 5;
+    constexpr int LEN = fft_ilog2(E * NS);
+    if constexpr (NS == 1) return p + (p >> LG);
+    else if constexpr (NS < G) return p + ((p >> LEN) * NS);
+    else return p;
 }
 
 // v[r] *= W^(r k STEP), r = 1..R-1, from four table reads (idx1 = k STEP)
@@ -230,26 +241,29 @@ __device__ __forceinline__ void fft_pass(cx<T> (&v)[E], int t, T* lre, T* lim,
         dftR<T, R>(&v[b * R]);
     }
     if constexpr (P + 1 < S::NP) {
-        // intermediate passes are radix E (B == 1); the LDS indices are
-        // recomputed from an opaque t rather than hoisted into registers
+        // intermediate passes are radix E (B == 1); one base word per side,
+        // the rest are immediate offsets.  The bases are recomputed from an
+        // opaque t in every exchange instead of being hoisted into registers
+        // for the whole kernel.
         asm volatile("" : "+v"(t));
         constexpr int R2 = S::radix(P + 1);
         constexpr int B2 = E / R2;
-        const int base = (t / NS) * NS * R + (t & (NS - 1));
+        T* const wre = lre + fft_lds<E, NS>((t / NS) * NS * R + (t & (NS - 1)));
+        T* const wim = lim + (wre - lre);
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int q = fft_swz<E>(base + r * NS);
-            lre[q] = v[r].re;
-            lim[q] = v[r].im;
+            wre[fft_lds<E, NS>(r * NS)] = v[r].re;
+            wim[fft_lds<E, NS>(r * NS)] = v[r].im;
         }
         __syncthreads();
+        const T* const rre = lre + fft_lds<E, NS>(t);
+        const T* const rim = lim + fft_lds<E, NS>(t);
 #pragma unroll
         for (int b = 0; b < B2; b++) {
-            const int j = t + b * TPR;
 #pragma unroll
             for (int r = 0; r < R2; r++) {
-                const int q = fft_swz<E>(j + r * (N / R2));
-                v[b * R2 + r] = cx<T>{lre[q], lim[q]};
+                const int o = fft_lds<E, NS>(b * TPR + r * (N / R2));
+                v[b * R2 + r] = cx<T>{rre[o], rim[o]};
             }
         }
         __syncthreads();

@@ -503,6 +503,15 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
     constexpr int PA[4] = {RHO, UY, BX, BZ};
     constexpr int PB[4] = {UX, UZ, BY, PR};
     if (a.filter || a.derivs) {
+        // The next pair is loaded while the last transform of the current
+        // pair runs (its registers replace the saved spectrum Z), so the HBM
+        // latency of every pair but the first is hidden behind FFT work.
+        cx<T> nxt[E];
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const int k = t + m * TPR;
+            nxt[m] = cx<T>{u[PA[0] * n + row + k], u[PB[0] * n + row + k]};
+        }
 #pragma unroll 1
         for (int p = 0; p < 4; p++) {
             const int fa = PA[p], fb = PB[p];
@@ -514,10 +523,7 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
             asm volatile("" : "+v"(tq));
             cx<T> v[E];
 #pragma unroll
-            for (int m = 0; m < E; m++) {
-                const int k = t + m * TPR;
-                v[m] = cx<T>{ua[k], ub[k]};
-            }
+            for (int m = 0; m < E; m++) v[m] = nxt[m];
             fft_fwd<T, N, E>(v, t, lre, lim, tw);
             if (a.filter) {
                 cx<T> Z[E];
@@ -543,6 +549,15 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
             } else {
 #pragma unroll
                 for (int m = 0; m < E; m++) v[m] = mul_i_scal(v[m], (T)a.tb.md[tq + m * TPR]);
+            }
+            if (p < 3) {
+                const T* na = u + PA[p + 1] * n + row;
+                const T* nb = u + PB[p + 1] * n + row;
+#pragma unroll
+                for (int m = 0; m < E; m++) {
+                    const int k = t + m * TPR;
+                    nxt[m] = cx<T>{na[k], nb[k]};
+                }
             }
             if (a.derivs) {
                 fft_inv<T, N, E>(v, t, lre, lim, tw);
