@@ -114,6 +114,7 @@ __device__ __forceinline__ void st_stream(T* p, T v)
 }
 
 // ============================================================== x solves
+constexpr bool xsolve_lds_tables(int C) { return C <= 32; }
 #ifndef MHD2D_X_MINB
 #define MHD2D_X_MINB 1
 #endif
@@ -238,9 +239,34 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
     if (first) w[0] = rfirst;
     if (last) w[C - 1] = rlast;
 
-    const double* __restrict__ L = a.tb.l + s;
-    const double* __restrict__ ID = a.tb.invd + s;
-    const double* __restrict__ Q = a.tb.q + s;
+    // Factor rows.  For lines of up to 2048 rows (C <= 32) the three tables
+    // of the whole line go to LDS (3 nx words, dynamic) and the recurrences
+    // read them as broadcasts instead of 6 C vector loads per thread
+    // (measured -8 % x-solve time at nx = 2048, -11 % at 512); at nx = 4096
+    // the 96 KB fill per workgroup costs more than it saves.
+    using FT = typename std::conditional<xsolve_lds_tables(C), T, double>::type;
+    const FT* L;
+    const FT* ID;
+    const FT* Q;
+    if constexpr (xsolve_lds_tables(C)) {
+        extern __shared__ unsigned char x_dyn[];
+        T* const tl = reinterpret_cast<T*>(x_dyn);
+        T* const tid = tl + nx;
+        T* const tqq = tid + nx;
+        for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+            tl[i] = (T)a.tb.l[i];
+            tid[i] = (T)a.tb.invd[i];
+            tqq[i] = (T)a.tb.q[i];
+        }
+        __syncthreads();
+        L = tl + s;
+        ID = tid + s;
+        Q = tqq + s;
+    } else {
+        L = a.tb.l + s;
+        ID = a.tb.invd + s;
+        Q = a.tb.q + s;
+    }
 
     // Rows from tb.kconv to N-2 carry the factors' exact FP64 fixed point,
     // so a chunk inside that range uses the scalar constants (no loads).
@@ -362,15 +388,16 @@ void xsolve_shape(int nx, int elem_bytes, int* chunk, int* width, int* groups)
 template <typename T, typename TI, int C, int W, bool BLK>
 static void launch_xsolve_kind(const XSolveArgs<T, TI>& a, dim3 grid, dim3 block, hipStream_t s)
 {
+    const size_t shm = xsolve_lds_tables(C) ? 3 * (size_t)a.nx * sizeof(T) : 0;
     if constexpr (std::is_same<T, TI>::value) {
         switch (a.kind) {
-        case XD1: hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD1, BLK>), grid, block, 0, s, a); break;
-        case XD2: hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD2, BLK>), grid, block, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XFILT, BLK>), grid, block, 0, s, a); break;
+        case XD1: hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD1, BLK>), grid, block, shm, s, a); break;
+        case XD2: hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD2, BLK>), grid, block, shm, s, a); break;
+        default: hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XFILT, BLK>), grid, block, shm, s, a); break;
         }
     } else {
         // converting inputs: first derivatives of a FP64 state only
-        hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD1, BLK>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD1, BLK>), grid, block, shm, s, a);
     }
 }
 
@@ -424,9 +451,11 @@ struct YElems {
     static constexpr int E = sizeof(T) == 8 ? MHD2D_Y_E64 : 16;
 };
 
-// optional occupancy request for the fused y-sweep (0: compiler's choice)
+// occupancy request for the fused y-sweep: 4 waves per SIMD (128 VGPRs, a
+// few spills) measured -17 % y-sweep time at 4096^2 and neutral at 2048^2
+// (0: compiler's choice)
 #ifndef MHD2D_Y_WAVES
-#define MHD2D_Y_WAVES 0
+#define MHD2D_Y_WAVES 4
 #endif
 #if MHD2D_Y_WAVES > 0
 #define MHD2D_Y_ATTR __attribute__((amdgpu_waves_per_eu(MHD2D_Y_WAVES)))
