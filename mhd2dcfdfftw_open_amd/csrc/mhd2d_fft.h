@@ -13,8 +13,9 @@
 // E words from one base register with immediate offsets.
 //
 // Twiddles come from a table of exp(-2 pi i t/N) (FP64, host-computed in
-// long double): per butterfly four entries are read and the rest formed by
-// at most three complex products.
+// long double): per butterfly the FP64 transform reads one entry and forms
+// the rest by squarings and complex products; the FP32 transform reads
+// three (four for radix 16).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -157,8 +158,8 @@ struct FFTShape {
 
 // LDS word of line position p in the exchange that follows a pass with
 // sub-transform length NS (1, E, E^2, ...).  The LDS serves G lanes per
-// cycle: 16 lanes of 8-byte words (E = 8, FP64) or 32 lanes of 4-byte words
-// (E = 16, FP32), one word per slot of G.  Such a group writes positions
+// cycle: 16 lanes of 8-byte words (FP64) or 32 lanes of 4-byte words (FP32),
+// one word per slot of G.  Such a group writes positions
 // (t / NS) NS E + t % NS + r NS: runs of NS consecutive positions E NS
 // apart (stride E for NS == 1), and reads G consecutive positions.
 //   NS == 1:  p + p / G          stride-E writes land on distinct slots
@@ -167,21 +168,29 @@ struct FFTShape {
 // Every term is additive in the per-thread base and the compile-time
 // offsets r NS (writes) and b N/E + r N/R (reads), so each thread forms one
 // address per exchange side.  Injective, below N + N/E, conflict-free and
-// additive for every exchange of N = 16..4096 (checked exhaustively by the
-// CPU tests).
-template <int E, int NS>
+// additive for every exchange of N = 16..4096 with E = 8 or 16 (checked
+// exhaustively by the CPU tests).
+template <int E, int NS, int G>
 __host__ __device__ constexpr int fft_lds(int p)
 {
-    constexpr int G = E == 8 ? 16 : 32;
-    constexpr int LG = E == 8 ? 4 : 5;
+    constexpr int LG = fft_ilog2(G);
     constexpr int LEN = fft_ilog2(E * NS);
     if constexpr (NS == 1) return p + (p >> LG);
     else if constexpr (NS < G) return p + ((p >> LEN) * NS);
     else return p;
 }
 
-// v[r] *= W^(r k STEP), r = 1..R-1, from four table reads (idx1 = k STEP)
-// and at most three complex products per factor
+// w^2 of a unit complex number: (re + im)(re - im) keeps cos 2a accurate
+template <typename T>
+__device__ __forceinline__ cx<T> csq(cx<T> w)
+{
+    return {(w.re + w.im) * (w.re - w.im), (T)2 * (w.re * w.im)};
+}
+
+// v[r] *= W^(r k STEP), r = 1..R-1 (idx1 = k STEP).  FP64: one table read
+// per butterfly, W^2k and W^4k by squaring and the rest by complex
+// products (a few ulp; the table reads were a quarter of the y-sweep's
+// transform time); FP32: W, W^2, W^4 (and W^8) from the FP64 table.
 template <typename T, int R>
 __device__ __forceinline__ void twiddle(cx<T>* v, int idx1, const double2* __restrict__ tw)
 {
@@ -189,18 +198,29 @@ __device__ __forceinline__ void twiddle(cx<T>* v, int idx1, const double2* __res
     // index keeps the compiler from hoisting them all (tens of registers)
     // out of the callers' loops.  The empty asm only moves a register.
     asm volatile("" : "+v"(idx1));
+    constexpr bool SQ = sizeof(T) == 8;
     const double2 a1 = tw[idx1];
     const cx<T> w1{(T)a1.x, (T)a1.y};
     v[1] = cmul(v[1], w1);
     if constexpr (R > 2) {
-        const double2 a2 = tw[2 * idx1];
-        const cx<T> w2{(T)a2.x, (T)a2.y};
+        cx<T> w2;
+        if constexpr (SQ) {
+            w2 = csq(w1);
+        } else {
+            const double2 a2 = tw[2 * idx1];
+            w2 = cx<T>{(T)a2.x, (T)a2.y};
+        }
         const cx<T> w3 = cmul(w2, w1);
         v[2] = cmul(v[2], w2);
         v[3] = cmul(v[3], w3);
         if constexpr (R > 4) {
-            const double2 a4 = tw[4 * idx1];
-            const cx<T> w4{(T)a4.x, (T)a4.y};
+            cx<T> w4;
+            if constexpr (SQ) {
+                w4 = csq(w2);
+            } else {
+                const double2 a4 = tw[4 * idx1];
+                w4 = cx<T>{(T)a4.x, (T)a4.y};
+            }
             v[4] = cmul(v[4], w4);
             v[5] = cmul(v[5], cmul(w4, w1));
             v[6] = cmul(v[6], cmul(w4, w2));
@@ -248,21 +268,22 @@ __device__ __forceinline__ void fft_pass(cx<T> (&v)[E], int t, T* lre, T* lim,
         asm volatile("" : "+v"(t));
         constexpr int R2 = S::radix(P + 1);
         constexpr int B2 = E / R2;
-        T* const wre = lre + fft_lds<E, NS>((t / NS) * NS * R + (t & (NS - 1)));
+        constexpr int G = 128 / sizeof(T);  // lanes per LDS cycle
+        T* const wre = lre + fft_lds<E, NS, G>((t / NS) * NS * R + (t & (NS - 1)));
         T* const wim = lim + (wre - lre);
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            wre[fft_lds<E, NS>(r * NS)] = v[r].re;
-            wim[fft_lds<E, NS>(r * NS)] = v[r].im;
+            wre[fft_lds<E, NS, G>(r * NS)] = v[r].re;
+            wim[fft_lds<E, NS, G>(r * NS)] = v[r].im;
         }
         __syncthreads();
-        const T* const rre = lre + fft_lds<E, NS>(t);
-        const T* const rim = lim + fft_lds<E, NS>(t);
+        const T* const rre = lre + fft_lds<E, NS, G>(t);
+        const T* const rim = lim + fft_lds<E, NS, G>(t);
 #pragma unroll
         for (int b = 0; b < B2; b++) {
 #pragma unroll
             for (int r = 0; r < R2; r++) {
-                const int o = fft_lds<E, NS>(b * TPR + r * (N / R2));
+                const int o = fft_lds<E, NS, G>(b * TPR + r * (N / R2));
                 v[b * R2 + r] = cx<T>{rre[o], rim[o]};
             }
         }

@@ -6,8 +6,8 @@ word per thread and exchange side and reaches the other words through
 compile-time offsets, so the layout must be additive in those offsets; it
 must also be injective within the padded array of N + N/E words and free of
 LDS bank conflicts (16 lanes of 8-byte words per LDS cycle for the FP64
-transform, E = 8; 32 lanes of 4-byte words for FP32, E = 16; the lane
-grouping was measured with SQ_LDS_BANK_CONFLICT on gfx950).  The formula is
+transform; 32 lanes of 4-byte words for FP32; the lane grouping was
+measured with SQ_LDS_BANK_CONFLICT on gfx950).  The formula is
 restated here from the header and checked exhaustively for every supported
 line length.
 """
@@ -16,8 +16,7 @@ import math
 import pytest
 
 
-def fft_lds(E, NS, p):
-    G = 16 if E == 8 else 32
+def fft_lds(E, NS, G, p):
     if NS == 1:
         return p + p // G
     if NS < G:
@@ -53,26 +52,26 @@ def worst_conflict(words, lanes):
     return worst
 
 
-CASES = [(8, N) for N in (16, 32, 64, 128, 256, 512, 1024, 2048, 4096)] + \
-        [(16, N) for N in (32, 64, 128, 256, 512, 1024, 2048, 4096)]
+# (elements per thread, LDS lanes per cycle = 128 B / word size, N)
+CASES = [(E, G, N) for E in (8, 16) for G in (16, 32)
+         for N in (16, 32, 64, 128, 256, 512, 1024, 2048, 4096) if N >= E]
 
 
-@pytest.mark.parametrize("E,N", CASES)
-def test_layout_injective_additive_conflict_free(E, N):
-    lanes = 16 if E == 8 else 32
+@pytest.mark.parametrize("E,G,N", CASES)
+def test_layout_injective_additive_conflict_free(E, G, N):
     for NS, TPR, writes, reads in exchanges(N, E):
-        L = [fft_lds(E, NS, p) for p in range(N)]
+        L = [fft_lds(E, NS, G, p) for p in range(N)]
         assert len(set(L)) == N and max(L) < N + N // E
         for group in (writes, reads):
-            base = [fft_lds(E, NS, p) for p in group[0]]
+            base = [fft_lds(E, NS, G, p) for p in group[0]]
             for pat in group:
                 # the offset the kernel uses is the layout of the t = 0 position
-                off = fft_lds(E, NS, pat[0])
-                assert all(fft_lds(E, NS, p) - b == off for p, b in zip(pat, base))
-                if TPR >= lanes:
+                off = fft_lds(E, NS, G, pat[0])
+                assert all(fft_lds(E, NS, G, p) - b == off for p, b in zip(pat, base))
+                if TPR >= G:
                     # rows of fewer than one lane group share groups; only
                     # full groups are priced
-                    assert worst_conflict([fft_lds(E, NS, p) for p in pat], lanes) == 1
+                    assert worst_conflict([fft_lds(E, NS, G, p) for p in pat], G) == 1
 
 
 def test_conflict_counter_catches_a_strided_layout():
