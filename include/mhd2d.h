@@ -115,8 +115,8 @@ typedef struct mhd2d_params {
     double cfl;
     int hall;               /* 1: whistler CFL limit with di (timeAdvance.c:97) */
     double di;
-    double resistivity;     /* must be 0 in the context API */
-    double viscosity;       /* must be 0 in the context API */
+    double resistivity;     /* >= 0: eta Lap B (addSourceTerm, calcRHS.c:498-531) */
+    double viscosity;       /* >= 0: nu/rho Lap u (same) */
     int divbclean;          /* must be 0 in the context API */
     int precision;          /* MHD2D_FP64, MHD2D_FP32 or MHD2D_MIXED */
     int xblocks;            /* 0 or 1: x is one block (the reference on one MPI
@@ -271,8 +271,8 @@ enum { MHD2D_DRIVE_AUTO = 0, MHD2D_DRIVE_FUSED = 1, MHD2D_DRIVE_DROPIN = 2 };
 
 typedef struct mhd2d_drive_opts {
     const char *outdir;     /* directory of the output files ("" = cwd) */
-    int mode;               /* MHD2D_DRIVE_*: AUTO = FUSED for ideal MHD,
-                               DROPIN otherwise */
+    int mode;               /* MHD2D_DRIVE_*: AUTO = FUSED without div B
+                               cleaning, DROPIN otherwise */
     int precision;          /* MHD2D_FP64 / MHD2D_FP32 (FUSED only) */
     int verbose;            /* print the reference's progress lines */
     long long max_steps;    /* stop after this many steps (0: no cap) */

@@ -60,6 +60,10 @@ struct mhd2d_ctx {
     void* dudx = nullptr;
     void* dudy = nullptr;
     void* rprev = nullptr;  // [B][8][nx][ny]
+    // [B][6][nx][ny] (derivative type), with viscosity or resistivity: the
+    // Laplacians of ux, uy, uz and Bx, By, Bz (d2/dy2 from the y-sweep, then
+    // d2/dx2 added by the x-solve)
+    void* lap = nullptr;
     RunScalars* rs = nullptr;  // [B]
     double* partial = nullptr;
     int npartial = 0;
@@ -71,6 +75,9 @@ struct mhd2d_ctx {
     size_t su() const { return 11 * n; }  // per-instance strides (elements)
     size_t sd() const { return 11 * n; }
     size_t sr() const { return 8 * n; }
+    size_t sl() const { return 6 * n; }
+    bool lap_u() const { return p.viscosity > 0; }
+    bool lap_b() const { return p.resistivity > 0; }
 };
 
 namespace {
@@ -108,6 +115,12 @@ hipError_t ysweep(mhd2d_ctx* c, int filter, int derivs, int dt_limits)
     a.dt_limits = dt_limits;
     a.bs_u = c->su();
     a.bs_d = c->sd();
+    if (c->lap && derivs) {
+        a.lap = (TD*)c->lap;
+        a.lap_u = c->lap_u();
+        a.lap_b = c->lap_b();
+        a.bs_l = c->sl();
+    }
     return launch_ysweep<T, TD>(a, c->stream, c->B);
 }
 
@@ -159,8 +172,27 @@ template <typename T, typename TD>
 hipError_t xderiv(mhd2d_ctx* c)
 {
     // 8 state fields, Ey and Ez, solved in the derivative type
-    return xderiv_fields<T, TD>(c, (const T*)c->u, (TD*)c->dudx, 10, c->su(), c->sd(), c->B,
-                                c->rs);
+    hipError_t e = xderiv_fields<T, TD>(c, (const T*)c->u, (TD*)c->dudx, 10, c->su(), c->sd(),
+                                        c->B, c->rs);
+    if (e != hipSuccess || !c->lap) return e;
+    // second x-derivatives of ux..uz (viscosity) and / or Bx..Bz
+    // (resistivity), added onto the second y-derivatives in lap
+    const int f0 = c->lap_u() ? UX : BX, f1 = c->lap_b() ? BZ : UZ;
+    XSolveArgs<TD, T> a;
+    for (int v = 0; v <= f1 - f0; v++) {
+        a.in[v] = (const T*)c->u + (f0 + v) * c->n;
+        a.out[v] = (TD*)c->lap + (f0 + v - UX) * c->n;
+    }
+    a.rs = c->rs;
+    a.st = c->tb.st[XD2];
+    a.tb = c->tb.xt[XD2];
+    a.kind = XD2;
+    a.nx = c->p.nx;
+    a.ny = c->p.ny;
+    a.accumulate = 1;
+    a.bs_in = c->su();
+    a.bs_out = c->sl();
+    return launch_xsolve<TD, T>(a, f1 - f0 + 1, c->stream, c->B);
 }
 
 template <typename T>
@@ -202,6 +234,12 @@ hipError_t rhs_rk(mhd2d_ctx* c, int stage)
     a.open_x = c->p.open_x;
     a.nx = c->p.nx;
     a.ny = c->p.ny;
+    if (c->lap) {
+        a.lap = (const TD*)c->lap;
+        a.bs_l = c->sl();
+        a.viscosity = c->p.viscosity;
+        a.resistivity = c->p.resistivity;
+    }
     hipError_t e = launch_rhs_rk<T, TD>(a, c->stream, c->B);
     if (e == hipSuccess && c->p.open_x) e = launch_rhs_rk_open<T, TD>(a, c->stream, c->B);
     return e;
@@ -309,10 +347,12 @@ int check_params(const mhd2d_params* p)
         return fail(MHD2D_EINVAL, "nx and ny must be powers of two in [16, 4096]");
     if (!(p->Lx > 0) || !(p->Ly > 0) || !(p->cfl > 0) || !(p->gamma > 1))
         return fail(MHD2D_EINVAL, "Lx, Ly, cfl must be > 0 and gamma > 1");
-    if (p->resistivity != 0 || p->viscosity != 0 || p->divbclean != 0)
+    if (!(p->resistivity >= 0) || !(p->viscosity >= 0))
+        return fail(MHD2D_EINVAL, "resistivity and viscosity must be >= 0");
+    if (p->divbclean != 0)
         return fail(MHD2D_EINVAL,
-                    "the context API runs the ideal path only (resistivity = viscosity = "
-                    "divbclean = 0); use the drop-in entry points for the dissipative terms");
+                    "the context API runs divbclean = 0 only; use the drop-in entry points "
+                    "for div B cleaning");
     if (p->precision != MHD2D_FP64 && p->precision != MHD2D_FP32 &&
         p->precision != MHD2D_MIXED)
         return fail(MHD2D_EINVAL, "precision must be MHD2D_FP64, MHD2D_FP32 or MHD2D_MIXED");
@@ -374,6 +414,14 @@ int mhd2d_create(const mhd2d_params* p, mhd2d_ctx** out)
         }
         c->bytes += sizes[k];
     }
+    if (c->lap_u() || c->lap_b()) {
+        e = hipMalloc(&c->lap, 6 * db);
+        if (e != hipSuccess) {
+            mhd2d_destroy(c);
+            return fail(MHD2D_ENOMEM, "device allocation", e);
+        }
+        c->bytes += 6 * db;
+    }
     e = hipMalloc(&c->rs, sizeof(RunScalars) * c->B);
     if (e == hipSuccess) e = hipMalloc(&c->partial, 2 * c->npartial * sizeof(double));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -390,7 +438,7 @@ void mhd2d_destroy(mhd2d_ctx* c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
-    void* ptrs[] = {c->u, c->dudx, c->dudy, c->rprev, c->rs, c->partial};
+    void* ptrs[] = {c->u, c->dudx, c->dudy, c->rprev, c->lap, c->rs, c->partial};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -2,7 +2,8 @@
 // the GPU: output cadence dtOut, log cadence dtOut/10, the dt < 1e-8 stop,
 // and the final output at Tmax.  Two engines:
 //   MHD2D_DRIVE_FUSED  - the context API (fused kernels, dt on the device);
-//                        ideal MHD only.
+//                        Hall MHD with viscosity and resistivity, no div B
+//                        cleaning.
 //   MHD2D_DRIVE_DROPIN - the reference's own call sequence through the
 //                        drop-in entry points on the host globals; any
 //                        supported setting (viscosity, resistivity,
@@ -56,6 +57,8 @@ int drive_fused(const mhd2d_input* in, const double* uu0, const mhd2d_drive_opts
     p.cfl = in->cfl;
     p.hall = in->hall;
     p.di = in->di;
+    p.resistivity = in->resistivity;
+    p.viscosity = in->viscosity;
     p.precision = o->precision;
     p.xblocks = o->xblocks;
     p.open_x = in->boundaryTypeX == 1;
@@ -254,16 +257,16 @@ extern "C" int mhd2d_drive(const mhd2d_input* in, const double* uu0, const mhd2d
                              "npe >= 2 x blocks\n");
         return MHD2D_EINVAL;
     }
-    const bool ideal = in->resistivity == 0 && in->viscosity == 0 && in->divbclean == 0;
+    const bool fusable = in->divbclean == 0;
     int mode = o->mode;
-    if (mode == MHD2D_DRIVE_AUTO) mode = ideal ? MHD2D_DRIVE_FUSED : MHD2D_DRIVE_DROPIN;
+    if (mode == MHD2D_DRIVE_AUTO) mode = fusable ? MHD2D_DRIVE_FUSED : MHD2D_DRIVE_DROPIN;
     if (mode == MHD2D_DRIVE_DROPIN && o->xblocks > 1) {
         std::fprintf(stderr, "mhd2d_drive: x blocks (npe > 1) need the fused engine\n");
         return MHD2D_EINVAL;
     }
     if (mode == MHD2D_DRIVE_FUSED) {
-        if (!ideal) {
-            std::fprintf(stderr, "mhd2d_drive: the fused engine runs ideal MHD only\n");
+        if (!fusable) {
+            std::fprintf(stderr, "mhd2d_drive: the fused engine runs divBcleaning = 0 only\n");
             return MHD2D_EINVAL;
         }
         return drive_fused(in, uu0, o, rep);

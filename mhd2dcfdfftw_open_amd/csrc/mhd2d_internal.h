@@ -62,6 +62,7 @@ struct XSolveArgs {
     const RunScalars* rs = nullptr;  // optional: skip the launch once the run has stopped
     int kind;
     int nx, ny;
+    int accumulate = 0;      // out += result (second x-derivatives onto the y part)
     // instance z of a batch (blockIdx.z) reads in[v] + z * bs_in, writes
     // out[v] + z * bs_out and checks rs[z]
     size_t bs_in = 0, bs_out = 0;
@@ -101,6 +102,7 @@ struct YTables {
     const double* mdd;
     const double* mf;
     const double* mfd;
+    const double* mfdd;      // filter times second derivative
     const double2* tw;
 };
 
@@ -149,9 +151,14 @@ struct YSweepArgs {          // fused y-sweep of the driver
     int filter;              // apply the pending y filter first
     int dt_limits;           // reduce the local dt limits into rs
     int derivs;              // compute the y derivatives and E
+    // second y-derivatives (with derivs) for the source terms: of ux, uy, uz
+    // (viscosity) into lap[0..2] and of Bx, By, Bz (resistivity) into
+    // lap[3..5]; the x-solve adds the second x-derivatives
+    TD* lap = nullptr;       // [6][nx][ny]
+    int lap_u = 0, lap_b = 0;
     // instance z of a batch (blockIdx.y): u, E + z * bs_u; dudy, dEdy +
-    // z * bs_d; rs + z (whose di replaces dp.di)
-    size_t bs_u = 0, bs_d = 0;
+    // z * bs_d; lap + z * bs_l; rs + z (whose di replaces dp.di)
+    size_t bs_u = 0, bs_d = 0, bs_l = 0;
 };
 
 template <typename T>
@@ -183,14 +190,16 @@ struct RhsRkArgs {           // fused RHS + low-storage RK update (driver)
     const TD* dEdx;          // [2][nx][ny]: dEy/dx, dEz/dx
     const TD* dEdy;          // [2][nx][ny]: dEx/dy, dEz/dy
     TD* rprev;
+    const TD* lap = nullptr; // [6][nx][ny] Laplacians of u and B (source terms)
+    double viscosity = 0, resistivity = 0;
     const RunScalars* rs;
     double gamma;
     int stage;
     int write_rprev;
     size_t n;
     // instance z of a batch (blockIdx.y): u + z * bs_u, dudx/dudy/dEdx/dEdy
-    // + z * bs_d, rprev + z * bs_r, rs + z
-    size_t bs_u = 0, bs_d = 0, bs_r = 0;
+    // + z * bs_d, rprev + z * bs_r, lap + z * bs_l, rs + z
+    size_t bs_u = 0, bs_d = 0, bs_r = 0, bs_l = 0;
     // open x boundary: rows 0 and nx-1 are left to launch_rhs_rk_open
     int open_x = 0;
     int nx = 0, ny = 0;

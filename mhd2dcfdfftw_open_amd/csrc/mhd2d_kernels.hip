@@ -364,8 +364,18 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
     // row segments of at least 64 bytes; narrower ones (8 FP32 columns)
     // measured 1.5x slower as nontemporal stores
     constexpr bool NT = MHD2D_NT_X && W * sizeof(T) >= 64;
+    if (a.accumulate) {
+        // second x-derivative onto the second y-derivative already there
+        // (addSourceTerm's d2/dx2 + d2/dy2, calcRHS.c:498-531)
 #pragma unroll
-    for (int k = 0; k < C; k++) st_stream<NT>(po + k * ny, w[k]);
+        for (int k = 0; k < C; k++) {
+            const T yy = po[k * ny];
+            st_stream<NT>(po + k * ny, w[k] + yy);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < C; k++) st_stream<NT>(po + k * ny, w[k]);
+    }
 }
 
 #ifndef MHD2D_X_W
@@ -396,8 +406,12 @@ static void launch_xsolve_kind(const XSolveArgs<T, TI>& a, dim3 grid, dim3 block
         default: hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XFILT, BLK>), grid, block, shm, s, a); break;
         }
     } else {
-        // converting inputs: first derivatives of a FP64 state only
-        hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD1, BLK>), grid, block, shm, s, a);
+        // converting inputs: derivatives of a FP64 state (both annihilate
+        // the subtracted column constant; the filter does not)
+        if (a.kind == XD2)
+            hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD2, BLK>), grid, block, shm, s, a);
+        else
+            hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD1, BLK>), grid, block, shm, s, a);
     }
 }
 
@@ -417,9 +431,9 @@ template <typename T, typename TI>
 hipError_t launch_xsolve(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int nb)
 {
     if (nb < 1 || nb > 65535) return hipErrorInvalidValue;
-    // converting inputs: first derivatives only (they annihilate the
-    // subtracted constant; the filter does not)
-    if (!std::is_same<T, TI>::value && a.kind != XD1) return hipErrorInvalidValue;
+    // converting inputs: derivatives only (they annihilate the subtracted
+    // constant; the filter does not)
+    if (!std::is_same<T, TI>::value && a.kind == XFILT) return hipErrorInvalidValue;
     int C, W, G;
     xsolve_shape(a.nx, sizeof(T), &C, &W, &G);
     if (a.nx % C || G > 64 || G < 2 || a.ny % W || nv < 1 || nv > 16) return hipErrorInvalidValue;
@@ -527,7 +541,7 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) k_yapply(YApplyArgs<T
 // Fused y-sweep of one RK stage (driver).  Each thread owns the same 16
 // positions of its row in physical and spectral space, so the filtered state
 // it writes is re-read by the same thread only (E and the CFL limits).
-template <typename T, typename TD, int N>
+template <typename T, typename TD, int N, bool LAP>
 __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep(YSweepArgs<T, TD> a)
 {
     using Y = YShape<T, N>;
@@ -552,9 +566,11 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
     TD* const dudy = a.dudy + z * a.bs_d;
     TD* const dEdy = a.dEdy + z * a.bs_d;
 
-    // field pairs packed as re/im of one complex line
-    constexpr int PA[4] = {RHO, UY, BX, BZ};
-    constexpr int PB[4] = {UX, UZ, BY, PR};
+    // field pairs packed as re/im of one complex line; the velocity and field
+    // components sit in pairs 0-2 so that the second derivatives of the
+    // source terms take at most three more inverse transforms
+    constexpr int PA[4] = {UX, UZ, BX, RHO};
+    constexpr int PB[4] = {UY, BZ, BY, PR};
     if (a.filter || a.derivs) {
         // The next pair is loaded while the last transform of the current
         // pair runs (its registers replace the saved spectrum Z), so the HBM
@@ -565,11 +581,17 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
             const int k = t + m * TPR;
             nxt[m] = cx<T>{u[PA[0] * n + row + k], u[PB[0] * n + row + k]};
         }
+        TD* const lapz = a.lap ? a.lap + z * a.bs_l : nullptr;
 #pragma unroll 1
         for (int p = 0; p < 4; p++) {
             const int fa = PA[p], fb = PB[p];
             T* ua = u + fa * n + row;
             T* ub = u + fb * n + row;
+            // second y-derivatives wanted from this pair (re: fa, im: fb)
+            const bool la = LAP && a.derivs &&
+                            ((a.lap_u && (fa == UX || fa == UZ)) || (a.lap_b && fa == BX));
+            const bool lb = LAP && a.derivs &&
+                            ((a.lap_u && fb == UY) || (a.lap_b && (fb == BZ || fb == BY)));
             // opaque copy of t: the multiplier loads stay inside the loop
             // instead of being hoisted into 24 live registers
             int tq = t;
@@ -578,13 +600,14 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
 #pragma unroll
             for (int m = 0; m < E; m++) v[m] = nxt[m];
             fft_fwd<T, N, E>(v, t, lre, lim, tw);
-            if (a.filter) {
-                cx<T> Z[E];
+            cx<T> Z[E];
+            if (LAP || a.filter) {
 #pragma unroll
-                for (int m = 0; m < E; m++) {
-                    Z[m] = v[m];
-                    v[m] = mul_scal(v[m], (T)a.tb.mf[tq + m * TPR]);
-                }
+                for (int m = 0; m < E; m++) Z[m] = v[m];
+            }
+            if (a.filter) {
+#pragma unroll
+                for (int m = 0; m < E; m++) v[m] = mul_scal(Z[m], (T)a.tb.mf[tq + m * TPR]);
                 fft_inv<T, N, E>(v, t, lre, lim, tw);
                 if (active) {
 #pragma unroll
@@ -594,14 +617,36 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
                         st_stream<MHD2D_NT_Y>(ub + k, v[m].im);
                     }
                 }
-                if (a.derivs) {
+                if (!LAP && a.derivs) {
 #pragma unroll
                     for (int m = 0; m < E; m++)
                         v[m] = mul_i_scal(Z[m], (T)a.tb.mfd[tq + m * TPR]);
                 }
-            } else {
+            } else if (!LAP) {
 #pragma unroll
                 for (int m = 0; m < E; m++) v[m] = mul_i_scal(v[m], (T)a.tb.md[tq + m * TPR]);
+            }
+            if (LAP && (la || lb)) {
+                // d2/dy2 of the filtered fields (derivYY, calcDerivs.c:132-140)
+                const double* __restrict__ m2 = a.filter ? a.tb.mfdd : a.tb.mdd;
+#pragma unroll
+                for (int m = 0; m < E; m++) v[m] = mul_scal(Z[m], (T)m2[tq + m * TPR]);
+                fft_inv<T, N, E>(v, t, lre, lim, tw);
+                if (active) {
+                    TD* pa = lapz + (fa - UX) * n + row;
+                    TD* pb = lapz + (fb - UX) * n + row;
+#pragma unroll
+                    for (int m = 0; m < E; m++) {
+                        const int k = t + m * TPR;
+                        if (la) pa[k] = (TD)v[m].re;
+                        if (lb) pb[k] = (TD)v[m].im;
+                    }
+                }
+            }
+            if (LAP && a.derivs) {
+                const double* __restrict__ m1 = a.filter ? a.tb.mfd : a.tb.md;
+#pragma unroll
+                for (int m = 0; m < E; m++) v[m] = mul_i_scal(Z[m], (T)m1[tq + m * TPR]);
             }
             if (p < 3) {
                 const T* na = u + PA[p + 1] * n + row;
@@ -692,7 +737,10 @@ static hipError_t launch_y_n(const YApplyArgs<T>* ya, const YSweepArgs<T, TD>* y
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((k_ysweep<T, TD, N>), grid, block, 0, s, *ys);
+    if (ys->lap)
+        hipLaunchKernelGGL((k_ysweep<T, TD, N, true>), grid, block, 0, s, *ys);
+    else
+        hipLaunchKernelGGL((k_ysweep<T, TD, N, false>), grid, block, 0, s, *ys);
     return hipGetLastError();
 }
 
@@ -930,6 +978,29 @@ __global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T, TD> a)
             R[BZ].x[w] = (T)ey[0].x[w] - (T)ex[0].x[w];
             R[PR].x[w] = -ux * d_x[PR] - uy * d_y[PR] - g * pr * divU;
         }
+        if (a.lap) {
+            // addSourceTerm (calcRHS.c:498-531): lap holds d2/dx2 + d2/dy2
+            const TD* const lz = a.lap + z * a.bs_l;
+            if (a.viscosity > 0) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const Pack<TD, W> l = ldp<W>(lz + c * n + p);
+#pragma unroll
+                    for (int w = 0; w < W; w++)
+                        R[UX + c].x[w] = R[UX + c].x[w] +
+                                         (T)a.viscosity / u[RHO].x[w] * (T)l.x[w];
+                }
+            }
+            if (a.resistivity > 0) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const Pack<TD, W> l = ldp<W>(lz + (3 + c) * n + p);
+#pragma unroll
+                    for (int w = 0; w < W; w++)
+                        R[BX + c].x[w] = R[BX + c].x[w] + (T)a.resistivity * (T)l.x[w];
+                }
+            }
+        }
 #pragma unroll
         for (int f = 0; f < 8; f++) {
             // rkExecute, timeAdvance.c:31-35 (R_prev is zero in stage 0)
@@ -1062,6 +1133,18 @@ __global__ void __launch_bounds__(256) k_rhs_rk_open(RhsRkArgs<T, TD> a)
     R[BY] += c[4];
     R[BZ] += c[5];
     R[PR] += c[6];
+    if (a.lap) {
+        // addSourceTerm covers the boundary lines too (calcRHS.c:498-531)
+        const TD* const lz = a.lap + z * a.bs_l;
+        if (a.viscosity > 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) R[UX + k] = R[UX + k] + (T)a.viscosity / rho * (T)lz[k * n + p];
+        }
+        if (a.resistivity > 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) R[BX + k] = R[BX + k] + (T)a.resistivity * (T)lz[(3 + k) * n + p];
+        }
+    }
     const bool use_prev = a.stage > 0;
 #pragma unroll
     for (int f = 0; f < 8; f++) {

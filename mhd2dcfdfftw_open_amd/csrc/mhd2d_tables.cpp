@@ -215,7 +215,7 @@ hipError_t GridTables::build(int nx_, int ny_, double Lx_, double Ly_, int elem_
     const int G = groups, C = chunk, N = ny;
     const size_t nxs = (size_t)nx, gs = (size_t)G;
     const size_t x_doubles = NXKIND * (3 * nxs + 2 * gs);
-    const size_t y_doubles = 4 * (size_t)N + 2 * (size_t)N;  // md, mdd, mf, mfd + tw
+    const size_t y_doubles = 5 * (size_t)N + 2 * (size_t)N;  // md, mdd, mf, mfd, mfdd + tw
     // rows with an explicit stencil: the edges of every block, and the three
     // rows of each open end; ascending, the same rows for every kind
     std::vector<int> orows;
@@ -285,7 +285,8 @@ hipError_t GridTables::build(int nx_, int ny_, double Lx_, double Ly_, int elem_
     }
     // ---- y multipliers and twiddles (long double trig, rounded once)
     const size_t off_md = o, off_mdd = o + N, off_mf = o + 2 * N, off_mfd = o + 3 * N;
-    const size_t off_tw = o + 4 * N;
+    const size_t off_mfdd = o + 4 * N;
+    const size_t off_tw = o + 5 * N;
     const long double PI = 3.141592653589793238462643383279502884L;
     const long double afy = al.afy;
     const long double aj = (5.0L + 6.0L * afy) / 8.0L, bj = (1.0L + 2.0L * afy) / 2.0L,
@@ -301,6 +302,7 @@ hipError_t GridTables::build(int nx_, int ny_, double Lx_, double Ly_, int elem_
         p[off_mdd + k] = (double)(-kapa * kapa / N);
         p[off_mf + k] = (double)(fy / N);
         p[off_mfd + k] = (double)(fy * kap / N);
+        p[off_mfdd + k] = (double)(-fy * kapa * kapa / N);
         const long double ang = 2.0L * PI * k / N;
         p[off_tw + 2 * k] = (double)cosl(ang);
         p[off_tw + 2 * k + 1] = (double)(-sinl(ang));
@@ -354,6 +356,7 @@ hipError_t GridTables::build(int nx_, int ny_, double Lx_, double Ly_, int elem_
     yt.mdd = d + off_mdd;
     yt.mf = d + off_mf;
     yt.mfd = d + off_mfd;
+    yt.mfdd = d + off_mfdd;
     yt.tw = (const double2*)(d + off_tw);
     return hipSuccess;
 }

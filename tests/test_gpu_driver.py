@@ -1,8 +1,8 @@
 """The reference's time-driven run loop (mhd.c:58-230) on the GPU against the
 same loop driven by the FP64 oracle: output times and files, step counts, the
-dt < 1e-8 stop, both engines (fused context; drop-in entry points with
-viscosity, resistivity and diffusive div B cleaning) and the mhd2d_run
-executable."""
+dt < 1e-8 stop, both engines (fused context, also with viscosity and
+resistivity; drop-in entry points with viscosity, resistivity and both div B
+cleanings) and the mhd2d_run executable."""
 import array
 import os
 import subprocess
@@ -70,6 +70,7 @@ def _inp(m, **kw):
 
 @pytest.mark.parametrize("engine,extra", [
     ("fused", {}),
+    ("fused", dict(viscosity=2e-3, resistivity=1e-3, hall=0)),
     ("dropin", {}),
     ("dropin", dict(viscosity=2e-3, resistivity=1e-3, hall=0)),
     ("dropin", dict(divbclean=1)),
