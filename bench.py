@@ -158,10 +158,10 @@ def _bytes(name):
     return {"fp64": (8, 8), "fp32": (4, 4), "mixed": (8, 4)}[name]
 
 
-def run_single(m, u0, nx, ny, prec, steps, warmup, graph, hall, di):
+def run_single(m, u0, nx, ny, prec, steps, warmup, graph, hall, di, **dissipation):
     """(cell-updates/s, ms, context) of one run; the context holds the state
     after warmup + steps"""
-    ctx = m.Context(nx, ny, hall=hall, di=di, precision=_precision(m, prec))
+    ctx = m.Context(nx, ny, hall=hall, di=di, precision=_precision(m, prec), **dissipation)
     ctx.set_state(u0)
     ctx.run(warmup, graph=graph)
     ctx.sync()
@@ -394,6 +394,18 @@ def main():
                 "after_steps": nsteps, "meets_1e-6": bool(max(e) <= 1e-6),
                 "step_roofline_frac": v * B_ALG[prec] / HBM_PEAK, "roofline": roof,
                 "kernels": kern}
+            c.close()
+        # the same workload with viscosity and resistivity (addSourceTerm):
+        # the cost of the Laplacians in the fused engine
+        for prec in ("fp64", "mixed"):
+            v, ms_p, c = run_single(m, u0, nx, ny, prec, args.steps, args.warmup, graph, hall,
+                                    di, viscosity=1e-4, resistivity=1e-4)
+            pe, pd = _bytes(prec)
+            _, kern = roofline_of(c, nx, ny, pe, pd, 1, args.profile_steps, copy_gbps)
+            forms[f"dissipative_{prec}_{nx}"] = {
+                "workload": f"turbulence {nx}x{ny}, {prec}, viscosity = resistivity = 1e-4",
+                "cell_updates_per_s": v, "ms_per_step": ms_p / args.steps,
+                "vs_ideal_fp64": v / value, "kernels": kern}
             c.close()
         # batched ensemble: 64 x 512^2 linear waves, d_i per instance
         enx = eny = 512
