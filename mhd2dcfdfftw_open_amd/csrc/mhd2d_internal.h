@@ -249,6 +249,7 @@ template <typename T>
 hipError_t launch_add(const T* a, const T* b, T* out, size_t n, hipStream_t s);
 
 // chunk length (rows per thread) and strip width used by the x solves
+// (elem_bytes: size of the input word)
 void xsolve_shape(int nx, int elem_bytes, int* chunk, int* width, int* groups);
 int ysweep_supported(int ny);
 

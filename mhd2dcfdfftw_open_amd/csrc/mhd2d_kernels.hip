@@ -387,9 +387,11 @@ void xsolve_shape(int nx, int elem_bytes, int* chunk, int* width, int* groups)
     int C = nx / 64;
     if (C < 8) C = 8;
     int W = MHD2D_X_W;
-    // 64-row chunks: 8-column strips keep 512 threads per block; lines of
-    // <= 512 rows (64-thread columns) measured faster in 8-column strips too
-    if (C > 32 || nx <= 512) W = 8;
+    // 64-row chunks: 8-column strips keep 512 threads per block.  Lines of
+    // <= 512 rows read as FP32 (elem_bytes: the input word) measured faster
+    // in 8-column strips too (64 x 512^2 FP32: -5 %); with FP64 inputs they
+    // take 16 columns (128-byte rows: -3 % FP64, -6 % mixed).
+    if (C > 32 || (nx <= 512 && elem_bytes == 4)) W = 8;
     *chunk = C;
     *width = W;
     *groups = nx / C;
@@ -435,7 +437,7 @@ hipError_t launch_xsolve(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int 
     // constant; the filter does not)
     if (!std::is_same<T, TI>::value && a.kind == XFILT) return hipErrorInvalidValue;
     int C, W, G;
-    xsolve_shape(a.nx, sizeof(T), &C, &W, &G);
+    xsolve_shape(a.nx, sizeof(TI), &C, &W, &G);
     if (a.nx % C || G > 64 || G < 2 || a.ny % W || nv < 1 || nv > 16) return hipErrorInvalidValue;
     if (W == 8) {
         switch (C) {
