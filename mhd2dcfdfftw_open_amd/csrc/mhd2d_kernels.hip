@@ -915,9 +915,19 @@ __device__ __forceinline__ void stp(T* p, const Pack<T, W>& v)
 // fused RHS + RK update; each thread handles W = 16 / sizeof(T) consecutive
 // cells (one 16-byte vector of each state field) per field (n is a multiple
 // of W: nx, ny are powers of two).  Derivatives and R_prev are read and
-// written as TD, the arithmetic and the state update run in T.
+// written as TD, the arithmetic and the state update run in T.  LAP adds the
+// viscous and resistive source terms (a separate instance, so the ideal
+// kernel keeps its register allocation).  The FP64 instance is held at 2
+// waves per SIMD (measured -2.5 % against the 3 its registers allow; the
+// mixed instance measured +7 % at 2, so the others keep the compiler's
+// choice).
 template <typename T, typename TD>
-__global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T, TD> a)
+struct RhsRkWaves {
+    static constexpr int MAX = (sizeof(T) == 8 && sizeof(TD) == 8) ? 2 : 10;
+};
+template <typename T, typename TD, bool LAP>
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(1, RhsRkWaves<T, TD>::MAX))) k_rhs_rk(RhsRkArgs<T, TD> a)
 {
     constexpr int W = 16 / sizeof(T);
     const int z = blockIdx.y;  // instance of a batch
@@ -980,7 +990,7 @@ __global__ void __launch_bounds__(256) k_rhs_rk(RhsRkArgs<T, TD> a)
             R[BZ].x[w] = (T)ey[0].x[w] - (T)ex[0].x[w];
             R[PR].x[w] = -ux * d_x[PR] - uy * d_y[PR] - g * pr * divU;
         }
-        if (a.lap) {
+        if (LAP) {
             // addSourceTerm (calcRHS.c:498-531): lap holds d2/dx2 + d2/dy2
             const TD* const lz = a.lap + z * a.bs_l;
             if (a.viscosity > 0) {
@@ -1325,7 +1335,10 @@ hipError_t launch_rhs_rk(const RhsRkArgs<T, TD>& a, hipStream_t s, int nb)
     if (nb < 1 || nb > 65535) return hipErrorInvalidValue;
     dim3 grid = pw_grid(a.n / (16 / sizeof(T)), 256);
     grid.y = nb;
-    hipLaunchKernelGGL((k_rhs_rk<T, TD>), grid, dim3(256), 0, s, a);
+    if (a.lap)
+        hipLaunchKernelGGL((k_rhs_rk<T, TD, true>), grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_rhs_rk<T, TD, false>), grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
