@@ -74,6 +74,8 @@ def cpu_baseline(nx, ny, u0, hall, di):
 
 
 COPY_BYTES = 1 << 30
+# every context first runs the x-solve shape autotuner (mhd2d_autotune)
+AUTOTUNE = True
 
 
 def _pmc_rows(path):
@@ -163,6 +165,7 @@ def run_single(m, u0, nx, ny, prec, steps, warmup, graph, hall, di, **dissipatio
     after warmup + steps"""
     ctx = m.Context(nx, ny, hall=hall, di=di, precision=_precision(m, prec), **dissipation)
     ctx.set_state(u0)
+    ctx.x_shapes = ctx.autotune() if AUTOTUNE else None
     ctx.run(warmup, graph=graph)
     ctx.sync()
     ms = ctx.time_run(steps, graph=graph)
@@ -199,6 +202,7 @@ def run_batch(m, states, dis, nx, ny, prec, steps, warmup, hall=1):
     for b in range(B):
         ctx.set_di(b, float(dis[b]))
         ctx.set_state(states[b], inst=b)
+    ctx.x_shapes = ctx.autotune() if AUTOTUNE else None
     ctx.run(warmup, graph=True)
     ctx.sync()
     ms = ctx.time_run(steps, graph=True)
@@ -250,6 +254,8 @@ def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50,
     for name, prec, xg in variants:
         ctx = m.Context(n, n, hall=hall, di=di, precision=prec, xgemm=xg)
         ctx.set_state(u0)
+        if AUTOTUNE and not xg:
+            ctx.autotune()
         ctx.run(1, graph=True)   # the first step (no pending filter) and the capture
         ctx.sync()
         ctx.set_state(u0)
@@ -304,6 +310,8 @@ def main():
                     help="steps of the 64 x 512^2 ensemble")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default x-solve launch shapes")
     ap.add_argument("--c3-only", action="store_true",
                     help="only the 4096^2 precision / x-derivative comparison (prints its JSON)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -312,6 +320,8 @@ def main():
                                                          "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="turn rocprofv3 --pmc counter CSVs into profiles/traffic_*.json")
     args = ap.parse_args()
+    global AUTOTUNE
+    AUTOTUNE = not args.no_autotune
 
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, nx=args.nx, ny=args.ny)
@@ -369,6 +379,7 @@ def main():
         "roofline": roofline,
         "step_roofline_frac": value * B_ALG[args.precision] / HBM_PEAK,
         "kernels": kernels,
+        "x_solve_shapes": ctx.x_shapes,
         "copy_probe_GBps": copy_gbps,
         "diagnostics": {"energy": energy, "max_divb": divb},
     }
@@ -442,6 +453,8 @@ def main():
             E = m.Ensemble(runs, precision=_precision(m, prec))
             for k in range(len(runs)):
                 E.set_state(k, ustates[k])
+            if AUTOTUNE:
+                E.autotune()
             E.run(args.warmup, graph=True)
             E.sync()
             ms_p = E.time_run(args.steps, graph=True)

@@ -132,6 +132,8 @@ def lib():
         "mhd2d_stopped_at": (i, [vp, i, p]),
         "mhd2d_xderiv": (i, [vp, i, p, p]),
         "mhd2d_xgemm_panels": (ctypes.c_longlong, [vp]),
+        "mhd2d_autotune": (i, [vp, i, ctypes.POINTER(i)]),
+        "mhd2d_ens_autotune": (i, [vp, i, ctypes.POINTER(i)]),
         "mhd2d_ens_create": (i, [ctypes.POINTER(Params), i, ctypes.POINTER(vp)]),
         "mhd2d_ens_destroy": (None, [vp]),
         "mhd2d_ens_groups": (i, [vp]),
@@ -272,6 +274,13 @@ class Context:
     def xgemm_panels(self):
         return lib().mhd2d_xgemm_panels(self.h)
 
+    def autotune(self, reps=5):
+        """time every x-solve launch shape on this grid and keep the fastest;
+        returns {'deriv': (width, xcd_order), 'filter': (width, xcd_order)}"""
+        c = (ctypes.c_int * 4)()
+        _check(lib().mhd2d_autotune(self.h, reps, c))
+        return {"deriv": (c[0], c[1]), "filter": (c[2], c[3])}
+
     def set_di(self, inst, di):
         _check(lib().mhd2d_set_di(self.h, inst, di))
 
@@ -323,14 +332,16 @@ class Context:
 class Ensemble:
     """Runs on grids of different sizes advanced together
     (mhd2d_ens_create ... mhd2d_ens_destroy).  `runs` is a list of dicts with
-    nx, ny and optionally Lx, Ly, gamma, cfl, hall, di, npe."""
+    nx, ny and optionally Lx, Ly, gamma, cfl, hall, di, npe, resistivity,
+    viscosity."""
 
     def __init__(self, runs, precision=FP64):
         arr = (Params * len(runs))()
         for k, r in enumerate(runs):
             arr[k] = Params(r["nx"], r["ny"], r.get("Lx", 2 * np.pi), r.get("Ly", 2 * np.pi),
                             r.get("gamma", 5.0 / 3.0), r.get("cfl", 0.3), r.get("hall", 0),
-                            r.get("di", 0.0), 0.0, 0.0, 0, precision, r.get("npe", 1), 0,
+                            r.get("di", 0.0), r.get("resistivity", 0.0),
+                            r.get("viscosity", 0.0), 0, precision, r.get("npe", 1), 0,
                             r.get("xgemm", 0), int(r.get("open_x", False)))
         self.shapes = [(r["nx"], r["ny"]) for r in runs]
         h = ctypes.c_void_p()
@@ -361,6 +372,14 @@ class Ensemble:
 
     def run(self, nsteps, graph=False):
         _check(lib().mhd2d_ens_run(self.h, nsteps, int(graph)))
+
+    def autotune(self, reps=5):
+        """Context.autotune for every group; one dict per group"""
+        g = lib().mhd2d_ens_groups(self.h)
+        c = (ctypes.c_int * (4 * g))()
+        _check(lib().mhd2d_ens_autotune(self.h, reps, c))
+        return [{"deriv": (c[4 * k], c[4 * k + 1]), "filter": (c[4 * k + 2], c[4 * k + 3])}
+                for k in range(g)]
 
     def sync(self):
         _check(lib().mhd2d_ens_sync(self.h))

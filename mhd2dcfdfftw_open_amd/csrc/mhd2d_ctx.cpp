@@ -11,6 +11,7 @@
 // instance index is a grid dimension) and each has its own RunScalars.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -70,6 +71,9 @@ struct mhd2d_ctx {
     bool pending_filter = false;
     size_t bytes = 0;
     hipGraphExec_t graph = nullptr;
+    // launch shapes of the derivative and filter x-solves (strip width,
+    // XCD-contiguous order); defaults until mhd2d_autotune measures them
+    int xw_d = 0, xr_d = -1, xw_f = 0, xr_f = -1;
     std::vector<char> have_state;
     std::vector<double> di;
     size_t su() const { return 11 * n; }  // per-instance strides (elements)
@@ -165,6 +169,8 @@ hipError_t xderiv_fields(mhd2d_ctx* c, const T* in, TD* out, int nv, size_t bs_i
     a.ny = c->p.ny;
     a.bs_in = bs_in;
     a.bs_out = bs_out;
+    a.width = c->xw_d;
+    a.remap = c->xr_d;
     return launch_xsolve<TD, T>(a, nv, c->stream, nb);
 }
 
@@ -190,6 +196,8 @@ hipError_t xderiv(mhd2d_ctx* c)
     a.nx = c->p.nx;
     a.ny = c->p.ny;
     a.accumulate = 1;
+    a.width = c->xw_d;
+    a.remap = c->xr_d;
     a.bs_in = c->su();
     a.bs_out = c->sl();
     return launch_xsolve<TD, T>(a, f1 - f0 + 1, c->stream, c->B);
@@ -208,6 +216,8 @@ hipError_t xfilter(mhd2d_ctx* c)
     a.nx = c->p.nx;
     a.ny = c->p.ny;
     a.bs_in = a.bs_out = c->su();
+    a.width = c->xw_f;
+    a.remap = c->xr_f;
     return launch_xsolve<T>(a, 8, c->stream, c->B);
 }
 
@@ -804,9 +814,134 @@ int xderiv_host(mhd2d_ctx* c, int nv, const double* in, double* out)
     if (e != hipSuccess) return fail(MHD2D_EHIP, "x-derivative", e);
     return MHD2D_OK;
 }
+// mean time of `reps` launches of f() after one untimed launch
+template <typename F>
+hipError_t time_launches(hipStream_t s, int reps, F f, float* ms)
+{
+    hipEvent_t a, b;
+    hipError_t e = hipEventCreate(&a);
+    if (e != hipSuccess) return e;
+    e = hipEventCreate(&b);
+    if (e == hipSuccess) e = f();
+    if (e == hipSuccess) e = hipEventRecord(a, s);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = f();
+    if (e == hipSuccess) e = hipEventRecord(b, s);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float t = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+    *ms = t / reps;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return e;
+}
+
+// Autotuner of the x-solve launch shapes: every valid (strip width, XCD
+// order) is timed on this context's own grid, precision and batch, for the
+// derivative solve (all fields of a stage, into the derivative buffer, which
+// holds no live data between steps) and for the filter (in place on a
+// zeroed scratch copy, so the state is untouched); the candidates are timed
+// in two interleaved rounds, each keeps its faster mean, and the fastest
+// candidate is kept.
+template <typename T, typename TD>
+hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
+{
+    struct Cand {
+        int w, r;
+    };
+    int C0, W0, G0;
+    xsolve_shape(c->p.nx, 8, &C0, &W0, &G0);
+    std::vector<Cand> cands;
+    for (int w : {16, 8})
+        for (int r : {0, 1}) {
+            if ((w == 16 && C0 > 32) || c->p.ny % w) continue;
+            cands.push_back({w, r});
+        }
+    hipError_t e = hipStreamSynchronize(c->stream);
+    T* scratch = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&scratch, 8 * c->n * sizeof(T) * c->B);
+    if (e == hipSuccess) e = hipMemsetAsync(scratch, 0, 8 * c->n * sizeof(T) * c->B, c->stream);
+    std::vector<float> td(cands.size(), 1e30f), tf(cands.size(), 1e30f);
+    for (size_t it = 0; it < 2 * cands.size(); it++) {  // two rounds over the candidates
+        const size_t ci = it % cands.size();
+        const Cand& k = cands[ci];
+        if (e != hipSuccess) break;
+        c->xw_d = k.w;
+        c->xr_d = k.r;
+        float t = 0;
+        e = time_launches(c->stream, reps, [&]() {
+            return xderiv_fields<T, TD>(c, (const T*)c->u, (TD*)c->dudx, 10, c->su(), c->sd(),
+                                        c->B, nullptr);
+        }, &t);
+        if (e == hipSuccess) td[ci] = std::min(td[ci], t);
+        XSolveArgs<T> a;
+        for (int v = 0; v < 8; v++) a.in[v] = a.out[v] = scratch + v * c->n;
+        a.st = c->tb.st[XFILT];
+        a.tb = c->tb.xt[XFILT];
+        a.kind = XFILT;
+        a.nx = c->p.nx;
+        a.ny = c->p.ny;
+        a.bs_in = a.bs_out = 8 * c->n;
+        a.width = k.w;
+        a.remap = k.r;
+        if (e == hipSuccess)
+            e = time_launches(c->stream, reps,
+                              [&]() { return launch_xsolve<T>(a, 8, c->stream, c->B); }, &t);
+        if (e == hipSuccess) tf[ci] = std::min(tf[ci], t);
+    }
+    int wd = 0, rd = -1, wf = 0, rf = -1;
+    float best_d = 1e30f, best_f = 1e30f;
+    for (size_t ci = 0; ci < cands.size(); ci++) {
+        if (td[ci] < best_d) {
+            best_d = td[ci];
+            wd = cands[ci].w;
+            rd = cands[ci].r;
+        }
+        if (tf[ci] < best_f) {
+            best_f = tf[ci];
+            wf = cands[ci].w;
+            rf = cands[ci].r;
+        }
+    }
+    if (scratch) (void)hipFree(scratch);
+    if (e != hipSuccess) {
+        c->xw_d = 0;
+        c->xr_d = -1;
+        return e;
+    }
+    c->xw_d = wd;
+    c->xr_d = rd;
+    c->xw_f = wf;
+    c->xr_f = rf;
+    if (c->graph) {  // the captured step holds the old launch shapes
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    if (out) {
+        out[0] = wd;
+        out[1] = rd;
+        out[2] = wf;
+        out[3] = rf;
+    }
+    return hipSuccess;
+}
 }  // namespace
 
 extern "C" {
+
+int mhd2d_autotune(mhd2d_ctx* c, int reps, int* choice)
+{
+    if (!c || reps < 1) return fail(MHD2D_EINVAL, "bad argument");
+    if (c->p.xgemm) return fail(MHD2D_EINVAL, "the GEMM x-derivative has no shapes to tune");
+    hipError_t e;
+    if (c->elem == 4)
+        e = autotune_impl<float, float>(c, reps, choice);
+    else if (c->delem == 8)
+        e = autotune_impl<double, double>(c, reps, choice);
+    else
+        e = autotune_impl<double, float>(c, reps, choice);
+    if (e != hipSuccess) return fail(MHD2D_EHIP, "autotune", e);
+    return MHD2D_OK;
+}
 
 long long mhd2d_xgemm_panels(const mhd2d_ctx* c) { return c && c->p.xgemm ? c->xg.panels_total : 0; }
 
@@ -1034,6 +1169,23 @@ int mhd2d_ens_sync(mhd2d_ens* e)
     HIPCHK(hipStreamSynchronize(e->master));
     for (auto* c : e->groups)
         if (int rc = mhd2d_sync(c)) return rc;
+    return MHD2D_OK;
+}
+
+// every group tunes its own x-solve shapes (its grid, precision and batch);
+// choice[4 g ..] receives group g's
+int mhd2d_ens_autotune(mhd2d_ens* e, int reps, int* choice)
+{
+    if (!e || reps < 1) return fail(MHD2D_EINVAL, "bad argument");
+    HIPCHK(join_into_master(e));
+    HIPCHK(hipStreamSynchronize(e->master));
+    for (size_t g = 0; g < e->groups.size(); g++)
+        if (int rc = mhd2d_autotune(e->groups[g], reps, choice ? choice + 4 * g : nullptr))
+            return rc;
+    if (e->graph) {
+        (void)hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
     return MHD2D_OK;
 }
 

@@ -63,6 +63,11 @@ struct XSolveArgs {
     int kind;
     int nx, ny;
     int accumulate = 0;      // out += result (second x-derivatives onto the y part)
+    // launch shape: strip width (0: the default rule of xsolve_shape, else 8
+    // or 16 columns) and the XCD-contiguous workgroup order (-1: default,
+    // on for 8-column strips; 0 / 1: off / on), as chosen by the autotuner
+    int width = 0;
+    int remap = -1;
     // instance z of a batch (blockIdx.z) reads in[v] + z * bs_in, writes
     // out[v] + z * bs_out and checks rs[z]
     size_t bs_in = 0, bs_out = 0;

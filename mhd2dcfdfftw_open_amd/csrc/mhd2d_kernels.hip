@@ -119,18 +119,18 @@ constexpr bool xsolve_lds_tables(int C) { return C <= 32; }
 #define MHD2D_X_MINB 1
 #endif
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-// L2).  For 8-column strips (two strips per 128-byte FP64 line segment) the
-// linear workgroup id is remapped so that every XCD takes one contiguous
-// range of (strip, field, instance) tiles and neighbouring strips meet in
-// one L2 (measured -12 % x-derivative, -17 % x-filter time at nx = 4096).
-// 16-column FP64 strips fill whole lines; there the remap measured slower.
-template <int W>
-__device__ __forceinline__ void xcd_tile(int& x, int& y, int& z)
+// L2).  With remap the linear workgroup id is mapped so that every XCD takes
+// one contiguous range of (strip, field, instance) tiles and neighbouring
+// strips meet in one L2.  It pays for 8-column strips (two strips per
+// 128-byte FP64 line segment: -12 % x-derivative, -17 % x-filter time at
+// nx = 4096) and measured slower for 16-column strips, so that is the
+// default; the autotuner may choose either.
+__device__ __forceinline__ void xcd_tile(int& x, int& y, int& z, bool remap)
 {
     x = blockIdx.x;
     y = blockIdx.y;
     z = blockIdx.z;
-    if constexpr (W == 8) {
+    if (remap) {
         const unsigned S = gridDim.x, V = gridDim.y;
         const unsigned T = S * V * gridDim.z;
         if ((T & 7u) == 0) {
@@ -150,7 +150,7 @@ template <typename T, typename TI, int C, int W, int KIND, bool BLK>
 __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, TI> a)
 {
     int bx, v, z;  // column strip, field, instance of a batch
-    xcd_tile<W>(bx, v, z);
+    xcd_tile(bx, v, z, a.remap > 0);
     if (a.rs && a.rs[z].stop) return;  // the run stopped (dt < 1e-8): uniform per block
     const int G = blockDim.x / W;
     const int c = threadIdx.x % W;
@@ -438,20 +438,28 @@ hipError_t launch_xsolve(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int 
     if (!std::is_same<T, TI>::value && a.kind == XFILT) return hipErrorInvalidValue;
     int C, W, G;
     xsolve_shape(a.nx, sizeof(TI), &C, &W, &G);
+    if (a.width) {
+        // autotuned width: 8 or 16 columns; 64-row chunks only fit 8
+        if ((a.width != 8 && a.width != 16) || (a.width == 16 && C > 32))
+            return hipErrorInvalidValue;
+        W = a.width;
+    }
     if (a.nx % C || G > 64 || G < 2 || a.ny % W || nv < 1 || nv > 16) return hipErrorInvalidValue;
+    XSolveArgs<T, TI> b = a;
+    b.remap = a.remap < 0 ? (W == 8) : a.remap;
     if (W == 8) {
         switch (C) {
-        case 8: return launch_xsolve_cw<T, TI, 8, 8>(a, nv, s, nb);
-        case 16: return launch_xsolve_cw<T, TI, 16, 8>(a, nv, s, nb);
-        case 32: return launch_xsolve_cw<T, TI, 32, 8>(a, nv, s, nb);
-        case 64: return launch_xsolve_cw<T, TI, 64, 8>(a, nv, s, nb);
+        case 8: return launch_xsolve_cw<T, TI, 8, 8>(b, nv, s, nb);
+        case 16: return launch_xsolve_cw<T, TI, 16, 8>(b, nv, s, nb);
+        case 32: return launch_xsolve_cw<T, TI, 32, 8>(b, nv, s, nb);
+        case 64: return launch_xsolve_cw<T, TI, 64, 8>(b, nv, s, nb);
         default: return hipErrorInvalidValue;
         }
     }
     switch (C) {
-    case 8: return launch_xsolve_cw<T, TI, 8, 16>(a, nv, s, nb);
-    case 16: return launch_xsolve_cw<T, TI, 16, 16>(a, nv, s, nb);
-    case 32: return launch_xsolve_cw<T, TI, 32, 16>(a, nv, s, nb);
+    case 8: return launch_xsolve_cw<T, TI, 8, 16>(b, nv, s, nb);
+    case 16: return launch_xsolve_cw<T, TI, 16, 16>(b, nv, s, nb);
+    case 32: return launch_xsolve_cw<T, TI, 32, 16>(b, nv, s, nb);
     default: return hipErrorInvalidValue;
     }
 }
