@@ -838,10 +838,11 @@ hipError_t time_launches(hipStream_t s, int reps, F f, float* ms)
 // Autotuner of the x-solve launch shapes: every valid (strip width, XCD
 // order) is timed on this context's own grid, precision and batch, for the
 // derivative solve (all fields of a stage, into the derivative buffer, which
-// holds no live data between steps) and for the filter (in place on a
-// zeroed scratch copy, so the state is untouched); the candidates are timed
-// in two interleaved rounds, each keeps its faster mean, and the fastest
-// candidate is kept.
+// holds no live data between steps) and for the filter (in place on the
+// state itself, which is saved before and restored after, so the timed
+// launches see the real layout); the candidates are timed in two
+// interleaved rounds, each keeps its faster mean, and the fastest candidate
+// is kept.
 template <typename T, typename TD>
 hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
 {
@@ -857,9 +858,10 @@ hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
             cands.push_back({w, r});
         }
     hipError_t e = hipStreamSynchronize(c->stream);
-    T* scratch = nullptr;
-    if (e == hipSuccess) e = hipMalloc(&scratch, 8 * c->n * sizeof(T) * c->B);
-    if (e == hipSuccess) e = hipMemsetAsync(scratch, 0, 8 * c->n * sizeof(T) * c->B, c->stream);
+    const size_t ubytes = c->su() * sizeof(T) * c->B;
+    T* saved = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&saved, ubytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(saved, c->u, ubytes, hipMemcpyDeviceToDevice, c->stream);
     std::vector<float> td(cands.size(), 1e30f), tf(cands.size(), 1e30f);
     for (size_t it = 0; it < 2 * cands.size(); it++) {  // two rounds over the candidates
         const size_t ci = it % cands.size();
@@ -874,13 +876,13 @@ hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
         }, &t);
         if (e == hipSuccess) td[ci] = std::min(td[ci], t);
         XSolveArgs<T> a;
-        for (int v = 0; v < 8; v++) a.in[v] = a.out[v] = scratch + v * c->n;
+        for (int v = 0; v < 8; v++) a.in[v] = a.out[v] = (T*)c->u + v * c->n;
         a.st = c->tb.st[XFILT];
         a.tb = c->tb.xt[XFILT];
         a.kind = XFILT;
         a.nx = c->p.nx;
         a.ny = c->p.ny;
-        a.bs_in = a.bs_out = 8 * c->n;
+        a.bs_in = a.bs_out = c->su();
         a.width = k.w;
         a.remap = k.r;
         if (e == hipSuccess)
@@ -902,7 +904,13 @@ hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
             rf = cands[ci].r;
         }
     }
-    if (scratch) (void)hipFree(scratch);
+    if (saved) {
+        const hipError_t e2 =
+            hipMemcpyAsync(c->u, saved, ubytes, hipMemcpyDeviceToDevice, c->stream);
+        const hipError_t e3 = hipStreamSynchronize(c->stream);
+        (void)hipFree(saved);
+        if (e == hipSuccess) e = e2 != hipSuccess ? e2 : e3;
+    }
     if (e != hipSuccess) {
         c->xw_d = 0;
         c->xr_d = -1;
