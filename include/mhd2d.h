@@ -117,7 +117,7 @@ typedef struct mhd2d_params {
     double di;
     double resistivity;     /* >= 0: eta Lap B (addSourceTerm, calcRHS.c:498-531) */
     double viscosity;       /* >= 0: nu/rho Lap u (same) */
-    int divbclean;          /* must be 0 in the context API */
+    int divbclean;          /* 0 or 1 (diffusive) in the context API */
     int precision;          /* MHD2D_FP64, MHD2D_FP32 or MHD2D_MIXED */
     int xblocks;            /* 0 or 1: x is one block (the reference on one MPI
                                rank); npe > 1: x split into npe blocks sized as
@@ -283,8 +283,8 @@ enum { MHD2D_DRIVE_AUTO = 0, MHD2D_DRIVE_FUSED = 1, MHD2D_DRIVE_DROPIN = 2 };
 
 typedef struct mhd2d_drive_opts {
     const char *outdir;     /* directory of the output files ("" = cwd) */
-    int mode;               /* MHD2D_DRIVE_*: AUTO = FUSED without div B
-                               cleaning, DROPIN otherwise */
+    int mode;               /* MHD2D_DRIVE_*: AUTO = FUSED unless the
+                               cleaning is hyperbolic (GLM), DROPIN then */
     int precision;          /* MHD2D_FP64 / MHD2D_FP32 (FUSED only) */
     int verbose;            /* print the reference's progress lines */
     long long max_steps;    /* stop after this many steps (0: no cap) */

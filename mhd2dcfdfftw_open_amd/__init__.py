@@ -232,15 +232,15 @@ class Context:
 
     def __init__(self, nx, ny, Lx=2 * np.pi, Ly=2 * np.pi, gamma=5.0 / 3.0, cfl=0.3,
                  hall=0, di=0.0, precision=FP64, npe=1, batch=1, xgemm=0, open_x=False,
-                 resistivity=0.0, viscosity=0.0):
+                 resistivity=0.0, viscosity=0.0, divbclean=0):
         """npe > 1 splits x into npe blocks as a reference run on npe MPI
         ranks does (decomposeGrid1D, initialize.c:547-569); xgemm 3 or 6
         takes x-derivatives as split-BF16 MFMA GEMMs with the dense A^-1 B;
         resistivity and viscosity add the reference's addSourceTerm
-        (calcRHS.c:498-531)."""
+        (calcRHS.c:498-531); divbclean = 1 the diffusive div B cleaning."""
         self.nx, self.ny, self.batch = nx, ny, batch
-        self.params = Params(nx, ny, Lx, Ly, gamma, cfl, hall, di, resistivity, viscosity, 0,
-                             precision, npe, batch, xgemm, int(open_x))
+        self.params = Params(nx, ny, Lx, Ly, gamma, cfl, hall, di, resistivity, viscosity,
+                             divbclean, precision, npe, batch, xgemm, int(open_x))
         h = ctypes.c_void_p()
         _check(lib().mhd2d_create(ctypes.byref(self.params), ctypes.byref(h)))
         self.h = h

@@ -65,6 +65,9 @@ struct mhd2d_ctx {
     // Laplacians of ux, uy, uz and Bx, By, Bz (d2/dy2 from the y-sweep, then
     // d2/dx2 added by the x-solve)
     void* lap = nullptr;
+    // [B][3][nx][ny] (derivative type), with diffusive div B cleaning: div B,
+    // d(div B)/dx, d(div B)/dy
+    void* divb = nullptr;
     RunScalars* rs = nullptr;  // [B]
     double* partial = nullptr;
     int npartial = 0;
@@ -74,12 +77,14 @@ struct mhd2d_ctx {
     // launch shapes of the derivative and filter x-solves (strip width,
     // XCD-contiguous order); defaults until mhd2d_autotune measures them
     int xw_d = 0, xr_d = -1, xw_f = 0, xr_f = -1;
+    double cdiff = 0;  // constDiffDivB = 2 max(dx, dy)^2 (initialize.c:17, 154)
     std::vector<char> have_state;
     std::vector<double> di;
     size_t su() const { return 11 * n; }  // per-instance strides (elements)
     size_t sd() const { return 11 * n; }
     size_t sr() const { return 8 * n; }
     size_t sl() const { return 6 * n; }
+    size_t sb() const { return 3 * n; }
     bool lap_u() const { return p.viscosity > 0; }
     bool lap_b() const { return p.resistivity > 0; }
 };
@@ -250,7 +255,44 @@ hipError_t rhs_rk(mhd2d_ctx* c, int stage)
         a.viscosity = c->p.viscosity;
         a.resistivity = c->p.resistivity;
     }
-    hipError_t e = launch_rhs_rk<T, TD>(a, c->stream, c->B);
+    hipError_t e = hipSuccess;
+    if (c->divb) {
+        // diffusive cleaning (mhd.c:175-182): div B of this stage's
+        // derivatives, then its x- and y-derivatives
+        TD* const db = (TD*)c->divb;
+        e = launch_divb_b<TD>(dx, dy, db, c->n, c->sd(), c->sb(), c->B, c->stream);
+        if (e == hipSuccess) {
+            YApplyArgs<TD> ya;
+            ya.in[0] = db;
+            ya.out[0] = db + 2 * c->n;
+            ya.tb = c->tb.yt;
+            ya.nv = 1;
+            ya.mode = Y_D1;
+            ya.nx = c->p.nx;
+            ya.ny = c->p.ny;
+            ya.bs_in = ya.bs_out = c->sb();
+            e = launch_yapply<TD>(ya, c->stream, c->B);
+        }
+        if (e == hipSuccess) {
+            XSolveArgs<TD> xa;
+            xa.in[0] = db;
+            xa.out[0] = db + c->n;
+            xa.rs = c->rs;
+            xa.st = c->tb.st[XD1];
+            xa.tb = c->tb.xt[XD1];
+            xa.kind = XD1;
+            xa.nx = c->p.nx;
+            xa.ny = c->p.ny;
+            xa.bs_in = xa.bs_out = c->sb();
+            xa.width = c->xw_d;
+            xa.remap = c->xr_d;
+            e = launch_xsolve<TD>(xa, 1, c->stream, c->B);
+        }
+        a.ddivb = db + c->n;
+        a.bs_b = c->sb();
+        a.cdiff = c->cdiff;
+    }
+    if (e == hipSuccess) e = launch_rhs_rk<T, TD>(a, c->stream, c->B);
     if (e == hipSuccess && c->p.open_x) e = launch_rhs_rk_open<T, TD>(a, c->stream, c->B);
     return e;
 }
@@ -359,10 +401,10 @@ int check_params(const mhd2d_params* p)
         return fail(MHD2D_EINVAL, "Lx, Ly, cfl must be > 0 and gamma > 1");
     if (!(p->resistivity >= 0) || !(p->viscosity >= 0))
         return fail(MHD2D_EINVAL, "resistivity and viscosity must be >= 0");
-    if (p->divbclean != 0)
+    if (p->divbclean != 0 && p->divbclean != 1)
         return fail(MHD2D_EINVAL,
-                    "the context API runs divbclean = 0 only; use the drop-in entry points "
-                    "for div B cleaning");
+                    "the context API runs divbclean 0 or 1; use the drop-in entry points "
+                    "for hyperbolic (GLM) cleaning");
     if (p->precision != MHD2D_FP64 && p->precision != MHD2D_FP32 &&
         p->precision != MHD2D_MIXED)
         return fail(MHD2D_EINVAL, "precision must be MHD2D_FP64, MHD2D_FP32 or MHD2D_MIXED");
@@ -412,6 +454,10 @@ int mhd2d_create(const mhd2d_params* p, mhd2d_ctx** out)
         return fail(MHD2D_EHIP, "table build", e);
     }
     c->bytes += c->xg.bytes;
+    {
+        const double h = std::max(c->tb.dx, c->tb.dy);
+        c->cdiff = 2.0 * h * h;
+    }
     const size_t fb = c->n * c->elem * c->B, db = c->n * c->delem * c->B;
     c->npartial = 1024;
     size_t sizes[] = {11 * fb, 11 * db, 11 * db, 8 * db};
@@ -423,6 +469,14 @@ int mhd2d_create(const mhd2d_params* p, mhd2d_ctx** out)
             return fail(MHD2D_ENOMEM, "device allocation", e);
         }
         c->bytes += sizes[k];
+    }
+    if (p->divbclean == 1) {
+        e = hipMalloc(&c->divb, 3 * db);
+        if (e != hipSuccess) {
+            mhd2d_destroy(c);
+            return fail(MHD2D_ENOMEM, "device allocation", e);
+        }
+        c->bytes += 3 * db;
     }
     if (c->lap_u() || c->lap_b()) {
         e = hipMalloc(&c->lap, 6 * db);
@@ -448,7 +502,7 @@ void mhd2d_destroy(mhd2d_ctx* c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
-    void* ptrs[] = {c->u, c->dudx, c->dudy, c->rprev, c->lap, c->rs, c->partial};
+    void* ptrs[] = {c->u, c->dudx, c->dudy, c->rprev, c->lap, c->divb, c->rs, c->partial};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -2,8 +2,8 @@
 // the GPU: output cadence dtOut, log cadence dtOut/10, the dt < 1e-8 stop,
 // and the final output at Tmax.  Two engines:
 //   MHD2D_DRIVE_FUSED  - the context API (fused kernels, dt on the device);
-//                        Hall MHD with viscosity and resistivity, no div B
-//                        cleaning.
+//                        Hall MHD with viscosity, resistivity and diffusive
+//                        div B cleaning.
 //   MHD2D_DRIVE_DROPIN - the reference's own call sequence through the
 //                        drop-in entry points on the host globals; any
 //                        supported setting (viscosity, resistivity,
@@ -59,6 +59,7 @@ int drive_fused(const mhd2d_input* in, const double* uu0, const mhd2d_drive_opts
     p.di = in->di;
     p.resistivity = in->resistivity;
     p.viscosity = in->viscosity;
+    p.divbclean = in->divbclean;
     p.precision = o->precision;
     p.xblocks = o->xblocks;
     p.open_x = in->boundaryTypeX == 1;
@@ -257,7 +258,7 @@ extern "C" int mhd2d_drive(const mhd2d_input* in, const double* uu0, const mhd2d
                              "npe >= 2 x blocks\n");
         return MHD2D_EINVAL;
     }
-    const bool fusable = in->divbclean == 0;
+    const bool fusable = in->divbclean != 2;
     int mode = o->mode;
     if (mode == MHD2D_DRIVE_AUTO) mode = fusable ? MHD2D_DRIVE_FUSED : MHD2D_DRIVE_DROPIN;
     if (mode == MHD2D_DRIVE_DROPIN && o->xblocks > 1) {
@@ -266,7 +267,7 @@ extern "C" int mhd2d_drive(const mhd2d_input* in, const double* uu0, const mhd2d
     }
     if (mode == MHD2D_DRIVE_FUSED) {
         if (!fusable) {
-            std::fprintf(stderr, "mhd2d_drive: the fused engine runs divBcleaning = 0 only\n");
+            std::fprintf(stderr, "mhd2d_drive: the fused engine runs divBcleaning 0 or 1\n");
             return MHD2D_EINVAL;
         }
         return drive_fused(in, uu0, o, rep);

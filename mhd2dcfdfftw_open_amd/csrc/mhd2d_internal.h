@@ -120,6 +120,8 @@ struct YApplyArgs {          // generic multiplier pass (drop-in entry points)
     YTables tb;
     int nv, mode;
     int nx, ny;
+    // instance z of a batch (blockIdx.y): in[v] + z * bs_in, out[v] + z * bs_out
+    size_t bs_in = 0, bs_out = 0;
 };
 
 struct DtParams {
@@ -197,14 +199,17 @@ struct RhsRkArgs {           // fused RHS + low-storage RK update (driver)
     TD* rprev;
     const TD* lap = nullptr; // [6][nx][ny] Laplacians of u and B (source terms)
     double viscosity = 0, resistivity = 0;
+    // diffusive div B cleaning (divBcleaning == 1): d(div B)/dx, d(div B)/dy
+    const TD* ddivb = nullptr;  // [2][nx][ny]
+    double cdiff = 0;           // constDiffDivB
     const RunScalars* rs;
     double gamma;
     int stage;
     int write_rprev;
     size_t n;
     // instance z of a batch (blockIdx.y): u + z * bs_u, dudx/dudy/dEdx/dEdy
-    // + z * bs_d, rprev + z * bs_r, lap + z * bs_l, rs + z
-    size_t bs_u = 0, bs_d = 0, bs_r = 0, bs_l = 0;
+    // + z * bs_d, rprev + z * bs_r, lap + z * bs_l, ddivb + z * bs_b, rs + z
+    size_t bs_u = 0, bs_d = 0, bs_r = 0, bs_l = 0, bs_b = 0;
     // open x boundary: rows 0 and nx-1 are left to launch_rhs_rk_open
     int open_x = 0;
     int nx = 0, ny = 0;
@@ -214,7 +219,7 @@ struct RhsRkArgs {           // fused RHS + low-storage RK update (driver)
 template <typename T, typename TI = T>
 hipError_t launch_xsolve(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int nb = 1);
 template <typename T>
-hipError_t launch_yapply(const YApplyArgs<T>& a, hipStream_t s);
+hipError_t launch_yapply(const YApplyArgs<T>& a, hipStream_t s, int nb = 1);
 template <typename T, typename TD = T>
 hipError_t launch_ysweep(const YSweepArgs<T, TD>& a, hipStream_t s, int nb = 1);
 template <typename TI>
@@ -231,6 +236,10 @@ template <typename T>
 hipError_t launch_efield(const T* u, T* E, size_t n, hipStream_t s);
 template <typename T>
 hipError_t launch_divb(const T* dudx, const T* dudy, T* divB, size_t n, hipStream_t s);
+// batched: instance z reads dudx, dudy + z * bs_d and writes divB + z * bs_o
+template <typename T>
+hipError_t launch_divb_b(const T* dudx, const T* dudy, T* divB, size_t n, size_t bs_d,
+                         size_t bs_o, int nb, hipStream_t s);
 template <typename T>
 hipError_t launch_rk(T* u, const T* dudt, T* dudtRK, double cc, double dd, size_t n,
                      hipStream_t s);

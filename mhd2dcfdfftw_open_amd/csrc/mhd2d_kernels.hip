@@ -514,14 +514,15 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) k_yapply(YApplyArgs<T
     const int i_raw = blockIdx.x * Y::ROWS + rl;
     const bool active = i_raw < a.nx;
     const size_t row = (size_t)(active ? i_raw : a.nx - 1) * N;
+    const size_t zi = blockIdx.y * a.bs_in, zo = blockIdx.y * a.bs_out;  // instance
     T* lre = s_re + rl * Y::LDSW;
     T* lim = s_im + rl * Y::LDSW;
     const double* __restrict__ mul =
         a.mode == Y_D1 ? a.tb.md : (a.mode == Y_D2 ? a.tb.mdd : a.tb.mf);
     for (int p = 0; p < a.nv; p += 2) {
         const bool two = (p + 1) < a.nv;
-        const T* fa = a.in[p] + row;
-        const T* fb = two ? a.in[p + 1] + row : fa;
+        const T* fa = a.in[p] + zi + row;
+        const T* fb = two ? a.in[p + 1] + zi + row : fa;
         cx<T> v[E];
 #pragma unroll
         for (int m = 0; m < E; m++) {
@@ -536,11 +537,11 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) k_yapply(YApplyArgs<T
         }
         fft_inv<T, N, E>(v, t, lre, lim, a.tb.tw);
         if (active) {
-            T* oa = a.out[p] + row;
+            T* oa = a.out[p] + zo + row;
 #pragma unroll
             for (int m = 0; m < E; m++) oa[t + m * TPR] = v[m].re;
             if (two) {
-                T* ob = a.out[p + 1] + row;
+                T* ob = a.out[p + 1] + zo + row;
 #pragma unroll
                 for (int m = 0; m < E; m++) ob[t + m * TPR] = v[m].im;
             }
@@ -740,7 +741,7 @@ static hipError_t launch_y_n(const YApplyArgs<T>* ya, const YSweepArgs<T, TD>* y
 {
     const int nx = ya ? ya->nx : ys->nx;
     using Y = YShape<T, N>;
-    dim3 grid((nx + Y::ROWS - 1) / Y::ROWS, ya ? 1 : nb), block(Y::THREADS);
+    dim3 grid((nx + Y::ROWS - 1) / Y::ROWS, nb), block(Y::THREADS);
     if constexpr (std::is_same<T, TD>::value) {
         if (ya) {
             hipLaunchKernelGGL((k_yapply<T, N>), grid, block, 0, s, *ya);
@@ -774,10 +775,10 @@ static hipError_t launch_y(int ny, const YApplyArgs<T>* ya, const YSweepArgs<T, 
 }
 
 template <typename T>
-hipError_t launch_yapply(const YApplyArgs<T>& a, hipStream_t s)
+hipError_t launch_yapply(const YApplyArgs<T>& a, hipStream_t s, int nb)
 {
     if (a.nv < 1 || a.nv > 16) return hipErrorInvalidValue;
-    return launch_y<T, T>(a.ny, &a, nullptr, s);
+    return launch_y<T, T>(a.ny, &a, nullptr, s, nb);
 }
 
 template <typename T, typename TD>
@@ -814,6 +815,19 @@ __global__ void k_divb(const T* __restrict__ dudx, const T* __restrict__ dudy,
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
          p += (size_t)gridDim.x * blockDim.x)
         divB[p] = dudx[BX * n + p] + dudy[BY * n + p];
+}
+
+// calcDivB (calcRHS.c:11-21) for instance blockIdx.y of a batch
+template <typename T>
+__global__ void k_divb_b(const T* __restrict__ dudx, const T* __restrict__ dudy,
+                         T* __restrict__ divB, size_t n, size_t bs_d, size_t bs_o)
+{
+    const T* const dx = dudx + blockIdx.y * bs_d;
+    const T* const dy = dudy + blockIdx.y * bs_d;
+    T* const o = divB + blockIdx.y * bs_o;
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
+         p += (size_t)gridDim.x * blockDim.x)
+        o[p] = dx[BX * n + p] + dy[BY * n + p];
 }
 
 // ideal-MHD right-hand side at one cell, calcRHS.c:85-142
@@ -998,7 +1012,7 @@ __global__ void __launch_bounds__(256)
             R[BZ].x[w] = (T)ey[0].x[w] - (T)ex[0].x[w];
             R[PR].x[w] = -ux * d_x[PR] - uy * d_y[PR] - g * pr * divU;
         }
-        if (LAP) {
+        if (LAP && a.lap) {
             // addSourceTerm (calcRHS.c:498-531): lap holds d2/dx2 + d2/dy2
             const TD* const lz = a.lap + z * a.bs_l;
             if (a.viscosity > 0) {
@@ -1019,6 +1033,17 @@ __global__ void __launch_bounds__(256)
                     for (int w = 0; w < W; w++)
                         R[BX + c].x[w] = R[BX + c].x[w] + (T)a.resistivity * (T)l.x[w];
                 }
+            }
+        }
+        if (LAP && a.ddivb) {
+            // diffusive div B cleaning (calcRHS.c:533-543)
+            const TD* const gz = a.ddivb + z * a.bs_b;
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const Pack<TD, W> gd = ldp<W>(gz + c * n + p);
+#pragma unroll
+                for (int w = 0; w < W; w++)
+                    R[BX + c].x[w] = R[BX + c].x[w] + (T)a.cdiff * (T)gd.x[w];
             }
         }
 #pragma unroll
@@ -1164,6 +1189,11 @@ __global__ void __launch_bounds__(256) k_rhs_rk_open(RhsRkArgs<T, TD> a)
 #pragma unroll
             for (int k = 0; k < 3; k++) R[BX + k] = R[BX + k] + (T)a.resistivity * (T)lz[(3 + k) * n + p];
         }
+    }
+    if (a.ddivb) {
+        const TD* const gz = a.ddivb + z * a.bs_b;
+        R[BX] = R[BX] + (T)a.cdiff * (T)gz[p];
+        R[BY] = R[BY] + (T)a.cdiff * (T)gz[n + p];
     }
     const bool use_prev = a.stage > 0;
 #pragma unroll
@@ -1322,6 +1352,17 @@ hipError_t launch_divb(const T* dudx, const T* dudy, T* divB, size_t n, hipStrea
 }
 
 template <typename T>
+hipError_t launch_divb_b(const T* dudx, const T* dudy, T* divB, size_t n, size_t bs_d,
+                         size_t bs_o, int nb, hipStream_t s)
+{
+    if (nb < 1 || nb > 65535) return hipErrorInvalidValue;
+    dim3 grid = pw_grid(n, 256);
+    grid.y = nb;
+    hipLaunchKernelGGL((k_divb_b<T>), grid, dim3(256), 0, s, dudx, dudy, divB, n, bs_d, bs_o);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_rhs(const RhsArgs<T>& a, hipStream_t s)
 {
     hipLaunchKernelGGL((k_rhs<T>), pw_grid(a.n, 256), dim3(256), 0, s, a);
@@ -1343,7 +1384,7 @@ hipError_t launch_rhs_rk(const RhsRkArgs<T, TD>& a, hipStream_t s, int nb)
     if (nb < 1 || nb > 65535) return hipErrorInvalidValue;
     dim3 grid = pw_grid(a.n / (16 / sizeof(T)), 256);
     grid.y = nb;
-    if (a.lap)
+    if (a.lap || a.ddivb)
         hipLaunchKernelGGL((k_rhs_rk<T, TD, true>), grid, dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL((k_rhs_rk<T, TD, false>), grid, dim3(256), 0, s, a);
@@ -1420,13 +1461,15 @@ hipError_t launch_diag(const T* u, const T* dbx, const T* dby, double* partial, 
 // explicit instantiations
 #define MHD2D_INST(T)                                                                        \
     template hipError_t launch_xsolve<T, T>(const XSolveArgs<T, T>&, int, hipStream_t, int); \
-    template hipError_t launch_yapply<T>(const YApplyArgs<T>&, hipStream_t);                 \
+    template hipError_t launch_yapply<T>(const YApplyArgs<T>&, hipStream_t, int);            \
     template hipError_t launch_ysweep<T, T>(const YSweepArgs<T, T>&, hipStream_t, int);     \
     template hipError_t launch_rhs<T>(const RhsArgs<T>&, hipStream_t);                       \
     template hipError_t launch_rhs_rk<T, T>(const RhsRkArgs<T, T>&, hipStream_t, int);      \
     template hipError_t launch_rhs_rk_open<T, T>(const RhsRkArgs<T, T>&, hipStream_t, int); \
     template hipError_t launch_efield<T>(const T*, T*, size_t, hipStream_t);                 \
     template hipError_t launch_divb<T>(const T*, const T*, T*, size_t, hipStream_t);         \
+    template hipError_t launch_divb_b<T>(const T*, const T*, T*, size_t, size_t, size_t, int, \
+                                         hipStream_t);                                       \
     template hipError_t launch_rk<T>(T*, const T*, T*, double, double, size_t, hipStream_t); \
     template hipError_t launch_dt_limits<T>(const T*, const DtParams&, RunScalars*, int, int, \
                                             hipStream_t);                                    \

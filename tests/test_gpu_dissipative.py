@@ -1,7 +1,9 @@
-"""Viscosity and resistivity in the fused engine (addSourceTerm,
-calcRHS.c:498-531): the second derivatives of u and B come from the y-sweep
-(filter x d2/dy2 of the spectrum) and an x-solve that adds d2/dx2, on every
-cell including the open-boundary lines.
+"""Viscosity, resistivity and diffusive div B cleaning in the fused engine
+(addSourceTerm, calcRHS.c:498-543): the second derivatives of u and B come
+from the y-sweep (filter x d2/dy2 of the spectrum) and an x-solve that adds
+d2/dx2; div B of the stage's derivatives gets its own x-solve and y
+transform (mhd.c:175-182); all on every cell including the open-boundary
+lines.
 
 * FP64: per-field rel-L2 <= 1e-12 against the oracle every step, periodic
   and open x (npe 2 and 3), with viscosity only, resistivity only, both;
@@ -33,20 +35,24 @@ def _state(m, nx, ny, flow=0.0, seed=3):
 
 
 CASES = [
-    # nx, ny, npe, open, viscosity, resistivity, hall, steps
-    (64, 32, 1, False, 2e-3, 0.0, 1, 6),
-    (64, 64, 1, False, 0.0, 1e-3, 1, 6),
-    (128, 32, 1, False, 5e-3, 3e-3, 0, 6),
-    (64, 32, 2, True, 2e-3, 1e-3, 1, 6),
-    (128, 32, 3, True, 0.0, 2e-3, 0, 5),
+    # nx, ny, npe, open, viscosity, resistivity, divbclean, hall, steps
+    (64, 32, 1, False, 2e-3, 0.0, 0, 1, 6),
+    (64, 64, 1, False, 0.0, 1e-3, 0, 1, 6),
+    (128, 32, 1, False, 5e-3, 3e-3, 0, 0, 6),
+    (64, 32, 2, True, 2e-3, 1e-3, 0, 1, 6),
+    (128, 32, 3, True, 0.0, 2e-3, 0, 0, 5),
+    (64, 64, 1, False, 0.0, 0.0, 1, 1, 6),
+    (128, 32, 1, False, 2e-3, 1e-3, 1, 0, 5),
+    (64, 32, 2, True, 0.0, 1e-3, 1, 1, 6),
 ]
 
 
-@pytest.mark.parametrize("nx,ny,npe,open_x,nu,eta,hall,steps", CASES)
-def test_dissipative_matches_oracle(m, nx, ny, npe, open_x, nu, eta, hall, steps):
-    print(f"[dissipative] {nx}x{ny} npe={npe} open={open_x} nu={nu} eta={eta}", flush=True)
+@pytest.mark.parametrize("nx,ny,npe,open_x,nu,eta,clean,hall,steps", CASES)
+def test_dissipative_matches_oracle(m, nx, ny, npe, open_x, nu, eta, clean, hall, steps):
+    print(f"[dissipative] {nx}x{ny} npe={npe} open={open_x} nu={nu} eta={eta} clean={clean}",
+          flush=True)
     u0 = _state(m, nx, ny, 0.3 if open_x else 0.0)
-    kw = dict(hall=hall, di=0.05, viscosity=nu, resistivity=eta)
+    kw = dict(hall=hall, di=0.05, viscosity=nu, resistivity=eta, divbclean=clean)
     ctx = m.Context(nx, ny, npe=npe, open_x=open_x, **kw)
     ctx.set_state(u0)
     S = orc.Sim(nx, ny, npe=npe, open_x=open_x, **kw)
@@ -58,21 +64,27 @@ def test_dissipative_matches_oracle(m, nx, ny, npe, open_x, nu, eta, hall, steps
         assert abs(ctx.scalars()[1] - S.time) <= 1e-12 * S.time
 
 
-def test_dissipative_changes_the_state(m):
-    # the source terms are really applied: a resistive run departs from the
-    # ideal one by far more than round-off
+@pytest.mark.parametrize("kw", [dict(resistivity=5e-3, viscosity=5e-3), dict(divbclean=1)])
+def test_dissipative_changes_the_state(m, kw):
+    # the source terms are really applied: the run departs from the ideal one
+    # by far more than round-off (div B of the seeded state is round-off
+    # only, so cleaning gets a state with a div B part added)
     u0 = _state(m, 64, 64)
+    if kw.get("divbclean"):
+        x = np.arange(64) * 2 * np.pi / 64
+        u0[4] += 0.01 * np.sin(2 * x)[:, None]
     a = m.Context(64, 64, hall=1, di=0.05)
-    b = m.Context(64, 64, hall=1, di=0.05, resistivity=5e-3, viscosity=5e-3)
+    b = m.Context(64, 64, hall=1, di=0.05, **kw)
     for c in (a, b):
         c.set_state(u0)
         c.run(4)
     assert field_errors(a.state(), b.state()).max() > 1e-6
 
 
-def test_dissipative_graph_and_batch(m):
+@pytest.mark.parametrize("clean", [0, 1])
+def test_dissipative_graph_and_batch(m, clean):
     u0, u1 = _state(m, 64, 32, seed=4), _state(m, 64, 32, seed=5)
-    kw = dict(hall=1, di=0.05, viscosity=2e-3, resistivity=1e-3)
+    kw = dict(hall=1, di=0.05, viscosity=2e-3, resistivity=1e-3, divbclean=clean)
     eager = m.Context(64, 32, **kw)
     eager.set_state(u0)
     eager.run(5)
@@ -87,9 +99,10 @@ def test_dissipative_graph_and_batch(m):
     np.testing.assert_array_equal(b.state(inst=1), eager.state())
 
 
-def test_dissipative_mixed_precision(m):
+@pytest.mark.parametrize("clean", [0, 1])
+def test_dissipative_mixed_precision(m, clean):
     u0 = _state(m, 128, 64, seed=6)
-    kw = dict(hall=1, di=0.05, viscosity=2e-3, resistivity=1e-3)
+    kw = dict(hall=1, di=0.05, viscosity=2e-3, resistivity=1e-3, divbclean=clean)
     ref = m.Context(128, 64, **kw)
     mix = m.Context(128, 64, precision=m.MIXED, **kw)
     for c in (ref, mix):

@@ -71,6 +71,7 @@ def _inp(m, **kw):
 @pytest.mark.parametrize("engine,extra", [
     ("fused", {}),
     ("fused", dict(viscosity=2e-3, resistivity=1e-3, hall=0)),
+    ("fused", dict(divbclean=1)),
     ("dropin", {}),
     ("dropin", dict(viscosity=2e-3, resistivity=1e-3, hall=0)),
     ("dropin", dict(divbclean=1)),
