@@ -333,7 +333,7 @@ class Ensemble:
     """Runs on grids of different sizes advanced together
     (mhd2d_ens_create ... mhd2d_ens_destroy).  `runs` is a list of dicts with
     nx, ny and optionally Lx, Ly, gamma, cfl, hall, di, npe, resistivity,
-    viscosity."""
+    viscosity, divbclean."""
 
     def __init__(self, runs, precision=FP64):
         arr = (Params * len(runs))()
@@ -341,7 +341,8 @@ class Ensemble:
             arr[k] = Params(r["nx"], r["ny"], r.get("Lx", 2 * np.pi), r.get("Ly", 2 * np.pi),
                             r.get("gamma", 5.0 / 3.0), r.get("cfl", 0.3), r.get("hall", 0),
                             r.get("di", 0.0), r.get("resistivity", 0.0),
-                            r.get("viscosity", 0.0), 0, precision, r.get("npe", 1), 0,
+                            r.get("viscosity", 0.0), r.get("divbclean", 0), precision,
+                            r.get("npe", 1), 0,
                             r.get("xgemm", 0), int(r.get("open_x", False)))
         self.shapes = [(r["nx"], r["ny"]) for r in runs]
         h = ctypes.c_void_p()

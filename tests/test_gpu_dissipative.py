@@ -133,3 +133,22 @@ def test_reference_input_runs_fused(m, tmp_path):
     assert abs(a[0] - S.time) <= 1e-12 * S.time
     assert_fields_close(np.reshape(np.asarray(a[1:]), [8, 64, 32]), S.state(),
                         what="reference input, final output")
+
+
+def test_dissipative_ensemble_matches_single_runs(m):
+    # an ensemble of two grid shapes with the source terms and the diffusive
+    # cleaning equals the same runs made one context at a time
+    kw = dict(hall=1, viscosity=2e-3, resistivity=1e-3, divbclean=1)
+    runs = [dict(nx=64, ny=32, di=0.02, **kw), dict(nx=128, ny=64, di=0.05, **kw),
+            dict(nx=64, ny=32, di=0.04, **kw)]
+    states = [_state(m, r["nx"], r["ny"], seed=20 + k) for k, r in enumerate(runs)]
+    ens = m.Ensemble(runs)
+    for k, u in enumerate(states):
+        ens.set_state(k, u)
+    ens.run(4, graph=True)
+    for k, r in enumerate(runs):
+        c = m.Context(r["nx"], r["ny"], hall=1, di=r["di"], viscosity=2e-3, resistivity=1e-3,
+                      divbclean=1)
+        c.set_state(states[k])
+        c.run(4)
+        np.testing.assert_array_equal(ens.state(k), c.state())
