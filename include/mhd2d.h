@@ -117,7 +117,8 @@ typedef struct mhd2d_params {
     double di;
     double resistivity;     /* >= 0: eta Lap B (addSourceTerm, calcRHS.c:498-531) */
     double viscosity;       /* >= 0: nu/rho Lap u (same) */
-    int divbclean;          /* 0 or 1 (diffusive) in the context API */
+    int divbclean;          /* 0, 1 (diffusive) or 2 (GLM: the state gains
+                               psi as a ninth field, host arrays [9][nx][ny]) */
     int precision;          /* MHD2D_FP64, MHD2D_FP32 or MHD2D_MIXED */
     int xblocks;            /* 0 or 1: x is one block (the reference on one MPI
                                rank); npe > 1: x split into npe blocks sized as
@@ -156,8 +157,9 @@ enum {
 
 int mhd2d_create(const mhd2d_params *p, mhd2d_ctx **out);
 void mhd2d_destroy(mhd2d_ctx *c);
-/* upload a state [8][nx][ny] (FP64 host); resets time, step and dt, and the
- * first step's dt is calcDt(0) of this state (mhd.c:58) */
+/* upload a state [8][nx][ny] (FP64 host; [9][nx][ny] with psi last when
+ * divbclean == 2); resets time, step and dt, and the first step's dt is
+ * calcDt(0) of this state (mhd.c:58) */
 int mhd2d_set_state(mhd2d_ctx *c, const double *uu);
 /* download the state after the last step (applies the pending y filter) */
 int mhd2d_get_state(mhd2d_ctx *c, double *uu);
@@ -283,8 +285,7 @@ enum { MHD2D_DRIVE_AUTO = 0, MHD2D_DRIVE_FUSED = 1, MHD2D_DRIVE_DROPIN = 2 };
 
 typedef struct mhd2d_drive_opts {
     const char *outdir;     /* directory of the output files ("" = cwd) */
-    int mode;               /* MHD2D_DRIVE_*: AUTO = FUSED unless the
-                               cleaning is hyperbolic (GLM), DROPIN then */
+    int mode;               /* MHD2D_DRIVE_*: AUTO = FUSED */
     int precision;          /* MHD2D_FP64 / MHD2D_FP32 (FUSED only) */
     int verbose;            /* print the reference's progress lines */
     long long max_steps;    /* stop after this many steps (0: no cap) */

@@ -237,8 +237,11 @@ class Context:
         ranks does (decomposeGrid1D, initialize.c:547-569); xgemm 3 or 6
         takes x-derivatives as split-BF16 MFMA GEMMs with the dense A^-1 B;
         resistivity and viscosity add the reference's addSourceTerm
-        (calcRHS.c:498-531); divbclean = 1 the diffusive div B cleaning."""
+        (calcRHS.c:498-531); divbclean = 1 the diffusive div B cleaning,
+        divbclean = 2 the GLM cleaning, whose states carry psi as a ninth
+        field [9][nx][ny]."""
         self.nx, self.ny, self.batch = nx, ny, batch
+        self.nvar = 9 if divbclean == 2 else 8
         self.params = Params(nx, ny, Lx, Ly, gamma, cfl, hall, di, resistivity, viscosity,
                              divbclean, precision, npe, batch, xgemm, int(open_x))
         h = ctypes.c_void_p()
@@ -255,11 +258,11 @@ class Context:
 
     def set_state(self, uu, inst=0):
         uu = np.ascontiguousarray(uu, dtype=np.float64)
-        assert uu.shape == (8, self.nx, self.ny)
+        assert uu.shape == (self.nvar, self.nx, self.ny)
         _check(lib().mhd2d_set_state_at(self.h, inst, _ptr(uu)))
 
     def state(self, inst=0):
-        uu = np.zeros((8, self.nx, self.ny))
+        uu = np.zeros((self.nvar, self.nx, self.ny))
         _check(lib().mhd2d_get_state_at(self.h, inst, _ptr(uu)))
         return uu
 
@@ -345,6 +348,7 @@ class Ensemble:
                             r.get("npe", 1), 0,
                             r.get("xgemm", 0), int(r.get("open_x", False)))
         self.shapes = [(r["nx"], r["ny"]) for r in runs]
+        self.nvars = [9 if r.get("divbclean", 0) == 2 else 8 for r in runs]
         h = ctypes.c_void_p()
         _check(lib().mhd2d_ens_create(arr, len(runs), ctypes.byref(h)))
         self.h = h
@@ -363,11 +367,11 @@ class Ensemble:
 
     def set_state(self, inst, uu):
         uu = np.ascontiguousarray(uu, dtype=np.float64)
-        assert uu.shape == (8,) + self.shapes[inst]
+        assert uu.shape == (self.nvars[inst],) + self.shapes[inst]
         _check(lib().mhd2d_ens_set_state(self.h, inst, _ptr(uu)))
 
     def state(self, inst):
-        uu = np.zeros((8,) + self.shapes[inst])
+        uu = np.zeros((self.nvars[inst],) + self.shapes[inst])
         _check(lib().mhd2d_ens_get_state(self.h, inst, _ptr(uu)))
         return uu
 

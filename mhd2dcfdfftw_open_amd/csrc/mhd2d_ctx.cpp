@@ -68,6 +68,8 @@ struct mhd2d_ctx {
     // [B][3][nx][ny] (derivative type), with diffusive div B cleaning: div B,
     // d(div B)/dx, d(div B)/dy
     void* divb = nullptr;
+    // [B][xblocks][2] FP64, with GLM cleaning: Ch, Cp of every x block
+    double* glm_cp = nullptr;
     RunScalars* rs = nullptr;  // [B]
     double* partial = nullptr;
     int npartial = 0;
@@ -80,9 +82,13 @@ struct mhd2d_ctx {
     double cdiff = 0;  // constDiffDivB = 2 max(dx, dy)^2 (initialize.c:17, 154)
     std::vector<char> have_state;
     std::vector<double> di;
-    size_t su() const { return 11 * n; }  // per-instance strides (elements)
+    // per-instance strides (elements); with GLM cleaning psi takes slot
+    // PSI_SLOT of u and the diagnostics scratch moves to slot 11
+    bool glm() const { return p.divbclean == 2; }
+    size_t su() const { return (glm() ? 12 : 11) * n; }
+    int nv_state() const { return glm() ? 9 : 8; }
     size_t sd() const { return 11 * n; }
-    size_t sr() const { return 8 * n; }
+    size_t sr() const { return (glm() ? 9 : 8) * n; }
     size_t sl() const { return 6 * n; }
     size_t sb() const { return 3 * n; }
     bool lap_u() const { return p.viscosity > 0; }
@@ -124,6 +130,7 @@ hipError_t ysweep(mhd2d_ctx* c, int filter, int derivs, int dt_limits)
     a.dt_limits = dt_limits;
     a.bs_u = c->su();
     a.bs_d = c->sd();
+    a.glm = c->glm();
     if (c->lap && derivs) {
         a.lap = (TD*)c->lap;
         a.lap_u = c->lap_u();
@@ -183,8 +190,9 @@ template <typename T, typename TD>
 hipError_t xderiv(mhd2d_ctx* c)
 {
     // 8 state fields, Ey and Ez, solved in the derivative type
-    hipError_t e = xderiv_fields<T, TD>(c, (const T*)c->u, (TD*)c->dudx, 10, c->su(), c->sd(),
-                                        c->B, c->rs);
+    // (and psi in slot PSI_SLOT = 10 with GLM cleaning)
+    hipError_t e = xderiv_fields<T, TD>(c, (const T*)c->u, (TD*)c->dudx, c->glm() ? 11 : 10,
+                                        c->su(), c->sd(), c->B, c->rs);
     if (e != hipSuccess || !c->lap) return e;
     // second x-derivatives of ux..uz (viscosity) and / or Bx..Bz
     // (resistivity), added onto the second y-derivatives in lap
@@ -214,6 +222,7 @@ hipError_t xfilter(mhd2d_ctx* c)
     XSolveArgs<T> a;
     T* u = (T*)c->u;
     for (int v = 0; v < 8; v++) a.in[v] = a.out[v] = u + v * c->n;
+    if (c->glm()) a.in[8] = a.out[8] = u + PSI_SLOT * c->n;
     a.rs = c->rs;
     a.st = c->tb.st[XFILT];
     a.tb = c->tb.xt[XFILT];
@@ -223,7 +232,7 @@ hipError_t xfilter(mhd2d_ctx* c)
     a.bs_in = a.bs_out = c->su();
     a.width = c->xw_f;
     a.remap = c->xr_f;
-    return launch_xsolve<T>(a, 8, c->stream, c->B);
+    return launch_xsolve<T>(a, c->nv_state(), c->stream, c->B);
 }
 
 template <typename T, typename TD>
@@ -292,6 +301,11 @@ hipError_t rhs_rk(mhd2d_ctx* c, int stage)
         a.bs_b = c->sb();
         a.cdiff = c->cdiff;
     }
+    if (c->glm()) {
+        a.glm = 1;
+        a.npe = c->p.xblocks > 1 ? c->p.xblocks : 1;
+        a.glm_cp = c->glm_cp;
+    }
     if (e == hipSuccess) e = launch_rhs_rk<T, TD>(a, c->stream, c->B);
     if (e == hipSuccess && c->p.open_x) e = launch_rhs_rk_open<T, TD>(a, c->stream, c->B);
     return e;
@@ -309,6 +323,14 @@ hipError_t step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
         e = ysweep<T, TD>(c, (s > 0 || c->pending_filter) ? 1 : 0, 1, s == 0 ? 1 : 0);
         if (e != hipSuccess) return e;
         mark(0);
+        if (s == 0 && c->glm()) {
+            // calcDt's Ch / Cp of the state it sees (timeAdvance.c:107-112)
+            DtParams dp = dt_params(c);
+            e = launch_glm_blocks<T>((const T*)c->u, c->su(), dp, c->rs, c->glm_cp, c->p.nx,
+                                     c->p.ny, c->p.xblocks > 1 ? c->p.xblocks : 1, c->B,
+                                     c->stream);
+            if (e != hipSuccess) return e;
+        }
         if (s == 0) {
             e = launch_dt_finalize(c->rs, c->p.cfl, 0, c->stream, c->B);
             if (e != hipSuccess) return e;
@@ -401,10 +423,8 @@ int check_params(const mhd2d_params* p)
         return fail(MHD2D_EINVAL, "Lx, Ly, cfl must be > 0 and gamma > 1");
     if (!(p->resistivity >= 0) || !(p->viscosity >= 0))
         return fail(MHD2D_EINVAL, "resistivity and viscosity must be >= 0");
-    if (p->divbclean != 0 && p->divbclean != 1)
-        return fail(MHD2D_EINVAL,
-                    "the context API runs divbclean 0 or 1; use the drop-in entry points "
-                    "for hyperbolic (GLM) cleaning");
+    if (p->divbclean < 0 || p->divbclean > 2)
+        return fail(MHD2D_EINVAL, "divbclean must be 0, 1 or 2");
     if (p->precision != MHD2D_FP64 && p->precision != MHD2D_FP32 &&
         p->precision != MHD2D_MIXED)
         return fail(MHD2D_EINVAL, "precision must be MHD2D_FP64, MHD2D_FP32 or MHD2D_MIXED");
@@ -460,7 +480,8 @@ int mhd2d_create(const mhd2d_params* p, mhd2d_ctx** out)
     }
     const size_t fb = c->n * c->elem * c->B, db = c->n * c->delem * c->B;
     c->npartial = 1024;
-    size_t sizes[] = {11 * fb, 11 * db, 11 * db, 8 * db};
+    size_t sizes[] = {(p->divbclean == 2 ? 12 : 11) * fb, 11 * db, 11 * db,
+                      (p->divbclean == 2 ? 9 : 8) * db};
     void** ptrs[] = {&c->u, &c->dudx, &c->dudy, &c->rprev};
     for (int k = 0; k < 4; k++) {
         e = hipMalloc(ptrs[k], sizes[k]);
@@ -469,6 +490,15 @@ int mhd2d_create(const mhd2d_params* p, mhd2d_ctx** out)
             return fail(MHD2D_ENOMEM, "device allocation", e);
         }
         c->bytes += sizes[k];
+    }
+    if (p->divbclean == 2) {
+        const size_t gb = 2 * sizeof(double) * c->B * (p->xblocks > 1 ? p->xblocks : 1);
+        e = hipMalloc(&c->glm_cp, gb);
+        if (e != hipSuccess) {
+            mhd2d_destroy(c);
+            return fail(MHD2D_ENOMEM, "device allocation", e);
+        }
+        c->bytes += gb;
     }
     if (p->divbclean == 1) {
         e = hipMalloc(&c->divb, 3 * db);
@@ -502,7 +532,8 @@ void mhd2d_destroy(mhd2d_ctx* c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
-    void* ptrs[] = {c->u, c->dudx, c->dudy, c->rprev, c->lap, c->divb, c->rs, c->partial};
+    void* ptrs[] = {c->u,    c->dudx, c->dudy, c->rprev, c->lap,
+                    c->divb, c->glm_cp, c->rs, c->partial};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -547,17 +578,28 @@ int mhd2d_set_state_at(mhd2d_ctx* c, int inst, const double* uu)
     if (inst < 0 || inst >= c->B) return fail(MHD2D_EINVAL, "instance out of range");
     // the other instances keep their pending y filter: complete it first
     HIPCHK(do_flush(c));
+    // host fields 0..7 -> slots 0..7; psi (host field 8, GLM) -> PSI_SLOT
     const size_t cnt = 8 * c->n;
+    const double* hpsi = uu + cnt;
     if (c->elem == 8) {
         double* dst = (double*)c->u + inst * c->su();
         HIPCHK(hipMemcpyAsync(dst, uu, cnt * 8, hipMemcpyHostToDevice, c->stream));
+        if (c->glm())
+            HIPCHK(hipMemcpyAsync(dst + PSI_SLOT * c->n, hpsi, c->n * 8, hipMemcpyHostToDevice,
+                                  c->stream));
     } else {
         double* tmp = nullptr;
-        HIPCHK(hipMalloc(&tmp, cnt * 8));
-        HIPCHK(hipMemcpyAsync(tmp, uu, cnt * 8, hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_to_float, dim3(2048), dim3(256), 0, c->stream, tmp,
-                           (float*)c->u + inst * c->su(), cnt);
+        const size_t tot = cnt + (c->glm() ? c->n : 0);
+        HIPCHK(hipMalloc(&tmp, tot * 8));
+        HIPCHK(hipMemcpyAsync(tmp, uu, tot * 8, hipMemcpyHostToDevice, c->stream));
+        float* dst = (float*)c->u + inst * c->su();
+        hipLaunchKernelGGL(k_to_float, dim3(2048), dim3(256), 0, c->stream, tmp, dst, cnt);
         HIPCHK(hipGetLastError());
+        if (c->glm()) {
+            hipLaunchKernelGGL(k_to_float, dim3(2048), dim3(256), 0, c->stream, tmp + cnt,
+                               dst + PSI_SLOT * c->n, c->n);
+            HIPCHK(hipGetLastError());
+        }
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipFree(tmp));
     }
@@ -594,13 +636,22 @@ int mhd2d_get_state_at(mhd2d_ctx* c, int inst, double* uu)
     if (c->elem == 8) {
         const double* src = (const double*)c->u + inst * c->su();
         HIPCHK(hipMemcpyAsync(uu, src, cnt * 8, hipMemcpyDeviceToHost, c->stream));
+        if (c->glm())
+            HIPCHK(hipMemcpyAsync(uu + cnt, src + PSI_SLOT * c->n, c->n * 8,
+                                  hipMemcpyDeviceToHost, c->stream));
     } else {
         double* tmp = nullptr;
-        HIPCHK(hipMalloc(&tmp, cnt * 8));
-        hipLaunchKernelGGL(k_to_double, dim3(2048), dim3(256), 0, c->stream,
-                           (const float*)c->u + inst * c->su(), tmp, cnt);
+        const size_t tot = cnt + (c->glm() ? c->n : 0);
+        HIPCHK(hipMalloc(&tmp, tot * 8));
+        const float* src = (const float*)c->u + inst * c->su();
+        hipLaunchKernelGGL(k_to_double, dim3(2048), dim3(256), 0, c->stream, src, tmp, cnt);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(uu, tmp, cnt * 8, hipMemcpyDeviceToHost, c->stream));
+        if (c->glm()) {
+            hipLaunchKernelGGL(k_to_double, dim3(2048), dim3(256), 0, c->stream,
+                               src + PSI_SLOT * c->n, tmp + cnt, c->n);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipMemcpyAsync(uu, tmp, tot * 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipFree(tmp));
     }
@@ -721,8 +772,9 @@ int mhd2d_get_divb(mhd2d_ctx* c, double* divb)
         ya.nx = c->p.nx;
         ya.ny = c->p.ny;
         HIPCHK(launch_yapply<double>(ya, c->stream));
-        HIPCHK(launch_add<double>(sc, sc + n, sc + 2 * n, n, c->stream));
-        HIPCHK(hipMemcpyAsync(divb, sc + 2 * n, n * 8, hipMemcpyDeviceToHost, c->stream));
+        double* s3 = u + (c->glm() ? 11 : 10) * n;  // not psi
+        HIPCHK(launch_add<double>(sc, sc + n, s3, n, c->stream));
+        HIPCHK(hipMemcpyAsync(divb, s3, n * 8, hipMemcpyDeviceToHost, c->stream));
     } else {
         float* u = (float*)c->u;
         float* sc = u + 8 * n;
@@ -744,9 +796,10 @@ int mhd2d_get_divb(mhd2d_ctx* c, double* divb)
         ya.nx = c->p.nx;
         ya.ny = c->p.ny;
         HIPCHK(launch_yapply<float>(ya, c->stream));
-        HIPCHK(launch_add<float>(sc, sc + n, sc + 2 * n, n, c->stream));
+        float* s3 = u + (c->glm() ? 11 : 10) * n;  // not psi
+        HIPCHK(launch_add<float>(sc, sc + n, s3, n, c->stream));
         std::vector<float> tmp(n);
-        HIPCHK(hipMemcpyAsync(tmp.data(), sc + 2 * n, n * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(tmp.data(), s3, n * 4, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         for (size_t i = 0; i < n; i++) divb[i] = tmp[i];
     }

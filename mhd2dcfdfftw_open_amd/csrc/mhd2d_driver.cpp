@@ -3,7 +3,7 @@
 // and the final output at Tmax.  Two engines:
 //   MHD2D_DRIVE_FUSED  - the context API (fused kernels, dt on the device);
 //                        Hall MHD with viscosity, resistivity and diffusive
-//                        div B cleaning.
+//                        or GLM div B cleaning.
 //   MHD2D_DRIVE_DROPIN - the reference's own call sequence through the
 //                        drop-in entry points on the host globals; any
 //                        supported setting (viscosity, resistivity,
@@ -67,9 +67,11 @@ int drive_fused(const mhd2d_input* in, const double* uu0, const mhd2d_drive_opts
     int rc = mhd2d_create(&p, &c);
     if (rc) return rc;
     const size_t n = (size_t)in->nx * in->ny;
-    std::vector<double> uu(uu0, uu0 + 8 * n), divb(n);
+    const int nvar = in->divbclean == 2 ? 9 : 8;      // psi starts at 0
+    std::vector<double> uu(nvar * n, 0.0), divb(n);
+    std::memcpy(uu.data(), uu0, 8 * n * sizeof(double));
     rc = mhd2d_set_state(c, uu.data());
-    Out w{in, o, 0};
+    Out w{in, o, 0, nvar};
     const double t0 = now_s();
     double t = 0.0, dt = 0.0, tOut = 0.0, tLog = 0.0, maxdivb = 0.0, energy = 0.0;
     const double dtLog = in->dtOut / 10.0;  // mhd.c:66
@@ -258,19 +260,12 @@ extern "C" int mhd2d_drive(const mhd2d_input* in, const double* uu0, const mhd2d
                              "npe >= 2 x blocks\n");
         return MHD2D_EINVAL;
     }
-    const bool fusable = in->divbclean != 2;
     int mode = o->mode;
-    if (mode == MHD2D_DRIVE_AUTO) mode = fusable ? MHD2D_DRIVE_FUSED : MHD2D_DRIVE_DROPIN;
+    if (mode == MHD2D_DRIVE_AUTO) mode = MHD2D_DRIVE_FUSED;
     if (mode == MHD2D_DRIVE_DROPIN && o->xblocks > 1) {
         std::fprintf(stderr, "mhd2d_drive: x blocks (npe > 1) need the fused engine\n");
         return MHD2D_EINVAL;
     }
-    if (mode == MHD2D_DRIVE_FUSED) {
-        if (!fusable) {
-            std::fprintf(stderr, "mhd2d_drive: the fused engine runs divBcleaning 0 or 1\n");
-            return MHD2D_EINVAL;
-        }
-        return drive_fused(in, uu0, o, rep);
-    }
+    if (mode == MHD2D_DRIVE_FUSED) return drive_fused(in, uu0, o, rep);
     return drive_dropin(in, uu0, o, rep);
 }

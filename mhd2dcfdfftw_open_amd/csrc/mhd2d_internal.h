@@ -10,6 +10,9 @@ namespace mhd2d {
 
 // field indices of the state, same order as the reference (macros.h:4-11)
 enum Field { RHO = 0, UX, UY, UZ, BX, BY, BZ, PR, NFIELD = 8 };
+// psi of the hyperbolic (GLM) div B cleaning in the fused engine's state
+// buffer (after the 8 fields and E_y, E_z), and of its derivative buffers
+constexpr int PSI_SLOT = 10;
 
 struct RunScalars;
 
@@ -163,6 +166,7 @@ struct YSweepArgs {          // fused y-sweep of the driver
     // lap[3..5]; the x-solve adds the second x-derivatives
     TD* lap = nullptr;       // [6][nx][ny]
     int lap_u = 0, lap_b = 0;
+    int glm = 0;             // psi (slot PSI_SLOT) is filtered and differentiated too
     // instance z of a batch (blockIdx.y): u, E + z * bs_u; dudy, dEdy +
     // z * bs_d; lap + z * bs_l; rs + z (whose di replaces dp.di)
     size_t bs_u = 0, bs_d = 0, bs_l = 0;
@@ -202,6 +206,12 @@ struct RhsRkArgs {           // fused RHS + low-storage RK update (driver)
     // diffusive div B cleaning (divBcleaning == 1): d(div B)/dx, d(div B)/dy
     const TD* ddivb = nullptr;  // [2][nx][ny]
     double cdiff = 0;           // constDiffDivB
+    // hyperbolic cleaning (divBcleaning == 2): psi in slot PSI_SLOT of u,
+    // dudx, dudy and in field 8 of rprev; Ch, Cp of each of the npe x blocks
+    // of instance z at glm_cp[2 (z npe + b)]
+    int glm = 0;
+    int npe = 1;
+    const double* glm_cp = nullptr;
     const RunScalars* rs;
     double gamma;
     int stage;
@@ -236,6 +246,12 @@ template <typename T>
 hipError_t launch_efield(const T* u, T* E, size_t n, hipStream_t s);
 template <typename T>
 hipError_t launch_divb(const T* dudx, const T* dudy, T* divB, size_t n, hipStream_t s);
+// GLM Ch of each x block (calcDt's value: the fastest speed of the block's
+// last cell, timeAdvance.c:107-112) and Cp = sqrt(0.18 Ch) (divBcleaning.c:
+// 7-10), for every instance: out[2 (z npe + b)] = {Ch, Cp}
+template <typename T>
+hipError_t launch_glm_blocks(const T* u, size_t bs_u, const DtParams& dp, const RunScalars* rs,
+                             double* out, int nx, int ny, int npe, int nb, hipStream_t s);
 // batched: instance z reads dudx, dudy + z * bs_d and writes divB + z * bs_o
 template <typename T>
 hipError_t launch_divb_b(const T* dudx, const T* dudy, T* divB, size_t n, size_t bs_d,

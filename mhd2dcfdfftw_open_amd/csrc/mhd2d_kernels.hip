@@ -579,9 +579,12 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
 
     // field pairs packed as re/im of one complex line; the velocity and field
     // components sit in pairs 0-2 so that the second derivatives of the
-    // source terms take at most three more inverse transforms
-    constexpr int PA[4] = {UX, UZ, BX, RHO};
-    constexpr int PB[4] = {UY, BZ, BY, PR};
+    // source terms take at most three more inverse transforms; the GLM psi
+    // (slot PSI_SLOT) is a fifth line of its own, in the LAP instance only,
+    // so the ideal instance keeps four full pairs and its register budget
+    constexpr int PA[5] = {UX, UZ, BX, RHO, PSI_SLOT};
+    constexpr int PB[5] = {UY, BZ, BY, PR, -1};
+    const int npairs = (LAP && a.glm) ? 5 : 4;
     if (a.filter || a.derivs) {
         // The next pair is loaded while the last transform of the current
         // pair runs (its registers replace the saved spectrum Z), so the HBM
@@ -594,10 +597,11 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
         }
         TD* const lapz = a.lap ? a.lap + z * a.bs_l : nullptr;
 #pragma unroll 1
-        for (int p = 0; p < 4; p++) {
+        for (int p = 0; p < npairs; p++) {
             const int fa = PA[p], fb = PB[p];
+            const bool two = !LAP || fb >= 0;
             T* ua = u + fa * n + row;
-            T* ub = u + fb * n + row;
+            T* ub = u + (two ? fb : fa) * n + row;
             // second y-derivatives wanted from this pair (re: fa, im: fb)
             const bool la = LAP && a.derivs &&
                             ((a.lap_u && (fa == UX || fa == UZ)) || (a.lap_b && fa == BX));
@@ -625,7 +629,7 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
                     for (int m = 0; m < E; m++) {
                         const int k = t + m * TPR;
                         st_stream<MHD2D_NT_Y>(ua + k, v[m].re);
-                        st_stream<MHD2D_NT_Y>(ub + k, v[m].im);
+                        if (two) st_stream<MHD2D_NT_Y>(ub + k, v[m].im);
                     }
                 }
                 if (!LAP && a.derivs) {
@@ -659,25 +663,26 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
 #pragma unroll
                 for (int m = 0; m < E; m++) v[m] = mul_i_scal(Z[m], (T)m1[tq + m * TPR]);
             }
-            if (p < 3) {
+            if (p + 1 < npairs) {
                 const T* na = u + PA[p + 1] * n + row;
-                const T* nb = u + PB[p + 1] * n + row;
+                const bool two_n = !LAP || PB[p + 1] >= 0;
+                const T* nb = u + (two_n ? PB[p + 1] : PA[p + 1]) * n + row;
 #pragma unroll
                 for (int m = 0; m < E; m++) {
                     const int k = t + m * TPR;
-                    nxt[m] = cx<T>{na[k], nb[k]};
+                    nxt[m] = cx<T>{na[k], two_n ? nb[k] : (T)0};
                 }
             }
             if (a.derivs) {
                 fft_inv<T, N, E>(v, t, lre, lim, tw);
                 if (active) {
                     TD* da = dudy + fa * n + row;
-                    TD* db = dudy + fb * n + row;
+                    TD* db = dudy + (two ? fb : fa) * n + row;
 #pragma unroll
                     for (int m = 0; m < E; m++) {
                         const int k = t + m * TPR;
                         st_stream<MHD2D_NT_Y>(da + k, (TD)v[m].re);
-                        st_stream<MHD2D_NT_Y>(db + k, (TD)v[m].im);
+                        if (two) st_stream<MHD2D_NT_Y>(db + k, (TD)v[m].im);
                     }
                 }
             }
@@ -748,7 +753,7 @@ static hipError_t launch_y_n(const YApplyArgs<T>* ya, const YSweepArgs<T, TD>* y
             return hipGetLastError();
         }
     }
-    if (ys->lap)
+    if (ys->lap || ys->glm)
         hipLaunchKernelGGL((k_ysweep<T, TD, N, true>), grid, block, 0, s, *ys);
     else
         hipLaunchKernelGGL((k_ysweep<T, TD, N, false>), grid, block, 0, s, *ys);
@@ -895,6 +900,39 @@ __global__ void k_rhs(RhsArgs<T> a)
     }
 }
 
+// x block (MPI rank of decomposeGrid1D, initialize.c:547-569) of row i
+__device__ __forceinline__ int x_block_of(int i, int nx, int npe)
+{
+    const int base = nx / npe, rem = nx % npe;
+    const int big = rem * (base + 1);
+    return i < big ? i / (base + 1) : rem + (i - big) / base;
+}
+
+// Ch of every x block of every instance: calcDt on each rank leaves the
+// fastest speed of its own last cell (timeAdvance.c:107-112), and
+// calc_Cp_fix_Cr sets Cp = sqrt(Cr Ch) with Cr = 0.18 (divBcleaning.c:4-10)
+template <typename T>
+__global__ void k_glm_blocks(const T* __restrict__ u, size_t bs_u, DtParams dp,
+                             const RunScalars* __restrict__ rs, double* __restrict__ out,
+                             int nx, int ny, int npe)
+{
+    const int z = blockIdx.x;
+    const T* const uz = u + z * bs_u;
+    const size_t n = (size_t)nx * ny;
+    dp.di = rs[z].di;
+    for (int b = threadIdx.x; b < npe; b += blockDim.x) {
+        const int base = nx / npe, rem = nx % npe;
+        const int off = b * base + (b < rem ? b : rem), N = base + (b < rem ? 1 : 0);
+        const size_t p = (size_t)(off + N - 1) * ny + (ny - 1);
+        double cmx, cmy;
+        cell_speeds(uz[RHO * n + p], uz[UX * n + p], uz[UY * n + p], uz[BX * n + p],
+                    uz[BY * n + p], uz[BZ * n + p], uz[PR * n + p], dp, cmx, cmy);
+        const double ch = fmax(cmx, cmy);
+        out[2 * ((size_t)z * npe + b)] = ch;
+        out[2 * ((size_t)z * npe + b) + 1] = sqrt(0.18 * ch);
+    }
+}
+
 // Ch of calcDt's divBcleaning == 2 branch (timeAdvance.c:107-112): the value
 // the cell loop leaves, i.e. max(cmaxx, cmaxy) of the last cell (n - 1)
 template <typename T>
@@ -1012,6 +1050,27 @@ __global__ void __launch_bounds__(256)
             R[BZ].x[w] = (T)ey[0].x[w] - (T)ex[0].x[w];
             R[PR].x[w] = -ux * d_x[PR] - uy * d_y[PR] - g * pr * divU;
         }
+        Pack<T, W> rpsi;
+        if (LAP && a.glm) {
+            // hyperbolic cleaning, Dedner et al. 2002 eq. 24 (calcRHS.c:144-158),
+            // with the Ch, Cp of this row's x block
+            const int blk = x_block_of((int)(p / (size_t)a.ny), a.nx, a.npe);
+            const double* const gc = a.glm_cp + 2 * ((size_t)z * a.npe + blk);
+            const T ch = (T)gc[0], cp = (T)gc[1];
+            const Pack<TD, W> px = ldp<W>(dxz + PSI_SLOT * n + p);
+            const Pack<TD, W> py = ldp<W>(dyz + PSI_SLOT * n + p);
+            const Pack<T, W> ps = ldp<W>(uz + PSI_SLOT * n + p);
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                const T divB = (T)dx[BX].x[w] + (T)dy[BY].x[w];
+                R[UX].x[w] = R[UX].x[w] - divB * u[BX].x[w];
+                R[UY].x[w] = R[UY].x[w] - divB * u[BY].x[w];
+                R[UZ].x[w] = R[UZ].x[w] - divB * u[BZ].x[w];
+                R[BX].x[w] = R[BX].x[w] - (T)px.x[w];
+                R[BY].x[w] = R[BY].x[w] - (T)py.x[w];
+                rpsi.x[w] = -ch * ch * (divB + ps.x[w] / cp / cp);
+            }
+        }
         if (LAP && a.lap) {
             // addSourceTerm (calcRHS.c:498-531): lap holds d2/dx2 + d2/dy2
             const TD* const lz = a.lap + z * a.bs_l;
@@ -1060,6 +1119,21 @@ __global__ void __launch_bounds__(256)
             }
             stp<MHD2D_NT_R, W>(uz + f * n + p, o);
             if (a.write_rprev) stp<MHD2D_NT_R, W>(rpz + f * n + p, rs_out);
+        }
+        if (LAP && a.glm) {
+            // psi: field 8 of R_prev, slot PSI_SLOT of the state
+            const Pack<T, W> ps = ldp<W>(uz + PSI_SLOT * n + p);
+            Pack<TD, W> rp;
+            if (use_prev) rp = ldp<W>(rpz + 8 * n + p);
+            Pack<T, W> o;
+            Pack<TD, W> rs_out;
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                o.x[w] = ps.x[w] + cc * rpsi.x[w] + dd * (use_prev ? (T)rp.x[w] : (T)0);
+                rs_out.x[w] = (TD)rpsi.x[w];
+            }
+            stp<MHD2D_NT_R, W>(uz + PSI_SLOT * n + p, o);
+            if (a.write_rprev) stp<MHD2D_NT_R, W>(rpz + 8 * n + p, rs_out);
         }
     }
 }
@@ -1178,6 +1252,17 @@ __global__ void __launch_bounds__(256) k_rhs_rk_open(RhsRkArgs<T, TD> a)
     R[BY] += c[4];
     R[BZ] += c[5];
     R[PR] += c[6];
+    T rpsi = 0;
+    if (a.glm) {
+        // hyperbolic cleaning at the open line: psi terms only (calcRHS.c:
+        // 241-249), with the Ch, Cp of the first / last x block
+        const double* const gc = a.glm_cp + 2 * ((size_t)z * a.npe + (boundary == 0 ? 0 : a.npe - 1));
+        const T ch = (T)gc[0], cp = (T)gc[1];
+        const T divB = d[BX] + e[BY];
+        R[BX] = R[BX] - (T)dxz[PSI_SLOT * n + p];
+        R[BY] = R[BY] - (T)dyz[PSI_SLOT * n + p];
+        rpsi = -ch * ch * (divB + uz[PSI_SLOT * n + p] / cp / cp);
+    }
     if (a.lap) {
         // addSourceTerm covers the boundary lines too (calcRHS.c:498-531)
         const TD* const lz = a.lap + z * a.bs_l;
@@ -1201,6 +1286,11 @@ __global__ void __launch_bounds__(256) k_rhs_rk_open(RhsRkArgs<T, TD> a)
         const T rp = use_prev ? (T)rpz[f * n + p] : (T)0;
         uz[f * n + p] = w[f] + cc * R[f] + dd * rp;
         if (a.write_rprev) rpz[f * n + p] = (TD)R[f];
+    }
+    if (a.glm) {
+        const T rp = use_prev ? (T)rpz[8 * n + p] : (T)0;
+        uz[PSI_SLOT * n + p] = uz[PSI_SLOT * n + p] + cc * rpsi + dd * rp;
+        if (a.write_rprev) rpz[8 * n + p] = (TD)rpsi;
     }
 }
 
@@ -1352,6 +1442,16 @@ hipError_t launch_divb(const T* dudx, const T* dudy, T* divB, size_t n, hipStrea
 }
 
 template <typename T>
+hipError_t launch_glm_blocks(const T* u, size_t bs_u, const DtParams& dp, const RunScalars* rs,
+                             double* out, int nx, int ny, int npe, int nb, hipStream_t s)
+{
+    if (nb < 1 || npe < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_glm_blocks<T>), dim3(nb), dim3(64), 0, s, u, bs_u, dp, rs, out, nx, ny,
+                       npe);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_divb_b(const T* dudx, const T* dudy, T* divB, size_t n, size_t bs_d,
                          size_t bs_o, int nb, hipStream_t s)
 {
@@ -1384,7 +1484,7 @@ hipError_t launch_rhs_rk(const RhsRkArgs<T, TD>& a, hipStream_t s, int nb)
     if (nb < 1 || nb > 65535) return hipErrorInvalidValue;
     dim3 grid = pw_grid(a.n / (16 / sizeof(T)), 256);
     grid.y = nb;
-    if (a.lap || a.ddivb)
+    if (a.lap || a.ddivb || a.glm)
         hipLaunchKernelGGL((k_rhs_rk<T, TD, true>), grid, dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL((k_rhs_rk<T, TD, false>), grid, dim3(256), 0, s, a);
@@ -1470,6 +1570,9 @@ hipError_t launch_diag(const T* u, const T* dbx, const T* dby, double* partial, 
     template hipError_t launch_divb<T>(const T*, const T*, T*, size_t, hipStream_t);         \
     template hipError_t launch_divb_b<T>(const T*, const T*, T*, size_t, size_t, size_t, int, \
                                          hipStream_t);                                       \
+    template hipError_t launch_glm_blocks<T>(const T*, size_t, const DtParams&,               \
+                                             const RunScalars*, double*, int, int, int, int,  \
+                                             hipStream_t);                                    \
     template hipError_t launch_rk<T>(T*, const T*, T*, double, double, size_t, hipStream_t); \
     template hipError_t launch_dt_limits<T>(const T*, const DtParams&, RunScalars*, int, int, \
                                             hipStream_t);                                    \

@@ -3,14 +3,19 @@
 from the y-sweep (filter x d2/dy2 of the spectrum) and an x-solve that adds
 d2/dx2; div B of the stage's derivatives gets its own x-solve and y
 transform (mhd.c:175-182); all on every cell including the open-boundary
-lines.
+lines.  The GLM cleaning (divBcleaning 2, calcRHS.c:144-158) adds psi as a
+ninth field with Ch / Cp per x block from the last cell of the block
+(timeAdvance.c:107-112); the open-boundary lines get the psi terms only.
 
 * FP64: per-field rel-L2 <= 1e-12 against the oracle every step, periodic
   and open x (npe 2 and 3), with viscosity only, resistivity only, both;
 * graph replay and batches equal eager single runs;
 * the mixed form within 1e-6 of FP64 after 40 steps;
 * the reference's own `input` (open x, resistivity 1e-4) runs in the fused
-  engine of the run loop and matches the oracle.
+  engine of the run loop and matches the oracle;
+* GLM: psi itself relatively within 1e-8 of the oracle (its conditioning
+  floor is ~1e-9: it is driven by div B, a cancellation of O(1)
+  derivatives), the other fields at 1e-12.
 """
 import numpy as np
 import pytest
@@ -44,7 +49,20 @@ CASES = [
     (64, 64, 1, False, 0.0, 0.0, 1, 1, 6),
     (128, 32, 1, False, 2e-3, 1e-3, 1, 0, 5),
     (64, 32, 2, True, 0.0, 1e-3, 1, 1, 6),
+    (64, 32, 1, False, 0.0, 0.0, 2, 1, 6),
+    (64, 64, 1, False, 2e-3, 1e-3, 2, 0, 6),
+    (128, 32, 3, False, 0.0, 0.0, 2, 1, 5),
+    (64, 32, 2, True, 0.0, 0.0, 2, 0, 6),
+    (128, 32, 3, True, 2e-3, 1e-3, 2, 1, 5),
 ]
+
+
+def _with_divb(u0, nx, amp=0.01):
+    # a div B part (dBx/dx != 0) so that psi is driven by more than round-off
+    u = u0.copy()
+    x = np.arange(nx) * 2 * np.pi / nx
+    u[4] += amp * np.sin(2 * x)[:, None]
+    return u
 
 
 @pytest.mark.parametrize("nx,ny,npe,open_x,nu,eta,clean,hall,steps", CASES)
@@ -54,36 +72,51 @@ def test_dissipative_matches_oracle(m, nx, ny, npe, open_x, nu, eta, clean, hall
     u0 = _state(m, nx, ny, 0.3 if open_x else 0.0)
     kw = dict(hall=hall, di=0.05, viscosity=nu, resistivity=eta, divbclean=clean)
     ctx = m.Context(nx, ny, npe=npe, open_x=open_x, **kw)
-    ctx.set_state(u0)
     S = orc.Sim(nx, ny, npe=npe, open_x=open_x, **kw)
+    if clean == 2:
+        u0 = np.concatenate([_with_divb(u0, nx), np.zeros((1, nx, ny))])
+    ctx.set_state(u0)
     S.set_state(u0)
     for k in range(steps):
         ctx.run(1)
         S.run(1)
-        assert_fields_close(ctx.state(), S.state(), what=f"step {k + 1}")
+        got, want = ctx.state(), S.state()
+        assert_fields_close(got, want, what=f"step {k + 1}")
+        if clean == 2:
+            psi_err = np.linalg.norm(got[8] - want[8]) / np.linalg.norm(want[8])
+            assert psi_err <= 1e-8, f"step {k + 1}: psi rel-L2 {psi_err}"
         assert abs(ctx.scalars()[1] - S.time) <= 1e-12 * S.time
 
 
-@pytest.mark.parametrize("kw", [dict(resistivity=5e-3, viscosity=5e-3), dict(divbclean=1)])
+@pytest.mark.parametrize("kw", [dict(resistivity=5e-3, viscosity=5e-3), dict(divbclean=1),
+                                dict(divbclean=2)])
 def test_dissipative_changes_the_state(m, kw):
     # the source terms are really applied: the run departs from the ideal one
     # by far more than round-off (div B of the seeded state is round-off
     # only, so cleaning gets a state with a div B part added)
     u0 = _state(m, 64, 64)
     if kw.get("divbclean"):
-        x = np.arange(64) * 2 * np.pi / 64
-        u0[4] += 0.01 * np.sin(2 * x)[:, None]
+        u0 = _with_divb(u0, 64)
     a = m.Context(64, 64, hall=1, di=0.05)
     b = m.Context(64, 64, hall=1, di=0.05, **kw)
+    a.set_state(u0)
+    b.set_state(np.concatenate([u0, np.zeros((1, 64, 64))]) if kw.get("divbclean") == 2
+                else u0)
     for c in (a, b):
-        c.set_state(u0)
         c.run(4)
-    assert field_errors(a.state(), b.state()).max() > 1e-6
+    assert field_errors(a.state(), b.state()[:8]).max() > 1e-6
 
 
-@pytest.mark.parametrize("clean", [0, 1])
+def _glm_state(u, clean):
+    if clean != 2:
+        return u
+    return np.concatenate([_with_divb(u, u.shape[1]), np.zeros((1,) + u.shape[1:])])
+
+
+@pytest.mark.parametrize("clean", [0, 1, 2])
 def test_dissipative_graph_and_batch(m, clean):
-    u0, u1 = _state(m, 64, 32, seed=4), _state(m, 64, 32, seed=5)
+    u0 = _glm_state(_state(m, 64, 32, seed=4), clean)
+    u1 = _glm_state(_state(m, 64, 32, seed=5), clean)
     kw = dict(hall=1, di=0.05, viscosity=2e-3, resistivity=1e-3, divbclean=clean)
     eager = m.Context(64, 32, **kw)
     eager.set_state(u0)
@@ -99,9 +132,9 @@ def test_dissipative_graph_and_batch(m, clean):
     np.testing.assert_array_equal(b.state(inst=1), eager.state())
 
 
-@pytest.mark.parametrize("clean", [0, 1])
+@pytest.mark.parametrize("clean", [0, 1, 2])
 def test_dissipative_mixed_precision(m, clean):
-    u0 = _state(m, 128, 64, seed=6)
+    u0 = _glm_state(_state(m, 128, 64, seed=6), clean)
     kw = dict(hall=1, di=0.05, viscosity=2e-3, resistivity=1e-3, divbclean=clean)
     ref = m.Context(128, 64, **kw)
     mix = m.Context(128, 64, precision=m.MIXED, **kw)
@@ -136,19 +169,23 @@ def test_reference_input_runs_fused(m, tmp_path):
 
 
 def test_dissipative_ensemble_matches_single_runs(m):
-    # an ensemble of two grid shapes with the source terms and the diffusive
-    # cleaning equals the same runs made one context at a time
-    kw = dict(hall=1, viscosity=2e-3, resistivity=1e-3, divbclean=1)
-    runs = [dict(nx=64, ny=32, di=0.02, **kw), dict(nx=128, ny=64, di=0.05, **kw),
-            dict(nx=64, ny=32, di=0.04, **kw)]
-    states = [_state(m, r["nx"], r["ny"], seed=20 + k) for k, r in enumerate(runs)]
+    # an ensemble of grid shapes with the source terms and the diffusive or
+    # GLM cleaning equals the same runs made one context at a time
+    kw = dict(hall=1, viscosity=2e-3, resistivity=1e-3)
+    runs = [dict(nx=64, ny=32, di=0.02, divbclean=1, **kw),
+            dict(nx=128, ny=64, di=0.05, divbclean=1, **kw),
+            dict(nx=64, ny=32, di=0.04, divbclean=1, **kw),
+            dict(nx=64, ny=32, di=0.03, divbclean=2, **kw)]
+    states = [_glm_state(_state(m, r["nx"], r["ny"], seed=20 + k), r["divbclean"])
+              for k, r in enumerate(runs)]
     ens = m.Ensemble(runs)
+    assert ens.groups == 3
     for k, u in enumerate(states):
         ens.set_state(k, u)
     ens.run(4, graph=True)
     for k, r in enumerate(runs):
         c = m.Context(r["nx"], r["ny"], hall=1, di=r["di"], viscosity=2e-3, resistivity=1e-3,
-                      divbclean=1)
+                      divbclean=r["divbclean"])
         c.set_state(states[k])
         c.run(4)
         np.testing.assert_array_equal(ens.state(k), c.state())

@@ -1,8 +1,8 @@
 """The reference's time-driven run loop (mhd.c:58-230) on the GPU against the
 same loop driven by the FP64 oracle: output times and files, step counts, the
-dt < 1e-8 stop, both engines (fused context, also with viscosity and
-resistivity; drop-in entry points with viscosity, resistivity and both div B
-cleanings) and the mhd2d_run executable."""
+dt < 1e-8 stop, both engines (fused context and drop-in entry points, each
+with viscosity, resistivity and both div B cleanings) and the mhd2d_run
+executable."""
 import array
 import os
 import subprocess
@@ -72,6 +72,7 @@ def _inp(m, **kw):
     ("fused", {}),
     ("fused", dict(viscosity=2e-3, resistivity=1e-3, hall=0)),
     ("fused", dict(divbclean=1)),
+    ("fused", dict(divbclean=2, viscosity=2e-3)),
     ("dropin", {}),
     ("dropin", dict(viscosity=2e-3, resistivity=1e-3, hall=0)),
     ("dropin", dict(divbclean=1)),
