@@ -38,21 +38,26 @@ HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
 B_ALG = {"fp64": 1680.0, "fp32": 840.0, "mixed": 3 * (32 * 8 + 38 * 4)}
 
 
-def kernel_alg_bytes(nx, ny, elem=8, delem=None, batch=1):
+def kernel_alg_bytes(nx, ny, elem=8, delem=None, batch=1, divb=False):
     """Algorithmic HBM bytes per launch of each kernel class (inputs read once,
     outputs written once), per cell times the cells of one launch.  elem:
-    bytes of a state (and E) value, delem: of a derivative / R_prev value."""
+    bytes of a state (and E) value, delem: of a derivative / R_prev value.
+    divb: dBx/dx and dBy/dy are formed (div B cleaning or an open x
+    boundary); the ideal periodic RHS reads neither."""
     n = nx * ny * batch
     s, d = elem, (elem if delem is None else delem)
+    k = 1 if divb else 0
     return {
-        # y-sweep: read u (8), write filtered u (8), Ey and Ez (2), dudy (8),
-        # dEx/dy and dEz/dy (2)
-        "ysweep": n * (18 * s + 10 * d),
-        # x-derivative: read u (8), Ey and Ez (2), write d/dx of all (10)
-        "xderiv": n * (10 * s + 10 * d),
+        # y-sweep: read u (8), write filtered u (8), Ey and Ez (2), dudy (7,
+        # all but dBy/dy, which the kernel stores all the same: it comes out
+        # of the (Bx, By) transform pair), dEx/dy and dEz/dy (2)
+        "ysweep": n * (18 * s + (9 + k) * d),
+        # x-derivative: read u (7, all but Bx), Ey and Ez (2), write their
+        # d/dx (9)
+        "xderiv": n * ((9 + k) * s + (9 + k) * d),
         # RHS + RK, average of the three stages: read u (8) and dudx, dudy
-        # (20), R_prev in stages 2-3 (8); write u (8), R_prev in stages 1-2 (8)
-        "rhs_rk": n * (16 * s + 20 * d + (16.0 / 3.0) * 2 * d),
+        # (18), R_prev in stages 2-3 (8); write u (8), R_prev in stages 1-2 (8)
+        "rhs_rk": n * (16 * s + (18 + 2 * k) * d + (16.0 / 3.0) * 2 * d),
         # x-filter in place: read 8, write 8
         "xfilter": n * 16 * s,
     }
@@ -91,6 +96,19 @@ def _pmc_rows(path):
     return out
 
 
+PMC_HEADER = """# HBM traffic per launch from rocprofv3 PMC counters, {nx}x{nx} FP64 turbulence
+# commands (GPU host, repo root $REPO; cd /tmp && export TMPDIR=/tmp first):
+#   rocprofv3 --pmc FETCH_SIZE --output-format csv -d $REPO/gpurun_out/pmc_fetch -o run -- python3 $REPO/bench.py --steps 2 --warmup 1 --no-cpu --no-forms --profile-steps 1 --graph 0
+#   rocprofv3 --pmc WRITE_SIZE --output-format csv -d $REPO/gpurun_out/pmc_write -o run -- python3 $REPO/bench.py --steps 2 --warmup 1 --no-cpu --no-forms --profile-steps 1 --graph 0
+#   rocprofv3 --pmc FETCH_SIZE --output-format csv -d $REPO/gpurun_out/pmc_copy_fetch -o run -- python3 $REPO/bench.py --copy-probe
+#   rocprofv3 --pmc WRITE_SIZE --output-format csv -d $REPO/gpurun_out/pmc_copy_write -o run -- python3 $REPO/bench.py --copy-probe
+#   python bench.py --summarize-pmc <the four run_counter_collection.csv>
+# calibration: the 1 GiB copy kernel's counters scale FETCH by {fs:.3f}, WRITE by {ws:.3f}
+# per launch: median over dispatches; the RHS + RK: mean over whole steps
+# (its R_prev traffic differs by stage)
+"""
+
+
 def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, nx, ny):
     """HBM bytes per launch of each kernel class from two --pmc passes
     (FETCH_SIZE, WRITE_SIZE in KiB per dispatch), scaled by what the same
@@ -103,8 +121,18 @@ def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, nx, ny):
                 continue
             acc.setdefault(name, []).append(val * 1024.0)
         # median over dispatches: drops the odd launches of the same kernel
-        # (first-stage y-sweep without filter, 1-field diagnostics solves)
-        return {k: float(np.median(v)) for k, v in acc.items()}
+        # (first-stage y-sweep without filter, 1-field diagnostics solves);
+        # the RHS + RK moves different bytes per stage (R_prev is not read in
+        # stage 1 nor written in stage 3), so it takes the mean over its
+        # launches, whole steps of three each
+        out = {}
+        for k, v in acc.items():
+            if "k_rhs_rk" in k:
+                v = v[len(v) % 3:]
+                out[k] = float(np.mean(v))
+            else:
+                out[k] = float(np.median(v))
+        return out
 
     cf = per_kernel(copy_fetch_csv, "FETCH_SIZE")
     cw = per_kernel(copy_write_csv, "WRITE_SIZE")
@@ -118,6 +146,7 @@ def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, nx, ny):
                "xfilter": "k_xsolve"}
     alg = kernel_alg_bytes(nx, ny)
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    rows = []
     for cls, key in classes.items():
         names = [k for k in fetch if key in k]
         # k_xsolve<T, TI, C, W, KIND, BLK>: KIND 0 derivative, 2 filter
@@ -138,6 +167,12 @@ def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, nx, ny):
         with open(os.path.join(ROOT, "profiles", f"traffic_{cls}_{nx}.json"), "w") as f:
             json.dump(rec, f, indent=1)
         print(json.dumps(rec))
+        rows.append(f"{cls} | {rd:.3e} | {wr:.3e} | {rd + wr:.3e} | {alg[cls]:.3e} | "
+                    f"{(rd + wr) / alg[cls]:.3f}")
+    with open(os.path.join(ROOT, "profiles", f"traffic_fp64_{nx}.txt"), "w") as f:
+        f.write(PMC_HEADER.format(nx=nx, fs=fscale, ws=wscale))
+        f.write("# kernel | fetch B/launch | write B/launch | total | algorithmic | total/alg\n")
+        f.write("\n".join(rows) + "\n")
 
 
 def load_traffic(kernel):

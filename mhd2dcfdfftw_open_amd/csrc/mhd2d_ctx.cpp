@@ -85,6 +85,11 @@ struct mhd2d_ctx {
     // per-instance strides (elements); with GLM cleaning psi takes slot
     // PSI_SLOT of u and the diagnostics scratch moves to slot 11
     bool glm() const { return p.divbclean == 2; }
+    // dBx/dx of a stage is read only by the cleaning terms and the
+    // open-boundary lines; the ideal periodic RHS does without it (dBy/dy
+    // comes out of the (Bx, By) transform pair and is stored regardless:
+    // skipping the store cost the y-sweep registers and time)
+    bool divb_derivs() const { return p.divbclean != 0 || p.open_x; }
     size_t su() const { return (glm() ? 12 : 11) * n; }
     int nv_state() const { return glm() ? 9 : 8; }
     size_t sd() const { return 11 * n; }
@@ -141,17 +146,18 @@ hipError_t ysweep(mhd2d_ctx* c, int filter, int derivs, int dt_limits)
 }
 
 // x-derivative of nv fields: in (state type, stride bs_in per instance) ->
-// out (derivative type); the tridiagonal solve or the MFMA GEMM form
+// out (derivative type); the tridiagonal solve or the MFMA GEMM form.  Field
+// v is slot fl[v] of in and out (slot v without a list).
 template <typename T, typename TD>
 hipError_t xderiv_fields(mhd2d_ctx* c, const T* in, TD* out, int nv, size_t bs_in,
-                         size_t bs_out, int nb, const RunScalars* rs)
+                         size_t bs_out, int nb, const RunScalars* rs, const int* fl = nullptr)
 {
     if constexpr (std::is_same<TD, float>::value) {
         if (c->p.xgemm) {
             XGemmArgs<T> g;
             for (int v = 0; v < nv; v++) {
-                g.in[v] = in + v * c->n;
-                g.out[v] = out + v * c->n;
+                g.in[v] = in + (fl ? fl[v] : v) * c->n;
+                g.out[v] = out + (fl ? fl[v] : v) * c->n;
             }
             g.mh = c->xg.mh;
             g.mm = c->xg.mm;
@@ -170,8 +176,8 @@ hipError_t xderiv_fields(mhd2d_ctx* c, const T* in, TD* out, int nv, size_t bs_i
     }
     XSolveArgs<TD, T> a;
     for (int v = 0; v < nv; v++) {
-        a.in[v] = in + v * c->n;
-        a.out[v] = out + v * c->n;
+        a.in[v] = in + (fl ? fl[v] : v) * c->n;
+        a.out[v] = out + (fl ? fl[v] : v) * c->n;
     }
     a.rs = rs;
     a.st = c->tb.st[XD1];
@@ -189,10 +195,15 @@ hipError_t xderiv_fields(mhd2d_ctx* c, const T* in, TD* out, int nv, size_t bs_i
 template <typename T, typename TD>
 hipError_t xderiv(mhd2d_ctx* c)
 {
-    // 8 state fields, Ey and Ez, solved in the derivative type
-    // (and psi in slot PSI_SLOT = 10 with GLM cleaning)
-    hipError_t e = xderiv_fields<T, TD>(c, (const T*)c->u, (TD*)c->dudx, c->glm() ? 11 : 10,
-                                        c->su(), c->sd(), c->B, c->rs);
+    // 8 state fields, Ey and Ez, solved in the derivative type (and psi in
+    // slot PSI_SLOT = 10 with GLM cleaning); Bx only where dBx/dx is read
+    static const int ideal[9] = {RHO, UX, UY, UZ, BY, BZ, PR, 8, 9};
+    hipError_t e =
+        c->divb_derivs()
+            ? xderiv_fields<T, TD>(c, (const T*)c->u, (TD*)c->dudx, c->glm() ? 11 : 10, c->su(),
+                                   c->sd(), c->B, c->rs)
+            : xderiv_fields<T, TD>(c, (const T*)c->u, (TD*)c->dudx, 9, c->su(), c->sd(), c->B,
+                                   c->rs, ideal);
     if (e != hipSuccess || !c->lap) return e;
     // second x-derivatives of ux..uz (viscosity) and / or Bx..Bz
     // (resistivity), added onto the second y-derivatives in lap
@@ -978,8 +989,9 @@ hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
         c->xr_d = k.r;
         float t = 0;
         e = time_launches(c->stream, reps, [&]() {
-            return xderiv_fields<T, TD>(c, (const T*)c->u, (TD*)c->dudx, 10, c->su(), c->sd(),
-                                        c->B, nullptr);
+            return xderiv_fields<T, TD>(c, (const T*)c->u, (TD*)c->dudx,
+                                        c->divb_derivs() ? 10 : 9, c->su(), c->sd(), c->B,
+                                        nullptr);
         }, &t);
         if (e == hipSuccess) td[ci] = std::min(td[ci], t);
         XSolveArgs<T> a;

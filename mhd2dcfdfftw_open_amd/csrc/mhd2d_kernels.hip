@@ -958,6 +958,31 @@ __device__ __forceinline__ Pack<T, W> ldp(const T* __restrict__ p)
     return *reinterpret_cast<const Pack<T, W>*>(p);
 }
 
+// streaming inputs of the RHS + RK (derivatives and R_prev, read once per
+// stage) as nontemporal loads (MHD2D_NT_LOAD): measured -15 % RHS + RK time
+// at 2048^2 FP64 (it then streams at the copy kernel's rate), -13 % mixed,
+// -7 % at 4096^2.  The same for the x-solves' rows and the y-sweep's field
+// pairs measured no gain, and for the state itself a loss (the x-filter that
+// follows reads it back 12 % slower).
+#ifndef MHD2D_NT_LOAD
+#define MHD2D_NT_LOAD 1
+#endif
+
+template <int W, typename T>
+__device__ __forceinline__ Pack<T, W> ldn(const T* __restrict__ p)
+{
+    if constexpr (MHD2D_NT_LOAD) {
+        typedef T V __attribute__((ext_vector_type(W)));
+        const V x = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+        Pack<T, W> r;
+#pragma unroll
+        for (int w = 0; w < W; w++) r.x[w] = x[w];
+        return r;
+    } else {
+        return ldp<W>(p);
+    }
+}
+
 template <bool NT, int W, typename T>
 __device__ __forceinline__ void stp(T* p, const Pack<T, W>& v)
 {
@@ -1015,13 +1040,13 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int f = 0; f < 8; f++) {
             u[f] = ldp<W>(uz + f * n + p);
-            dx[f] = ldp<W>(dxz + f * n + p);
-            dy[f] = ldp<W>(dyz + f * n + p);
+            dx[f] = ldn<W>(dxz + f * n + p);
+            dy[f] = ldn<W>(dyz + f * n + p);
         }
 #pragma unroll
         for (int f = 0; f < 2; f++) {
-            ex[f] = ldp<W>(exz + f * n + p);
-            ey[f] = ldp<W>(eyz + f * n + p);
+            ex[f] = ldn<W>(exz + f * n + p);
+            ey[f] = ldn<W>(eyz + f * n + p);
         }
         Pack<T, W> R[8];
 #pragma unroll
@@ -1109,7 +1134,7 @@ __global__ void __launch_bounds__(256)
         for (int f = 0; f < 8; f++) {
             // rkExecute, timeAdvance.c:31-35 (R_prev is zero in stage 0)
             Pack<TD, W> rp;
-            if (use_prev) rp = ldp<W>(rpz + f * n + p);
+            if (use_prev) rp = ldn<W>(rpz + f * n + p);
             Pack<T, W> o;
             Pack<TD, W> rs_out;
 #pragma unroll
