@@ -97,6 +97,9 @@ __device__ __forceinline__ void block_min_dt(double v, RunScalars* rs, double* r
 // (MHD2D_NT_X / _Y / _R: the x-solves, the y-sweep, the RHS + RK).  They do
 // not leave dirty lines in L2 for the next kernel to write back: measured
 // per step -2.8 % (2048^2 FP64), -5.6 % (mixed), -4.3 % (4096^2 FP64).
+// The exception is the y-sweep's filtered state (MHD2D_NT_YS): the same
+// kernel reads six of its fields back to form E, and ordinary stores let
+// part of that come from cache (-3.3 % y-sweep time at 2048^2 FP64).
 #ifndef MHD2D_NT_X
 #define MHD2D_NT_X 1
 #endif
@@ -105,6 +108,9 @@ __device__ __forceinline__ void block_min_dt(double v, RunScalars* rs, double* r
 #endif
 #ifndef MHD2D_NT_R
 #define MHD2D_NT_R 1
+#endif
+#ifndef MHD2D_NT_YS
+#define MHD2D_NT_YS 0
 #endif
 template <bool NT, typename T>
 __device__ __forceinline__ void st_stream(T* p, T v)
@@ -578,12 +584,13 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
     TD* const dEdy = a.dEdy + z * a.bs_d;
 
     // field pairs packed as re/im of one complex line; the velocity and field
-    // components sit in pairs 0-2 so that the second derivatives of the
-    // source terms take at most three more inverse transforms; the GLM psi
+    // components share pairs so that the second derivatives of the source
+    // terms take at most three more inverse transforms, and they come last so
+    // that E finds them as recent as possible in cache; the GLM psi
     // (slot PSI_SLOT) is a fifth line of its own, in the LAP instance only,
     // so the ideal instance keeps four full pairs and its register budget
-    constexpr int PA[5] = {UX, UZ, BX, RHO, PSI_SLOT};
-    constexpr int PB[5] = {UY, BZ, BY, PR, -1};
+    constexpr int PA[5] = {RHO, UX, UZ, BX, PSI_SLOT};
+    constexpr int PB[5] = {PR, UY, BZ, BY, -1};
     const int npairs = (LAP && a.glm) ? 5 : 4;
     if (a.filter || a.derivs) {
         // The next pair is loaded while the last transform of the current
@@ -628,8 +635,8 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
 #pragma unroll
                     for (int m = 0; m < E; m++) {
                         const int k = t + m * TPR;
-                        st_stream<MHD2D_NT_Y>(ua + k, v[m].re);
-                        if (two) st_stream<MHD2D_NT_Y>(ub + k, v[m].im);
+                        st_stream<MHD2D_NT_YS>(ua + k, v[m].re);
+                        if (two) st_stream<MHD2D_NT_YS>(ub + k, v[m].im);
                     }
                 }
                 if (!LAP && a.derivs) {
