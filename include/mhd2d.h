@@ -199,14 +199,15 @@ int mhd2d_xderiv(mhd2d_ctx *c, int nv, const double *in, double *out);
  * all 64-row tiles (the MFMA work is panels * 64 * 16 * 2 * ny flops per
  * field and split product); 0 for the tridiagonal form */
 long long mhd2d_xgemm_panels(const mhd2d_ctx *c);
-/* Autotuner (no counterpart in the reference; the north_star's "tile and
- * line-per-workgroup shapes chosen by an autotuner"): times every valid
- * x-solve launch shape - strip width of 8 or 16 y-lines per workgroup and
- * the XCD-contiguous workgroup order - on this context's grid, precision and
- * batch (reps timed launches each; the state is untouched) and keeps the
- * fastest for the derivative and the filter solves.  choice (may be NULL)
- * receives {width_d, remap_d, width_f, remap_f}.  Results do not depend on
- * the shape (each y-line is solved the same way).  Tridiagonal form only. */
+/* Autotuner (no counterpart in the reference): times every valid x-solve
+ * launch shape - strip width of 8 or 16 y-lines per workgroup and the
+ * XCD-contiguous workgroup order - for the derivative solves, then for the
+ * filter, as reps whole RK3 steps of this context's own run, each from the
+ * same state, and keeps the fastest.  Needs a state for every instance; the
+ * state, R_prev, time, dt and step count are restored afterwards.  choice
+ * (may be NULL) receives {width_d, remap_d, width_f, remap_f}.  Results do
+ * not depend on the shape (each y-line is solved the same way).
+ * Tridiagonal form only. */
 int mhd2d_autotune(mhd2d_ctx *c, int reps, int *choice);
 /* wall time of nsteps on the context's stream (hipEvents), in ms */
 int mhd2d_time_run(mhd2d_ctx *c, int nsteps, int use_graph, double *ms);

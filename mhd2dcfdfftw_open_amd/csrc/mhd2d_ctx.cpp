@@ -954,13 +954,12 @@ hipError_t time_launches(hipStream_t s, int reps, F f, float* ms)
 }
 
 // Autotuner of the x-solve launch shapes: every valid (strip width, XCD
-// order) is timed on this context's own grid, precision and batch, for the
-// derivative solve (all fields of a stage, into the derivative buffer, which
-// holds no live data between steps) and for the filter (in place on the
-// state itself, which is saved before and restored after, so the timed
-// launches see the real layout); the candidates are timed in two
-// interleaved rounds, each keeps its faster mean, and the fastest candidate
-// is kept.
+// order) of the derivative solves, then of the filter, is timed as whole RK3
+// steps of this context's own run (grid, precision, batch, settings), each
+// from the same saved state (state, R_prev, run scalars, pending filter),
+// which is restored afterwards.  A shape is judged where it runs: alone, the
+// kernels' own timings ranked the XCD orders differently from the step.  Two
+// interleaved rounds, each candidate keeps its faster mean.
 template <typename T, typename TD>
 hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
 {
@@ -976,78 +975,87 @@ hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
             cands.push_back({w, r});
         }
     hipError_t e = hipStreamSynchronize(c->stream);
-    const size_t ubytes = c->su() * sizeof(T) * c->B;
-    T* saved = nullptr;
-    if (e == hipSuccess) e = hipMalloc(&saved, ubytes);
-    if (e == hipSuccess) e = hipMemcpyAsync(saved, c->u, ubytes, hipMemcpyDeviceToDevice, c->stream);
-    std::vector<float> td(cands.size(), 1e30f), tf(cands.size(), 1e30f);
-    for (size_t it = 0; it < 2 * cands.size(); it++) {  // two rounds over the candidates
-        const size_t ci = it % cands.size();
-        const Cand& k = cands[ci];
-        if (e != hipSuccess) break;
-        c->xw_d = k.w;
-        c->xr_d = k.r;
-        float t = 0;
-        e = time_launches(c->stream, reps, [&]() {
-            return xderiv_fields<T, TD>(c, (const T*)c->u, (TD*)c->dudx,
-                                        c->divb_derivs() ? 10 : 9, c->su(), c->sd(), c->B,
-                                        nullptr);
-        }, &t);
-        if (e == hipSuccess) td[ci] = std::min(td[ci], t);
-        XSolveArgs<T> a;
-        for (int v = 0; v < 8; v++) a.in[v] = a.out[v] = (T*)c->u + v * c->n;
-        a.st = c->tb.st[XFILT];
-        a.tb = c->tb.xt[XFILT];
-        a.kind = XFILT;
-        a.nx = c->p.nx;
-        a.ny = c->p.ny;
-        a.bs_in = a.bs_out = c->su();
-        a.width = k.w;
-        a.remap = k.r;
-        if (e == hipSuccess)
-            e = time_launches(c->stream, reps,
-                              [&]() { return launch_xsolve<T>(a, 8, c->stream, c->B); }, &t);
-        if (e == hipSuccess) tf[ci] = std::min(tf[ci], t);
-    }
-    int wd = 0, rd = -1, wf = 0, rf = -1;
-    float best_d = 1e30f, best_f = 1e30f;
-    for (size_t ci = 0; ci < cands.size(); ci++) {
-        if (td[ci] < best_d) {
-            best_d = td[ci];
-            wd = cands[ci].w;
-            rd = cands[ci].r;
+    const size_t ub = c->su() * sizeof(T) * c->B, rb = c->sr() * sizeof(TD) * c->B;
+    const size_t sb = sizeof(RunScalars) * c->B;
+    const bool pf = c->pending_filter;
+    char* saved = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&saved, ub + rb + sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(saved, c->u, ub, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(saved + ub, c->rprev, rb, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(saved + ub + rb, c->rs, sb, hipMemcpyDeviceToDevice, c->stream);
+    auto restore = [&]() {
+        hipError_t r = hipMemcpyAsync(c->u, saved, ub, hipMemcpyDeviceToDevice, c->stream);
+        if (r == hipSuccess)
+            r = hipMemcpyAsync(c->rprev, saved + ub, rb, hipMemcpyDeviceToDevice, c->stream);
+        if (r == hipSuccess)
+            r = hipMemcpyAsync(c->rs, saved + ub + rb, sb, hipMemcpyDeviceToDevice, c->stream);
+        c->pending_filter = pf;
+        return r;
+    };
+    // ms per step of `reps` steps after one untimed step, from the saved state
+    auto time_steps = [&](float* ms) {
+        hipError_t r = restore();
+        if (r == hipSuccess)
+            r = time_launches(c->stream, reps, [&]() { return step<T, TD>(c, nullptr, nullptr); },
+                              ms);
+        return r;
+    };
+    const int xw_f0 = c->xw_f, xr_f0 = c->xr_f;
+    int pick[2][2] = {{0, -1}, {0, -1}};  // [deriv, filter][width, order]
+    for (int phase = 0; phase < 2 && e == hipSuccess; phase++) {
+        std::vector<float> tm(cands.size(), 1e30f);
+        for (size_t it = 0; it < 2 * cands.size() && e == hipSuccess; it++) {
+            const size_t ci = it % cands.size();
+            if (phase == 0) {
+                c->xw_d = cands[ci].w;
+                c->xr_d = cands[ci].r;
+                c->xw_f = xw_f0;
+                c->xr_f = xr_f0;
+            } else {
+                c->xw_d = pick[0][0];
+                c->xr_d = pick[0][1];
+                c->xw_f = cands[ci].w;
+                c->xr_f = cands[ci].r;
+            }
+            float t = 0;
+            e = time_steps(&t);
+            if (e == hipSuccess) tm[ci] = std::min(tm[ci], t);
         }
-        if (tf[ci] < best_f) {
-            best_f = tf[ci];
-            wf = cands[ci].w;
-            rf = cands[ci].r;
-        }
+        float best = 1e30f;
+        for (size_t ci = 0; ci < cands.size(); ci++)
+            if (tm[ci] < best) {
+                best = tm[ci];
+                pick[phase][0] = cands[ci].w;
+                pick[phase][1] = cands[ci].r;
+            }
     }
     if (saved) {
-        const hipError_t e2 =
-            hipMemcpyAsync(c->u, saved, ubytes, hipMemcpyDeviceToDevice, c->stream);
+        const hipError_t e2 = restore();
         const hipError_t e3 = hipStreamSynchronize(c->stream);
         (void)hipFree(saved);
         if (e == hipSuccess) e = e2 != hipSuccess ? e2 : e3;
     }
+    c->pending_filter = pf;
     if (e != hipSuccess) {
-        c->xw_d = 0;
-        c->xr_d = -1;
+        c->xw_d = c->xw_f = 0;
+        c->xr_d = c->xr_f = -1;
         return e;
     }
-    c->xw_d = wd;
-    c->xr_d = rd;
-    c->xw_f = wf;
-    c->xr_f = rf;
+    c->xw_d = pick[0][0];
+    c->xr_d = pick[0][1];
+    c->xw_f = pick[1][0];
+    c->xr_f = pick[1][1];
     if (c->graph) {  // the captured step holds the old launch shapes
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
     }
     if (out) {
-        out[0] = wd;
-        out[1] = rd;
-        out[2] = wf;
-        out[3] = rf;
+        out[0] = c->xw_d;
+        out[1] = c->xr_d;
+        out[2] = c->xw_f;
+        out[3] = c->xr_f;
     }
     return hipSuccess;
 }
@@ -1059,6 +1067,7 @@ int mhd2d_autotune(mhd2d_ctx* c, int reps, int* choice)
 {
     if (!c || reps < 1) return fail(MHD2D_EINVAL, "bad argument");
     if (c->p.xgemm) return fail(MHD2D_EINVAL, "the GEMM x-derivative has no shapes to tune");
+    if (int rc = check_ready(c)) return rc;  // the candidates are timed as steps of the run
     hipError_t e;
     if (c->elem == 4)
         e = autotune_impl<float, float>(c, reps, choice);
