@@ -318,8 +318,9 @@ def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50,
             rec["rel_l2_vs_fp64_max"] = max(errs)
             rec["meets_1e-6"] = bool(max(errs) <= 1e-6)
         if xg:
-            # MFMA work of one x-derivative launch (10 fields) over its time
-            flops = ctx.xgemm_panels() * 64 * 16 * 2 * n * 10 * xg
+            # MFMA work of one x-derivative launch (9 fields: the ideal
+            # periodic RHS needs no dBx/dx) over its time
+            flops = ctx.xgemm_panels() * 64 * 16 * 2 * n * 9 * xg
             t = kern["xderiv"]["avg_ms"] * 1e-3
             rec["xderiv_mfma"] = {"bf16_flops_per_launch": flops, "TFLOP/s": flops / t / 1e12,
                                   "frac_of_2500_TFLOPs": flops / t / 2.5e15,

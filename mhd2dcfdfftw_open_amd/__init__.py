@@ -278,8 +278,9 @@ class Context:
         return lib().mhd2d_xgemm_panels(self.h)
 
     def autotune(self, reps=5):
-        """time every x-solve launch shape on this grid and keep the fastest;
-        returns {'deriv': (width, xcd_order), 'filter': (width, xcd_order)}"""
+        """time every x-solve launch shape as whole steps of this run (after
+        set_state; the run is restored) and keep the fastest; returns
+        {'deriv': (width, xcd_order), 'filter': (width, xcd_order)}"""
         c = (ctypes.c_int * 4)()
         _check(lib().mhd2d_autotune(self.h, reps, c))
         return {"deriv": (c[0], c[1]), "filter": (c[2], c[3])}
