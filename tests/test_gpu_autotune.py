@@ -1,7 +1,8 @@
 """The x-solve shape autotuner (mhd2d_autotune / mhd2d_ens_autotune): it
-returns valid shapes, leaves the state untouched, and a tuned run equals an
-untuned one bit for bit (every y-line is solved the same way whatever the
-strip width or workgroup order), eagerly and through the captured graph."""
+times whole steps from the run's state, returns valid shapes, restores the
+run (state, R_prev, time, dt, steps), and a tuned run equals an untuned one
+bit for bit (every y-line is solved the same way whatever the strip width or
+workgroup order), eagerly and through the captured graph."""
 import numpy as np
 import pytest
 
@@ -15,28 +16,45 @@ def m():
     return m
 
 
-@pytest.mark.parametrize("nx,ny,prec,batch,npe", [
-    (256, 128, "fp64", 1, 1),
-    (512, 256, "mixed", 3, 1),
-    (128, 64, "fp32", 2, 1),
-    (128, 64, "fp64", 1, 2),
+@pytest.mark.parametrize("nx,ny,prec,batch,npe,kw", [
+    (256, 128, "fp64", 1, 1, {}),
+    (512, 256, "mixed", 3, 1, {}),
+    (128, 64, "fp32", 2, 1, {}),
+    (128, 64, "fp64", 1, 2, {}),
+    # GLM psi (a ninth state field and R_prev field), Laplacians, open ends
+    (128, 64, "fp64", 2, 2, dict(divbclean=2, viscosity=2e-3, open_x=True)),
 ])
-def test_autotune_keeps_results(m, nx, ny, prec, batch, npe):
-    print(f"[autotune] {nx}x{ny} {prec} batch={batch} npe={npe}", flush=True)
+def test_autotune_keeps_results(m, nx, ny, prec, batch, npe, kw):
+    print(f"[autotune] {nx}x{ny} {prec} batch={batch} npe={npe} {kw}", flush=True)
     p = {"fp64": m.FP64, "mixed": m.MIXED, "fp32": m.FP32}[prec]
     u = [m.turbulence(nx, ny, seed=10 + b, kmax=8) for b in range(batch)]
-    ref = m.Context(nx, ny, hall=1, di=0.03, precision=p, batch=batch, npe=npe)
-    tun = m.Context(nx, ny, hall=1, di=0.03, precision=p, batch=batch, npe=npe)
+    if kw.get("open_x"):
+        for x in u:
+            x[1] += 0.3
+    if kw.get("divbclean") == 2:
+        xs = np.arange(nx) * 2 * np.pi / nx
+        u = [np.concatenate([x, 1e-3 * np.sin(xs)[None, :, None] + np.zeros((1, nx, ny))])
+             for x in u]
+    ref = m.Context(nx, ny, hall=1, di=0.03, precision=p, batch=batch, npe=npe, **kw)
+    tun = m.Context(nx, ny, hall=1, di=0.03, precision=p, batch=batch, npe=npe, **kw)
     for c in (ref, tun):
         for b in range(batch):
             c.set_state(u[b], inst=b)
+    # a step first, so that R_prev and the time are live (reading the state
+    # completes the pending y filter, on both contexts alike)
+    for c in (ref, tun):
+        c.run(1)
     before = [tun.state(inst=b) for b in range(batch)]
+    for b in range(batch):
+        np.testing.assert_array_equal(ref.state(inst=b), before[b])
+    sc = [tun.scalars(inst=b) for b in range(batch)]
     ch = tun.autotune(reps=2)
     for k in ("deriv", "filter"):
         w, r = ch[k]
         assert w in (8, 16) and r in (0, 1), ch
     for b in range(batch):
         np.testing.assert_array_equal(tun.state(inst=b), before[b])
+        assert tun.scalars(inst=b) == sc[b]
     ref.run(4)
     tun.run(4)
     for b in range(batch):
@@ -73,5 +91,11 @@ def test_ensemble_autotune(m):
 
 def test_autotune_rejects_gemm_form(m):
     c = m.Context(128, 64, precision=m.MIXED, xgemm=6)
+    with pytest.raises(Exception):
+        c.autotune()
+
+
+def test_autotune_needs_a_state(m):
+    c = m.Context(128, 64)
     with pytest.raises(Exception):
         c.autotune()
