@@ -558,9 +558,12 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) k_yapply(YApplyArgs<T
 // Fused y-sweep of one RK stage (driver).  Each thread owns the same 16
 // positions of its row in physical and spectral space, so the filtered state
 // it writes is re-read by the same thread only (E and the CFL limits).
-template <typename T, typename TD, int N, bool LAP>
+// MODE 0: ideal; 1: with the second derivatives of the source terms; 2: also
+// the GLM psi line (separate instances, so each keeps its own registers)
+template <typename T, typename TD, int N, int MODE>
 __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep(YSweepArgs<T, TD> a)
 {
+    constexpr bool LAP = MODE >= 1, GLM = MODE == 2;
     using Y = YShape<T, N>;
     constexpr int TPR = Y::TPR;
     constexpr int E = Y::E;
@@ -587,11 +590,11 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
     // components share pairs so that the second derivatives of the source
     // terms take at most three more inverse transforms, and they come last so
     // that E finds them as recent as possible in cache; the GLM psi
-    // (slot PSI_SLOT) is a fifth line of its own, in the LAP instance only,
-    // so the ideal instance keeps four full pairs and its register budget
+    // (slot PSI_SLOT) is a fifth line of its own, in the GLM instance only,
+    // so the others keep four full pairs and their register budgets
     constexpr int PA[5] = {RHO, UX, UZ, BX, PSI_SLOT};
     constexpr int PB[5] = {PR, UY, BZ, BY, -1};
-    const int npairs = (LAP && a.glm) ? 5 : 4;
+    constexpr int npairs = GLM ? 5 : 4;
     if (a.filter || a.derivs) {
         // The next pair is loaded while the last transform of the current
         // pair runs (its registers replace the saved spectrum Z), so the HBM
@@ -606,7 +609,7 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
 #pragma unroll 1
         for (int p = 0; p < npairs; p++) {
             const int fa = PA[p], fb = PB[p];
-            const bool two = !LAP || fb >= 0;
+            const bool two = !GLM || fb >= 0;
             T* ua = u + fa * n + row;
             T* ub = u + (two ? fb : fa) * n + row;
             // second y-derivatives wanted from this pair (re: fa, im: fb)
@@ -672,7 +675,7 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
             }
             if (p + 1 < npairs) {
                 const T* na = u + PA[p + 1] * n + row;
-                const bool two_n = !LAP || PB[p + 1] >= 0;
+                const bool two_n = !GLM || PB[p + 1] >= 0;
                 const T* nb = u + (two_n ? PB[p + 1] : PA[p + 1]) * n + row;
 #pragma unroll
                 for (int m = 0; m < E; m++) {
@@ -760,10 +763,12 @@ static hipError_t launch_y_n(const YApplyArgs<T>* ya, const YSweepArgs<T, TD>* y
             return hipGetLastError();
         }
     }
-    if (ys->lap || ys->glm)
-        hipLaunchKernelGGL((k_ysweep<T, TD, N, true>), grid, block, 0, s, *ys);
+    if (ys->glm)
+        hipLaunchKernelGGL((k_ysweep<T, TD, N, 2>), grid, block, 0, s, *ys);
+    else if (ys->lap)
+        hipLaunchKernelGGL((k_ysweep<T, TD, N, 1>), grid, block, 0, s, *ys);
     else
-        hipLaunchKernelGGL((k_ysweep<T, TD, N, false>), grid, block, 0, s, *ys);
+        hipLaunchKernelGGL((k_ysweep<T, TD, N, 0>), grid, block, 0, s, *ys);
     return hipGetLastError();
 }
 
@@ -1007,9 +1012,10 @@ __device__ __forceinline__ void stp(T* p, const Pack<T, W>& v)
 // fused RHS + RK update; each thread handles W = 16 / sizeof(T) consecutive
 // cells (one 16-byte vector of each state field) per field (n is a multiple
 // of W: nx, ny are powers of two).  Derivatives and R_prev are read and
-// written as TD, the arithmetic and the state update run in T.  LAP adds the
-// viscous and resistive source terms (a separate instance, so the ideal
-// kernel keeps its register allocation).  The FP64 instance is held at 2
+// written as TD, the arithmetic and the state update run in T.  MODE 1 adds
+// the viscous and resistive source terms and the diffusive cleaning, MODE 2
+// also the GLM terms (separate instances, so each keeps its own register
+// allocation).  The FP64 instance is held at 2
 // waves per SIMD (measured -2.5 % against the 3 its registers allow; the
 // mixed instance measured +7 % at 2, so the others keep the compiler's
 // choice).
@@ -1017,10 +1023,11 @@ template <typename T, typename TD>
 struct RhsRkWaves {
     static constexpr int MAX = (sizeof(T) == 8 && sizeof(TD) == 8) ? 2 : 10;
 };
-template <typename T, typename TD, bool LAP>
+template <typename T, typename TD, int MODE>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(1, RhsRkWaves<T, TD>::MAX))) k_rhs_rk(RhsRkArgs<T, TD> a)
 {
+    constexpr bool LAP = MODE >= 1, GLM = MODE == 2;
     constexpr int W = 16 / sizeof(T);
     const int z = blockIdx.y;  // instance of a batch
     const RunScalars* const rs = a.rs + z;
@@ -1083,7 +1090,7 @@ __global__ void __launch_bounds__(256)
             R[PR].x[w] = -ux * d_x[PR] - uy * d_y[PR] - g * pr * divU;
         }
         Pack<T, W> rpsi;
-        if (LAP && a.glm) {
+        if (GLM) {
             // hyperbolic cleaning, Dedner et al. 2002 eq. 24 (calcRHS.c:144-158),
             // with the Ch, Cp of this row's x block
             const int blk = x_block_of((int)(p / (size_t)a.ny), a.nx, a.npe);
@@ -1152,7 +1159,7 @@ __global__ void __launch_bounds__(256)
             stp<MHD2D_NT_R, W>(uz + f * n + p, o);
             if (a.write_rprev) stp<MHD2D_NT_R, W>(rpz + f * n + p, rs_out);
         }
-        if (LAP && a.glm) {
+        if (GLM) {
             // psi: field 8 of R_prev, slot PSI_SLOT of the state
             const Pack<T, W> ps = ldp<W>(uz + PSI_SLOT * n + p);
             Pack<TD, W> rp;
@@ -1516,10 +1523,12 @@ hipError_t launch_rhs_rk(const RhsRkArgs<T, TD>& a, hipStream_t s, int nb)
     if (nb < 1 || nb > 65535) return hipErrorInvalidValue;
     dim3 grid = pw_grid(a.n / (16 / sizeof(T)), 256);
     grid.y = nb;
-    if (a.lap || a.ddivb || a.glm)
-        hipLaunchKernelGGL((k_rhs_rk<T, TD, true>), grid, dim3(256), 0, s, a);
+    if (a.glm)
+        hipLaunchKernelGGL((k_rhs_rk<T, TD, 2>), grid, dim3(256), 0, s, a);
+    else if (a.lap || a.ddivb)
+        hipLaunchKernelGGL((k_rhs_rk<T, TD, 1>), grid, dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL((k_rhs_rk<T, TD, false>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_rhs_rk<T, TD, 0>), grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
