@@ -72,10 +72,14 @@ __device__ __forceinline__ double cell_dt(double rho, double ux, double uy, doub
 
 // block-wide min of one value per thread, then one atomic per block.
 // Positive doubles order like their bit patterns, so the running minimum is
-// an integer atomicMin.  A non-finite or non-positive limit raises a flag.
-__device__ __forceinline__ void block_min_dt(double v, RunScalars* rs, double* red)
+// an integer atomicMin.  A non-finite or non-positive limit of a thread that
+// holds cells (valid) raises a flag; threads without cells (the spare rows
+// of a y-sweep workgroup) take no part.
+__device__ __forceinline__ void block_min_dt(double v, bool valid, RunScalars* rs, double* red)
 {
-    if (!(v > 0.0) || !isfinite(v)) {
+    if (!valid) {
+        v = INFINITY;
+    } else if (!(v > 0.0) || !isfinite(v)) {
         atomicOr(&rs->nonfinite, 1);
         v = INFINITY;
     }
@@ -734,6 +738,7 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
     }
 
     if (a.dt_limits) {
+        // CFL limits of the filtered state (stage 1; timeAdvance.c:50-136)
         DtParams dp = a.dp;
         dp.di = rs->di;
         double mn = INFINITY;
@@ -746,7 +751,7 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
                                       u[PR * n + k], dp));
             }
         }
-        block_min_dt(mn, rs, red);
+        block_min_dt(mn, active, rs, red);
     }
 }
 
@@ -1350,11 +1355,11 @@ __global__ void __launch_bounds__(256) k_dt_limits(const T* __restrict__ u, DtPa
 {
     __shared__ double red[16];
     double mn = INFINITY;
-    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
-         p += (size_t)gridDim.x * blockDim.x)
+    const size_t p0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    for (size_t p = p0; p < n; p += (size_t)gridDim.x * blockDim.x)
         mn = fmin(mn, cell_dt(u[RHO * n + p], u[UX * n + p], u[UY * n + p], u[BX * n + p],
                               u[BY * n + p], u[BZ * n + p], u[PR * n + p], dp));
-    block_min_dt(mn, rs, red);
+    block_min_dt(mn, p0 < n, rs, red);
 }
 
 // dt for the coming step: cfl * global min, with the +-2 % hysteresis of

@@ -171,3 +171,23 @@ def test_full_size_properties_2048(m):
     c2.set_state(u0)
     c2.run(4, graph=True)
     np.testing.assert_array_equal(c2.state(), u)
+
+
+@pytest.mark.parametrize("nx,ny", [(16, 16), (32, 16), (16, 32)])
+def test_small_grids_with_spare_rows(m, nx, ny):
+    # short lines put several rows in one y-sweep workgroup (64 for ny = 16),
+    # more than the grid has: the spare rows take no part in the CFL
+    # reduction, so the run reports no non-finite limit, and it matches the
+    # oracle
+    print(f"[context] small grid {nx}x{ny}", flush=True)
+    u0 = m.turbulence(nx, ny, seed=3, kmax=2)
+    ctx = m.Context(nx, ny, hall=1, di=0.05)
+    ctx.set_state(u0)
+    S = orc.Sim(nx, ny, hall=1, di=0.05)
+    S.set_state(u0)
+    for k in range(4):
+        ctx.run(1)
+        ctx.sync()
+        S.run(1)
+        assert_fields_close(ctx.state(), S.state(), what=f"{nx}x{ny} step {k + 1}")
+        assert abs(ctx.scalars()[1] - S.time) <= 1e-12 * S.time
