@@ -191,3 +191,10 @@ def test_small_grids_with_spare_rows(m, nx, ny):
         S.run(1)
         assert_fields_close(ctx.state(), S.state(), what=f"{nx}x{ny} step {k + 1}")
         assert abs(ctx.scalars()[1] - S.time) <= 1e-12 * S.time
+    e, dmax = ctx.diagnostics()
+    u = ctx.state()
+    dx, dy = 2 * np.pi / nx, 2 * np.pi / ny
+    assert abs(e - orc.energy(u, dx, dy)) <= 1e-12 * e
+    div = orc.derivX(u[4:5], dx)[0] + orc.derivY(u[5:6], 2 * np.pi)[0]
+    assert abs(dmax - np.abs(div).max()) <= 1e-9 * np.abs(div).max() + 1e-14
+    np.testing.assert_allclose(ctx.divb(), div, rtol=0, atol=1e-9 * np.abs(div).max() + 1e-14)
