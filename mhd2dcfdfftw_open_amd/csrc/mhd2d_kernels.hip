@@ -101,9 +101,12 @@ __device__ __forceinline__ void block_min_dt(double v, bool valid, RunScalars* r
 // (MHD2D_NT_X / _Y / _R: the x-solves, the y-sweep, the RHS + RK).  They do
 // not leave dirty lines in L2 for the next kernel to write back: measured
 // per step -2.8 % (2048^2 FP64), -5.6 % (mixed), -4.3 % (4096^2 FP64).
-// The exception is the y-sweep's filtered state (MHD2D_NT_YS): the same
+// The exceptions are the y-sweep's filtered state (MHD2D_NT_YS): the same
 // kernel reads six of its fields back to form E, and ordinary stores let
-// part of that come from cache (-3.3 % y-sweep time at 2048^2 FP64).
+// part of that come from cache (-3.3 % y-sweep time at 2048^2 FP64); and
+// the x-filter's output (MHD2D_NT_XF), which the next y-sweep reads first
+// (-4 % y-sweep time; the derivative solves keep nontemporal stores, whose
+// reader, the RHS + RK, measured 7 % slower after ordinary ones).
 #ifndef MHD2D_NT_X
 #define MHD2D_NT_X 1
 #endif
@@ -115,6 +118,9 @@ __device__ __forceinline__ void block_min_dt(double v, bool valid, RunScalars* r
 #endif
 #ifndef MHD2D_NT_YS
 #define MHD2D_NT_YS 0
+#endif
+#ifndef MHD2D_NT_XF
+#define MHD2D_NT_XF 0
 #endif
 template <bool NT, typename T>
 __device__ __forceinline__ void st_stream(T* p, T v)
@@ -373,7 +379,7 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
     T* __restrict__ po = o + (size_t)s * ny;
     // row segments of at least 64 bytes; narrower ones (8 FP32 columns)
     // measured 1.5x slower as nontemporal stores
-    constexpr bool NT = MHD2D_NT_X && W * sizeof(T) >= 64;
+    constexpr bool NT = (KIND == XFILT ? MHD2D_NT_XF : MHD2D_NT_X) && W * sizeof(T) >= 64;
     if (a.accumulate) {
         // second x-derivative onto the second y-derivative already there
         // (addSourceTerm's d2/dx2 + d2/dy2, calcRHS.c:498-531)
