@@ -105,8 +105,10 @@ __device__ __forceinline__ void block_min_dt(double v, bool valid, RunScalars* r
 // kernel reads six of its fields back to form E, and ordinary stores let
 // part of that come from cache (-3.3 % y-sweep time at 2048^2 FP64); and
 // the x-filter's output (MHD2D_NT_XF), which the next y-sweep reads first
-// (-4 % y-sweep time; the derivative solves keep nontemporal stores, whose
-// reader, the RHS + RK, measured 7 % slower after ordinary ones).
+// (-7 % y-sweep time; the derivative solves keep nontemporal stores, whose
+// reader, the RHS + RK, measured 7 % slower after ordinary ones).  Ordinary
+// stores of E (read by the x-derivative next) measured +9 % y-sweep and
+// x-derivative time, of the RHS + RK's state no change.
 #ifndef MHD2D_NT_X
 #define MHD2D_NT_X 1
 #endif
