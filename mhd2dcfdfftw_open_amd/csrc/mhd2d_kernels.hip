@@ -207,6 +207,37 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
         for (int k = 0; k < 3; k++) w[C + 3 + k] = (T)(hi[k * ny] - c0);
     }
 
+    // Factor rows.  For lines of up to 2048 rows (C <= 32) the three tables
+    // of the whole line go to LDS (3 nx words, dynamic) and the recurrences
+    // read them as broadcasts instead of 6 C vector loads per thread
+    // (measured -8 % x-solve time at nx = 2048, -11 % at 512); at nx = 4096
+    // the 96 KB fill per workgroup costs more than it saves.  The fill is
+    // issued right behind the line's loads, so both wait out one memory
+    // latency (-1.5 % x-solve time at nx = 2048); the barrier before the
+    // recurrences publishes it.
+    using FT = typename std::conditional<xsolve_lds_tables(C), T, double>::type;
+    const FT* L;
+    const FT* ID;
+    const FT* Q;
+    if constexpr (xsolve_lds_tables(C)) {
+        extern __shared__ unsigned char x_dyn[];
+        T* const tl = reinterpret_cast<T*>(x_dyn);
+        T* const tid = tl + nx;
+        T* const tqq = tid + nx;
+        for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+            tl[i] = (T)a.tb.l[i];
+            tid[i] = (T)a.tb.invd[i];
+            tqq[i] = (T)a.tb.q[i];
+        }
+        L = tl + s;
+        ID = tid + s;
+        Q = tqq + s;
+    } else {
+        L = a.tb.l + s;
+        ID = a.tb.invd + s;
+        Q = a.tb.q + s;
+    }
+
     const XStencil& st = a.st;
     const T c0 = (T)st.c0, c1 = (T)st.c1, c2 = (T)st.c2, c3 = (T)st.c3;
     const bool first = !BLK && (g == 0), last = !BLK && (g == G - 1);
@@ -257,35 +288,7 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
     if (first) w[0] = rfirst;
     if (last) w[C - 1] = rlast;
 
-    // Factor rows.  For lines of up to 2048 rows (C <= 32) the three tables
-    // of the whole line go to LDS (3 nx words, dynamic) and the recurrences
-    // read them as broadcasts instead of 6 C vector loads per thread
-    // (measured -8 % x-solve time at nx = 2048, -11 % at 512); at nx = 4096
-    // the 96 KB fill per workgroup costs more than it saves.
-    using FT = typename std::conditional<xsolve_lds_tables(C), T, double>::type;
-    const FT* L;
-    const FT* ID;
-    const FT* Q;
-    if constexpr (xsolve_lds_tables(C)) {
-        extern __shared__ unsigned char x_dyn[];
-        T* const tl = reinterpret_cast<T*>(x_dyn);
-        T* const tid = tl + nx;
-        T* const tqq = tid + nx;
-        for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-            tl[i] = (T)a.tb.l[i];
-            tid[i] = (T)a.tb.invd[i];
-            tqq[i] = (T)a.tb.q[i];
-        }
-        __syncthreads();
-        L = tl + s;
-        ID = tid + s;
-        Q = tqq + s;
-    } else {
-        L = a.tb.l + s;
-        ID = a.tb.invd + s;
-        Q = a.tb.q + s;
-    }
-
+    if constexpr (xsolve_lds_tables(C)) __syncthreads();  // the tables are in LDS
     // Rows from tb.kconv to N-2 carry the factors' exact FP64 fixed point,
     // so a chunk inside that range uses the scalar constants (no loads).
     // (FP32 only: in FP64 the second code path costs more registers than
