@@ -37,6 +37,17 @@ int fail(int code, const char* what, hipError_t e = hipSuccess)
     }
     return code;
 }
+
+// a device temporary released when it goes out of scope (early error
+// returns included)
+template <typename T>
+struct DevTemp {
+    T* p = nullptr;
+    ~DevTemp()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
 }  // namespace
 
 #define HIPCHK(expr)                                                        \
@@ -599,7 +610,8 @@ int mhd2d_set_state_at(mhd2d_ctx* c, int inst, const double* uu)
             HIPCHK(hipMemcpyAsync(dst + PSI_SLOT * c->n, hpsi, c->n * 8, hipMemcpyHostToDevice,
                                   c->stream));
     } else {
-        double* tmp = nullptr;
+        DevTemp<double> t;  // freed on every return
+        double*& tmp = t.p;
         const size_t tot = cnt + (c->glm() ? c->n : 0);
         HIPCHK(hipMalloc(&tmp, tot * 8));
         HIPCHK(hipMemcpyAsync(tmp, uu, tot * 8, hipMemcpyHostToDevice, c->stream));
@@ -612,7 +624,6 @@ int mhd2d_set_state_at(mhd2d_ctx* c, int inst, const double* uu)
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipStreamSynchronize(c->stream));
-        HIPCHK(hipFree(tmp));
     }
     RunScalars z;
     std::memset(&z, 0, sizeof(z));
@@ -651,7 +662,8 @@ int mhd2d_get_state_at(mhd2d_ctx* c, int inst, double* uu)
             HIPCHK(hipMemcpyAsync(uu + cnt, src + PSI_SLOT * c->n, c->n * 8,
                                   hipMemcpyDeviceToHost, c->stream));
     } else {
-        double* tmp = nullptr;
+        DevTemp<double> t;  // freed on every return
+        double*& tmp = t.p;
         const size_t tot = cnt + (c->glm() ? c->n : 0);
         HIPCHK(hipMalloc(&tmp, tot * 8));
         const float* src = (const float*)c->u + inst * c->su();
@@ -664,7 +676,6 @@ int mhd2d_get_state_at(mhd2d_ctx* c, int inst, double* uu)
         }
         HIPCHK(hipMemcpyAsync(uu, tmp, tot * 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        HIPCHK(hipFree(tmp));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return MHD2D_OK;
