@@ -276,7 +276,8 @@ def roofline_of(ctx, nx, ny, elem, delem, batch, profile_steps, copy_gbps):
     return roof, kernels
 
 
-def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50, chunks=4):
+def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50, chunks=4,
+             graph=True):
     """4096^2: mixed precision with the x-derivative as the tridiagonal solve
     and as the dense A^-1 B through split-BF16 MFMA GEMMs (3 and 6
     products), and plain FP32; per-field rel-L2 against FP64 every 50 steps
@@ -291,12 +292,12 @@ def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50,
         ctx.set_state(u0)
         if AUTOTUNE and not xg:
             ctx.autotune()
-        ctx.run(1, graph=True)   # the first step (no pending filter) and the capture
+        ctx.run(1, graph=graph)  # the first step (no pending filter) and the capture
         ctx.sync()
         ctx.set_state(u0)
         ms, errs = 0.0, []
         for k in range(chunks):
-            ms += ctx.time_run(chunk, graph=True)
+            ms += ctx.time_run(chunk, graph=graph)
             ctx.sync()
             st = ctx.state()
             if name == "fp64":
@@ -377,7 +378,7 @@ def main():
 
     if args.c3_only:
         print(json.dumps(c3_forms(m, float("nan"), args.profile_steps, m.copy_probe(COPY_BYTES, 5),
-                                  1, 0.02)))
+                                  1, 0.02, graph=bool(args.graph))))
         return
 
     nx, ny = args.nx, args.ny
