@@ -197,25 +197,29 @@ def test_dissipative_mixed_precision(m, clean):
     assert e.max() <= 1e-6, e
 
 
-def test_reference_input_runs_fused(m, tmp_path):
+@pytest.mark.parametrize("clean", [0, 2])
+def test_reference_input_runs_fused(m, tmp_path, clean):
     # the reference's own `input`: open x (npe >= 2), resistivity 1e-4,
-    # Hall off; here on a 64 x 32 grid of its Lx = 2, Ly = 4 box
+    # Hall off; here on a 64 x 32 grid of its Lx = 2, Ly = 4 box (and the
+    # same with GLM cleaning, whose output files hold psi as field 9)
     import array
     import os
     inp = m.Input(boundaryTypeX=1, cfl=0.5, Tmax=0.02, dtOut=0.01, nx=64, ny=32,
                   Lx=2.0, Ly=4.0, gamma=1.666666667, hall=0, di=0.002,
-                  resistivity=1e-4, viscosity=0.0, divbclean=0, outdivb=1)
+                  resistivity=1e-4, viscosity=0.0, divbclean=clean, outdivb=1)
     u0 = _state(m, 64, 32, 0.2)
     rep = m.drive(inp, u0, outdir=str(tmp_path), mode=m.DRIVE_AUTO, npe=2)
-    S = orc.Sim(64, 32, 2.0, 4.0, 1.666666667, 0.5, 0, 0.002, 1e-4, 0.0, 0, npe=2,
+    S = orc.Sim(64, 32, 2.0, 4.0, 1.666666667, 0.5, 0, 0.002, 1e-4, 0.0, clean, npe=2,
                 open_x=True)
     S.set_state(u0)
     S.run(rep["steps"])
+    nv = 9 if clean == 2 else 8
     a = array.array("d")
     with open(os.path.join(str(tmp_path), f"out{rep['outputs'] - 1:04d}.dat"), "rb") as f:
-        a.fromfile(f, 8 * 64 * 32 + 1)
+        a.fromfile(f, nv * 64 * 32 + 1)
+        assert f.read() == b""
     assert abs(a[0] - S.time) <= 1e-12 * S.time
-    assert_fields_close(np.reshape(np.asarray(a[1:]), [8, 64, 32]), S.state(),
+    assert_fields_close(np.reshape(np.asarray(a[1:]), [nv, 64, 32]), S.state(),
                         what="reference input, final output")
 
 
