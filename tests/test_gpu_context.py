@@ -173,6 +173,36 @@ def test_full_size_properties_2048(m):
     np.testing.assert_array_equal(c2.state(), u)
 
 
+def test_full_size_properties_4096_mixed(m):
+    # the 4096^2 workload of the precision comparison: the mixed form
+    # conserves energy like the FP64 run, stays within 1e-6 of it, and its
+    # graph replay equals its eager run bit for bit
+    n = 4096
+    print("[ctx] 4096^2 turbulence, mixed and FP64, 3 steps", flush=True)
+    u0 = m.turbulence(n, n, seed=42, rms=0.1, kmin=1, kmax=16)
+    ref = m.Context(n, n, hall=1, di=0.02)
+    mix = m.Context(n, n, hall=1, di=0.02, precision=m.MIXED)
+    for c in (ref, mix):
+        c.set_state(u0)
+    e0, _ = mix.diagnostics()
+    for c in (ref, mix):
+        c.run(3)
+        c.sync()
+    e1, d1 = mix.diagnostics()
+    u = mix.state()
+    assert np.isfinite(u).all()
+    assert abs(e1 - e0) <= 1e-6 * e0
+    assert d1 <= 1e-4
+    e = field_errors(u, ref.state())
+    print(f"[ctx] 4096^2 mixed vs FP64 after 3 steps: {e.max():.2e}")
+    assert e.max() <= 1e-6, e
+    del ref
+    g = m.Context(n, n, hall=1, di=0.02, precision=m.MIXED)
+    g.set_state(u0)
+    g.run(3, graph=True)
+    np.testing.assert_array_equal(g.state(), u)
+
+
 @pytest.mark.parametrize("nx,ny", [(16, 16), (32, 16), (16, 32)])
 def test_small_grids_with_spare_rows(m, nx, ny):
     # short lines put several rows in one y-sweep workgroup (64 for ny = 16),
