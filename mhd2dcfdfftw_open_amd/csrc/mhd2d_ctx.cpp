@@ -982,7 +982,7 @@ hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
     std::vector<Cand> cands;
     for (int w : {16, 8})
         for (int r : {0, 1}) {
-            if ((w == 16 && C0 > 32) || c->p.ny % w) continue;
+            if (w * G0 > 1024 || c->p.ny % w) continue;  // at most 1024 threads
             cands.push_back({w, r});
         }
     hipError_t e = hipStreamSynchronize(c->stream);

@@ -165,7 +165,7 @@ __device__ __forceinline__ void xcd_tile(int& x, int& y, int& z, bool remap)
 // tb.omask take explicit stencils over f[i-4 .. i+4] (tb.ocoef); otherwise
 // rows 0 and nx-1 are the only end rows.
 template <typename T, typename TI, int C, int W, int KIND, bool BLK>
-__global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, TI> a)
+__global__ void __launch_bounds__(1024, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, TI> a)
 {
     int bx, v, z;  // column strip, field, instance of a batch
     xcd_tile(bx, v, z, a.remap > 0);
@@ -183,9 +183,12 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
     // per-chunk carries, row g of W (+1 pad: the scans read them with
     // consecutive lanes one row apart, which the pad spreads over the banks)
     constexpr int WP = W + 1;
-    __shared__ T sh_in[WP * 64];
-    __shared__ T sh_fw[WP * 64];
-    __shared__ T sh_bw[WP * 64];
+    // up to 1024 threads: 64 chunks of 16 columns or 128 chunks of 8
+    constexpr int GMAX = 1024 / W;
+    __shared__ T sh_in[WP * GMAX];
+    __shared__ T sh_fw[WP * GMAX];
+    __shared__ T sh_bw[WP * GMAX];
+    __shared__ T sh_half[W];  // a column's value at the end of its lower 64 chunks
 
     // rows s-3 .. s+C+2 of this column; only the 3-row halos can wrap in x
     T w[C + 6];
@@ -207,11 +210,11 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
         for (int k = 0; k < 3; k++) w[C + 3 + k] = (T)(hi[k * ny] - c0);
     }
 
-    // Factor rows.  For lines of up to 2048 rows (C <= 32) the three tables
-    // of the whole line go to LDS (3 nx words, dynamic) and the recurrences
-    // read them as broadcasts instead of 6 C vector loads per thread
-    // (measured -8 % x-solve time at nx = 2048, -11 % at 512); at nx = 4096
-    // the 96 KB fill per workgroup costs more than it saves.  The fill is
+    // Factor rows.  With chunks of up to 32 rows (every line length now) the
+    // three tables of the whole line go to LDS (3 nx words, dynamic) and the
+    // recurrences read them as broadcasts instead of 6 C vector loads per
+    // thread (measured -8 % x-solve time at nx = 2048, -11 % at 512, -8 % at
+    // 4096 with 128 chunks).  The fill is
     // issued right behind the line's loads, so both wait out one memory
     // latency (-1.5 % x-solve time at nx = 2048); the barrier before the
     // recurrences publishes it.
@@ -308,19 +311,29 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
 #pragma unroll
         for (int k = 0; k < C; k++) y = w[k] - (T)L[k] * y;
     }
+    // The chunk scans run on wavefront shuffles over up to 64 chunks; with
+    // 128 chunks (8-column strips of 4096 rows) each half is scanned in its
+    // own wavefront and the upper half then starts from the lower half's end.
+    const int SW = G > 64 ? 64 : G;
     sh_in[g * WP + c] = y;
     __syncthreads();
     {
         const int c2i = threadIdx.x / G, g2 = threadIdx.x % G;
+        const int gl = g2 % SW;
         T B = sh_in[g2 * WP + c2i];
         T A = (T)a.tb.pf[g2];
-        for (int d = 1; d < G; d <<= 1) {
-            const T Ap = __shfl_up(A, d, G);
-            const T Bp = __shfl_up(B, d, G);
-            if (g2 >= d) {
+        for (int d = 1; d < SW; d <<= 1) {
+            const T Ap = __shfl_up(A, d, SW);
+            const T Bp = __shfl_up(B, d, SW);
+            if (gl >= d) {
                 B = B + A * Bp;
                 A = A * Ap;
             }
+        }
+        if (G > 64) {
+            if (g2 == 63) sh_half[c2i] = B;
+            __syncthreads();
+            if (g2 >= 64) B = B + A * sh_half[c2i];
         }
         sh_fw[g2 * WP + c2i] = B;  // y at the last row of chunk g2
     }
@@ -354,15 +367,21 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
     {
         const int c2i = threadIdx.x / G, g2 = G - 1 - (threadIdx.x % G);
         const int gr = threadIdx.x % G;  // position in the reversed scan
+        const int gl = gr % SW;
         T B = sh_in[g2 * WP + c2i];
         T A = (T)a.tb.qb[g2];
-        for (int d = 1; d < G; d <<= 1) {
-            const T Ap = __shfl_up(A, d, G);
-            const T Bp = __shfl_up(B, d, G);
-            if (gr >= d) {
+        for (int d = 1; d < SW; d <<= 1) {
+            const T Ap = __shfl_up(A, d, SW);
+            const T Bp = __shfl_up(B, d, SW);
+            if (gl >= d) {
                 B = B + A * Bp;
                 A = A * Ap;
             }
+        }
+        if (G > 64) {
+            if (gr == 63) sh_half[c2i] = B;
+            __syncthreads();
+            if (gr >= 64) B = B + A * sh_half[c2i];
         }
         sh_bw[g2 * WP + c2i] = B;  // x at the first row of chunk g2
     }
@@ -405,17 +424,21 @@ __global__ void __launch_bounds__(W * 64, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, T
 
 void xsolve_shape(int nx, int elem_bytes, int* chunk, int* width, int* groups)
 {
+    // 32-row chunks at most (64-row ones need ~170 VGPRs in FP64, one
+    // 512-thread workgroup per CU); 4096 rows are 128 chunks of 8 columns
     int C = nx / 64;
     if (C < 8) C = 8;
+    if (C > 32) C = 32;
+    const int G = nx / C;
     int W = MHD2D_X_W;
-    // 64-row chunks: 8-column strips keep 512 threads per block.  Lines of
+    // 128 chunks: 8-column strips keep 1024 threads per block.  Lines of
     // <= 512 rows read as FP32 (elem_bytes: the input word) measured faster
     // in 8-column strips too (64 x 512^2 FP32: -5 %); with FP64 inputs they
     // take 16 columns (128-byte rows: -3 % FP64, -6 % mixed).
-    if (C > 32 || (nx <= 512 && elem_bytes == 4)) W = 8;
+    if (G > 64 || (nx <= 512 && elem_bytes == 4)) W = 8;
     *chunk = C;
     *width = W;
-    *groups = nx / C;
+    *groups = G;
 }
 
 template <typename T, typename TI, int C, int W, bool BLK>
@@ -460,12 +483,13 @@ hipError_t launch_xsolve(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int 
     int C, W, G;
     xsolve_shape(a.nx, sizeof(TI), &C, &W, &G);
     if (a.width) {
-        // autotuned width: 8 or 16 columns; 64-row chunks only fit 8
-        if ((a.width != 8 && a.width != 16) || (a.width == 16 && C > 32))
-            return hipErrorInvalidValue;
+        // autotuned width: 8 or 16 columns, at most 1024 threads (128
+        // chunks only fit 8)
+        if ((a.width != 8 && a.width != 16) || a.width * G > 1024) return hipErrorInvalidValue;
         W = a.width;
     }
-    if (a.nx % C || G > 64 || G < 2 || a.ny % W || nv < 1 || nv > 16) return hipErrorInvalidValue;
+    if (a.nx % C || G > 128 || G < 2 || W * G > 1024 || a.ny % W || nv < 1 || nv > 16)
+        return hipErrorInvalidValue;
     XSolveArgs<T, TI> b = a;
     b.remap = a.remap < 0 ? (W == 8) : a.remap;
     if (W == 8) {
@@ -473,7 +497,6 @@ hipError_t launch_xsolve(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int 
         case 8: return launch_xsolve_cw<T, TI, 8, 8>(b, nv, s, nb);
         case 16: return launch_xsolve_cw<T, TI, 16, 8>(b, nv, s, nb);
         case 32: return launch_xsolve_cw<T, TI, 32, 8>(b, nv, s, nb);
-        case 64: return launch_xsolve_cw<T, TI, 64, 8>(b, nv, s, nb);
         default: return hipErrorInvalidValue;
         }
     }

@@ -66,7 +66,7 @@ def test_autotune_keeps_results(m, nx, ny, prec, batch, npe, kw):
 
 
 def test_autotune_4096_has_only_narrow_strips(m):
-    # 64-row chunks do not fit 16-column strips
+    # 128 chunks of 32 rows do not fit 16-column strips (1024 threads at most)
     c = m.Context(4096, 64, hall=1, di=0.02)
     c.set_state(m.turbulence(4096, 64, seed=3, kmax=8))
     ch = c.autotune(reps=1)
