@@ -231,16 +231,16 @@ def uneven_runs(m, count, seed):
     return runs, states
 
 
-def run_batch(m, states, dis, nx, ny, prec, steps, warmup, hall=1):
+def run_batch(m, states, dis, nx, ny, prec, steps, warmup, hall=1, graph=True):
     B = len(states)
     ctx = m.Context(nx, ny, hall=hall, precision=_precision(m, prec), batch=B)
     for b in range(B):
         ctx.set_di(b, float(dis[b]))
         ctx.set_state(states[b], inst=b)
     ctx.x_shapes = ctx.autotune() if AUTOTUNE else None
-    ctx.run(warmup, graph=True)
+    ctx.run(warmup, graph=graph)
     ctx.sync()
-    ms = ctx.time_run(steps, graph=True)
+    ms = ctx.time_run(steps, graph=graph)
     ctx.sync()
     return B * nx * ny * steps / (ms * 1e-3), ms, ctx
 
@@ -332,6 +332,43 @@ def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50,
     return out
 
 
+def batched_forms(m, value, args, copy_gbps):
+    """the 64 x 512^2 batched ensemble in the three precisions (errors
+    against the FP64 batch on four instances); value: the 2048^2 FP64 rate"""
+    out = {}
+    # batched ensemble: 64 x 512^2 linear waves, d_i per instance
+    enx = eny = 512
+    states, dis = ensemble_states(m, enx, eny, 64, 42)
+    res = {}
+    for prec in ("fp64", "mixed", "fp32"):
+        v, ms_p, c = run_batch(m, states, dis, enx, eny, prec, args.ens_steps, args.warmup,
+                               graph=bool(args.graph))
+        res[prec] = (v, ms_p, c)
+    vref, _, cref = res["fp64"]
+    checked = (0, 21, 42, 63)
+    # errors first: profiling advances the runs
+    ref_states = {b: cref.state(b) for b in checked}
+    errs_of = {prec: [max(field_errors(res[prec][2].state(b), ref_states[b]))
+                      for b in checked] for prec in res}
+    for prec in ("fp64", "mixed", "fp32"):
+        v, ms_p, c = res[prec]
+        errs = errs_of[prec]
+        pe, pd = _bytes(prec)
+        roof, kern = roofline_of(c, enx, eny, pe, pd, 64, args.profile_steps, copy_gbps)
+        out[f"batched64x512_{prec}"] = {
+            "workload": f"64 x {enx}x{eny} linear waves, d_i ~ U[0, 0.2], {prec}, "
+                        "one launch per kernel and stage, "
+                        + ("hipGraph step" if args.graph else "eager launches"),
+            "cell_updates_per_s": v, "ms_per_step": ms_p / args.ens_steps,
+            "speedup_vs_fp64_2048": v / value, "speedup_vs_fp64_batched": v / vref,
+            "rel_l2_vs_fp64_max": max(errs), "after_steps": args.warmup + args.ens_steps,
+            "instances_checked": list(checked), "meets_1e-6": bool(max(errs) <= 1e-6),
+            "step_roofline_frac": v * B_ALG[prec] / HBM_PEAK, "roofline": roof}
+    for prec in res:
+        res[prec][2].close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -351,6 +388,8 @@ def main():
                     help="keep the default x-solve launch shapes")
     ap.add_argument("--c3-only", action="store_true",
                     help="only the 4096^2 precision / x-derivative comparison (prints its JSON)")
+    ap.add_argument("--c4-only", action="store_true",
+                    help="only the 64 x 512^2 batched ensemble (prints its JSON; honours --graph)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the streaming-copy calibration kernel and print its rate")
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV",
@@ -379,6 +418,10 @@ def main():
     if args.c3_only:
         print(json.dumps(c3_forms(m, float("nan"), args.profile_steps, m.copy_probe(COPY_BYTES, 5),
                                   1, 0.02, graph=bool(args.graph))))
+        return
+
+    if args.c4_only:
+        print(json.dumps(batched_forms(m, float("nan"), args, m.copy_probe(COPY_BYTES, 5))))
         return
 
     nx, ny = args.nx, args.ny
@@ -455,34 +498,7 @@ def main():
                 "cell_updates_per_s": v, "ms_per_step": ms_p / args.steps,
                 "vs_ideal_fp64": v / value, "kernels": kern}
             c.close()
-        # batched ensemble: 64 x 512^2 linear waves, d_i per instance
-        enx = eny = 512
-        states, dis = ensemble_states(m, enx, eny, 64, 42)
-        res = {}
-        for prec in ("fp64", "mixed", "fp32"):
-            v, ms_p, c = run_batch(m, states, dis, enx, eny, prec, args.ens_steps, args.warmup)
-            res[prec] = (v, ms_p, c)
-        vref, _, cref = res["fp64"]
-        checked = (0, 21, 42, 63)
-        # errors first: profiling advances the runs
-        ref_states = {b: cref.state(b) for b in checked}
-        errs_of = {prec: [max(field_errors(res[prec][2].state(b), ref_states[b]))
-                          for b in checked] for prec in res}
-        for prec in ("fp64", "mixed", "fp32"):
-            v, ms_p, c = res[prec]
-            errs = errs_of[prec]
-            pe, pd = _bytes(prec)
-            roof, kern = roofline_of(c, enx, eny, pe, pd, 64, args.profile_steps, copy_gbps)
-            forms[f"batched64x512_{prec}"] = {
-                "workload": f"64 x {enx}x{eny} linear waves, d_i ~ U[0, 0.2], {prec}, "
-                            "one launch per kernel and stage, hipGraph step",
-                "cell_updates_per_s": v, "ms_per_step": ms_p / args.ens_steps,
-                "speedup_vs_fp64_2048": v / value, "speedup_vs_fp64_batched": v / vref,
-                "rel_l2_vs_fp64_max": max(errs), "after_steps": args.warmup + args.ens_steps,
-                "instances_checked": list(checked), "meets_1e-6": bool(max(errs) <= 1e-6),
-                "step_roofline_frac": v * B_ALG[prec] / HBM_PEAK, "roofline": roof}
-        for prec in res:
-            res[prec][2].close()
+        forms.update(batched_forms(m, value, args, copy_gbps))
         # uneven ensemble: 96 runs, sizes drawn from four shapes, per-run rms, d_i
         runs, ustates = uneven_runs(m, 96, 42)
         ures = {}
