@@ -369,6 +369,49 @@ def batched_forms(m, value, args, copy_gbps):
     return out
 
 
+def uneven_forms(m, value, args):
+    """the uneven ensemble of 96 turbulence runs on four grid sizes in the
+    three precisions (errors against the FP64 ensemble on four runs);
+    value: the 2048^2 FP64 rate"""
+    out = {}
+    graph = bool(args.graph)
+    # uneven ensemble: 96 runs, sizes drawn from four shapes, per-run rms, d_i
+    runs, ustates = uneven_runs(m, 96, 42)
+    ures = {}
+    for prec in ("fp64", "mixed", "fp32"):
+        E = m.Ensemble(runs, precision=_precision(m, prec))
+        for k in range(len(runs)):
+            E.set_state(k, ustates[k])
+        if AUTOTUNE:
+            E.autotune()
+        E.run(args.warmup, graph=graph)
+        E.sync()
+        ms_p = E.time_run(args.steps, graph=graph)
+        E.sync()
+        cells = sum(r["nx"] * r["ny"] for r in runs)
+        ures[prec] = (cells * args.steps / (ms_p * 1e-3), ms_p, E)
+    checked = (0, 31, 64, 95)
+    refs = {k: ures["fp64"][2].state(k) for k in checked}
+    for prec in ("fp64", "mixed", "fp32"):
+        v, ms_p, E = ures[prec]
+        errs = [max(field_errors(E.state(k), refs[k])) for k in checked]
+        out[f"uneven96_{prec}"] = {
+            "workload": "96 turbulence runs, sizes drawn from 256x256, 512x256, "
+                        "1024x1024, 2048x512, rms ~ U[0.05, 0.2], d_i ~ U[0, 0.1], "
+                        f"{prec}; one batched context per size, "
+                        + ("one hipGraph per step forking into the four size groups"
+                           if graph else "eager launches on the four group streams"),
+            "cells": sum(r["nx"] * r["ny"] for r in runs), "groups": E.groups,
+            "cell_updates_per_s": v, "ms_per_step": ms_p / args.steps,
+            "speedup_vs_fp64_2048": v / value, "speedup_vs_fp64_uneven": v / ures["fp64"][0],
+            "rel_l2_vs_fp64_max": max(errs), "after_steps": args.warmup + args.steps,
+            "instances_checked": list(checked), "meets_1e-6": bool(max(errs) <= 1e-6),
+            "step_roofline_frac": v * B_ALG[prec] / HBM_PEAK}
+    for prec in ures:
+        ures[prec][2].close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -390,6 +433,8 @@ def main():
                     help="only the 4096^2 precision / x-derivative comparison (prints its JSON)")
     ap.add_argument("--c4-only", action="store_true",
                     help="only the 64 x 512^2 batched ensemble (prints its JSON; honours --graph)")
+    ap.add_argument("--c5-only", action="store_true",
+                    help="only the 96-run uneven ensemble (prints its JSON; honours --graph)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the streaming-copy calibration kernel and print its rate")
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV",
@@ -418,6 +463,10 @@ def main():
     if args.c3_only:
         print(json.dumps(c3_forms(m, float("nan"), args.profile_steps, m.copy_probe(COPY_BYTES, 5),
                                   1, 0.02, graph=bool(args.graph))))
+        return
+
+    if args.c5_only:
+        print(json.dumps(uneven_forms(m, float("nan"), args)))
         return
 
     if args.c4_only:
@@ -499,39 +548,7 @@ def main():
                 "vs_ideal_fp64": v / value, "kernels": kern}
             c.close()
         forms.update(batched_forms(m, value, args, copy_gbps))
-        # uneven ensemble: 96 runs, sizes drawn from four shapes, per-run rms, d_i
-        runs, ustates = uneven_runs(m, 96, 42)
-        ures = {}
-        for prec in ("fp64", "mixed", "fp32"):
-            E = m.Ensemble(runs, precision=_precision(m, prec))
-            for k in range(len(runs)):
-                E.set_state(k, ustates[k])
-            if AUTOTUNE:
-                E.autotune()
-            E.run(args.warmup, graph=True)
-            E.sync()
-            ms_p = E.time_run(args.steps, graph=True)
-            E.sync()
-            cells = sum(r["nx"] * r["ny"] for r in runs)
-            ures[prec] = (cells * args.steps / (ms_p * 1e-3), ms_p, E)
-        checked = (0, 31, 64, 95)
-        refs = {k: ures["fp64"][2].state(k) for k in checked}
-        for prec in ("fp64", "mixed", "fp32"):
-            v, ms_p, E = ures[prec]
-            errs = [max(field_errors(E.state(k), refs[k])) for k in checked]
-            forms[f"uneven96_{prec}"] = {
-                "workload": "96 turbulence runs, sizes drawn from 256x256, 512x256, "
-                            "1024x1024, 2048x512, rms ~ U[0.05, 0.2], d_i ~ U[0, 0.1], "
-                            f"{prec}; one batched context per size, one hipGraph per step "
-                            "forking into the four size groups",
-                "cells": sum(r["nx"] * r["ny"] for r in runs), "groups": E.groups,
-                "cell_updates_per_s": v, "ms_per_step": ms_p / args.steps,
-                "speedup_vs_fp64_2048": v / value, "speedup_vs_fp64_uneven": v / ures["fp64"][0],
-                "rel_l2_vs_fp64_max": max(errs), "after_steps": args.warmup + args.steps,
-                "instances_checked": list(checked), "meets_1e-6": bool(max(errs) <= 1e-6),
-                "step_roofline_frac": v * B_ALG[prec] / HBM_PEAK}
-        for prec in ures:
-            ures[prec][2].close()
+        forms.update(uneven_forms(m, value, args))
         forms.update(c3_forms(m, value, args.profile_steps, copy_gbps, hall, di))
         out["forms"] = forms
     if not args.no_cpu:
