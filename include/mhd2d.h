@@ -206,9 +206,20 @@ long long mhd2d_xgemm_panels(const mhd2d_ctx *c);
  * same state, and keeps the fastest.  Needs a state for every instance; the
  * state, R_prev, time, dt and step count are restored afterwards.  choice
  * (may be NULL) receives {width_d, remap_d, width_f, remap_f}.  Results do
- * not depend on the shape (each y-line is solved the same way).
+ * not depend on the shape (each y-line is solved the same way).  For a
+ * batch it then times instance groups (mhd2d_set_subbatch) of half, a
+ * quarter, ... of the batch, down to 2^21 cells a group, and keeps a group
+ * size only where it beats the whole batch by 1 %.
  * Tridiagonal form only. */
 int mhd2d_autotune(mhd2d_ctx *c, int reps, int *choice);
+/* Instances per launch of a batch (no counterpart in the reference, whose
+ * runs are one grid each): a step runs the batch as groups of k instances,
+ * one whole RK3 step per group, so that a group's fields stay in the GPU's
+ * 256 MB Infinity Cache from kernel to kernel.  k = 0 or k >= batch: the
+ * whole batch in every launch (the default).  Results are bitwise the same
+ * for every k.  mhd2d_get_subbatch returns the group size in use. */
+int mhd2d_set_subbatch(mhd2d_ctx *c, int k);
+int mhd2d_get_subbatch(const mhd2d_ctx *c);
 /* wall time of nsteps on the context's stream (hipEvents), in ms */
 int mhd2d_time_run(mhd2d_ctx *c, int nsteps, int use_graph, double *ms);
 /* average device time per launch of each kernel class over nsteps, in ms:

@@ -127,6 +127,8 @@ def lib():
         "mhd2d_set_state_at": (i, [vp, i, p]),
         "mhd2d_get_state_at": (i, [vp, i, p]),
         "mhd2d_set_di": (i, [vp, i, d]),
+        "mhd2d_set_subbatch": (i, [vp, i]),
+        "mhd2d_get_subbatch": (i, [vp]),
         "mhd2d_get_scalars_at": (i, [vp, i, p, p, ctypes.POINTER(ll)]),
         "mhd2d_diagnostics_at": (i, [vp, i, p, p]),
         "mhd2d_stopped_at": (i, [vp, i, p]),
@@ -287,6 +289,18 @@ class Context:
 
     def set_di(self, inst, di):
         _check(lib().mhd2d_set_di(self.h, inst, di))
+
+    def set_subbatch(self, k):
+        """run the batch as groups of k instances per launch (0: all at once)"""
+        _check(lib().mhd2d_set_subbatch(self.h, k))
+
+    @property
+    def subbatch(self):
+        """instances per launch in use"""
+        r = lib().mhd2d_get_subbatch(self.h)
+        if r < 0:
+            _check(r)
+        return r
 
     def run(self, nsteps, graph=False):
         f = lib().mhd2d_run_graph if graph else lib().mhd2d_run

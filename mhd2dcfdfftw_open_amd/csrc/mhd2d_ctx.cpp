@@ -90,6 +90,9 @@ struct mhd2d_ctx {
     // launch shapes of the derivative and filter x-solves (strip width,
     // XCD-contiguous order); defaults until mhd2d_autotune measures them
     int xw_d = 0, xr_d = -1, xw_f = 0, xr_f = -1;
+    // instances per launch: a step runs the batch as groups of this many
+    // instances, one whole step per group (0: the whole batch at once)
+    int subbatch = 0;
     double cdiff = 0;  // constDiffDivB = 2 max(dx, dy)^2 (initialize.c:17, 154)
     std::vector<char> have_state;
     std::vector<double> di;
@@ -333,9 +336,16 @@ hipError_t rhs_rk(mhd2d_ctx* c, int stage)
     return e;
 }
 
-// one RK3 step; when ev is given, an event is recorded after every launch
+int nsub(const mhd2d_ctx* c)
+{
+    const int k = c->subbatch > 0 && c->subbatch < c->B ? c->subbatch : c->B;
+    return (c->B + k - 1) / k;
+}
+
+// one RK3 step of instances [z0, z0 + c->B) of the buffers (the caller
+// shifts them); when ev is given, an event is recorded after every launch
 template <typename T, typename TD>
-hipError_t step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
+hipError_t step_group(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
 {
     hipError_t e;
     auto mark = [&](int) {
@@ -370,6 +380,54 @@ hipError_t step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
     }
     c->pending_filter = true;
     return hipSuccess;
+}
+
+// One RK3 step of the batch.  With subbatch k < B the instances go through
+// the step k at a time, each group's data still in the 256 MB Infinity Cache
+// from kernel to kernel; the instances are independent, so the results are
+// those of one launch over the batch (bit for bit).
+template <typename T, typename TD>
+hipError_t step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
+{
+    const int ng = nsub(c);
+    if (ng == 1) return step_group<T, TD>(c, ev, nev);
+    const int B = c->B, k = (B + ng - 1) / ng;
+    const int npe = c->p.xblocks > 1 ? c->p.xblocks : 1;
+    T* const u = (T*)c->u;
+    TD* const dx = (TD*)c->dudx;
+    TD* const dy = (TD*)c->dudy;
+    TD* const rp = (TD*)c->rprev;
+    TD* const lp = (TD*)c->lap;
+    TD* const db = (TD*)c->divb;
+    double* const cp = c->glm_cp;
+    RunScalars* const rs = c->rs;
+    const bool pf = c->pending_filter;
+    hipError_t e = hipSuccess;
+    for (int z0 = 0; z0 < B && e == hipSuccess; z0 += k) {
+        const size_t z = (size_t)z0;
+        c->B = std::min(k, B - z0);
+        c->u = u + z * c->su();
+        c->dudx = dx + z * c->sd();
+        c->dudy = dy + z * c->sd();
+        c->rprev = rp + z * c->sr();
+        c->lap = lp ? lp + z * c->sl() : nullptr;
+        c->divb = db ? db + z * c->sb() : nullptr;
+        c->glm_cp = cp ? cp + 2 * z * npe : nullptr;
+        c->rs = rs + z;
+        c->pending_filter = pf;
+        e = step_group<T, TD>(c, ev, nev);
+    }
+    c->B = B;
+    c->u = u;
+    c->dudx = dx;
+    c->dudy = dy;
+    c->rprev = rp;
+    c->lap = lp;
+    c->divb = db;
+    c->glm_cp = cp;
+    c->rs = rs;
+    c->pending_filter = e == hipSuccess ? true : pf;
+    return e;
 }
 
 template <typename T, typename TD>
@@ -1042,6 +1100,36 @@ hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
                 pick[phase][1] = cands[ci].r;
             }
     }
+    // instances per launch (batches only): the whole batch, then groups of
+    // half as many while a group keeps at least 2^21 cells (fewer leave the
+    // chip short of work); a group size must beat the whole batch by 1 %
+    int pick_sb = c->subbatch;
+    if (e == hipSuccess && c->B > 1) {
+        c->xw_d = pick[0][0];
+        c->xr_d = pick[0][1];
+        c->xw_f = pick[1][0];
+        c->xr_f = pick[1][1];
+        std::vector<int> sbc = {0};
+        for (int k = (c->B + 1) / 2; (size_t)k * c->n >= ((size_t)1 << 21); k = (k + 1) / 2) {
+            sbc.push_back(k);
+            if (k == 1) break;
+        }
+        std::vector<float> tm(sbc.size(), 1e30f);
+        for (size_t it = 0; sbc.size() > 1 && it < 2 * sbc.size() && e == hipSuccess; it++) {
+            const size_t ci = it % sbc.size();
+            c->subbatch = sbc[ci];
+            float t = 0;
+            e = time_steps(&t);
+            if (e == hipSuccess) tm[ci] = std::min(tm[ci], t);
+        }
+        pick_sb = 0;
+        float best = tm[0] * 0.99f;
+        for (size_t ci = 1; ci < sbc.size(); ci++)
+            if (tm[ci] < best) {
+                best = tm[ci];
+                pick_sb = sbc[ci];
+            }
+    }
     if (saved) {
         const hipError_t e2 = restore();
         const hipError_t e3 = hipStreamSynchronize(c->stream);
@@ -1052,8 +1140,10 @@ hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
     if (e != hipSuccess) {
         c->xw_d = c->xw_f = 0;
         c->xr_d = c->xr_f = -1;
+        c->subbatch = 0;
         return e;
     }
+    c->subbatch = pick_sb;
     c->xw_d = pick[0][0];
     c->xr_d = pick[0][1];
     c->xw_f = pick[1][0];
@@ -1090,6 +1180,23 @@ int mhd2d_autotune(mhd2d_ctx* c, int reps, int* choice)
     return MHD2D_OK;
 }
 
+int mhd2d_set_subbatch(mhd2d_ctx* c, int k)
+{
+    if (!c || k < 0) return fail(MHD2D_EINVAL, "bad argument");
+    c->subbatch = k >= c->B ? 0 : k;
+    if (c->graph) {  // the captured step holds the old launches
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    return MHD2D_OK;
+}
+
+int mhd2d_get_subbatch(const mhd2d_ctx* c)
+{
+    if (!c) return fail(MHD2D_EINVAL, "bad argument");
+    return c->subbatch > 0 && c->subbatch < c->B ? c->subbatch : c->B;
+}
+
 long long mhd2d_xgemm_panels(const mhd2d_ctx* c) { return c && c->p.xgemm ? c->xg.panels_total : 0; }
 
 int mhd2d_xderiv(mhd2d_ctx* c, int nv, const double* in, double* out)
@@ -1123,10 +1230,12 @@ int mhd2d_time_run(mhd2d_ctx* c, int nsteps, int use_graph, double* ms)
 int mhd2d_profile_run(mhd2d_ctx* c, int nsteps, double* out_ms, int* out_launches)
 {
     if (!c || !out_ms || nsteps < 1) return fail(MHD2D_EINVAL, "bad argument");
-    // per step: 3 stages x (ysweep, xderiv, rhs_rk, xfilter) + one dt finalize;
-    // event k+1 closes the launch opened by event k.
-    const int per_step = 3 * 5 + 1;
-    std::vector<hipEvent_t> ev(per_step + 1);
+    // per step and instance group: 3 stages x (ysweep, xderiv, rhs_rk,
+    // xfilter) + one dt finalize; event k+1 closes the launch opened by event
+    // k.  A class's time sums its launches over the groups of a step, so it
+    // is the time of one launch over the whole batch.
+    const int per_step = 3 * 4 + 1, ng = nsub(c);
+    std::vector<hipEvent_t> ev(per_step * ng + 1);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     double acc[5] = {0, 0, 0, 0, 0};
     int cnt[5] = {0, 0, 0, 0, 0};
@@ -1139,16 +1248,17 @@ int mhd2d_profile_run(mhd2d_ctx* c, int nsteps, double* out_ms, int* out_launche
         HIPCHK(do_step(c, ev.data(), &nev));
         HIPCHK(hipStreamSynchronize(c->stream));
         int idx = 0;
-        for (int s = 0; s < 3; s++) {
-            const int* cl = s == 0 ? cls0 : cls;
-            const int m = s == 0 ? 5 : 4;
-            for (int q = 0; q < m; q++, idx++) {
-                float f = 0;
-                HIPCHK(hipEventElapsedTime(&f, ev[idx], ev[idx + 1]));
-                acc[cl[q]] += f;
-                cnt[cl[q]] += 1;
+        for (int g = 0; g < ng; g++)
+            for (int s = 0; s < 3; s++) {
+                const int* cl = s == 0 ? cls0 : cls;
+                const int m = s == 0 ? 5 : 4;
+                for (int q = 0; q < m; q++, idx++) {
+                    float f = 0;
+                    HIPCHK(hipEventElapsedTime(&f, ev[idx], ev[idx + 1]));
+                    acc[cl[q]] += f;
+                    if (g == 0) cnt[cl[q]] += 1;
+                }
             }
-        }
     }
     for (int q = 0; q < 5; q++) {
         out_ms[q] = cnt[q] ? acc[q] / cnt[q] : 0.0;
