@@ -357,8 +357,9 @@ def batched_forms(m, value, args, copy_gbps):
         roof, kern = roofline_of(c, enx, eny, pe, pd, 64, args.profile_steps, copy_gbps)
         out[f"batched64x512_{prec}"] = {
             "workload": f"64 x {enx}x{eny} linear waves, d_i ~ U[0, 0.2], {prec}, "
-                        "one launch per kernel and stage, "
+                        "one launch per kernel, stage and instance group, "
                         + ("hipGraph step" if args.graph else "eager launches"),
+            "instances_per_launch": c.subbatch,
             "cell_updates_per_s": v, "ms_per_step": ms_p / args.ens_steps,
             "speedup_vs_fp64_2048": v / value, "speedup_vs_fp64_batched": v / vref,
             "rel_l2_vs_fp64_max": max(errs), "after_steps": args.warmup + args.ens_steps,
