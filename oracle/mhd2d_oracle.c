@@ -1146,6 +1146,15 @@ void orc_sim_run_fixed_dt(orc_sim *s, int nsteps, const double *dts)
     for (int k = 0; k < nsteps; k++) sim_step(s, &dts[k]);
 }
 
+/* use dt for the next step (replaces the calcDt value of set_state or of
+ * the last step): rkInit(dt), timeAdvance.c:20-27.  A test hook, so that
+ * runs from slightly different states advance with the same dt sequence. */
+void orc_sim_set_dt(orc_sim *s, double dt)
+{
+    s->dt = dt;
+    rk_init(s, dt);
+}
+
 /* emulate the reference run on npe MPI ranks along x (block closure) */
 void orc_sim_set_npe(orc_sim *s, int npe) { s->npe = npe < 1 ? 1 : npe; }
 

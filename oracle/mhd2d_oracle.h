@@ -120,6 +120,7 @@ void orc_sim_get_derivs(const orc_sim *s, double *dudx, double *dudy);
 void orc_sim_run(orc_sim *s, int nsteps);
 void orc_sim_run_fixed_dt(orc_sim *s, int nsteps, const double *dts);
 void orc_sim_stage(orc_sim *s, int iRK);                /* one RK stage */
+void orc_sim_set_dt(orc_sim *s, double dt);            /* dt of the next step */
 void orc_sim_set_npe(orc_sim *s, int npe);             /* x blocks (MPI ranks) */
 int orc_sim_nvar(const orc_sim *s);                    /* 9 with divbclean == 2 */
 void orc_sim_set_open(orc_sim *s, int open);           /* open x boundary */

@@ -74,6 +74,7 @@ def lib():
             "orc_sim_run": (None, [ctypes.c_void_p, i]),
             "orc_sim_run_fixed_dt": (None, [ctypes.c_void_p, i, p]),
             "orc_sim_stage": (None, [ctypes.c_void_p, i]),
+            "orc_sim_set_dt": (None, [ctypes.c_void_p, d]),
             "orc_sim_dt": (d, [ctypes.c_void_p]),
             "orc_sim_time": (d, [ctypes.c_void_p]),
             "orc_sim_steps": (ctypes.c_long, [ctypes.c_void_p]),
@@ -341,6 +342,10 @@ class Sim:
 
     def stage(self, irk):
         lib().orc_sim_stage(self.h, irk)
+
+    def set_dt(self, dt):
+        """dt of the next step (instead of the calcDt value)"""
+        lib().orc_sim_set_dt(self.h, dt)
 
     def glm(self, i=0):
         """(Ch, Cp) in force for x row i (hyperbolic cleaning)."""
