@@ -63,19 +63,59 @@ def kernel_alg_bytes(nx, ny, elem=8, delem=None, batch=1, divb=False):
     }
 
 
-def cpu_baseline(nx, ny, u0, hall, di):
-    """FP64 C restatement (the checker) timed on this host, 1 thread, on a
-    bounded sample of the same workload: one RK3 step of the full grid."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(nx, ny, u0, hall, di, c1_steps=200, c1_steps_1t=40):
+    """The FP64 C restatement (the checker) timed on this host's cores:
+
+    * the Orszag-Tang workload (256 x 256, Hall CFL term with d_i = 0.05) with
+      OpenMP over independent lines and cells, n threads, 200 RK3 steps, and
+      single-threaded for a bounded 40 steps (the same code; results equal
+      bit for bit);
+    * one RK3 step of the timed GPU workload (nx x ny turbulence), n threads.
+    n = OMP_NUM_THREADS (at most 16, this host's share), else the CPU count."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc
-    s = orc.Sim(nx, ny, hall=hall, di=di)
-    s.set_state(u0)
-    t0 = time.perf_counter()
-    s.run(1)
-    dt = time.perf_counter() - t0
-    return {"value": nx * ny / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"1 RK3 step of the {nx}x{ny} FP64 turbulence state ({dt:.1f} s, "
-                      "single-threaded oracle restatement)"}
+    n = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    n = max(1, min(n, 16))
+    import mhd2dcfdfftw_open_amd as m
+
+    def timed(lib_threads, gx, gy, state, steps, **kw):
+        orc.lib().orc_set_threads(lib_threads)
+        s = orc.Sim(gx, gy, **kw)
+        s.set_state(state)
+        t0 = time.perf_counter()
+        s.run(steps)
+        dt = time.perf_counter() - t0
+        del s
+        return gx * gy * steps / dt, dt
+
+    ot = m.orszag_tang(256, 256)
+    with orc.threaded():
+        v1, t1 = timed(1, 256, 256, ot, c1_steps_1t, hall=1, di=0.05)
+        vn, tn = timed(n, 256, 256, ot, c1_steps, hall=1, di=0.05)
+        vb, tb = timed(n, nx, ny, u0, 1, hall=hall, di=di)
+        threads = orc.lib().orc_threads()
+    return {"value": vn, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"256x256 Orszag-Tang (d_i = 0.05 Hall CFL), {c1_steps} RK3 steps of the "
+                      f"FP64 oracle restatement, OpenMP over lines and cells, {threads} threads "
+                      f"({tn:.1f} s)",
+            "cpu_model": _cpu_model(), "cores_visible": os.cpu_count(),
+            "threads_1": {"value": v1, "steps": c1_steps_1t, "seconds": t1,
+                          "workload": "256x256 Orszag-Tang, 1 thread"},
+            "threads_n": {"value": vn, "threads": threads, "steps": c1_steps, "seconds": tn,
+                          "workload": "256x256 Orszag-Tang"},
+            "gpu_workload_1_step": {"value": vb, "threads": threads, "seconds": tb,
+                                    "workload": f"{nx}x{ny} turbulence, 1 RK3 step"}}
 
 
 COPY_BYTES = 1 << 30
@@ -245,6 +285,24 @@ def run_batch(m, states, dis, nx, ny, prec, steps, warmup, hall=1, graph=True):
     return B * nx * ny * steps / (ms * 1e-3), ms, ctx
 
 
+def advance_to(obj, done, target, reset, graph=True):
+    """step obj (a context or an ensemble) until `target` steps have been
+    taken since its initial state; `reset` restores that state when more
+    than `target` steps are done already"""
+    if done > target:
+        reset()
+        done = 0
+    if target > done:
+        obj.run(target - done, graph=graph)
+    obj.sync()
+    return target
+
+
+# steps after which each workload's error against FP64 is taken: the
+# 2048^2 and uneven-ensemble runs are 200 steps, the 64 x 512^2 batch 500
+ERR_STEPS = {"single": 200, "batched": 500, "uneven": 200}
+
+
 def field_errors(a, b):
     """per-field rel-L2 of a against b (fields of b below 1e-6 of its rms
     measured in absolute L2 over that rms)"""
@@ -257,18 +315,57 @@ def field_errors(a, b):
     return out
 
 
-def roofline_of(ctx, nx, ny, elem, delem, batch, profile_steps, copy_gbps):
+def sweep_alg_bytes(nx, ny, elem=8, delem=None, batch=1):
+    """Algorithmic bytes per stage of the canonical two-sweep schedule, per
+    cell times the cells of one launch: the y-sweep reads u (8 words) and
+    writes the filtered u (8), d/dy of u (8) and of E (3) = 27 words; the
+    x-sweep reads u (8), the y-derivatives (11) and R_prev (8) and writes u
+    (8) and R_prev (8) = 43 words.  State words count elem bytes, derivative
+    and R_prev words delem bytes."""
+    n = nx * ny * batch
+    s, d = elem, (elem if delem is None else delem)
+    return {"ysweep": n * (16 * s + 11 * d), "xsweep": n * (16 * s + 27 * d)}
+
+
+# kernels of each canonical sweep as launched here: the y-sweep is one
+# kernel; the x-sweep is the derivative x-solve, the RHS + RK update and the
+# filter x-solve (three kernels, since full x-lines of all fields do not fit
+# on one CU)
+SWEEP_KERNELS = {"ysweep": ("ysweep",), "xsweep": ("xderiv", "rhs_rk", "xfilter")}
+
+
+def roofline_of(ctx, nx, ny, elem, delem, batch, profile_steps, copy_gbps, traffic=None):
+    """roofline of the dominant canonical sweep (the larger device time per
+    step): its algorithmic bytes over its average time per stage, measured
+    with hipEvents around every launch on the kernels' stream.  The largest
+    single kernel against its own reads and writes is reported beside it
+    (kernel_io_frac).  traffic: HBM bytes per launch of each kernel class
+    from the committed PMC summary (or None)."""
     prof = ctx.profile_run(profile_steps)
     ctx.sync()
     alg = kernel_alg_bytes(nx, ny, elem, delem, batch)
+    salg = sweep_alg_bytes(nx, ny, elem, delem, batch)
+    # time per stage of each sweep: the sum of its kernels' average launches
+    st_ms = {sw: sum(prof[k][0] for k in ks) for sw, ks in SWEEP_KERNELS.items()}
+    dom = "ysweep" if st_ms["ysweep"] >= st_ms["xsweep"] else "xsweep"
+    achieved = salg[dom] / (st_ms[dom] * 1e-3)
+    roof = {"bound": "hbm", "kernel": dom, "kernels": list(SWEEP_KERNELS[dom]),
+            "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "avg_ms_per_stage": st_ms[dom],
+            "alg_bytes_per_stage": salg[dom],
+            "frac_vs_copy_probe": achieved / (copy_gbps * 1e9), "traffic": None}
+    if traffic is not None and all(k in traffic for k in SWEEP_KERNELS[dom]):
+        roof["traffic"] = sum(traffic[k] for k in SWEEP_KERNELS[dom])
+    roof["by_sweep"] = {sw: {"avg_ms_per_stage": st_ms[sw], "alg_bytes_per_stage": salg[sw],
+                             "GB/s": salg[sw] / (st_ms[sw] * 1e-3) / 1e9,
+                             "frac": salg[sw] / (st_ms[sw] * 1e-3) / HBM_PEAK}
+                        for sw in SWEEP_KERNELS}
     share = {k: prof[k][0] * prof[k][1] for k in alg}
-    dom = max(share, key=share.get)
-    dom_ms = prof[dom][0]
-    achieved = alg[dom] / (dom_ms * 1e-3)
-    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK, "avg_launch_ms": dom_ms,
-            "alg_bytes_per_launch": alg[dom],
-            "frac_vs_copy_probe": achieved / (copy_gbps * 1e9)}
+    big = max(share, key=share.get)
+    kio = alg[big] / (prof[big][0] * 1e-3)
+    roof["kernel_io_frac"] = {"kernel": big, "avg_launch_ms": prof[big][0],
+                              "io_bytes_per_launch": alg[big], "GB/s": kio / 1e9,
+                              "frac": kio / HBM_PEAK}
     kernels = {k: {"avg_ms": prof[k][0], "launches_per_step": prof[k][1] / profile_steps,
                    "GB/s": (alg[k] / (prof[k][0] * 1e-3) / 1e9) if k in alg and prof[k][0] > 0
                    else None}
@@ -340,9 +437,18 @@ def batched_forms(m, value, args, copy_gbps):
     enx = eny = 512
     states, dis = ensemble_states(m, enx, eny, 64, 42)
     res = {}
+    # timed steps: the warm-up plus the timed steps make the 500 of the
+    # workload, after which the errors are taken
+    tsteps = max(1, args.ens_steps - args.warmup)
+    target = ERR_STEPS["batched"]
     for prec in ("fp64", "mixed", "fp32"):
-        v, ms_p, c = run_batch(m, states, dis, enx, eny, prec, args.ens_steps, args.warmup,
+        v, ms_p, c = run_batch(m, states, dis, enx, eny, prec, tsteps, args.warmup,
                                graph=bool(args.graph))
+
+        def reset(c=c):
+            for b in range(len(states)):
+                c.set_state(states[b], inst=b)
+        advance_to(c, args.warmup + tsteps, target, reset, graph=bool(args.graph))
         res[prec] = (v, ms_p, c)
     vref, _, cref = res["fp64"]
     checked = (0, 21, 42, 63)
@@ -360,9 +466,9 @@ def batched_forms(m, value, args, copy_gbps):
                         "one launch per kernel, stage and instance group, "
                         + ("hipGraph step" if args.graph else "eager launches"),
             "instances_per_launch": c.subbatch,
-            "cell_updates_per_s": v, "ms_per_step": ms_p / args.ens_steps,
+            "cell_updates_per_s": v, "ms_per_step": ms_p / tsteps, "timed_steps": tsteps,
             "speedup_vs_fp64_2048": v / value, "speedup_vs_fp64_batched": v / vref,
-            "rel_l2_vs_fp64_max": max(errs), "after_steps": args.warmup + args.ens_steps,
+            "rel_l2_vs_fp64_max": max(errs), "after_steps": target,
             "instances_checked": list(checked), "meets_1e-6": bool(max(errs) <= 1e-6),
             "step_roofline_frac": v * B_ALG[prec] / HBM_PEAK, "roofline": roof}
     for prec in res:
@@ -389,6 +495,11 @@ def uneven_forms(m, value, args):
         E.sync()
         ms_p = E.time_run(args.steps, graph=graph)
         E.sync()
+
+        def reset(E=E):
+            for k in range(len(runs)):
+                E.set_state(k, ustates[k])
+        advance_to(E, args.warmup + args.steps, ERR_STEPS["uneven"], reset, graph=graph)
         cells = sum(r["nx"] * r["ny"] for r in runs)
         ures[prec] = (cells * args.steps / (ms_p * 1e-3), ms_p, E)
     checked = (0, 31, 64, 95)
@@ -405,7 +516,8 @@ def uneven_forms(m, value, args):
             "cells": sum(r["nx"] * r["ny"] for r in runs), "groups": E.groups,
             "cell_updates_per_s": v, "ms_per_step": ms_p / args.steps,
             "speedup_vs_fp64_2048": v / value, "speedup_vs_fp64_uneven": v / ures["fp64"][0],
-            "rel_l2_vs_fp64_max": max(errs), "after_steps": args.warmup + args.steps,
+            "rel_l2_vs_fp64_max": max(errs), "after_steps": ERR_STEPS["uneven"],
+            "timed_steps": args.steps,
             "instances_checked": list(checked), "meets_1e-6": bool(max(errs) <= 1e-6),
             "step_roofline_frac": v * B_ALG[prec] / HBM_PEAK}
     for prec in ures:
@@ -483,8 +595,12 @@ def main():
 
     value, ms, ctx = run_single(m, u0, nx, ny, args.precision, args.steps, args.warmup, graph,
                                 hall, di)
-    roofline, kernels = roofline_of(ctx, nx, ny, elem, delem, 1, args.profile_steps, copy_gbps)
-    roofline["traffic"] = load_traffic(roofline["kernel"]) if args.precision == "fp64" else None
+    traffic = None
+    if args.precision == "fp64" and nx == 2048 and ny == 2048:
+        traffic = {k: load_traffic(k) for k in ("ysweep", "xderiv", "rhs_rk", "xfilter")}
+        traffic = {k: v for k, v in traffic.items() if v is not None}
+    roofline, kernels = roofline_of(ctx, nx, ny, elem, delem, 1, args.profile_steps, copy_gbps,
+                                    traffic)
     energy, divb = ctx.diagnostics()
     # the state after warmup + steps (profile_run advanced it further: rerun)
     ref_state = None
@@ -504,7 +620,8 @@ def main():
             args.precision],
         "data": "synthetic (seeded turbulence initial state, seed 42)",
         "config": {"workload": f"hall-mhd2d turbulence {nx}x{ny}, 8 fields, "
-                               f"{args.precision}, d_i={di}, 200-step run",
+                               f"{args.precision}, d_i={di}, {args.steps} timed RK3 steps "
+                               f"after {args.warmup} warm-up steps",
                    "nx": nx, "ny": ny, "graph": graph, "precision": args.precision},
         "roofline": roofline,
         "step_roofline_frac": value * B_ALG[args.precision] / HBM_PEAK,
@@ -517,14 +634,19 @@ def main():
 
     if not args.no_forms and args.precision == "fp64":
         forms = {}
-        nsteps = args.warmup + args.steps
-        # FP64 reference state after warmup + steps, for the error columns
-        _, _, ref = run_single(m, u0, nx, ny, "fp64", args.steps, args.warmup, graph, hall, di)
+        nsteps = ERR_STEPS["single"]
+        # FP64 reference state after the workload's 200 steps, for the error
+        # columns
+        ref = m.Context(nx, ny, hall=hall, di=di, precision=m.FP64)
+        ref.set_state(u0)
+        advance_to(ref, 0, nsteps, None, graph=graph)
         ref_state = ref.state()
         ref.close()
         for prec in ("mixed", "fp32"):
             v, ms_p, c = run_single(m, u0, nx, ny, prec, args.steps, args.warmup, graph, hall,
                                     di)
+            advance_to(c, args.warmup + args.steps, nsteps, lambda c=c: c.set_state(u0),
+                       graph=graph)
             e = field_errors(c.state(), ref_state)
             pe, pd = _bytes(prec)
             roof, kern = roofline_of(c, nx, ny, pe, pd, 1, args.profile_steps, copy_gbps)

@@ -18,6 +18,29 @@
 
 #define ORC_PI 3.14159265358979323846264338327950288
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* threads of the OpenMP timing build (no effect in the plain build) */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int orc_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 static inline double dmax(double a, double b) { return a > b ? a : b; }
 static inline double dmin(double a, double b) { return a < b ? a : b; }
 
@@ -385,14 +408,25 @@ static void x_apply_npe_bc(line_op op, int nx, int ny, double dx, int nv, int np
     orc_coefs c;
     orc_coefs_default(&c);
     /* whole column first: halos are the neighbours' unfiltered values, and
-     * `in` may alias `out` */
+     * `in` may alias `out`.  Columns are independent: with OpenMP (the
+     * CPU-baseline timing build) they are shared over threads, each with its
+     * own buffers, with the same result bit for bit as one thread. */
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
     double *col = (double *)malloc(sizeof(double) * nx);
     double *sol = (double *)malloc(sizeof(double) * nx);
     double H0[2], H1[2];
-    for (int v = 0; v < nv; v++) {
+    long nl = (long)nv * ny;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (long l = 0; l < nl; l++) {
+        int v = (int)(l / ny), j = (int)(l % ny);
         const double *f = in + (size_t)v * nx * ny;
         double *g = out + (size_t)v * nx * ny;
-        for (int j = 0; j < ny; j++) {
+        {
             for (int i = 0; i < nx; i++) col[i] = f[(size_t)i * ny + j];
             for (int b = 0; b < npe; b++) {
                 int off, N;
@@ -412,6 +446,7 @@ static void x_apply_npe_bc(line_op op, int nx, int ny, double dx, int nv, int np
     }
     free(col);
     free(sol);
+    }
 }
 
 static void x_apply(line_op op, int nx, int ny, double dx, int nv,
@@ -470,7 +505,21 @@ void orc_filterX(int nx, int ny, int nv, double *inout)
 /* (stands in for FFTW's r2c/c2r, initialize.c:323-326, which are            */
 /* unnormalised; c2r ignores the imaginary parts of bins 0 and n/2).         */
 /* ------------------------------------------------------------------------ */
-static void cfft(int n, double *re, double *im, int sign)
+/* twiddles w[k] = exp(sign 2 pi i k / n), k < n/2, with the angle formed as
+ * the transform's stages would form it for (k, len = n).  A stage of length
+ * len = n/s takes w[k s]: (2 pi k s)/(len s) rounds exactly like
+ * (2 pi k)/len (scaling by a power of two), so the table gives the same
+ * twiddles, bit for bit, as evaluating cos/sin per stage. */
+static void twiddles(int n, int sign, double *wr, double *wi)
+{
+    for (int k = 0; k < n / 2; k++) {
+        double ang = sign * 2.0 * ORC_PI * (double)k / (double)n;
+        wr[k] = cos(ang);
+        wi[k] = sin(ang);
+    }
+}
+
+static void cfft_t(int n, double *re, double *im, const double *twr, const double *twi)
 {
     /* bit reversal */
     for (int i = 1, j = 0; i < n; i++) {
@@ -483,11 +532,9 @@ static void cfft(int n, double *re, double *im, int sign)
         }
     }
     for (int len = 2; len <= n; len <<= 1) {
-        int half = len >> 1;
+        int half = len >> 1, stride = n / len;
         for (int k = 0; k < half; k++) {
-            /* w = exp(sign * 2 pi i k / len), evaluated per k */
-            double ang = sign * 2.0 * ORC_PI * (double)k / (double)len;
-            double wr = cos(ang), wi = sin(ang);
+            double wr = twr[k * stride], wi = twi[k * stride];
             for (int s = 0; s < n; s += len) {
                 int a = s + k, b = s + k + half;
                 double tr = re[b] * wr - im[b] * wi;
@@ -497,6 +544,16 @@ static void cfft(int n, double *re, double *im, int sign)
             }
         }
     }
+}
+
+static void cfft(int n, double *re, double *im, int sign)
+{
+    double *wr = (double *)calloc(n / 2 + 1, sizeof(double));
+    double *wi = (double *)calloc(n / 2 + 1, sizeof(double));
+    twiddles(n, sign, wr, wi);
+    cfft_t(n, re, im, wr, wi);
+    free(wr);
+    free(wi);
 }
 
 void orc_rfft(int n, const double *x, double *re, double *im)
@@ -542,22 +599,42 @@ void orc_filtY_table(int ny, double afy, double *filtY)
 
 typedef enum { Y_D1, Y_D2, Y_F } y_op;
 
-/* derivY calcDerivs.c:75-109, derivYY :114-150, filteringY filtering.c:40-94 */
+/* derivY calcDerivs.c:75-109, derivYY :114-150, filteringY filtering.c:40-94.
+ * Lines are independent: with OpenMP (the CPU-baseline timing build) they
+ * are shared over threads, each with its own buffers; the result is the
+ * same bit for bit as one thread. */
 static void y_apply(y_op op, int nx, int ny, double Ly, int nv,
                     const double *in, double *out)
 {
-    int nky = ny / 2 + 1;
-    double *re = (double *)malloc(sizeof(double) * nky);
-    double *im = (double *)malloc(sizeof(double) * nky);
-    double *line = (double *)malloc(sizeof(double) * ny);
+    int nky = ny / 2 + 1, h = ny / 2;
     double *filtY = (double *)malloc(sizeof(double) * nky);
+    double *twf_r = (double *)malloc(sizeof(double) * (h + 1));
+    double *twf_i = (double *)malloc(sizeof(double) * (h + 1));
+    double *twb_r = (double *)malloc(sizeof(double) * (h + 1));
+    double *twb_i = (double *)malloc(sizeof(double) * (h + 1));
     orc_filtY_table(ny, 0.45, filtY);  /* afy, initialize.c:44 */
-    for (int v = 0; v < nv; v++) {
-        for (int i = 0; i < nx; i++) {
-            const double *f = in + ((size_t)v * nx + i) * ny;
-            double *g = out + ((size_t)v * nx + i) * ny;
-            memcpy(line, f, sizeof(double) * ny);
-            orc_rfft(ny, line, re, im);
+    twiddles(ny, -1, twf_r, twf_i);
+    twiddles(ny, +1, twb_r, twb_i);
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+        double *r = (double *)malloc(sizeof(double) * ny);
+        double *q = (double *)malloc(sizeof(double) * ny);
+        double *re = (double *)malloc(sizeof(double) * nky);
+        double *im = (double *)malloc(sizeof(double) * nky);
+        long nl = (long)nv * nx;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (long l = 0; l < nl; l++) {
+            const double *f = in + (size_t)l * ny;
+            double *g = out + (size_t)l * ny;
+            /* r2c (orc_rfft) */
+            memcpy(r, f, sizeof(double) * ny);
+            memset(q, 0, sizeof(double) * ny);
+            cfft_t(ny, r, q, twf_r, twf_i);
+            for (int k = 0; k < nky; k++) { re[k] = r[k]; im[k] = q[k]; }
             for (int k = 0; k < nky; k++) {
                 double a = re[k], b = im[k];
                 if (op == Y_D1) {
@@ -573,11 +650,20 @@ static void y_apply(y_op op, int nx, int ny, double Ly, int nv,
                     im[k] = b / ny * filtY[k];
                 }
             }
-            orc_irfft(ny, re, im, line);
-            memcpy(g, line, sizeof(double) * ny);
+            /* c2r (orc_irfft): the imaginary parts of bins 0 and n/2 are
+             * ignored */
+            r[0] = re[0]; q[0] = 0.0;
+            r[h] = re[h]; q[h] = 0.0;
+            for (int k = 1; k < h; k++) {
+                r[k] = re[k];       q[k] = im[k];
+                r[ny - k] = re[k];  q[ny - k] = -im[k];
+            }
+            cfft_t(ny, r, q, twb_r, twb_i);
+            memcpy(g, r, sizeof(double) * ny);
         }
+        free(r); free(q); free(re); free(im);
     }
-    free(re); free(im); free(line); free(filtY);
+    free(filtY); free(twf_r); free(twf_i); free(twb_r); free(twb_i);
 }
 
 void orc_derivY(int nx, int ny, double Ly, int nv, const double *in, double *out)
@@ -602,6 +688,9 @@ void orc_filterY(int nx, int ny, int nv, double *inout)
 void orc_efield(int nx, int ny, const double *uu, double *E)
 {
     size_t n = (size_t)nx * ny;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (size_t p = 0; p < n; p++) {
         double ux = uu[UX * n + p], uy = uu[UY * n + p], uz = uu[UZ * n + p];
         double bx = uu[BX * n + p], by = uu[BY * n + p], bz = uu[BZ * n + p];
@@ -774,6 +863,9 @@ static void rhs_impl_bc(int nx, int ny, double gamma, double resistivity, double
                         const double *ddivBdx, const double *ddivBdy, double *dudt, int open)
 {
     size_t n = (size_t)nx * ny;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (size_t p = 0; p < n; p++) {
         size_t row = p / (size_t)ny;
         if (open && (row == 0 || row == (size_t)nx - 1)) {
@@ -865,6 +957,9 @@ void orc_rhs_glm(int nx, int ny, double gamma, double resistivity, double viscos
 void orc_rk_execute(size_t n, double cc, double dd, double *uu,
                     const double *dudt, double *dudtRK)
 {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (size_t i = 0; i < n; i++) {
         uu[i] = uu[i] + cc * dudt[i] + dd * dudtRK[i];
         dudtRK[i] = dudt[i];
@@ -922,6 +1017,16 @@ double orc_calc_dt(int nx, int ny, double dx, double dy, double gamma,
 {
     size_t n = (size_t)nx * ny;
     double dtmin = -1;
+    /* the minimum does not depend on the order of the cells, so the OpenMP
+     * timing build finds the same value */
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+    double local = -1;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
     for (size_t p = 0; p < n; p++) {
         double cmx, cmy;
         cell_speeds(n, p, dx, dy, gamma, hall, di, uu, &cmx, &cmy);
@@ -935,7 +1040,12 @@ double orc_calc_dt(int nx, int ny, double dx, double dy, double gamma,
             dty = dmin(dty, dy * dy / viscosity);
         }
         double it = dmin(dtx, dty);
-        if (dtmin < 0 || it < dtmin) dtmin = it;
+        if (local < 0 || it < local) local = it;
+    }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+    if (local >= 0 && (dtmin < 0 || local < dtmin)) dtmin = local;
     }
     dtmin = dtmin * cfl;
     /* hysteresis, timeAdvance.c:144-151 */

@@ -20,6 +20,11 @@
 extern "C" {
 #endif
 
+/* threads of the OpenMP timing build (libmhd2d_oracle_omp.so); the plain
+ * build ignores the setting and reports 1 */
+void orc_set_threads(int n);
+int orc_threads(void);
+
 /* ---- scheme coefficients (initialize.c:572-631, parameter.h:24-34) ---- */
 typedef struct {
     double alpha1, a1, b1, c1;          /* 1st derivative, interior */

@@ -12,6 +12,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_SO = os.path.join(ORACLE_DIR, "libmhd2d_oracle.so")
+# the same code built with OpenMP over independent lines and cells (results
+# equal bit for bit); used where a large grid would make the checker slow
+ORACLE_OMP_SO = os.path.join(ORACLE_DIR, "libmhd2d_oracle_omp.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _lib = None
@@ -75,6 +78,8 @@ def lib():
             "orc_sim_run_fixed_dt": (None, [ctypes.c_void_p, i, p]),
             "orc_sim_stage": (None, [ctypes.c_void_p, i]),
             "orc_sim_set_dt": (None, [ctypes.c_void_p, d]),
+            "orc_set_threads": (None, [i]),
+            "orc_threads": (i, []),
             "orc_sim_dt": (d, [ctypes.c_void_p]),
             "orc_sim_time": (d, [ctypes.c_void_p]),
             "orc_sim_steps": (ctypes.c_long, [ctypes.c_void_p]),
@@ -85,6 +90,24 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+class threaded:
+    """Context manager: calls made inside it go to the OpenMP build of the
+    checker (same results bit for bit as the single-threaded build)."""
+
+    def __enter__(self):
+        global _lib, ORACLE_SO
+        if not os.path.exists(ORACLE_OMP_SO):
+            build()
+        self._saved = (_lib, ORACLE_SO)
+        _lib, ORACLE_SO = None, ORACLE_OMP_SO
+        return self
+
+    def __exit__(self, *exc):
+        global _lib, ORACLE_SO
+        _lib, ORACLE_SO = self._saved
+        return False
 
 
 def _c(a):
