@@ -234,6 +234,27 @@ size_t mhd2d_device_bytes(const mhd2d_ctx *c);
 int mhd2d_copy_probe(size_t bytes, int iters, double *gbps);
 const char *mhd2d_last_error(void);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same as mhd2d_create + mhd2d_set_state + mhd2d_run
+ * (nsteps) + mhd2d_get_state, bit for bit, for one FP64 run whose state
+ * [8][nx][ny] ([9][nx][ny] with psi last when divbclean == 2) is in caller
+ * device memory d_uu and is advanced in place, from time 0 (the first dt is
+ * calcDt(0) of the state).  The working arrays (derivatives, E, R_prev,
+ * source-term buffers, run scalars, operator tables) live in one fixed
+ * library array of MHD2D_NOALLOC_BYTES on each GPU the library runs on, held
+ * until the process ends; nothing is allocated or freed per call.  Limits:
+ * precision MHD2D_FP64, batch <= 1, xgemm 0, nx * ny <= MHD2D_NOALLOC_MAX_CELLS
+ * (larger grids: MHD2D_EINVAL with a message).  Every call on one GPU uses
+ * that GPU's array, so calls that overlap on one GPU (other host threads,
+ * other streams) race.  stream: a hipStream_t, or NULL for the null stream;
+ * the call returns when the work is done.  dt_out, time_out (may be NULL):
+ * the dt of the last step and the simulated time reached. */
+#define MHD2D_NOALLOC_MAX_CELLS 65536
+#define MHD2D_NOALLOC_BYTES (28u << 20)
+int mhd2d_run_noalloc(const mhd2d_params *p, double *d_uu, int nsteps, void *stream,
+                      double *dt_out, double *time_out);
+
 /* Ensembles of runs on grids of different sizes.  p[0..count) gives each
  * run's parameters (the batch field is ignored); runs whose parameters agree
  * except for di are grouped into one batched context per grid shape, and a

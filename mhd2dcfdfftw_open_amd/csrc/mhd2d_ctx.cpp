@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1507,3 +1508,138 @@ int mhd2d_ens_time_run(mhd2d_ens* e, int nsteps, int use_graph, double* ms)
 }
 
 }  // extern "C"
+
+// =====================================================================
+// mhd2d_run_noalloc: the context API's set_state + run + get_state for one
+// FP64 run held in caller device memory, allocating nothing.  Every working
+// array lives in `scratch`, one fixed array per GPU for the life of the
+// process.
+// =====================================================================
+namespace {
+
+constexpr size_t NA_MISC = 64u << 10;     // run scalars, GLM Ch / Cp per x block
+constexpr size_t NA_TABLES = 1u << 20;    // operator tables of the grid
+// per cell at most 50 words: state + E + scratch (11, 12 with GLM), dudx and
+// dudy (11 each), R_prev (8, 9), Laplacians (6), div B and its derivatives (3)
+constexpr size_t NA_WORDS = 50;
+static_assert(NA_MISC + NA_TABLES + NA_WORDS * 8 * (size_t)MHD2D_NOALLOC_MAX_CELLS <=
+                  (size_t)MHD2D_NOALLOC_BYTES,
+              "scratch too small for the largest grid");
+
+// Host code never uses this name as an address: each device's address comes
+// from hipGetSymbolAddress (noalloc_base).
+static __device__ __align__(256) double scratch[MHD2D_NOALLOC_BYTES / sizeof(double)];
+
+constexpr int NA_MAXDEV = 64;
+std::mutex g_na_mu;
+void* g_na_base[NA_MAXDEV] = {};
+
+// the address of the current device's copy of scratch, looked up once per
+// device with that device current
+hipError_t noalloc_base(void** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= NA_MAXDEV) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_base[dev]) {
+        void* q = nullptr;
+        e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!q) return hipErrorInvalidSymbol;
+        g_na_base[dev] = q;
+    }
+    *out = g_na_base[dev];
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" int mhd2d_run_noalloc(const mhd2d_params* p, double* d_uu, int nsteps, void* stream,
+                                 double* dt_out, double* time_out)
+{
+    if (!p || !d_uu || nsteps < 0) return fail(MHD2D_EINVAL, "bad argument");
+    if (int rc = check_params(p)) return rc;
+    if (p->precision != MHD2D_FP64 || p->batch > 1 || p->xgemm)
+        return fail(MHD2D_EINVAL,
+                    "mhd2d_run_noalloc runs one FP64 run (precision MHD2D_FP64, batch <= 1) "
+                    "with the tridiagonal x-derivative (xgemm 0)");
+    const size_t n = (size_t)p->nx * p->ny;
+    if (n > (size_t)MHD2D_NOALLOC_MAX_CELLS)
+        return fail(MHD2D_EINVAL,
+                    "mhd2d_run_noalloc: nx * ny exceeds MHD2D_NOALLOC_MAX_CELLS (65536 cells; "
+                    "its working arrays are a fixed 28 MiB per GPU)");
+    void* base = nullptr;
+    HIPCHK(noalloc_base(&base));
+    hipStream_t st = (hipStream_t)stream;
+
+    mhd2d_ctx c;  // views into scratch only: nothing to release
+    c.p = *p;
+    c.elem = c.delem = 8;
+    c.B = 1;
+    c.n = n;
+    c.stream = st;
+    char* const q = (char*)base;
+    c.rs = (RunScalars*)q;
+    if (c.glm()) c.glm_cp = (double*)(q + 4096);  // 2 npe <= nx / 2 doubles: 16 KiB at most
+    HIPCHK(c.tb.build(p->nx, p->ny, p->Lx, p->Ly, 8, SchemeAlphas(), p->xblocks, p->open_x,
+                      q + NA_MISC, NA_TABLES));
+    {
+        // constDiffDivB = 2 max(dx, dy)^2 (initialize.c:17, 154)
+        const double h = std::max(c.tb.dx, c.tb.dy);
+        c.cdiff = 2.0 * h * h;
+    }
+    double* f = (double*)(q + NA_MISC + NA_TABLES);
+    c.u = f;
+    f += c.su();
+    c.dudx = f;
+    f += c.sd();
+    c.dudy = f;
+    f += c.sd();
+    c.rprev = f;
+    f += c.sr();
+    if (c.lap_u() || c.lap_b()) {
+        c.lap = f;
+        f += c.sl();
+    }
+    if (p->divbclean == 1) {
+        c.divb = f;
+        f += c.sb();
+    }
+    if ((size_t)((char*)f - q) > (size_t)MHD2D_NOALLOC_BYTES)
+        return fail(MHD2D_EINVAL, "mhd2d_run_noalloc: working arrays exceed the scratch size");
+
+    // the state: host layout fields 0..7 (psi 8 with GLM) -> slots 0..7 (PSI_SLOT)
+    double* u = (double*)c.u;
+    HIPCHK(hipMemcpyAsync(u, d_uu, 8 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (c.glm())
+        HIPCHK(hipMemcpyAsync(u + PSI_SLOT * n, d_uu + 8 * n, n * sizeof(double),
+                              hipMemcpyDeviceToDevice, st));
+    // a fresh run: time 0, and the first dt is calcDt(0) of the state (mhd.c:58)
+    RunScalars z;
+    std::memset(&z, 0, sizeof(z));
+    z.dtmin_bits = 0x7ff0000000000000ull;
+    z.di = p->di;
+    HIPCHK(hipMemcpyAsync(c.rs, &z, sizeof(z), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    c.pending_filter = false;
+    for (int k = 0; k < nsteps; k++) HIPCHK((step<double, double>(&c, nullptr, nullptr)));
+    // the last stage's y filter (skipped when the run stopped: its stopping
+    // step's y-sweep applied it already)
+    RunScalars r;
+    HIPCHK(hipMemcpyAsync(&r, c.rs, sizeof(r), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (r.stop) c.pending_filter = false;
+    HIPCHK((flush<double, double>(&c)));
+    HIPCHK(hipMemcpyAsync(d_uu, u, 8 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (c.glm())
+        HIPCHK(hipMemcpyAsync(d_uu + 8 * n, u + PSI_SLOT * n, n * sizeof(double),
+                              hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(&r, c.rs, sizeof(r), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (dt_out) *dt_out = r.dt;
+    if (time_out) *time_out = r.step > 0 ? r.time + r.dt : 0.0;
+    if (r.nonfinite) return fail(MHD2D_ENONFINITE, "a CFL limit was not finite (state blew up)");
+    return MHD2D_OK;
+}

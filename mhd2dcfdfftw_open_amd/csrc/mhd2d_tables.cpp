@@ -171,7 +171,8 @@ void block_rows(int kind, int role, int n, const SchemeAlphas& al, double* lo, d
 }  // namespace
 
 hipError_t GridTables::build(int nx_, int ny_, double Lx_, double Ly_, int elem_bytes_,
-                             const SchemeAlphas& al, int xblocks_, int open_x_)
+                             const SchemeAlphas& al, int xblocks_, int open_x_, void* place,
+                             size_t place_bytes)
 {
     release();
     xblocks = xblocks_ < 1 ? 1 : xblocks_;
@@ -324,11 +325,19 @@ hipError_t GridTables::build(int nx_, int ny_, double Lx_, double Ly_, int elem_
             for (size_t r = 0; r < nov; r++)
                 for (int m = 0; m < 9; m++) p[off_oc + (k * nov + r) * 9 + m] = ocoef[k][r].c[m];
     }
-    hipError_t e = hipMalloc(&blob_, h.size() * sizeof(double));
+    // the tables go to caller-given device memory when `place` is set (no
+    // allocation; the caller keeps it alive), else to one allocation
+    void* dst = place;
+    if (place) {
+        if (h.size() * sizeof(double) > place_bytes) return hipErrorInvalidValue;
+    } else {
+        hipError_t e = hipMalloc(&blob_, h.size() * sizeof(double));
+        if (e != hipSuccess) return e;
+        dst = blob_;
+    }
+    hipError_t e = hipMemcpy(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
-    e = hipMemcpy(blob_, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return e;
-    const double* d = (const double*)blob_;
+    const double* d = (const double*)dst;
     for (int k = 0; k < NXKIND; k++) {
         // first row from which l, invd and q stay bitwise equal to their
         // values at row nx-3 up to row nx-2 (the Thomas factors' FP64 fixed

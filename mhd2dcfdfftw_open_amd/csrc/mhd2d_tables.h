@@ -27,10 +27,12 @@ public:
     // end rows and Thomas factors, as a run on xblocks MPI ranks has them.
     // open_x: the first and last blocks end in the reference's open-boundary
     // scheme (boundaryTypeX == 1, mathlib.c:87-159, :219-294, :355-440), and
-    // dx = Lx / (nx - 1); needs xblocks >= 2
+    // dx = Lx / (nx - 1); needs xblocks >= 2.  place (optional): device
+    // memory of place_bytes that receives the tables instead of an
+    // allocation of their own (hipErrorInvalidValue when they do not fit)
     hipError_t build(int nx, int ny, double Lx, double Ly, int elem_bytes,
                      const SchemeAlphas& al = SchemeAlphas(), int xblocks = 1,
-                     int open_x = 0);
+                     int open_x = 0, void* place = nullptr, size_t place_bytes = 0);
     void release();
 
     int nx = 0, ny = 0, elem_bytes = 0;

@@ -16,6 +16,7 @@ PKG = os.path.join(ROOT, "mhd2dcfdfftw_open_amd")
 def declared_symbols():
     src = open(HEADER).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"^\s*#.*$", "", src, flags=re.M)   # preprocessor lines
     funcs = set(re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", src))
     funcs -= {"if", "while", "for", "return", "sizeof"}
     data = set(re.findall(r"extern\s+real\s+(\w+)\s*;", src))
