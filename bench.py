@@ -493,6 +493,28 @@ def uneven_forms(m, value, args):
             E.autotune()
         E.run(args.warmup, graph=graph)
         E.sync()
+        # both launch modes, timed over the same steps from the same state;
+        # the autotuned one is the reported rate
+        chosen = E.launch_mode
+        by_mode = {}
+        for mode in (m.ENS_ONE_LAUNCH, m.ENS_PER_GROUP):
+            try:
+                E.set_launch(mode)
+            except m.MHD2DError:
+                continue
+            for k in range(len(runs)):
+                E.set_state(k, ustates[k])
+            E.run(args.warmup, graph=graph)
+            E.sync()
+            t = E.time_run(args.steps, graph=graph)
+            E.sync()
+            by_mode["one_launch" if mode == m.ENS_ONE_LAUNCH else "per_group"] = {
+                "ms_per_step": t / args.steps, "launches_per_step": E.launches_per_step}
+        E.set_launch(chosen)
+        for k in range(len(runs)):
+            E.set_state(k, ustates[k])
+        E.run(args.warmup, graph=graph)
+        E.sync()
         ms_p = E.time_run(args.steps, graph=graph)
         E.sync()
 
@@ -501,18 +523,23 @@ def uneven_forms(m, value, args):
                 E.set_state(k, ustates[k])
         advance_to(E, args.warmup + args.steps, ERR_STEPS["uneven"], reset, graph=graph)
         cells = sum(r["nx"] * r["ny"] for r in runs)
+        E.bench_modes = (chosen, by_mode)
         ures[prec] = (cells * args.steps / (ms_p * 1e-3), ms_p, E)
     checked = (0, 31, 64, 95)
     refs = {k: ures["fp64"][2].state(k) for k in checked}
     for prec in ("fp64", "mixed", "fp32"):
         v, ms_p, E = ures[prec]
         errs = [max(field_errors(E.state(k), refs[k])) for k in checked]
+        chosen, by_mode = E.bench_modes
+        how = ("one launch per kernel class and stage over the four size groups"
+               if chosen == m.ENS_ONE_LAUNCH else
+               "one batched context per size, launches forked into the four group streams")
         out[f"uneven96_{prec}"] = {
             "workload": "96 turbulence runs, sizes drawn from 256x256, 512x256, "
                         "1024x1024, 2048x512, rms ~ U[0.05, 0.2], d_i ~ U[0, 0.1], "
-                        f"{prec}; one batched context per size, "
-                        + ("one hipGraph per step forking into the four size groups"
-                           if graph else "eager launches on the four group streams"),
+                        f"{prec}; {how}" + ("; hipGraph step" if graph else "; eager launches"),
+            "launch_mode": "one_launch" if chosen == m.ENS_ONE_LAUNCH else "per_group",
+            "launches_per_step": E.launches_per_step, "by_launch_mode": by_mode,
             "cells": sum(r["nx"] * r["ny"] for r in runs), "groups": E.groups,
             "cell_updates_per_s": v, "ms_per_step": ms_p / args.steps,
             "speedup_vs_fp64_2048": v / value, "speedup_vs_fp64_uneven": v / ures["fp64"][0],

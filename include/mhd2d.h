@@ -277,6 +277,21 @@ int mhd2d_ens_get_scalars(mhd2d_ens *e, int inst, double *dt, double *time,
 int mhd2d_ens_diagnostics(mhd2d_ens *e, int inst, double *energy, double *max_divb);
 /* wall time of nsteps of every group (hipEvents around the fork / join) */
 int mhd2d_ens_time_run(mhd2d_ens *e, int nsteps, int use_graph, double *ms);
+/* How a step launches the groups (no counterpart in the reference):
+ * MHD2D_ENS_ONE_LAUNCH runs every group with one launch per kernel class
+ * and stage (13 launches a step, 16 with viscosity or resistivity), each
+ * block finding its group and line length; MHD2D_ENS_PER_GROUP runs each
+ * group's own batched launches on its own stream, forked from and joined
+ * into one master stream.  Results are the same bit for bit.  ONE_LAUNCH is
+ * the default where every group fits it: periodic x, no div B cleaning, one
+ * x block, the tridiagonal x-derivative, the same source terms in every
+ * group, nx in [256, 4096], ny in [128, 1024] (FP32 [128, 2048]); elsewhere
+ * it is refused with MHD2D_EINVAL.  mhd2d_ens_autotune keeps the faster. */
+enum { MHD2D_ENS_PER_GROUP = 0, MHD2D_ENS_ONE_LAUNCH = 1 };
+int mhd2d_ens_set_launch(mhd2d_ens *e, int mode);
+int mhd2d_ens_get_launch(const mhd2d_ens *e);
+/* kernel launches of one step in the current mode */
+int mhd2d_ens_launches_per_step(const mhd2d_ens *e);
 
 /* ===================================================================== */
 /* 3. the reference's run configuration, output files and run loop       */

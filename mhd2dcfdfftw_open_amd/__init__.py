@@ -58,6 +58,8 @@ class DriveReport(ctypes.Structure):
 
 
 DRIVE_AUTO, DRIVE_FUSED, DRIVE_DROPIN = 0, 1, 2
+# how an uneven ensemble launches its shape groups (mhd2d_ens_set_launch)
+ENS_PER_GROUP, ENS_ONE_LAUNCH = 0, 1
 
 
 class Params(ctypes.Structure):
@@ -146,6 +148,10 @@ def lib():
         "mhd2d_ens_get_scalars": (i, [vp, i, p, p, ctypes.POINTER(ll)]),
         "mhd2d_ens_diagnostics": (i, [vp, i, p, p]),
         "mhd2d_ens_time_run": (i, [vp, i, i, p]),
+        "mhd2d_ens_set_launch": (i, [vp, i]),
+        "mhd2d_ens_get_launch": (i, [vp]),
+        "mhd2d_ens_launches_per_step": (i, [vp]),
+        "mhd2d_run_noalloc": (i, [ctypes.POINTER(Params), vp, i, vp, p, p]),
         "mhd2d_read_input": (i, [ctypes.c_char_p, ctypes.POINTER(Input)]),
         "mhd2d_write_input": (i, [ctypes.c_char_p, ctypes.POINTER(Input)]),
         "mhd2d_write_grid": (i, [ctypes.c_char_p, i, i, d, d]),
@@ -418,6 +424,19 @@ class Ensemble:
         ms = ctypes.c_double()
         _check(lib().mhd2d_ens_time_run(self.h, nsteps, int(graph), _ptr1(ms)))
         return ms.value
+
+    def set_launch(self, mode):
+        """ENS_ONE_LAUNCH (one launch per kernel class and stage over every
+        group) or ENS_PER_GROUP (each group's own launches)"""
+        _check(lib().mhd2d_ens_set_launch(self.h, mode))
+
+    @property
+    def launch_mode(self):
+        return lib().mhd2d_ens_get_launch(self.h)
+
+    @property
+    def launches_per_step(self):
+        return lib().mhd2d_ens_launches_per_step(self.h)
 
 
 def _ptr1(c_double):

@@ -132,7 +132,7 @@ DtParams dt_params(const mhd2d_ctx* c)
 }
 
 template <typename T, typename TD>
-hipError_t ysweep(mhd2d_ctx* c, int filter, int derivs, int dt_limits)
+YSweepArgs<T, TD> ysweep_args(mhd2d_ctx* c, int filter, int derivs, int dt_limits)
 {
     YSweepArgs<T, TD> a;
     TD* dudy = (TD*)c->dudy;
@@ -157,7 +157,14 @@ hipError_t ysweep(mhd2d_ctx* c, int filter, int derivs, int dt_limits)
         a.lap_b = c->lap_b();
         a.bs_l = c->sl();
     }
-    return launch_ysweep<T, TD>(a, c->stream, c->B);
+    return a;
+}
+
+template <typename T, typename TD>
+hipError_t ysweep(mhd2d_ctx* c, int filter, int derivs, int dt_limits)
+{
+    return launch_ysweep<T, TD>(ysweep_args<T, TD>(c, filter, derivs, dt_limits), c->stream,
+                                c->B);
 }
 
 // x-derivative of nv fields: in (state type, stride bs_in per instance) ->
@@ -243,7 +250,7 @@ hipError_t xderiv(mhd2d_ctx* c)
 }
 
 template <typename T>
-hipError_t xfilter(mhd2d_ctx* c)
+XSolveArgs<T> xfilter_args(mhd2d_ctx* c)
 {
     XSolveArgs<T> a;
     T* u = (T*)c->u;
@@ -258,11 +265,17 @@ hipError_t xfilter(mhd2d_ctx* c)
     a.bs_in = a.bs_out = c->su();
     a.width = c->xw_f;
     a.remap = c->xr_f;
-    return launch_xsolve<T>(a, c->nv_state(), c->stream, c->B);
+    return a;
+}
+
+template <typename T>
+hipError_t xfilter(mhd2d_ctx* c)
+{
+    return launch_xsolve<T>(xfilter_args<T>(c), c->nv_state(), c->stream, c->B);
 }
 
 template <typename T, typename TD>
-hipError_t rhs_rk(mhd2d_ctx* c, int stage)
+RhsRkArgs<T, TD> rhs_rk_args(mhd2d_ctx* c, int stage)
 {
     RhsRkArgs<T, TD> a;
     TD* dx = (TD*)c->dudx;
@@ -290,6 +303,20 @@ hipError_t rhs_rk(mhd2d_ctx* c, int stage)
         a.viscosity = c->p.viscosity;
         a.resistivity = c->p.resistivity;
     }
+    if (c->glm()) {
+        a.glm = 1;
+        a.npe = c->p.xblocks > 1 ? c->p.xblocks : 1;
+        a.glm_cp = c->glm_cp;
+    }
+    return a;
+}
+
+template <typename T, typename TD>
+hipError_t rhs_rk(mhd2d_ctx* c, int stage)
+{
+    RhsRkArgs<T, TD> a = rhs_rk_args<T, TD>(c, stage);
+    TD* dx = (TD*)c->dudx;
+    TD* dy = (TD*)c->dudy;
     hipError_t e = hipSuccess;
     if (c->divb) {
         // diffusive cleaning (mhd.c:175-182): div B of this stage's
@@ -326,11 +353,6 @@ hipError_t rhs_rk(mhd2d_ctx* c, int stage)
         a.ddivb = db + c->n;
         a.bs_b = c->sb();
         a.cdiff = c->cdiff;
-    }
-    if (c->glm()) {
-        a.glm = 1;
-        a.npe = c->p.xblocks > 1 ? c->p.xblocks : 1;
-        a.glm_cp = c->glm_cp;
     }
     if (e == hipSuccess) e = launch_rhs_rk<T, TD>(a, c->stream, c->B);
     if (e == hipSuccess && c->p.open_x) e = launch_rhs_rk_open<T, TD>(a, c->stream, c->B);
@@ -1272,15 +1294,224 @@ int mhd2d_profile_run(mhd2d_ctx* c, int nsteps, double* out_ms, int* out_launche
 }  // extern "C"
 
 // ------------------------------------------------------------------ ensembles
+// The shape groups can advance per group (each group's batched context on
+// its own stream, forked from and joined back into the master stream) or
+// together: one launch per kernel class and stage over every group
+// (k_*_multi), with the groups' argument sets in a device table.
+struct EnsMulti {
+    void* tab = nullptr;       // device: the argument sets below
+    size_t off_y[3] = {0, 0, 0};  // y-sweeps: stage 0 without / with the pending filter, stages 1-2
+    size_t off_xd = 0, off_x2 = 0, off_xf = 0, off_dt = 0;
+    size_t off_r[3] = {0, 0, 0};  // RHS + RK of each stage
+    MultiMap my, mxd, mx2, mxf, mr, mdt;
+    size_t xdyn = 0;           // dynamic LDS of the x-solves: the largest group's tables
+    int lap = 0;               // viscosity / resistivity in every group
+};
+
 struct mhd2d_ens {
     std::vector<mhd2d_ctx*> groups;
     std::vector<std::pair<int, int>> where;  // instance -> (group, index in group)
     hipStream_t master = nullptr;
     std::vector<hipEvent_t> ev;             // [0] fork, [1 + g] join of group g
     hipGraphExec_t graph = nullptr;
+    EnsMulti multi;
+    bool multi_ok = false;     // every group fits the one-launch path
+    int mode = MHD2D_ENS_PER_GROUP;
 };
 
 namespace {
+
+// Argument tables and block maps of the one-launch path; false when some
+// group does not fit it (cleaning, open x, x blocks, GEMM x-derivative, a
+// line length outside the multi-shape kernels, or groups that differ in
+// having the Laplacian terms).  Groups go largest first, so the longest
+// blocks start first.
+template <typename T, typename TD>
+hipError_t build_multi_t(mhd2d_ens* e, bool* ok)
+{
+    *ok = false;
+    const int ng = (int)e->groups.size();
+    if (ng < 1 || ng > MULTI_MAXG) return hipSuccess;
+    std::vector<int> ord(ng);
+    for (int g = 0; g < ng; g++) ord[g] = g;
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) {
+        const mhd2d_ctx *x = e->groups[a], *y = e->groups[b];
+        return x->n * x->B > y->n * y->B;
+    });
+    const int lap = (e->groups[0]->lap_u() || e->groups[0]->lap_b()) ? 1 : 0;
+    for (auto* c : e->groups) {
+        const int cl = (c->lap_u() || c->lap_b()) ? 1 : 0;
+        if (c->p.divbclean || c->p.open_x || c->p.xblocks > 1 || c->p.xgemm || cl != lap)
+            return hipSuccess;
+        if (!ysweep_multi_rows(c->p.ny, c->elem) || xsolve_multi_code(c->p.nx, c->p.ny, 0, 0) < 0)
+            return hipSuccess;
+    }
+    EnsMulti& M = e->multi;
+    M.lap = lap;
+    std::vector<YSweepArgs<T, TD>> ya[3];
+    std::vector<XSolveArgs<TD, T>> xd, x2;
+    std::vector<XSolveArgs<T>> xf;
+    std::vector<RhsRkArgs<T, TD>> ra[3];
+    std::vector<DtMultiArgs> dta;
+    MultiMap my, mxd, mx2, mxf, mr, mdt;
+    for (MultiMap* m : {&my, &mxd, &mx2, &mxf, &mr, &mdt}) m->ng = ng;
+    size_t xdyn = 0;
+    static const int ideal[9] = {RHO, UX, UY, UZ, BY, BZ, PR, 8, 9};
+    for (int k = 0; k < ng; k++) {
+        mhd2d_ctx* c = e->groups[ord[k]];
+        ya[0].push_back(ysweep_args<T, TD>(c, 0, 1, 1));
+        ya[1].push_back(ysweep_args<T, TD>(c, 1, 1, 1));
+        ya[2].push_back(ysweep_args<T, TD>(c, 1, 1, 0));
+        for (int st = 0; st < 3; st++) ra[st].push_back(rhs_rk_args<T, TD>(c, st));
+        dta.push_back(DtMultiArgs{c->rs, c->p.cfl});
+        int C, W;
+        const int code = xsolve_multi_code(c->p.nx, c->p.ny, &C, &W);
+        const int strips = c->p.ny / W;
+        // x-derivative: the ideal periodic fields (no dBx/dx)
+        XSolveArgs<TD, T> a;
+        for (int v = 0; v < 9; v++) {
+            a.in[v] = (const T*)c->u + ideal[v] * c->n;
+            a.out[v] = (TD*)c->dudx + ideal[v] * c->n;
+        }
+        a.rs = c->rs;
+        a.st = c->tb.st[XD1];
+        a.tb = c->tb.xt[XD1];
+        a.kind = XD1;
+        a.nx = c->p.nx;
+        a.ny = c->p.ny;
+        a.bs_in = c->su();
+        a.bs_out = c->sd();
+        xd.push_back(a);
+        int nv2 = 0;
+        if (lap) {
+            const int f0 = c->lap_u() ? UX : BX, f1 = c->lap_b() ? BZ : UZ;
+            XSolveArgs<TD, T> b;
+            for (int v = 0; v <= f1 - f0; v++) {
+                b.in[v] = (const T*)c->u + (f0 + v) * c->n;
+                b.out[v] = (TD*)c->lap + (f0 + v - UX) * c->n;
+            }
+            b.rs = c->rs;
+            b.st = c->tb.st[XD2];
+            b.tb = c->tb.xt[XD2];
+            b.kind = XD2;
+            b.nx = c->p.nx;
+            b.ny = c->p.ny;
+            b.accumulate = 1;
+            b.bs_in = c->su();
+            b.bs_out = c->sl();
+            x2.push_back(b);
+            nv2 = f1 - f0 + 1;
+        }
+        xf.push_back(xfilter_args<T>(c));
+        xdyn = std::max(xdyn, 3 * (size_t)c->p.nx * std::max(sizeof(T), sizeof(TD)));
+        const int rows = ysweep_multi_rows(c->p.ny, c->elem);
+        const int B = c->B;
+        auto set = [&](MultiMap& m, int per, int aux, int cd) {
+            m.nb[k] = B;
+            m.per[k] = per;
+            m.aux[k] = aux;
+            m.code[k] = cd;
+            m.first[k + 1] = m.first[k] + per * B;
+        };
+        set(my, (c->p.nx + rows - 1) / rows, 0, 0);
+        set(mxd, strips * 9, strips, code);
+        set(mx2, strips * (nv2 > 0 ? nv2 : 1), strips, code);
+        set(mxf, strips * 8, strips, code);
+        {
+            // blocks per instance of the RHS + RK: as launch_rhs_rk sizes
+            // its grid (W cells per thread, 256 threads)
+            const size_t q = c->n / (16 / sizeof(T));
+            size_t b = (q + 255) / 256;
+            if (b > 65535u * 16u) b = 65535u * 16u;
+            set(mr, (int)b, 0, 0);
+        }
+        set(mdt, 1, 0, 0);
+    }
+    // one device table: each argument set 256-byte aligned
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    for (int v = 0; v < 3; v++) M.off_y[v] = take(ya[v].size() * sizeof(YSweepArgs<T, TD>));
+    M.off_xd = take(xd.size() * sizeof(XSolveArgs<TD, T>));
+    M.off_x2 = take(std::max<size_t>(1, x2.size()) * sizeof(XSolveArgs<TD, T>));
+    M.off_xf = take(xf.size() * sizeof(XSolveArgs<T>));
+    for (int st = 0; st < 3; st++) M.off_r[st] = take(ra[st].size() * sizeof(RhsRkArgs<T, TD>));
+    M.off_dt = take(dta.size() * sizeof(DtMultiArgs));
+    std::vector<char> h(off, 0);
+    for (int v = 0; v < 3; v++)
+        std::memcpy(h.data() + M.off_y[v], ya[v].data(), ya[v].size() * sizeof(ya[v][0]));
+    std::memcpy(h.data() + M.off_xd, xd.data(), xd.size() * sizeof(xd[0]));
+    if (!x2.empty()) std::memcpy(h.data() + M.off_x2, x2.data(), x2.size() * sizeof(x2[0]));
+    std::memcpy(h.data() + M.off_xf, xf.data(), xf.size() * sizeof(xf[0]));
+    for (int st = 0; st < 3; st++)
+        std::memcpy(h.data() + M.off_r[st], ra[st].data(), ra[st].size() * sizeof(ra[st][0]));
+    std::memcpy(h.data() + M.off_dt, dta.data(), dta.size() * sizeof(dta[0]));
+    hipError_t r = hipMalloc(&M.tab, off);
+    if (r != hipSuccess) return r;
+    r = hipMemcpy(M.tab, h.data(), off, hipMemcpyHostToDevice);
+    if (r != hipSuccess) return r;
+    M.my = my;
+    M.mxd = mxd;
+    M.mx2 = mx2;
+    M.mxf = mxf;
+    M.mr = mr;
+    M.mdt = mdt;
+    M.xdyn = xdyn;
+    *ok = true;
+    return hipSuccess;
+}
+
+hipError_t build_multi(mhd2d_ens* e, bool* ok)
+{
+    const mhd2d_ctx* c = e->groups[0];
+    if (c->elem == 4) return build_multi_t<float, float>(e, ok);
+    return c->delem == 8 ? build_multi_t<double, double>(e, ok) : build_multi_t<double, float>(e, ok);
+}
+
+// one RK3 step of every group with one launch per kernel class and stage,
+// on the master stream; every group must have the same pending filter
+template <typename T, typename TD>
+hipError_t ens_step_multi_t(mhd2d_ens* e)
+{
+    EnsMulti& M = e->multi;
+    char* tb = (char*)M.tab;
+    const bool pf = e->groups[0]->pending_filter;
+    const int mode = M.lap;
+    hipStream_t s = e->master;
+    hipError_t r = hipSuccess;
+    for (int st = 0; st < 3 && r == hipSuccess; st++) {
+        const size_t oy = st > 0 ? M.off_y[2] : (pf ? M.off_y[1] : M.off_y[0]);
+        r = launch_ysweep_multi<T, TD>((const YSweepArgs<T, TD>*)(tb + oy), M.my, mode, s);
+        if (r == hipSuccess && st == 0)
+            r = launch_dt_finalize_multi((const DtMultiArgs*)(tb + M.off_dt), M.mdt, s);
+        if (r == hipSuccess)
+            r = launch_xsolve_multi<TD, T>((const XSolveArgs<TD, T>*)(tb + M.off_xd), M.mxd, XD1,
+                                           M.xdyn, s);
+        if (r == hipSuccess && M.lap)
+            r = launch_xsolve_multi<TD, T>((const XSolveArgs<TD, T>*)(tb + M.off_x2), M.mx2, XD2,
+                                           M.xdyn, s);
+        if (r == hipSuccess)
+            r = launch_rhs_rk_multi<T, TD>((const RhsRkArgs<T, TD>*)(tb + M.off_r[st]), M.mr,
+                                           mode, s);
+        if (r == hipSuccess)
+            r = launch_xsolve_multi<T, T>((const XSolveArgs<T>*)(tb + M.off_xf), M.mxf, XFILT,
+                                          M.xdyn, s);
+    }
+    if (r == hipSuccess)
+        for (auto* c : e->groups) c->pending_filter = true;
+    return r;
+}
+
+hipError_t ens_step_multi(mhd2d_ens* e)
+{
+    const mhd2d_ctx* c = e->groups[0];
+    if (c->elem == 4) return ens_step_multi_t<float, float>(e);
+    return c->delem == 8 ? ens_step_multi_t<double, double>(e)
+                         : ens_step_multi_t<double, float>(e);
+}
 
 bool same_group(const mhd2d_params& a, const mhd2d_params& b)
 {
@@ -1310,13 +1541,50 @@ hipError_t fork_from_master(mhd2d_ens* e)
     return r;
 }
 
+// Before a one-launch step every group needs the same pending filter: a
+// group whose y filter is pending while another's is not (a state was set
+// on one group only) applies it now, on the master stream.  Not during
+// graph capture: the caller arranges the first (eager) step.
+hipError_t ens_align_filters(mhd2d_ens* e)
+{
+    bool any_fresh = false;
+    for (auto* c : e->groups) any_fresh = any_fresh || !c->pending_filter;
+    if (!any_fresh) return hipSuccess;
+    for (auto* c : e->groups) {
+        if (!c->pending_filter) continue;
+        hipStream_t own = c->stream;
+        c->stream = e->master;
+        hipError_t r = do_flush(c);
+        c->stream = own;
+        if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+}
+
 hipError_t ens_step_all(mhd2d_ens* e)
 {
+    if (e->mode == MHD2D_ENS_ONE_LAUNCH) return ens_step_multi(e);
     for (auto* c : e->groups) {
         hipError_t r = do_step(c, nullptr, nullptr);
         if (r != hipSuccess) return r;
     }
     return hipSuccess;
+}
+
+// kernel launches of one step (after the first) in the current mode
+int ens_launches(const mhd2d_ens* e)
+{
+    if (e->mode == MHD2D_ENS_ONE_LAUNCH) return 13 + (e->multi.lap ? 3 : 0);
+    int n = 0;
+    for (const auto* c : e->groups) {
+        int per = 13;
+        if (c->lap) per += 3;
+        if (c->divb) per += 9;
+        if (c->glm()) per += 1;
+        if (c->p.open_x) per += 3;
+        n += per * nsub(c);
+    }
+    return n;
 }
 
 }  // namespace
@@ -1367,13 +1635,37 @@ int mhd2d_ens_create(const mhd2d_params* p, int count, mhd2d_ens** out)
     e->ev.resize(1 + e->groups.size(), nullptr);
     for (auto& v : e->ev)
         if (r == hipSuccess) r = hipEventCreateWithFlags(&v, hipEventDisableTiming);
+    if (r == hipSuccess) r = build_multi(e, &e->multi_ok);
     if (r != hipSuccess) {
         mhd2d_ens_destroy(e);
         return fail(MHD2D_EHIP, "ensemble setup", r);
     }
+    e->mode = e->multi_ok ? MHD2D_ENS_ONE_LAUNCH : MHD2D_ENS_PER_GROUP;
     *out = e;
     return MHD2D_OK;
 }
+
+int mhd2d_ens_set_launch(mhd2d_ens* e, int mode)
+{
+    if (!e) return fail(MHD2D_EINVAL, "null argument");
+    if (mode != MHD2D_ENS_PER_GROUP && mode != MHD2D_ENS_ONE_LAUNCH)
+        return fail(MHD2D_EINVAL, "mode must be MHD2D_ENS_PER_GROUP or MHD2D_ENS_ONE_LAUNCH");
+    if (mode == MHD2D_ENS_ONE_LAUNCH && !e->multi_ok)
+        return fail(MHD2D_EINVAL,
+                    "one launch per kernel class needs every group periodic, without div B "
+                    "cleaning, x blocks or the GEMM x-derivative, with the same source terms, "
+                    "nx in [256, 4096] and ny in [128, 1024] (FP32: [128, 2048])");
+    if (mode != e->mode && e->graph) {
+        (void)hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+    e->mode = mode;
+    return MHD2D_OK;
+}
+
+int mhd2d_ens_get_launch(const mhd2d_ens* e) { return e ? e->mode : -1; }
+
+int mhd2d_ens_launches_per_step(const mhd2d_ens* e) { return e ? ens_launches(e) : -1; }
 
 void mhd2d_ens_destroy(mhd2d_ens* e)
 {
@@ -1384,6 +1676,7 @@ void mhd2d_ens_destroy(mhd2d_ens* e)
         if (v) (void)hipEventDestroy(v);
     for (auto* c : e->groups) mhd2d_destroy(c);
     if (e->master) (void)hipStreamDestroy(e->master);
+    if (e->multi.tab) (void)hipFree(e->multi.tab);
     delete e;
 }
 
@@ -1431,7 +1724,10 @@ int mhd2d_ens_sync(mhd2d_ens* e)
 }
 
 // every group tunes its own x-solve shapes (its grid, precision and batch);
-// choice[4 g ..] receives group g's
+// choice[4 g ..] receives group g's.  Where the one-launch path fits, the
+// two launch modes are then timed as whole steps of the ensemble from the
+// same saved state (restored afterwards); one launch per kernel class is
+// kept unless the per-group launches are faster by more than 1 %.
 int mhd2d_ens_autotune(mhd2d_ens* e, int reps, int* choice)
 {
     if (!e || reps < 1) return fail(MHD2D_EINVAL, "bad argument");
@@ -1444,6 +1740,71 @@ int mhd2d_ens_autotune(mhd2d_ens* e, int reps, int* choice)
         (void)hipGraphExecDestroy(e->graph);
         e->graph = nullptr;
     }
+    if (!e->multi_ok) return MHD2D_OK;
+    // snapshot of every group: state, R_prev, run scalars, pending filter
+    struct Snap {
+        char* p = nullptr;
+        size_t ub = 0, rb = 0, sb = 0;
+        bool pf = false;
+    };
+    std::vector<Snap> sn(e->groups.size());
+    auto release = [&]() {
+        for (auto& q : sn)
+            if (q.p) (void)hipFree(q.p);
+    };
+    hipError_t r = hipSuccess;
+    for (size_t g = 0; g < e->groups.size() && r == hipSuccess; g++) {
+        mhd2d_ctx* c = e->groups[g];
+        Snap& q = sn[g];
+        q.ub = c->su() * c->elem * c->B;
+        q.rb = c->sr() * c->delem * c->B;
+        q.sb = sizeof(RunScalars) * c->B;
+        q.pf = c->pending_filter;
+        r = hipMalloc(&q.p, q.ub + q.rb + q.sb);
+        if (r == hipSuccess) r = hipMemcpyAsync(q.p, c->u, q.ub, hipMemcpyDeviceToDevice, e->master);
+        if (r == hipSuccess)
+            r = hipMemcpyAsync(q.p + q.ub, c->rprev, q.rb, hipMemcpyDeviceToDevice, e->master);
+        if (r == hipSuccess)
+            r = hipMemcpyAsync(q.p + q.ub + q.rb, c->rs, q.sb, hipMemcpyDeviceToDevice, e->master);
+    }
+    auto restore = [&]() {
+        hipError_t x = hipSuccess;
+        for (size_t g = 0; g < e->groups.size() && x == hipSuccess; g++) {
+            mhd2d_ctx* c = e->groups[g];
+            const Snap& q = sn[g];
+            x = hipMemcpyAsync(c->u, q.p, q.ub, hipMemcpyDeviceToDevice, e->master);
+            if (x == hipSuccess)
+                x = hipMemcpyAsync(c->rprev, q.p + q.ub, q.rb, hipMemcpyDeviceToDevice, e->master);
+            if (x == hipSuccess)
+                x = hipMemcpyAsync(c->rs, q.p + q.ub + q.rb, q.sb, hipMemcpyDeviceToDevice,
+                                   e->master);
+            c->pending_filter = q.pf;
+        }
+        if (x == hipSuccess) x = fork_from_master(e);
+        if (x == hipSuccess) x = hipStreamSynchronize(e->master);
+        return x;
+    };
+    float best[2] = {1e30f, 1e30f};
+    const int saved_mode = e->mode;
+    for (int it = 0; it < 4 && r == hipSuccess; it++) {
+        const int mode = it % 2 ? MHD2D_ENS_PER_GROUP : MHD2D_ENS_ONE_LAUNCH;
+        e->mode = mode;
+        r = restore();
+        double ms = 0;
+        if (r == hipSuccess) {
+            // one untimed step (the first has no pending filter), then reps
+            if (mhd2d_ens_run(e, 1, 0) != MHD2D_OK) r = hipErrorUnknown;
+            if (r == hipSuccess && mhd2d_ens_time_run(e, reps, 0, &ms) != MHD2D_OK)
+                r = hipErrorUnknown;
+        }
+        if (r == hipSuccess) best[mode] = std::min(best[mode], (float)ms);
+    }
+    e->mode = saved_mode;
+    if (r == hipSuccess) r = restore();
+    release();
+    if (r != hipSuccess) return fail(MHD2D_EHIP, "ensemble autotune", r);
+    e->mode = best[MHD2D_ENS_PER_GROUP] < 0.99f * best[MHD2D_ENS_ONE_LAUNCH] ? MHD2D_ENS_PER_GROUP
+                                                                             : MHD2D_ENS_ONE_LAUNCH;
     return MHD2D_OK;
 }
 
@@ -1459,6 +1820,7 @@ int mhd2d_ens_run(mhd2d_ens* e, int nsteps, int use_graph)
     int k = 0;
     bool fresh = false;
     for (auto* c : e->groups) fresh = fresh || !c->pending_filter;
+    if (e->mode == MHD2D_ENS_ONE_LAUNCH) HIPCHK(ens_align_filters(e));
     if (!use_graph || fresh) {
         // eager steps (and the first step, which has no pending y filter)
         const int ne = use_graph ? 1 : nsteps;

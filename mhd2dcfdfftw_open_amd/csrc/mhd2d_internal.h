@@ -225,6 +225,26 @@ struct RhsRkArgs {           // fused RHS + low-storage RK update (driver)
     int nx = 0, ny = 0;
 };
 
+// ---------------------------------------------------------------- shape groups
+// One launch over the shape groups of an uneven ensemble: group g owns the
+// blocks [first[g], first[g + 1]) of a flattened grid, holds nb[g]
+// instances, per[g] blocks per instance; aux[g] and code[g] are kernel-class
+// specific (x-solves: column strips per field and the (chunk, width) code).
+constexpr int MULTI_MAXG = 16;
+struct MultiMap {
+    int ng = 0;
+    int first[MULTI_MAXG + 1] = {};
+    int nb[MULTI_MAXG] = {};
+    int per[MULTI_MAXG] = {};
+    int aux[MULTI_MAXG] = {};
+    int code[MULTI_MAXG] = {};
+};
+
+struct DtMultiArgs {
+    RunScalars* rs;  // the group's instances
+    double cfl;
+};
+
 // ---------------------------------------------------------------- launchers
 template <typename T, typename TI = T>
 hipError_t launch_xsolve(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int nb = 1);
@@ -277,6 +297,24 @@ hipError_t launch_diag(const T* u, const T* dbx, const T* dby, double* partial,
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 template <typename T>
 hipError_t launch_add(const T* a, const T* b, T* out, size_t n, hipStream_t s);
+
+// shape-group launches (tab: device array of one argument set per group)
+template <typename T, typename TD>
+hipError_t launch_ysweep_multi(const YSweepArgs<T, TD>* tab, const MultiMap& m, int mode,
+                               hipStream_t s);
+template <typename T, typename TI>
+hipError_t launch_xsolve_multi(const XSolveArgs<T, TI>* tab, const MultiMap& m, int kind,
+                               size_t dyn_lds, hipStream_t s);
+template <typename T, typename TD>
+hipError_t launch_rhs_rk_multi(const RhsRkArgs<T, TD>* tab, const MultiMap& m, int mode,
+                               hipStream_t s);
+hipError_t launch_dt_finalize_multi(const DtMultiArgs* tab, const MultiMap& m, hipStream_t s);
+// rows per y-sweep workgroup of a shape-group launch (0: ny not supported
+// there); elem_bytes: the state word
+int ysweep_multi_rows(int ny, int elem_bytes);
+// (chunk, width) code of an x-solve in a shape-group launch (1024 threads:
+// width = 1024 / (nx / chunk)), or -1 when the grid does not fit it
+int xsolve_multi_code(int nx, int ny, int* chunk, int* width);
 
 // chunk length (rows per thread) and strip width used by the x solves
 // (elem_bytes: size of the input word)

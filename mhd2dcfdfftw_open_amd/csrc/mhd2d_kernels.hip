@@ -169,11 +169,33 @@ __device__ __forceinline__ void xcd_tile(int& x, int& y, int& z, bool remap)
 // BLK: x is split into blocks (and may have open ends): the rows marked in
 // tb.omask take explicit stencils over f[i-4 .. i+4] (tb.ocoef); otherwise
 // rows 0 and nx-1 are the only end rows.
-template <typename T, typename TI, int C, int W, int KIND, bool BLK>
-__global__ void __launch_bounds__(1024, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, TI> a)
+// per-chunk carries, row g of W (+1 pad: the scans read them with
+// consecutive lanes one row apart, which the pad spreads over the banks);
+// up to 1024 threads: 32 chunks of 32 columns, 64 of 16 or 128 of 8
+template <int W>
+struct XShared {
+    static constexpr int WP = W + 1;
+    static constexpr int GMAX = 1024 / W;
+    static constexpr int WORDS = WP * GMAX;
+};
+constexpr int xshared_max_words()
 {
-    int bx, v, z;  // column strip, field, instance of a batch
-    xcd_tile(bx, v, z, a.remap > 0);
+    return XShared<8>::WORDS > XShared<16>::WORDS
+               ? (XShared<8>::WORDS > XShared<32>::WORDS ? XShared<8>::WORDS : XShared<32>::WORDS)
+               : (XShared<16>::WORDS > XShared<32>::WORDS ? XShared<16>::WORDS
+                                                           : XShared<32>::WORDS);
+}
+
+// One workgroup's compact solves: column strip bx, field v, instance z of
+// the launch described by `a`.  sh_in / sh_fw / sh_bw hold at least
+// XShared<W>::WORDS words, sh_half W words; x_dyn the factor tables (3 nx
+// words) when xsolve_lds_tables(C).
+template <typename T, typename TI, int C, int W, int KIND, bool BLK>
+__device__ __forceinline__ void xsolve_body(const XSolveArgs<T, TI>& a, int bx, int v, int z,
+                                            T* __restrict__ sh_in, T* __restrict__ sh_fw,
+                                            T* __restrict__ sh_bw, T* __restrict__ sh_half,
+                                            unsigned char* x_dyn)
+{
     if (a.rs && a.rs[z].stop) return;  // the run stopped (dt < 1e-8): uniform per block
     const int G = blockDim.x / W;
     const int c = threadIdx.x % W;
@@ -185,15 +207,7 @@ __global__ void __launch_bounds__(1024, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, TI>
     T* __restrict__ o = a.out[v] + z * a.bs_out + j;
     const int s = g * C;
 
-    // per-chunk carries, row g of W (+1 pad: the scans read them with
-    // consecutive lanes one row apart, which the pad spreads over the banks)
-    constexpr int WP = W + 1;
-    // up to 1024 threads: 64 chunks of 16 columns or 128 chunks of 8
-    constexpr int GMAX = 1024 / W;
-    __shared__ T sh_in[WP * GMAX];
-    __shared__ T sh_fw[WP * GMAX];
-    __shared__ T sh_bw[WP * GMAX];
-    __shared__ T sh_half[W];  // a column's value at the end of its lower 64 chunks
+    constexpr int WP = XShared<W>::WP;
 
     // rows s-3 .. s+C+2 of this column; only the 3-row halos can wrap in x
     T w[C + 6];
@@ -228,7 +242,6 @@ __global__ void __launch_bounds__(1024, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, TI>
     const FT* ID;
     const FT* Q;
     if constexpr (xsolve_lds_tables(C)) {
-        extern __shared__ unsigned char x_dyn[];
         T* const tl = reinterpret_cast<T*>(x_dyn);
         T* const tid = tl + nx;
         T* const tqq = tid + nx;
@@ -423,6 +436,19 @@ __global__ void __launch_bounds__(1024, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, TI>
     }
 }
 
+template <typename T, typename TI, int C, int W, int KIND, bool BLK>
+__global__ void __launch_bounds__(1024, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, TI> a)
+{
+    int bx, v, z;  // column strip, field, instance of a batch
+    xcd_tile(bx, v, z, a.remap > 0);
+    __shared__ T sh_in[XShared<W>::WORDS];
+    __shared__ T sh_fw[XShared<W>::WORDS];
+    __shared__ T sh_bw[XShared<W>::WORDS];
+    __shared__ T sh_half[W];  // a column's value at the end of its lower 64 chunks
+    extern __shared__ unsigned char x_dyn[];
+    xsolve_body<T, TI, C, W, KIND, BLK>(a, bx, v, z, sh_in, sh_fw, sh_bw, sh_half, x_dyn);
+}
+
 #ifndef MHD2D_X_W
 #define MHD2D_X_W 16
 #endif
@@ -603,25 +629,21 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) k_yapply(YApplyArgs<T
 // it writes is re-read by the same thread only (E and the CFL limits).
 // MODE 0: ideal; 1: with the second derivatives of the source terms; 2: also
 // the GLM psi line (separate instances, so each keeps its own registers)
+// One workgroup of the y-sweep: rows bx * ROWS .. of instance z; s_re and
+// s_im hold ROWS * LDSW words each, red 16 doubles.
 template <typename T, typename TD, int N, int MODE>
-__global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep(YSweepArgs<T, TD> a)
+__device__ __forceinline__ void ysweep_body(const YSweepArgs<T, TD>& a, int bx, int z,
+                                            T* __restrict__ s_re, T* __restrict__ s_im,
+                                            double* red)
 {
     constexpr bool LAP = MODE >= 1, GLM = MODE == 2;
     using Y = YShape<T, N>;
     constexpr int TPR = Y::TPR;
     constexpr int E = Y::E;
-    __shared__ T s_re[Y::ROWS * Y::LDSW];
-    __shared__ T s_im[Y::ROWS * Y::LDSW];
-    __shared__ double red[16];
-    // instance of a batch, taken last to first: the x-filter before this
-    // sweep ends on the last instance, whose data the 256 MB Infinity Cache
-    // still holds (with the same order in k_rhs_rk, -4 % step time for
-    // 2 x 2048^2 and 8 x 1024^2 batches; the results do not depend on it)
-    const int z = (int)(gridDim.y - 1 - blockIdx.y);
     RunScalars* const rs = a.rs + z;
     if (rs->stop) return;  // the run stopped (dt < 1e-8): uniform per block
     const int t = threadIdx.x % TPR, rl = threadIdx.x / TPR;
-    const int i_raw = blockIdx.x * Y::ROWS + rl;
+    const int i_raw = bx * Y::ROWS + rl;
     const bool active = i_raw < a.nx;
     const size_t row = (size_t)(active ? i_raw : a.nx - 1) * N;
     const size_t n = (size_t)a.nx * N;
@@ -796,6 +818,21 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep
         }
         block_min_dt(mn, active, rs, red);
     }
+}
+
+template <typename T, typename TD, int N, int MODE>
+__global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep(YSweepArgs<T, TD> a)
+{
+    using Y = YShape<T, N>;
+    __shared__ T s_re[Y::ROWS * Y::LDSW];
+    __shared__ T s_im[Y::ROWS * Y::LDSW];
+    __shared__ double red[16];
+    // instance of a batch, taken last to first: the x-filter before this
+    // sweep ends on the last instance, whose data the 256 MB Infinity Cache
+    // still holds (with the same order in k_rhs_rk, -4 % step time for
+    // 2 x 2048^2 and 8 x 1024^2 batches; the results do not depend on it)
+    const int z = (int)(gridDim.y - 1 - blockIdx.y);
+    ysweep_body<T, TD, N, MODE>(a, blockIdx.x, z, s_re, s_im, red);
 }
 
 template <typename T, typename TD, int N>
@@ -1071,16 +1108,12 @@ template <typename T, typename TD>
 struct RhsRkWaves {
     static constexpr int MAX = (sizeof(T) == 8 && sizeof(TD) == 8) ? 2 : 10;
 };
+// RHS + RK over the cells of instance z with blocks bid = 0 .. nblk-1
 template <typename T, typename TD, int MODE>
-__global__ void __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu(1, RhsRkWaves<T, TD>::MAX))) k_rhs_rk(RhsRkArgs<T, TD> a)
+__device__ __forceinline__ void rhs_rk_body(const RhsRkArgs<T, TD>& a, int z, int bid, int nblk)
 {
     constexpr bool LAP = MODE >= 1, GLM = MODE == 2;
     constexpr int W = 16 / sizeof(T);
-    // instance of a batch, last to first like k_ysweep (the x solves between
-    // them take the instances first to last), so that the next kernel
-    // starts on the instance this one touched last
-    const int z = (int)(gridDim.y - 1 - blockIdx.y);
     const RunScalars* const rs = a.rs + z;
     if (rs->stop) return;  // the run stopped (dt < 1e-8)
     const size_t n = a.n;
@@ -1097,8 +1130,8 @@ __global__ void __launch_bounds__(256)
     // with open x ends the first and last rows belong to the boundary kernel
     const size_t q0 = a.open_x ? (size_t)a.ny / W : 0;
     const size_t q1 = a.open_x ? (n - (size_t)a.ny) / W : n / W;
-    for (size_t q = q0 + blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < q1;
-         q += (size_t)gridDim.x * blockDim.x) {
+    for (size_t q = q0 + (size_t)bid * blockDim.x + threadIdx.x; q < q1;
+         q += (size_t)nblk * blockDim.x) {
         const size_t p = q * W;
         Pack<T, W> u[8];
         Pack<TD, W> dx[8], dy[8], ex[2], ey[2];
@@ -1226,6 +1259,17 @@ __global__ void __launch_bounds__(256)
             if (a.write_rprev) stp<MHD2D_NT_R, W>(rpz + 8 * n + p, rs_out);
         }
     }
+}
+
+template <typename T, typename TD, int MODE>
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(1, RhsRkWaves<T, TD>::MAX))) k_rhs_rk(RhsRkArgs<T, TD> a)
+{
+    // instance of a batch, last to first like k_ysweep (the x solves between
+    // them take the instances first to last), so that the next kernel
+    // starts on the instance this one touched last
+    const int z = (int)(gridDim.y - 1 - blockIdx.y);
+    rhs_rk_body<T, TD, MODE>(a, z, blockIdx.x, gridDim.x);
 }
 
 // calcCharacteristicsX (calcRHS.c:253-495) at one open boundary cell:
@@ -1411,9 +1455,8 @@ __global__ void __launch_bounds__(256) k_dt_limits(const T* __restrict__ u, DtPa
 // dt for the coming step: cfl * global min, with the +-2 % hysteresis of
 // timeAdvance.c:144-151 against the previous step's dt (0 before the first
 // step, mhd.c:58), then rkInit's coefficients (timeAdvance.c:20-27).
-__global__ void k_dt_finalize(RunScalars* rs0, double cfl)
+__device__ __forceinline__ void dt_finalize_body(RunScalars* rs, double cfl)
 {
-    RunScalars* rs = rs0 + blockIdx.x;  // one block per instance
     if (rs->stop) return;
     const double m = __longlong_as_double((long long)rs->dtmin_bits) * cfl;
     const bool first = rs->step == 0;
@@ -1439,6 +1482,11 @@ __global__ void k_dt_finalize(RunScalars* rs0, double cfl)
     rs->dd[2] = -5. / 12. * dt;
     rs->step += 1;
     rs->dtmin_bits = 0x7ff0000000000000ull;  // +inf for the next reduction
+}
+
+__global__ void k_dt_finalize(RunScalars* rs0, double cfl)
+{
+    dt_finalize_body(rs0 + blockIdx.x, cfl);  // one block per instance
 }
 
 __global__ void k_dt_reset(RunScalars* rs)
@@ -1650,6 +1698,209 @@ hipError_t launch_diag(const T* u, const T* dbx, const T* dby, double* partial, 
     return hipGetLastError();
 }
 
+// ============================================================== shape groups
+// One launch covers several grid shapes (an uneven ensemble): each block
+// finds its group from the flattened block index and runs that group's
+// code path, specialised on its line length; the shared memory is the
+// largest any of them needs.  Per cell and line the arithmetic is that of
+// the single-shape kernels, so results are the same bit for bit.
+__device__ __forceinline__ int multi_group(const MultiMap& m, int b)
+{
+    int g = 0;
+    while (g + 1 < m.ng && b >= m.first[g + 1]) g++;
+    return g;
+}
+
+// line lengths a 128-thread y-sweep workgroup covers
+template <typename T, int N>
+constexpr bool ymulti_ok()
+{
+    return YShape<T, N>::THREADS == 128;
+}
+template <typename T>
+constexpr int ymulti_lds()
+{
+    int m = 0;
+    if (ymulti_ok<T, 128>() && YShape<T, 128>::ROWS * YShape<T, 128>::LDSW > m)
+        m = YShape<T, 128>::ROWS * YShape<T, 128>::LDSW;
+    if (ymulti_ok<T, 256>() && YShape<T, 256>::ROWS * YShape<T, 256>::LDSW > m)
+        m = YShape<T, 256>::ROWS * YShape<T, 256>::LDSW;
+    if (ymulti_ok<T, 512>() && YShape<T, 512>::ROWS * YShape<T, 512>::LDSW > m)
+        m = YShape<T, 512>::ROWS * YShape<T, 512>::LDSW;
+    if (ymulti_ok<T, 1024>() && YShape<T, 1024>::ROWS * YShape<T, 1024>::LDSW > m)
+        m = YShape<T, 1024>::ROWS * YShape<T, 1024>::LDSW;
+    if (ymulti_ok<T, 2048>() && YShape<T, 2048>::ROWS * YShape<T, 2048>::LDSW > m)
+        m = YShape<T, 2048>::ROWS * YShape<T, 2048>::LDSW;
+    return m;
+}
+
+int ysweep_multi_rows(int ny, int elem_bytes)
+{
+    switch (ny) {
+    case 128: return elem_bytes == 8 ? YShape<double, 128>::ROWS : YShape<float, 128>::ROWS;
+    case 256: return elem_bytes == 8 ? YShape<double, 256>::ROWS : YShape<float, 256>::ROWS;
+    case 512: return elem_bytes == 8 ? YShape<double, 512>::ROWS : YShape<float, 512>::ROWS;
+    case 1024: return elem_bytes == 8 ? YShape<double, 1024>::ROWS : YShape<float, 1024>::ROWS;
+    case 2048: return elem_bytes == 8 ? 0 : YShape<float, 2048>::ROWS;
+    default: return 0;
+    }
+}
+
+template <typename T, typename TD, int MODE>
+__global__ void __launch_bounds__(128) MHD2D_Y_ATTR
+    k_ysweep_multi(const YSweepArgs<T, TD>* __restrict__ tab, MultiMap m)
+{
+    __shared__ T s_re[ymulti_lds<T>()];
+    __shared__ T s_im[ymulti_lds<T>()];
+    __shared__ double red[16];
+    const int b = blockIdx.x;
+    const int g = multi_group(m, b);
+    const YSweepArgs<T, TD>& a = tab[g];
+    const int local = b - m.first[g];
+    const int bx = local % m.per[g];
+    // instances last to first, as in k_ysweep
+    const int z = m.nb[g] - 1 - local / m.per[g];
+    switch (a.ny) {
+    case 128: ysweep_body<T, TD, 128, MODE>(a, bx, z, s_re, s_im, red); break;
+    case 256: ysweep_body<T, TD, 256, MODE>(a, bx, z, s_re, s_im, red); break;
+    case 512: ysweep_body<T, TD, 512, MODE>(a, bx, z, s_re, s_im, red); break;
+    case 1024: ysweep_body<T, TD, 1024, MODE>(a, bx, z, s_re, s_im, red); break;
+    case 2048:
+        if constexpr (ymulti_ok<T, 2048>()) ysweep_body<T, TD, 2048, MODE>(a, bx, z, s_re, s_im, red);
+        break;
+    default: break;
+    }
+}
+
+int xsolve_multi_code(int nx, int ny, int* chunk, int* width)
+{
+    int C, W, G;
+    xsolve_shape(nx, 8, &C, &W, &G);
+    const int w = G > 0 ? 1024 / G : 0;
+    int code = -1;
+    if (C == 8 && w == 32) code = 0;
+    else if (C == 8 && w == 16) code = 1;
+    else if (C == 16 && w == 16) code = 2;
+    else if (C == 32 && w == 16) code = 3;
+    else if (C == 32 && w == 8) code = 4;
+    if (code < 0 || w * G != 1024 || ny % w) return -1;
+    if (chunk) *chunk = C;
+    if (width) *width = w;
+    return code;
+}
+
+template <typename T, typename TI, int KIND>
+__global__ void __launch_bounds__(1024, MHD2D_X_MINB)
+    k_xsolve_multi(const XSolveArgs<T, TI>* __restrict__ tab, MultiMap m)
+{
+    __shared__ T sh_in[xshared_max_words()];
+    __shared__ T sh_fw[xshared_max_words()];
+    __shared__ T sh_bw[xshared_max_words()];
+    __shared__ T sh_half[32];
+    extern __shared__ unsigned char x_dyn[];
+    const int b = blockIdx.x;
+    const int g = multi_group(m, b);
+    const XSolveArgs<T, TI>& a = tab[g];
+    const int local = b - m.first[g];
+    const int strips = m.aux[g];
+    const int nv = m.per[g] / strips;
+    const int bx = local % strips;
+    const int v = (local / strips) % nv;
+    const int z = local / m.per[g];
+    switch (m.code[g]) {
+    case 0: xsolve_body<T, TI, 8, 32, KIND, false>(a, bx, v, z, sh_in, sh_fw, sh_bw, sh_half, x_dyn); break;
+    case 1: xsolve_body<T, TI, 8, 16, KIND, false>(a, bx, v, z, sh_in, sh_fw, sh_bw, sh_half, x_dyn); break;
+    case 2: xsolve_body<T, TI, 16, 16, KIND, false>(a, bx, v, z, sh_in, sh_fw, sh_bw, sh_half, x_dyn); break;
+    case 3: xsolve_body<T, TI, 32, 16, KIND, false>(a, bx, v, z, sh_in, sh_fw, sh_bw, sh_half, x_dyn); break;
+    case 4: xsolve_body<T, TI, 32, 8, KIND, false>(a, bx, v, z, sh_in, sh_fw, sh_bw, sh_half, x_dyn); break;
+    default: break;
+    }
+}
+
+template <typename T, typename TD, int MODE>
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(1, RhsRkWaves<T, TD>::MAX)))
+        k_rhs_rk_multi(const RhsRkArgs<T, TD>* __restrict__ tab, MultiMap m)
+{
+    const int b = blockIdx.x;
+    const int g = multi_group(m, b);
+    const int local = b - m.first[g];
+    const int per = m.per[g];
+    // instances last to first, as in k_rhs_rk
+    rhs_rk_body<T, TD, MODE>(tab[g], m.nb[g] - 1 - local / per, local % per, per);
+}
+
+__global__ void k_dt_finalize_multi(const DtMultiArgs* __restrict__ tab, MultiMap m)
+{
+    const int b = blockIdx.x;
+    const int g = multi_group(m, b);
+    dt_finalize_body(tab[g].rs + (b - m.first[g]), tab[g].cfl);
+}
+
+static int multi_blocks(const MultiMap& m)
+{
+    return m.ng > 0 ? m.first[m.ng] : 0;
+}
+
+template <typename T, typename TD>
+hipError_t launch_ysweep_multi(const YSweepArgs<T, TD>* tab, const MultiMap& m, int mode,
+                               hipStream_t s)
+{
+    const int nb = multi_blocks(m);
+    if (m.ng < 1 || m.ng > MULTI_MAXG || nb < 1 || mode < 0 || mode > 1) return hipErrorInvalidValue;
+    if (mode == 1)
+        hipLaunchKernelGGL((k_ysweep_multi<T, TD, 1>), dim3(nb), dim3(128), 0, s, tab, m);
+    else
+        hipLaunchKernelGGL((k_ysweep_multi<T, TD, 0>), dim3(nb), dim3(128), 0, s, tab, m);
+    return hipGetLastError();
+}
+
+template <typename T, typename TI>
+hipError_t launch_xsolve_multi(const XSolveArgs<T, TI>* tab, const MultiMap& m, int kind,
+                               size_t dyn_lds, hipStream_t s)
+{
+    const int nb = multi_blocks(m);
+    if (m.ng < 1 || m.ng > MULTI_MAXG || nb < 1) return hipErrorInvalidValue;
+    if constexpr (std::is_same<T, TI>::value) {
+        switch (kind) {
+        case XD1: hipLaunchKernelGGL((k_xsolve_multi<T, TI, XD1>), dim3(nb), dim3(1024), dyn_lds, s, tab, m); break;
+        case XD2: hipLaunchKernelGGL((k_xsolve_multi<T, TI, XD2>), dim3(nb), dim3(1024), dyn_lds, s, tab, m); break;
+        default: hipLaunchKernelGGL((k_xsolve_multi<T, TI, XFILT>), dim3(nb), dim3(1024), dyn_lds, s, tab, m); break;
+        }
+    } else {
+        // converting inputs: derivatives only (they annihilate the subtracted
+        // column constant; the filter does not)
+        if (kind == XD1)
+            hipLaunchKernelGGL((k_xsolve_multi<T, TI, XD1>), dim3(nb), dim3(1024), dyn_lds, s, tab, m);
+        else if (kind == XD2)
+            hipLaunchKernelGGL((k_xsolve_multi<T, TI, XD2>), dim3(nb), dim3(1024), dyn_lds, s, tab, m);
+        else
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename T, typename TD>
+hipError_t launch_rhs_rk_multi(const RhsRkArgs<T, TD>* tab, const MultiMap& m, int mode,
+                               hipStream_t s)
+{
+    const int nb = multi_blocks(m);
+    if (m.ng < 1 || m.ng > MULTI_MAXG || nb < 1 || mode < 0 || mode > 1) return hipErrorInvalidValue;
+    if (mode == 1)
+        hipLaunchKernelGGL((k_rhs_rk_multi<T, TD, 1>), dim3(nb), dim3(256), 0, s, tab, m);
+    else
+        hipLaunchKernelGGL((k_rhs_rk_multi<T, TD, 0>), dim3(nb), dim3(256), 0, s, tab, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_dt_finalize_multi(const DtMultiArgs* tab, const MultiMap& m, hipStream_t s)
+{
+    const int nb = multi_blocks(m);
+    if (m.ng < 1 || m.ng > MULTI_MAXG || nb < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_dt_finalize_multi, dim3(nb), dim3(1), 0, s, tab, m);
+    return hipGetLastError();
+}
+
 // explicit instantiations
 #define MHD2D_INST(T)                                                                        \
     template hipError_t launch_xsolve<T, T>(const XSolveArgs<T, T>&, int, hipStream_t, int); \
@@ -1675,6 +1926,23 @@ hipError_t launch_diag(const T* u, const T* dbx, const T* dby, double* partial, 
 
 MHD2D_INST(double)
 MHD2D_INST(float)
+
+#define MHD2D_INST_MULTI(T, TD)                                                              \
+    template hipError_t launch_ysweep_multi<T, TD>(const YSweepArgs<T, TD>*, const MultiMap&, \
+                                                   int, hipStream_t);                        \
+    template hipError_t launch_rhs_rk_multi<T, TD>(const RhsRkArgs<T, TD>*, const MultiMap&, \
+                                                   int, hipStream_t);
+MHD2D_INST_MULTI(double, double)
+MHD2D_INST_MULTI(float, float)
+MHD2D_INST_MULTI(double, float)
+template hipError_t launch_xsolve_multi<double, double>(const XSolveArgs<double, double>*,
+                                                        const MultiMap&, int, size_t,
+                                                        hipStream_t);
+template hipError_t launch_xsolve_multi<float, float>(const XSolveArgs<float, float>*,
+                                                      const MultiMap&, int, size_t, hipStream_t);
+template hipError_t launch_xsolve_multi<float, double>(const XSolveArgs<float, double>*,
+                                                       const MultiMap&, int, size_t,
+                                                       hipStream_t);
 
 // mixed form: FP64 state, FP32 derivatives and R_prev
 template hipError_t launch_xsolve<float, double>(const XSolveArgs<float, double>&, int,
