@@ -418,12 +418,12 @@ def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50,
         if xg:
             # MFMA work of one x-derivative launch (9 fields: the ideal
             # periodic RHS needs no dBx/dx) over its time
-            flops = ctx.xgemm_panels() * 64 * 16 * 2 * n * 9 * xg
+            flops = ctx.xgemm_panels() * 64 * 32 * 2 * n * 9 * xg
             t = kern["xderiv"]["avg_ms"] * 1e-3
             rec["xderiv_mfma"] = {"bf16_flops_per_launch": flops, "TFLOP/s": flops / t / 1e12,
                                   "frac_of_2500_TFLOPs": flops / t / 2.5e15,
                                   "panels": ctx.xgemm_panels(),
-                                  "dense_fraction": ctx.xgemm_panels() / ((n / 64) * (n / 16))}
+                                  "dense_fraction": ctx.xgemm_panels() / ((n / 64) * (n / 32))}
         out[f"{name}_{n}"] = rec
         ctx.close()
     return out

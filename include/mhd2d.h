@@ -195,8 +195,8 @@ int mhd2d_stopped_at(mhd2d_ctx *c, int inst, double *dt_stop);
  * x-derivative (tridiagonal or GEMM, in its precision), into FP64 host out;
  * the run's state is untouched (temporary device buffers) */
 int mhd2d_xderiv(mhd2d_ctx *c, int nv, const double *in, double *out);
-/* GEMM form: number of non-zero 64 x 16 panels of the split operator over
- * all 64-row tiles (the MFMA work is panels * 64 * 16 * 2 * ny flops per
+/* GEMM form: number of non-zero 64 x 32 panels of the split operator over
+ * all 64-row tiles (the MFMA work is panels * 64 * 32 * 2 * ny flops per
  * field and split product); 0 for the tridiagonal form */
 long long mhd2d_xgemm_panels(const mhd2d_ctx *c);
 /* Autotuner (no counterpart in the reference): times every valid x-solve

@@ -187,6 +187,9 @@ hipError_t xderiv_fields(mhd2d_ctx* c, const T* in, TD* out, int nv, size_t bs_i
             g.klist = c->xg.klist_d;
             g.kcount = c->xg.kcount_d;
             g.kmax = c->xg.kmax;
+            g.klist2 = c->xg.klist2_d;
+            g.kcount2 = c->xg.kcount2_d;
+            g.kmax2 = c->xg.kmax2;
             g.nsplit = c->xg.nsplit;
             g.rs = rs;
             g.nx = c->p.nx;

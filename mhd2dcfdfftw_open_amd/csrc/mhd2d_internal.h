@@ -78,8 +78,9 @@ struct XSolveArgs {
 
 // x-derivative through the dense operator M = A^-1 B split into BF16 planes
 // (mh, mm, ml: row-major nx x nx), applied as an MFMA GEMM over the y-lines;
-// klist[t * kmax ...] lists the 16-column panels of row tile t (64 rows)
-// that hold non-zeros, kcount[t] how many
+// klist[t * kmax ...] lists the XGEMM_KC-column chunks of row tile t (64
+// rows) that hold non-zeros, kcount[t] how many
+constexpr int XGEMM_KC = 32;
 template <typename TI>
 struct XGemmArgs {
     const TI* in[16];
@@ -90,6 +91,10 @@ struct XGemmArgs {
     const int* klist = nullptr;
     const int* kcount = nullptr;
     int kmax = 0;
+    // the same for 128-row tiles: entries (chunk << 2) | half mask
+    const int* klist2 = nullptr;
+    const int* kcount2 = nullptr;
+    int kmax2 = 0;
     int nsplit = 3;
     const RunScalars* rs = nullptr;
     int nx = 0, ny = 0;

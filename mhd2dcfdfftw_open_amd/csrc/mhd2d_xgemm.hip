@@ -8,16 +8,16 @@
 // and is stored split into BF16 planes, M ~ Mh + Mm (+ Ml), each plane the
 // BF16 rounding of what the previous planes leave.  The derivative of field
 // v is then the GEMM D_v = M U_v over all y-lines at once (U_v: nx x ny,
-// row-major, the field's own layout), computed with
-// v_mfma_f32_16x16x16bf16 and FP32 accumulation:
+// row-major, the field's own layout), computed with gfx950's
+// v_mfma_f32_16x16x32_bf16 and FP32 accumulation:
 //   NS = 3:  Mh Uh + Mh Um + Mm Uh                         (~2^-16 per term)
 //   NS = 6:  + Mh Ul + Ml Uh + Mm Um                       (~2^-24 per term)
-// with U split the same way on the fly.
+// with U split the same way on the fly, once per element and workgroup.
 //
 // M decays by ~0.45 per row away from the diagonal (the filter-free compact
 // scheme), so beyond ~120 rows its entries are below the BF16 range and all
-// planes hold exact zeros there.  Each 64-row tile keeps the list of 16-column
-// panels in which any plane is non-zero, and only those are multiplied: the
+// planes hold exact zeros there.  Each 64-row tile keeps the list of 32-column
+// chunks in which any plane is non-zero, and only those are multiplied: the
 // product is the full dense one of the split operator.
 #include "mhd2d_internal.h"
 #include "mhd2d_tables.h"
@@ -107,8 +107,8 @@ hipError_t XGemmOp::build(const GridTables& tb, int nsplit_)
             r -= bf16_value_host(b);
         }
     }
-    // non-zero 16-column panels of every 64-row tile
-    const int ntile = N / 64, npanel = N / 16;
+    // non-zero 32-column chunks of every 64-row tile
+    const int ntile = N / 64, npanel = N / XGEMM_KC;
     std::vector<std::vector<int>> lists(ntile);
     const int used = nsplit == 3 ? 2 : 3;
     kmax = 0;
@@ -116,7 +116,7 @@ hipError_t XGemmOp::build(const GridTables& tb, int nsplit_)
         for (int pnl = 0; pnl < npanel; pnl++) {
             bool nz = false;
             for (int i = t * 64; i < t * 64 + 64 && !nz; i++)
-                for (int k = pnl * 16; k < pnl * 16 + 16 && !nz; k++)
+                for (int k = pnl * XGEMM_KC; k < pnl * XGEMM_KC + XGEMM_KC && !nz; k++)
                     for (int s = 0; s < used && !nz; s++)
                         nz = (planes[s * NN + (size_t)i * N + k] & 0x7fff) != 0;
             if (nz) lists[t].push_back(pnl);
@@ -130,8 +130,29 @@ hipError_t XGemmOp::build(const GridTables& tb, int nsplit_)
         panels_total += kcount[t];
         for (int q2 = 0; q2 < kcount[t]; q2++) klist[(size_t)t * kmax + q2] = lists[t][q2];
     }
+    // 128-row tiles (pairs of 64-row tiles): the union of the pair's chunk
+    // lists, each entry (chunk << 2) | (bit 0: the first half multiplies it,
+    // bit 1: the second half)
+    const int ntile2 = ntile / 2;
+    std::vector<std::vector<int>> lists2(ntile2);
+    kmax2 = 0;
+    for (int t = 0; t < ntile2; t++) {
+        std::vector<int> mask(npanel, 0);
+        for (int h = 0; h < 2; h++)
+            for (int c : lists[2 * t + h]) mask[c] |= 1 << h;
+        for (int c = 0; c < npanel; c++)
+            if (mask[c]) lists2[t].push_back((c << 2) | mask[c]);
+        kmax2 = std::max(kmax2, (int)lists2[t].size());
+    }
+    std::vector<int> klist2((size_t)std::max(1, ntile2 * kmax2), 0), kcount2(std::max(1, ntile2), 0);
+    for (int t = 0; t < ntile2; t++) {
+        kcount2[t] = (int)lists2[t].size();
+        for (int q2 = 0; q2 < kcount2[t]; q2++) klist2[(size_t)t * kmax2 + q2] = lists2[t][q2];
+    }
+    klist.insert(klist.end(), klist2.begin(), klist2.end());
+    const size_t k2off = (size_t)ntile * kmax;
     const size_t pbytes = (size_t)used * NN * sizeof(unsigned short);
-    const size_t lbytes = (klist.size() + kcount.size()) * sizeof(int);
+    const size_t lbytes = (klist.size() + kcount.size() + kcount2.size()) * sizeof(int);
     hipError_t e = hipMalloc(&blob_, pbytes + lbytes);
     if (e != hipSuccess) return e;
     e = hipMemcpy(blob_, planes.data(), pbytes, hipMemcpyHostToDevice);
@@ -142,12 +163,17 @@ hipError_t XGemmOp::build(const GridTables& tb, int nsplit_)
     e = hipMemcpy(base + pbytes + klist.size() * sizeof(int), kcount.data(),
                   kcount.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
+    e = hipMemcpy(base + pbytes + (klist.size() + kcount.size()) * sizeof(int), kcount2.data(),
+                  kcount2.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
     const unsigned short* pl = (const unsigned short*)blob_;
     mh = pl;
     mm = pl + NN;
     ml = used == 3 ? pl + 2 * NN : nullptr;
     klist_d = (const int*)(base + pbytes);
     kcount_d = klist_d + klist.size();
+    klist2_d = ntile2 > 0 ? klist_d + k2off : nullptr;
+    kcount2_d = ntile2 > 0 ? kcount_d + kcount.size() : nullptr;
     bytes = pbytes + lbytes;
     return hipSuccess;
 }
@@ -157,12 +183,13 @@ void XGemmOp::release()
     if (blob_) (void)hipFree(blob_);
     blob_ = nullptr;
     mh = mm = ml = nullptr;
-    klist_d = kcount_d = nullptr;
+    klist_d = kcount_d = klist2_d = kcount2_d = nullptr;
 }
 
 XGemmOp::~XGemmOp() { release(); }
 
-typedef short s4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef short s8v __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ unsigned short bf16_bits(float x)
@@ -175,86 +202,177 @@ __device__ __forceinline__ float bf16_float(unsigned short b)
     return __uint_as_float((unsigned)b << 16);
 }
 
-// One workgroup: a 64 x 64 tile of D_v for one field of one instance; four
-// waves of 32 x 32 (2 x 2 blocks of 16 x 16).  MFMA 16x16x16 operand layout:
-// A[i][k] in lane (i + 16 * (k / 4)), element k % 4; B[k][j] in lane
-// (j + 16 * (k / 4)), element k % 4; D[i][j] in lane (j + 16 * (i / 4)),
-// element i % 4.
-template <typename TI, int NS>
-__global__ void __launch_bounds__(256) k_xgemm(XGemmArgs<TI> a)
+__device__ __forceinline__ f4v mfma_bf16(const s8v& a, const s8v& b, const f4v& c)
 {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a),
+                                                   __builtin_bit_cast(bf8v, b), c, 0, 0, 0);
+}
+
+// Workgroup: a (64 RH)-row x 256-column tile of D_v (one field v, one
+// instance), 4 RH waves of 64 x 64 each (4 x 4 blocks of 16 x 16, 16
+// accumulators).  For every 32-column chunk of M's row tile that is non-zero
+// in either 64-row half:
+//  * the 32 x 256 block of U (rows = the chunk's columns) is read once by
+//    the workgroup (each column by RH threads), the column's row-0 value
+//    subtracted (the operator annihilates constants), split into BF16
+//    planes and written to LDS transposed, [plane][column][k]; the
+//    (64 RH) x 32 block of each M plane goes to LDS as [plane][row][k].
+//    Rows and columns are padded to 40 elements (80 bytes: the 16-byte
+//    fragment reads of 16 consecutive rows or columns then cover all 64
+//    banks once);
+//  * each wave whose half has the chunk reads its A and B fragments from
+//    LDS and runs the split products, small terms first;
+//  * the next chunk's U and M blocks are loaded into registers while the
+//    MFMAs run.
+// With RH = 2 a U block serves 128 rows: the staging per output element
+// (loads, split, LDS writes) falls by about a third at nx = 4096.
+// MFMA 16x16x32 operands: lane (r = lane % 16, g = lane / 16) holds 8
+// consecutive k of row r (A) or column r (B), k = 8 g .. 8 g + 7; the same k
+// mapping on both sides pairs the products correctly.  D[i][j] is in lane
+// (j + 16 * (i / 4)), element i % 4.
+template <typename TI, int NS, int RH>
+__global__ void __launch_bounds__(256 * RH, 2 / RH) k_xgemm(XGemmArgs<TI> a)
+{
+    constexpr int NT = 256 * RH;         // threads
+    constexpr int NP = NS == 6 ? 3 : 2;  // planes of U and of M
+    constexpr int CT = 256;              // columns per workgroup
+    constexpr int KC = XGEMM_KC;         // k per chunk
+    constexpr int KT = KC / RH;          // k of a column staged by one thread
+    constexpr int CS = 40;               // LDS row / column stride (bf16 elements)
+    constexpr int RT = 64 * RH;          // rows per workgroup
+    constexpr int MP = NP * RT * (KC / 8);  // 16-byte pieces of an M block
+    constexpr int MPT = (MP + NT - 1) / NT; // per thread
+    __shared__ __attribute__((aligned(16))) unsigned short su[NP * CT * CS];
+    __shared__ __attribute__((aligned(16))) unsigned short sm[NP * RT * CS];
     const int z = blockIdx.z;
     if (a.rs && a.rs[z].stop) return;
     const int v = blockIdx.y;
     const int ny = a.ny;
-    const int jtiles = ny / 64;
-    const int it = blockIdx.x / jtiles, jt = blockIdx.x % jtiles;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wi = w >> 1, wj = w & 1;
-    const int row0 = it * 64 + wi * 32, col0 = jt * 64 + wj * 32;
-    const int lr = lane & 15, lk = (lane >> 4) * 4;
+    const int ctiles = (ny + CT - 1) / CT;
+    const int it = blockIdx.x / ctiles, ct = blockIdx.x % ctiles;
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const int wh = w >> 2, wc = w & 3;        // this wave's row half and column block
+    const int lr = lane & 15, lg = lane >> 4;
+    const int row0 = it * RT;
+    const int colw = ct * CT + wc * 64;       // this wave's first column
+    const bool wact = colw < ny;              // waves beyond a narrow grid only stage
+    const int mycol = ct * CT + (t % CT);     // the column this thread stages
+    const int kpart = t / CT;                 // and its part of the chunk's k
+    const bool cact = mycol < ny;
     const TI* __restrict__ U = a.in[v] + z * a.bs_in;
     float* __restrict__ D = a.out[v] + z * a.bs_out;
     const size_t N = (size_t)a.nx;
-    // column constant (the operator annihilates constants)
-    TI c0[2];
-#pragma unroll
-    for (int cb = 0; cb < 2; cb++) c0[cb] = U[col0 + cb * 16 + lr];
+    const unsigned short* const mpl[3] = {a.mh, a.mm, a.ml};
+    const TI c0 = cact ? U[mycol] : (TI)0;
 
-    f4v acc[2][2];
+    f4v acc[4][4];
 #pragma unroll
-    for (int rb = 0; rb < 2; rb++)
+    for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++) acc[rb][cb] = f4v{0.f, 0.f, 0.f, 0.f};
+        for (int cb = 0; cb < 4; cb++) acc[rb][cb] = f4v{0.f, 0.f, 0.f, 0.f};
 
-    const int np = a.kcount[it];
-    const int* __restrict__ kl = a.klist + (size_t)it * a.kmax;
-    for (int qn = 0; qn < np; qn++) {
-        const int k0 = kl[qn] * 16 + lk;
-        s4v ah[2], am[2], al[2];
+    const int* __restrict__ kl = RH == 2 ? a.klist2 + (size_t)it * a.kmax2
+                                         : a.klist + (size_t)it * a.kmax;
+    const int nch = RH == 2 ? a.kcount2[it] : a.kcount[it];
+    // chunk index and the halves that multiply it
+    auto chunk = [&](int q) { return RH == 2 ? (kl[q] >> 2) : kl[q]; };
+    auto mask = [&](int q) { return RH == 2 ? (kl[q] & 3) : 1; };
+    TI pre[KT];
+    s8v mpre[MPT];
+    // this thread's U values and M pieces of chunk q into registers
+    auto fetch = [&](int q) {
+        const int k0 = chunk(q) * KC;
+        const TI* col = U + (size_t)(k0 + kpart * KT) * ny + mycol;
 #pragma unroll
-        for (int rb = 0; rb < 2; rb++) {
-            const size_t off = (size_t)(row0 + rb * 16 + lr) * N + k0;
-            ah[rb] = *reinterpret_cast<const s4v*>(a.mh + off);
-            am[rb] = *reinterpret_cast<const s4v*>(a.mm + off);
-            if constexpr (NS == 6) al[rb] = *reinterpret_cast<const s4v*>(a.ml + off);
-        }
-        s4v bh[2], bm[2], bl[2];
+        for (int e = 0; e < KT; e++) pre[e] = cact ? col[(size_t)e * ny] : (TI)0;
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++) {
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const float x = (float)(U[(size_t)(k0 + e) * ny + col0 + cb * 16 + lr] - c0[cb]);
-                const unsigned short h = bf16_bits(x);
-                const float r1 = x - bf16_float(h);
-                const unsigned short m = bf16_bits(r1);
-                bh[cb][e] = (short)h;
-                bm[cb][e] = (short)m;
-                if constexpr (NS == 6) bl[cb][e] = (short)bf16_bits(r1 - bf16_float(m));
+        for (int i = 0; i < MPT; i++) {
+            const int pc = t + NT * i;  // piece: plane, row, k group
+            if (pc < MP) {
+                const int pl = pc / (RT * (KC / 8)), r = (pc / (KC / 8)) % RT, kg = pc % (KC / 8);
+                mpre[i] = *reinterpret_cast<const s8v*>(mpl[pl] + (size_t)(row0 + r) * N + k0 +
+                                                        kg * 8);
             }
         }
+    };
+    if (nch > 0) fetch(0);
+    for (int q = 0; q < nch; q++) {
+        __syncthreads();  // the previous chunk's fragments have been read
+        const int sc = t % CT;
 #pragma unroll
-        for (int rb = 0; rb < 2; rb++)
+        for (int g = 0; g < KT / 8; g++) {
+            s8v h, m, l;
 #pragma unroll
-            for (int cb = 0; cb < 2; cb++) {
-                // small terms first
-                if constexpr (NS == 6) {
-                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am[rb], bm[cb], acc[rb][cb], 0, 0, 0);
-                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[rb], bh[cb], acc[rb][cb], 0, 0, 0);
-                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[rb], bl[cb], acc[rb][cb], 0, 0, 0);
+            for (int e = 0; e < 8; e++) {
+                const float x = (float)(pre[g * 8 + e] - c0);
+                const unsigned short hb = bf16_bits(x);
+                const float r1 = x - bf16_float(hb);
+                const unsigned short mb = bf16_bits(r1);
+                h[e] = (short)hb;
+                m[e] = (short)mb;
+                if constexpr (NS == 6) l[e] = (short)bf16_bits(r1 - bf16_float(mb));
+            }
+            const int ko = kpart * KT + g * 8;
+            *reinterpret_cast<s8v*>(su + (0 * CT + sc) * CS + ko) = h;
+            *reinterpret_cast<s8v*>(su + (1 * CT + sc) * CS + ko) = m;
+            if constexpr (NS == 6) *reinterpret_cast<s8v*>(su + (2 * CT + sc) * CS + ko) = l;
+        }
+#pragma unroll
+        for (int i = 0; i < MPT; i++) {
+            const int pc = t + NT * i;
+            if (pc < MP) {
+                const int pl = pc / (RT * (KC / 8)), r = (pc / (KC / 8)) % RT, kg = pc % (KC / 8);
+                *reinterpret_cast<s8v*>(sm + (pl * RT + r) * CS + kg * 8) = mpre[i];
+            }
+        }
+        __syncthreads();
+        const bool mine = wact && ((mask(q) >> wh) & 1);
+        if (q + 1 < nch) fetch(q + 1);  // in flight during the MFMAs
+        if (mine) {
+            s8v bh[4], bm[4], bl[4];
+#pragma unroll
+            for (int cb = 0; cb < 4; cb++) {
+                const int c = wc * 64 + cb * 16 + lr;
+                bh[cb] = *reinterpret_cast<const s8v*>(su + (0 * CT + c) * CS + lg * 8);
+                bm[cb] = *reinterpret_cast<const s8v*>(su + (1 * CT + c) * CS + lg * 8);
+                if constexpr (NS == 6)
+                    bl[cb] = *reinterpret_cast<const s8v*>(su + (2 * CT + c) * CS + lg * 8);
+            }
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) {
+                const int r = wh * 64 + rb * 16 + lr;
+                const s8v ah = *reinterpret_cast<const s8v*>(sm + (0 * RT + r) * CS + lg * 8);
+                const s8v am = *reinterpret_cast<const s8v*>(sm + (1 * RT + r) * CS + lg * 8);
+                s8v al;
+                if constexpr (NS == 6)
+                    al = *reinterpret_cast<const s8v*>(sm + (2 * RT + r) * CS + lg * 8);
+#pragma unroll
+                for (int cb = 0; cb < 4; cb++) {
+                    // small terms first
+                    if constexpr (NS == 6) {
+                        acc[rb][cb] = mfma_bf16(am, bm[cb], acc[rb][cb]);
+                        acc[rb][cb] = mfma_bf16(al, bh[cb], acc[rb][cb]);
+                        acc[rb][cb] = mfma_bf16(ah, bl[cb], acc[rb][cb]);
+                    }
+                    acc[rb][cb] = mfma_bf16(am, bh[cb], acc[rb][cb]);
+                    acc[rb][cb] = mfma_bf16(ah, bm[cb], acc[rb][cb]);
+                    acc[rb][cb] = mfma_bf16(ah, bh[cb], acc[rb][cb]);
                 }
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am[rb], bh[cb], acc[rb][cb], 0, 0, 0);
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[rb], bm[cb], acc[rb][cb], 0, 0, 0);
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[rb], bh[cb], acc[rb][cb], 0, 0, 0);
             }
+        }
     }
+    if (!wact) return;
+    // 16 lanes write 64 contiguous bytes of a row; nontemporal (the RHS
+    // reads the derivatives once, later)
+    const int rw = row0 + wh * 64;
 #pragma unroll
-    for (int rb = 0; rb < 2; rb++)
+    for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++)
+        for (int cb = 0; cb < 4; cb++)
 #pragma unroll
             for (int e = 0; e < 4; e++)
-                D[(size_t)(row0 + rb * 16 + lk + e) * ny + col0 + cb * 16 + lr] = acc[rb][cb][e];
+                __builtin_nontemporal_store(
+                    acc[rb][cb][e], D + (size_t)(rw + rb * 16 + lg * 4 + e) * ny + colw + cb * 16 + lr);
 }
 
 template <typename TI>
@@ -262,12 +380,17 @@ hipError_t launch_xgemm(const XGemmArgs<TI>& a, int nv, hipStream_t s, int nb)
 {
     if (a.nx % 64 || a.ny % 64 || nv < 1 || nv > 16 || nb < 1 || nb > 65535 || !a.mh || !a.mm)
         return hipErrorInvalidValue;
-    dim3 grid((a.nx / 64) * (a.ny / 64), nv, nb), block(256);
-    if (a.nsplit == 6) {
-        if (!a.ml) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_xgemm<TI, 6>), grid, block, 0, s, a);
+    const int ctiles = (a.ny + 255) / 256;
+    // 128-row tiles where the lines hold whole pairs of 64-row tiles
+    const bool two = a.nx % 128 == 0 && a.klist2 && a.kcount2;
+    dim3 grid((a.nx / (two ? 128 : 64)) * ctiles, nv, nb), block(two ? 512 : 256);
+    if (a.nsplit == 6 && !a.ml) return hipErrorInvalidValue;
+    if (two) {
+        if (a.nsplit == 6) hipLaunchKernelGGL((k_xgemm<TI, 6, 2>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_xgemm<TI, 3, 2>), grid, block, 0, s, a);
     } else {
-        hipLaunchKernelGGL((k_xgemm<TI, 3>), grid, block, 0, s, a);
+        if (a.nsplit == 6) hipLaunchKernelGGL((k_xgemm<TI, 6, 1>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_xgemm<TI, 3, 1>), grid, block, 0, s, a);
     }
     return hipGetLastError();
 }

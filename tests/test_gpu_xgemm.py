@@ -6,8 +6,9 @@ the FP64 oracle.
   (FP64 at 1e-12; FP32 and mixed within their rounding);
 * the GEMM form with 6 split products matches the oracle within FP32-level
   error, with 3 split products within BF16-pair error, on random and smooth
-  lines of 64…1024 rows (checks the MFMA operand layout, the panel lists and
-  the periodic corners of the operator);
+  lines of 64…4096 rows and 64…512 columns (checks the MFMA operand layout,
+  the chunk lists, the periodic corners of the operator and column tiles
+  that only partly cover the grid);
 * whole runs with the GEMM form against FP64.
 """
 import numpy as np
@@ -52,7 +53,7 @@ def test_tridiagonal_xderiv_matches_oracle(m, prec, tol):
     assert e.max() <= tol, e
 
 
-@pytest.mark.parametrize("nx,ny", [(64, 64), (256, 128), (1024, 64)])
+@pytest.mark.parametrize("nx,ny", [(64, 64), (256, 128), (1024, 64), (4096, 64), (128, 512)])
 @pytest.mark.parametrize("kind", ["smooth", "random"])
 @pytest.mark.parametrize("ns", [3, 6])
 @pytest.mark.parametrize("prec", ["mixed", "fp32"])
@@ -66,7 +67,9 @@ def test_gemm_xderiv_matches_oracle(m, nx, ny, kind, ns, prec):
     tri = m.Context(nx, ny, precision=P).xderiv(f)
     et = field_errors(tri, want)
     print(f"[xgemm] {prec} ns={ns} {nx}x{ny} {kind}: gemm {e.max():.2e}, tridiagonal {et.max():.2e}")
-    tol = {3: 2e-3, 6: 2e-5}[ns]
+    # the split error of a derivative grows with the line length (the
+    # cancellation M U << |M| |U| deepens as nx grows): per 1024 rows
+    tol = {3: 2e-3, 6: 2e-5}[ns] * max(1.0, nx / 1024)
     assert e.max() <= tol, e
 
 
