@@ -47,6 +47,8 @@ def kernel_alg_bytes(nx, ny, elem=8, delem=None, batch=1, divb=False):
     n = nx * ny * batch
     s, d = elem, (elem if delem is None else delem)
     k = 1 if divb else 0
+    # x chunk of the solves (rows per thread): two carries per chunk and field
+    cr = 18.0 / min(32, max(8, nx // 64))
     return {
         # y-sweep: read u (8), write filtered u (8), Ey and Ez (2), dudy (7,
         # all but dBy/dy, which the kernel stores all the same: it comes out
@@ -60,6 +62,12 @@ def kernel_alg_bytes(nx, ny, elem=8, delem=None, batch=1, divb=False):
         "rhs_rk": n * (16 * s + (18 + 2 * k) * d + (16.0 / 3.0) * 2 * d),
         # x-filter in place: read 8, write 8
         "xfilter": n * 16 * s,
+        # fused path: the carry solve reads the 9 fields the x-derivative
+        # reads and writes 2 values per chunk and field; the chunk kernel
+        # reads u (8), Ey and Ez (2), dudy (9), R_prev in stages 2-3 (8) and
+        # the carries, writes u (8) and R_prev in stages 1-2 (8)
+        "xcarry": n * (9 * s + cr * d),
+        "xrhs_rk": n * (18 * s + 9 * d + cr * d + (16.0 / 3.0) * 2 * d),
     }
 
 
@@ -167,7 +175,7 @@ def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, nx, ny):
         # launches, whole steps of three each
         out = {}
         for k, v in acc.items():
-            if "k_rhs_rk" in k:
+            if "k_rhs_rk" in k or "k_xrhs_rk" in k:
                 v = v[len(v) % 3:]
                 out[k] = float(np.mean(v))
             else:
@@ -182,18 +190,28 @@ def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, nx, ny):
     wscale = COPY_BYTES / cw[ck[0]] if ck and cw[ck[0]] > 0 else 1.0
     fetch = per_kernel(fetch_csv, "FETCH_SIZE")
     write = per_kernel(write_csv, "WRITE_SIZE")
+    ndisp = {}
+    for name, ctr, _ in _pmc_rows(fetch_csv):
+        if ctr == "FETCH_SIZE":
+            ndisp[name] = ndisp.get(name, 0) + 1
     classes = {"ysweep": "k_ysweep", "xderiv": "k_xsolve", "rhs_rk": "k_rhs_rk",
-               "xfilter": "k_xsolve"}
+               "xfilter": "k_xsolve", "xcarry": "k_xsolve", "xrhs_rk": "k_xrhs_rk"}
     alg = kernel_alg_bytes(nx, ny)
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
     rows = []
     for cls, key in classes.items():
         names = [k for k in fetch if key in k]
-        # k_xsolve<T, TI, C, W, KIND, BLK>: KIND 0 derivative, 2 filter
+        # k_xsolve<T, TI, C, W, KIND, BLK>: KIND 0 derivative, 2 filter, 3
+        # carry solve, 4 filter with its chunk edge rows from the fused path
         if cls == "xderiv":
             names = [k for k in names if ", 0, false>(" in k]
+        if cls == "xcarry":
+            names = [k for k in names if ", 3, false>(" in k]
         if cls == "xfilter":
-            names = [k for k in names if ", 2, false>(" in k]
+            # the filter the timed steps ran: the one with more dispatches (the
+            # autotuner's steps run the other path too)
+            names = sorted([k for k in names if ", 2, false>(" in k or ", 4, false>(" in k],
+                           key=lambda k: -ndisp.get(k, 0))
         if not names:
             continue
         k = names[0]
@@ -330,8 +348,10 @@ def sweep_alg_bytes(nx, ny, elem=8, delem=None, batch=1):
 # kernels of each canonical sweep as launched here: the y-sweep is one
 # kernel; the x-sweep is the derivative x-solve, the RHS + RK update and the
 # filter x-solve (three kernels, since full x-lines of all fields do not fit
-# on one CU)
+# on one CU), or with the fused x-derivative + RHS the carry solve, the fused
+# chunk kernel and the filter x-solve
 SWEEP_KERNELS = {"ysweep": ("ysweep",), "xsweep": ("xderiv", "rhs_rk", "xfilter")}
+SWEEP_KERNELS_FUSED = {"ysweep": ("ysweep",), "xsweep": ("xcarry", "xrhs_rk", "xfilter")}
 
 
 def roofline_of(ctx, nx, ny, elem, delem, batch, profile_steps, copy_gbps, traffic=None):
@@ -345,22 +365,23 @@ def roofline_of(ctx, nx, ny, elem, delem, batch, profile_steps, copy_gbps, traff
     ctx.sync()
     alg = kernel_alg_bytes(nx, ny, elem, delem, batch)
     salg = sweep_alg_bytes(nx, ny, elem, delem, batch)
+    sk = SWEEP_KERNELS_FUSED if "xcarry" in prof else SWEEP_KERNELS
     # time per stage of each sweep: the sum of its kernels' average launches
-    st_ms = {sw: sum(prof[k][0] for k in ks) for sw, ks in SWEEP_KERNELS.items()}
+    st_ms = {sw: sum(prof[k][0] for k in ks) for sw, ks in sk.items()}
     dom = "ysweep" if st_ms["ysweep"] >= st_ms["xsweep"] else "xsweep"
     achieved = salg[dom] / (st_ms[dom] * 1e-3)
-    roof = {"bound": "hbm", "kernel": dom, "kernels": list(SWEEP_KERNELS[dom]),
+    roof = {"bound": "hbm", "kernel": dom, "kernels": list(sk[dom]),
             "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "avg_ms_per_stage": st_ms[dom],
             "alg_bytes_per_stage": salg[dom],
             "frac_vs_copy_probe": achieved / (copy_gbps * 1e9), "traffic": None}
-    if traffic is not None and all(k in traffic for k in SWEEP_KERNELS[dom]):
-        roof["traffic"] = sum(traffic[k] for k in SWEEP_KERNELS[dom])
+    if traffic is not None and all(k in traffic for k in sk[dom]):
+        roof["traffic"] = sum(traffic[k] for k in sk[dom])
     roof["by_sweep"] = {sw: {"avg_ms_per_stage": st_ms[sw], "alg_bytes_per_stage": salg[sw],
                              "GB/s": salg[sw] / (st_ms[sw] * 1e-3) / 1e9,
                              "frac": salg[sw] / (st_ms[sw] * 1e-3) / HBM_PEAK}
-                        for sw in SWEEP_KERNELS}
-    share = {k: prof[k][0] * prof[k][1] for k in alg}
+                        for sw in sk}
+    share = {k: prof[k][0] * prof[k][1] for k in alg if k in prof}
     big = max(share, key=share.get)
     kio = alg[big] / (prof[big][0] * 1e-3)
     roof["kernel_io_frac"] = {"kernel": big, "avg_launch_ms": prof[big][0],
@@ -624,7 +645,8 @@ def main():
                                 hall, di)
     traffic = None
     if args.precision == "fp64" and nx == 2048 and ny == 2048:
-        traffic = {k: load_traffic(k) for k in ("ysweep", "xderiv", "rhs_rk", "xfilter")}
+        traffic = {k: load_traffic(k)
+                   for k in ("ysweep", "xderiv", "rhs_rk", "xcarry", "xrhs_rk", "xfilter")}
         traffic = {k: v for k, v in traffic.items() if v is not None}
     roofline, kernels = roofline_of(ctx, nx, ny, elem, delem, 1, args.profile_steps, copy_gbps,
                                     traffic)
@@ -654,6 +676,7 @@ def main():
         "step_roofline_frac": value * B_ALG[args.precision] / HBM_PEAK,
         "kernels": kernels,
         "x_solve_shapes": ctx.x_shapes,
+        "x_fused": ctx.xfuse,
         "copy_probe_GBps": copy_gbps,
         "diagnostics": {"energy": energy, "max_divb": divb},
     }

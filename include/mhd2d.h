@@ -209,8 +209,10 @@ long long mhd2d_xgemm_panels(const mhd2d_ctx *c);
  * not depend on the shape (each y-line is solved the same way).  For a
  * batch it then times instance groups (mhd2d_set_subbatch) of half, a
  * quarter, ... of the batch, down to 2^21 cells a group, and keeps a group
- * size only where it beats the whole batch by 1 %.
- * Tridiagonal form only. */
+ * size only where it beats the whole batch by 1 %.  Where the fused
+ * x-derivative + RHS applies (mhd2d_set_xfuse) it times the step with and
+ * without it (before the group sizes) and keeps the faster; mhd2d_get_xfuse
+ * tells which.  Tridiagonal form only. */
 int mhd2d_autotune(mhd2d_ctx *c, int reps, int *choice);
 /* Instances per launch of a batch (no counterpart in the reference, whose
  * runs are one grid each): a step runs the batch as groups of k instances,
@@ -220,6 +222,17 @@ int mhd2d_autotune(mhd2d_ctx *c, int reps, int *choice);
  * for every k.  mhd2d_get_subbatch returns the group size in use. */
 int mhd2d_set_subbatch(mhd2d_ctx *c, int k);
 int mhd2d_get_subbatch(const mhd2d_ctx *c);
+/* Fused x-derivative + RHS (no counterpart in the reference, whose calcRHS
+ * reads the derivative arrays derivX wrote, mhd.c:186-198): on = 1 runs each
+ * stage's x-derivative as a solve that keeps only the values entering each
+ * chunk of rows, then one kernel that re-solves every chunk from them in
+ * on-chip memory and applies calcRHS + rkExecute there, so the x-derivative
+ * arrays never reach device memory.  Results are bitwise those of on = 0.
+ * Ideal MHD (no viscosity, resistivity or div B cleaning) on one periodic x
+ * block with the tridiagonal x-derivative only: MHD2D_EINVAL otherwise.
+ * mhd2d_get_xfuse returns the setting. */
+int mhd2d_set_xfuse(mhd2d_ctx *c, int on);
+int mhd2d_get_xfuse(const mhd2d_ctx *c);
 /* wall time of nsteps on the context's stream (hipEvents), in ms */
 int mhd2d_time_run(mhd2d_ctx *c, int nsteps, int use_graph, double *ms);
 /* average device time per launch of each kernel class over nsteps, in ms:

@@ -131,6 +131,8 @@ def lib():
         "mhd2d_set_di": (i, [vp, i, d]),
         "mhd2d_set_subbatch": (i, [vp, i]),
         "mhd2d_get_subbatch": (i, [vp]),
+        "mhd2d_set_xfuse": (i, [vp, i]),
+        "mhd2d_get_xfuse": (i, [vp]),
         "mhd2d_get_scalars_at": (i, [vp, i, p, p, ctypes.POINTER(ll)]),
         "mhd2d_diagnostics_at": (i, [vp, i, p, p]),
         "mhd2d_stopped_at": (i, [vp, i, p]),
@@ -308,6 +310,19 @@ class Context:
             _check(r)
         return r
 
+    def set_xfuse(self, on):
+        """x-derivative + RHS as the carry solve and the fused chunk kernel
+        (ideal MHD on one periodic x block; bitwise the same results)"""
+        _check(lib().mhd2d_set_xfuse(self.h, int(on)))
+
+    @property
+    def xfuse(self):
+        """1 when the fused x-derivative + RHS path is in use"""
+        r = lib().mhd2d_get_xfuse(self.h)
+        if r < 0:
+            _check(r)
+        return r
+
     def run(self, nsteps, graph=False):
         f = lib().mhd2d_run_graph if graph else lib().mhd2d_run
         _check(f(self.h, nsteps))
@@ -334,7 +349,10 @@ class Context:
         out = (ctypes.c_double * 5)()
         cnt = (ctypes.c_int * 5)()
         _check(lib().mhd2d_profile_run(self.h, nsteps, out, cnt))
-        names = ("ysweep", "xderiv", "rhs_rk", "xfilter", "dt_finalize")
+        # with the fused x-derivative + RHS the second and third classes are
+        # the carry solve and the fused chunk kernel
+        names = ("ysweep", "xcarry" if self.xfuse else "xderiv",
+                 "xrhs_rk" if self.xfuse else "rhs_rk", "xfilter", "dt_finalize")
         return {k: (out[q], cnt[q]) for q, k in enumerate(names)}
 
     def device_bytes(self):

@@ -94,6 +94,21 @@ struct mhd2d_ctx {
     // instances per launch: a step runs the batch as groups of this many
     // instances, one whole step per group (0: the whole batch at once)
     int subbatch = 0;
+    // x-derivative and RHS + RK as the carry solve and the fused chunk
+    // kernel (k_xrhs_rk) instead of the full solve and k_rhs_rk; only
+    // where xfuse_ok holds
+    int xfuse = 0;
+    // [B][8][nx / C][6][ny] (state type), allocated with xfuse: the fused
+    // kernel's updated state of the first and last 3 rows of every x chunk
+    // of C rows, which the neighbouring chunks' workgroups still read from
+    // u as halos; the x-filter reads them from here
+    void* xedge = nullptr;
+    size_t edge_fs() const
+    {
+        int C, W, G;
+        xsolve_shape(p.nx, elem, &C, &W, &G);
+        return (size_t)G * 6 * p.ny;
+    }
     double cdiff = 0;  // constDiffDivB = 2 max(dx, dy)^2 (initialize.c:17, 154)
     std::vector<char> have_state;
     std::vector<double> di;
@@ -252,12 +267,19 @@ hipError_t xderiv(mhd2d_ctx* c)
     return launch_xsolve<TD, T>(a, f1 - f0 + 1, c->stream, c->B);
 }
 
+// u in place; with edge, the chunk edge rows of the input come from the
+// fused path's edge buffer
 template <typename T>
-XSolveArgs<T> xfilter_args(mhd2d_ctx* c)
+XSolveArgs<T> xfilter_args(mhd2d_ctx* c, bool edge)
 {
     XSolveArgs<T> a;
     T* u = (T*)c->u;
     for (int v = 0; v < 8; v++) a.in[v] = a.out[v] = u + v * c->n;
+    if (edge) {
+        a.edge = (const T*)c->xedge;
+        a.edge_fs = c->edge_fs();
+        a.bs_edge = 8 * a.edge_fs;
+    }
     if (c->glm()) a.in[8] = a.out[8] = u + PSI_SLOT * c->n;
     a.rs = c->rs;
     a.st = c->tb.st[XFILT];
@@ -274,7 +296,7 @@ XSolveArgs<T> xfilter_args(mhd2d_ctx* c)
 template <typename T>
 hipError_t xfilter(mhd2d_ctx* c)
 {
-    return launch_xsolve<T>(xfilter_args<T>(c), c->nv_state(), c->stream, c->B);
+    return launch_xsolve<T>(xfilter_args<T>(c, c->xfuse != 0), c->nv_state(), c->stream, c->B);
 }
 
 template <typename T, typename TD>
@@ -362,6 +384,64 @@ hipError_t rhs_rk(mhd2d_ctx* c, int stage)
     return e;
 }
 
+// the fused x-derivative + RHS + RK applies: ideal MHD on one periodic x
+// block, the tridiagonal x-derivative, a grid its workgroups tile
+bool xfuse_ok(const mhd2d_ctx* c)
+{
+    return !c->lap && c->p.divbclean == 0 && !c->p.open_x && c->p.xblocks <= 1 &&
+           !c->p.xgemm && xrhs_supported(c->p.nx, c->p.ny, c->elem);
+}
+
+// the x-derivative solve of the fields the ideal RHS reads, stopping at the
+// chunk carries (into the dudx buffer, which the fused path leaves unused:
+// 18 nx ny / C <= 2.25 nx ny words per instance)
+template <typename T, typename TD>
+XSolveArgs<TD, T> xcarry_args(mhd2d_ctx* c)
+{
+    static const int ideal[9] = {RHO, UX, UY, UZ, BY, BZ, PR, 8, 9};
+    XSolveArgs<TD, T> a;
+    for (int v = 0; v < 9; v++) {
+        a.in[v] = (const T*)c->u + ideal[v] * c->n;
+        a.out[v] = nullptr;
+    }
+    a.rs = c->rs;
+    a.st = c->tb.st[XD1];
+    a.tb = c->tb.xt[XD1];
+    a.kind = XD1;
+    a.nx = c->p.nx;
+    a.ny = c->p.ny;
+    a.bs_in = c->su();
+    a.width = c->xw_d;
+    a.remap = c->xr_d;
+    a.carry = (TD*)c->dudx;
+    a.bs_carry = c->sd();
+    return a;
+}
+
+// the fused path's edge-row buffer, allocated on first use
+hipError_t ensure_xedge(mhd2d_ctx* c)
+{
+    if (c->xedge) return hipSuccess;
+    const size_t b = 8 * c->edge_fs() * c->elem * c->B;
+    const hipError_t e = hipMalloc(&c->xedge, b);
+    if (e == hipSuccess) c->bytes += b;
+    return e;
+}
+
+template <typename T, typename TD>
+XRhsArgs<T, TD> xrhs_args(mhd2d_ctx* c, int stage)
+{
+    XRhsArgs<T, TD> a;
+    a.x = xcarry_args<T, TD>(c);
+    a.r = rhs_rk_args<T, TD>(c, stage);
+    a.carry = (const TD*)c->dudx;
+    a.bs_carry = c->sd();
+    a.edge = (T*)c->xedge;
+    a.edge_fs = c->edge_fs();
+    a.bs_edge = 8 * a.edge_fs;
+    return a;
+}
+
 int nsub(const mhd2d_ctx* c)
 {
     const int k = c->subbatch > 0 && c->subbatch < c->B ? c->subbatch : c->B;
@@ -394,12 +474,21 @@ hipError_t step_group(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
             if (e != hipSuccess) return e;
             mark(0);
         }
-        e = xderiv<T, TD>(c);
-        if (e != hipSuccess) return e;
-        mark(0);
-        e = rhs_rk<T, TD>(c, s);
-        if (e != hipSuccess) return e;
-        mark(0);
+        if (c->xfuse) {
+            e = launch_xsolve<TD, T>(xcarry_args<T, TD>(c), 9, c->stream, c->B);
+            if (e != hipSuccess) return e;
+            mark(0);
+            e = launch_xrhs_rk<T, TD>(xrhs_args<T, TD>(c, s), c->stream, c->B);
+            if (e != hipSuccess) return e;
+            mark(0);
+        } else {
+            e = xderiv<T, TD>(c);
+            if (e != hipSuccess) return e;
+            mark(0);
+            e = rhs_rk<T, TD>(c, s);
+            if (e != hipSuccess) return e;
+            mark(0);
+        }
         e = xfilter<T>(c);
         if (e != hipSuccess) return e;
         mark(0);
@@ -425,6 +514,7 @@ hipError_t step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
     TD* const rp = (TD*)c->rprev;
     TD* const lp = (TD*)c->lap;
     TD* const db = (TD*)c->divb;
+    T* const xe = (T*)c->xedge;
     double* const cp = c->glm_cp;
     RunScalars* const rs = c->rs;
     const bool pf = c->pending_filter;
@@ -438,6 +528,7 @@ hipError_t step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
         c->rprev = rp + z * c->sr();
         c->lap = lp ? lp + z * c->sl() : nullptr;
         c->divb = db ? db + z * c->sb() : nullptr;
+        c->xedge = xe ? xe + z * 8 * c->edge_fs() : nullptr;
         c->glm_cp = cp ? cp + 2 * z * npe : nullptr;
         c->rs = rs + z;
         c->pending_filter = pf;
@@ -450,6 +541,7 @@ hipError_t step(mhd2d_ctx* c, hipEvent_t* ev, int* nev)
     c->rprev = rp;
     c->lap = lp;
     c->divb = db;
+    c->xedge = xe;
     c->glm_cp = cp;
     c->rs = rs;
     c->pending_filter = e == hipSuccess ? true : pf;
@@ -638,8 +730,8 @@ void mhd2d_destroy(mhd2d_ctx* c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
-    void* ptrs[] = {c->u,    c->dudx, c->dudy, c->rprev, c->lap,
-                    c->divb, c->glm_cp, c->rs, c->partial};
+    void* ptrs[] = {c->u,    c->dudx,   c->dudy, c->rprev,   c->lap,
+                    c->divb, c->glm_cp, c->rs,   c->partial, c->xedge};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1126,6 +1218,27 @@ hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
                 pick[phase][1] = cands[ci].r;
             }
     }
+    // the fused x-derivative + RHS, where it applies: the faster of the two
+    // (it moves ~13 words per cell and stage less, but its chunk kernel
+    // streams less evenly than the RHS + RK alone; which wins depends on
+    // the grid, the precision and the batch)
+    const int fuse0 = c->xfuse;
+    int pick_fuse = fuse0;
+    if (e == hipSuccess && xfuse_ok(c) && ensure_xedge(c) == hipSuccess) {
+        c->xw_d = pick[0][0];
+        c->xr_d = pick[0][1];
+        c->xw_f = pick[1][0];
+        c->xr_f = pick[1][1];
+        float tm[2] = {1e30f, 1e30f};
+        for (int it = 0; it < 4 && e == hipSuccess; it++) {
+            c->xfuse = it % 2;
+            float t = 0;
+            e = time_steps(&t);
+            if (e == hipSuccess) tm[it % 2] = std::min(tm[it % 2], t);
+        }
+        pick_fuse = tm[1] < tm[0] ? 1 : 0;
+        c->xfuse = pick_fuse;
+    }
     // instances per launch (batches only): the whole batch, then groups of
     // half as many while a group keeps at least 2^21 cells (fewer leave the
     // chip short of work); a group size must beat the whole batch by 1 %
@@ -1167,9 +1280,11 @@ hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
         c->xw_d = c->xw_f = 0;
         c->xr_d = c->xr_f = -1;
         c->subbatch = 0;
+        c->xfuse = fuse0;
         return e;
     }
     c->subbatch = pick_sb;
+    c->xfuse = pick_fuse;
     c->xw_d = pick[0][0];
     c->xr_d = pick[0][1];
     c->xw_f = pick[1][0];
@@ -1215,6 +1330,27 @@ int mhd2d_set_subbatch(mhd2d_ctx* c, int k)
         c->graph = nullptr;
     }
     return MHD2D_OK;
+}
+
+int mhd2d_set_xfuse(mhd2d_ctx* c, int on)
+{
+    if (!c || on < 0 || on > 1) return fail(MHD2D_EINVAL, "bad argument");
+    if (on && !xfuse_ok(c))
+        return fail(MHD2D_EINVAL, "fused x-derivative + RHS: needs ideal MHD on one periodic "
+                                  "x block, the tridiagonal x-derivative and a grid it tiles");
+    if (on) HIPCHK(ensure_xedge(c));
+    c->xfuse = on;
+    if (c->graph) {  // the captured step holds the old launches
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    return MHD2D_OK;
+}
+
+int mhd2d_get_xfuse(const mhd2d_ctx* c)
+{
+    if (!c) return fail(MHD2D_EINVAL, "bad argument");
+    return c->xfuse;
 }
 
 int mhd2d_get_subbatch(const mhd2d_ctx* c)
@@ -1405,7 +1541,7 @@ hipError_t build_multi_t(mhd2d_ens* e, bool* ok)
             x2.push_back(b);
             nv2 = f1 - f0 + 1;
         }
-        xf.push_back(xfilter_args<T>(c));
+        xf.push_back(xfilter_args<T>(c, false));
         xdyn = std::max(xdyn, 3 * (size_t)c->p.nx * std::max(sizeof(T), sizeof(TD)));
         const int rows = ysweep_multi_rows(c->p.ny, c->elem);
         const int B = c->B;

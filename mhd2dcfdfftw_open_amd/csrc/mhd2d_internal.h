@@ -74,6 +74,21 @@ struct XSolveArgs {
     // instance z of a batch (blockIdx.z) reads in[v] + z * bs_in, writes
     // out[v] + z * bs_out and checks rs[z]
     size_t bs_in = 0, bs_out = 0;
+    // derivative solves (XD1, one periodic block) that stop after the chunk
+    // scans: with carry set, out is not written; instead carry + z * bs_carry
+    // receives, per field v and chunk g of C rows, the values entering the
+    // chunk, [v][2][nx / C][ny]: y at row g C - 1 of the forward recurrence
+    // (0 for g = 0), then x at row g C + C of the backward one (0 for the
+    // last chunk)
+    T* carry = nullptr;
+    size_t bs_carry = 0;
+    // filter solves (XFILT, one periodic block, in == out) whose input's
+    // first and last 3 rows of every chunk of C rows (the rows other
+    // chunks read as halos) are not in in[v] but in edge + z * bs_edge +
+    // v * edge_fs, [v][nx / C][6][ny]: rows g C .. g C + 2, then g C + C - 3
+    // .. g C + C - 1 (what k_xrhs_rk leaves there)
+    const T* edge = nullptr;
+    size_t edge_fs = 0, bs_edge = 0;
 };
 
 // x-derivative through the dense operator M = A^-1 B split into BF16 planes
@@ -230,6 +245,25 @@ struct RhsRkArgs {           // fused RHS + low-storage RK update (driver)
     int nx = 0, ny = 0;
 };
 
+// fused x-derivative + RHS + RK (ideal MHD, one periodic x block): x holds
+// the derivative solve's arguments (in[0..8]: rho, ux, uy, uz, By, Bz, p,
+// Ey, Ez; st, tb of XD1), r the RHS + RK arguments (dudx, dEdx unused; the
+// state is read from r.u), carry / bs_carry what the carry solve
+// (XSolveArgs::carry) stored.  The updated state goes to u_out + z * bs_uo
+// The updated state of a chunk's rows that other chunks read as x halos
+// (its first and last 3) goes to edge + z * bs_edge + f * edge_fs in the
+// layout of XSolveArgs::edge, the others back to r.u: the workgroups of
+// neighbouring chunks may still be reading those rows of r.u.
+template <typename T, typename TD>
+struct XRhsArgs {
+    XSolveArgs<TD, T> x;
+    RhsRkArgs<T, TD> r;
+    const TD* carry = nullptr;
+    size_t bs_carry = 0;
+    T* edge = nullptr;
+    size_t edge_fs = 0, bs_edge = 0;
+};
+
 // ---------------------------------------------------------------- shape groups
 // One launch over the shape groups of an uneven ensemble: group g owns the
 // blocks [first[g], first[g + 1]) of a flattened grid, holds nb[g]
@@ -263,6 +297,12 @@ template <typename T>
 hipError_t launch_rhs(const RhsArgs<T>& a, hipStream_t s);
 template <typename T, typename TD = T>
 hipError_t launch_rhs_rk(const RhsRkArgs<T, TD>& a, hipStream_t s, int nb = 1);
+template <typename T, typename TD = T>
+hipError_t launch_xrhs_rk(const XRhsArgs<T, TD>& a, hipStream_t s, int nb = 1);
+// the fused x-derivative + RHS + RK covers this grid (elem_bytes: the state
+// word): the x chunks are C = xsolve_shape's rows, a workgroup's columns
+// 512 / C (FP64 state) or 1024 / C (FP32) must divide ny
+int xrhs_supported(int nx, int ny, int elem_bytes);
 // the two open boundary lines (x = 0, x = Lx): calcRHSOpenBoundaryX with the
 // characteristic terms, then the same RK update
 template <typename T, typename TD = T>

@@ -186,6 +186,180 @@ constexpr int xshared_max_words()
                                                            : XShared<32>::WORDS);
 }
 
+// Pieces of one column's compact solve over its chunk of C rows starting at
+// row s, shared by the full-line solve (k_xsolve) and the chunk re-solve of
+// the fused x-derivative + RHS (k_xrhs_rk), so that both produce the same
+// values bit for bit.  They (and the RHS / RK helpers below) contract a * b
+// + c into an FMA only within one source expression (fp contract(on)):
+// under the default cross-statement contraction the backend chooses, per
+// kernel, which product of an a * b + c * d becomes the FMA, and the two
+// kernels then differed in the last bits.
+
+// rows s-3 .. s+C+2 of the column at f; only the 3-row halos can wrap in x
+template <typename T, typename TI, int C>
+__device__ __forceinline__ void xcol_load(T (&w)[C + 6], const TI* __restrict__ f, int s, int nx,
+                                          size_t ny)
+{
+    const TI* __restrict__ lo = f + (size_t)(s > 0 ? s - 3 : nx - 3) * ny;
+    const TI* __restrict__ hi = f + (size_t)(s + C < nx ? s + C : 0) * ny;
+    const TI* __restrict__ mid = f + (size_t)s * ny;
+    // A narrowing load (FP64 state into an FP32 derivative solve) first
+    // subtracts the column's row-0 value in TI: the operator annihilates
+    // constants, and the rounding then scales with the variation along
+    // the line instead of with a large uniform background.
+    TI c0 = 0;
+    if constexpr (!std::is_same<T, TI>::value) c0 = f[0];
+#pragma unroll
+    for (int k = 0; k < 3; k++) w[k] = (T)(lo[k * ny] - c0);
+#pragma unroll
+    for (int k = 0; k < C; k++) w[k + 3] = (T)(mid[k * ny] - c0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) w[C + 3 + k] = (T)(hi[k * ny] - c0);
+}
+
+// the same rows for a filter input whose chunk edge rows (first and last 3
+// of every chunk of C rows) are in the edge buffer e (the column's word of
+// row 0 of chunk 0 there, [G][6][ny]) and the others in f
+template <typename T, int C>
+__device__ __forceinline__ void xcol_load_edge(T (&w)[C + 6], const T* __restrict__ f,
+                                               const T* __restrict__ e, int g, int G, size_t ny)
+{
+    const int s = g * C;
+    const T* __restrict__ lo = e + ((size_t)(g > 0 ? g - 1 : G - 1) * 6 + 3) * ny;
+    const T* __restrict__ own = e + (size_t)g * 6 * ny;
+    const T* __restrict__ hi = e + (size_t)(g + 1 < G ? g + 1 : 0) * 6 * ny;
+    const T* __restrict__ mid = f + (size_t)s * ny;
+#pragma unroll
+    for (int k = 0; k < 3; k++) w[k] = lo[k * ny];
+#pragma unroll
+    for (int k = 0; k < 3; k++) w[k + 3] = own[k * ny];
+#pragma unroll
+    for (int k = 3; k < C - 3; k++) w[k + 3] = mid[k * ny];
+#pragma unroll
+    for (int k = 0; k < 3; k++) w[C + k] = own[(3 + k) * ny];
+#pragma unroll
+    for (int k = 0; k < 3; k++) w[C + 3 + k] = hi[k * ny];
+}
+
+// right-hand side of chunk g in place: w[k] = r at row s + k.  first / last:
+// the chunk holds row 0 / row N-1 of a periodic line (explicit end rows).
+// BLK: x is split into blocks (and may have open ends): the rows marked in
+// tb.omask take explicit stencils over f[i-4 .. i+4] (tb.ocoef); otherwise
+// rows 0 and nx-1 are the only end rows.
+template <typename T, int C, int KIND, bool BLK>
+__device__ __forceinline__ void xcol_rhs(T (&w)[C + 6], const XStencil& st, const XTables& tb,
+                                         int g, bool first, bool last)
+{
+#pragma clang fp contract(on)
+    const T c0 = (T)st.c0, c1 = (T)st.c1, c2 = (T)st.c2, c3 = (T)st.c3;
+    unsigned long long om = 0;
+    int oi = 0;
+    if constexpr (BLK) {
+        om = tb.omask[g];
+        oi = tb.ofirst[g];
+    }
+    T saved = 0;  // f at row - 4 of the current row (w[k - 1] before it is overwritten)
+    T rfirst = 0, rlast = 0;
+    if (first)   // row 0 over f[-2..2]
+        rfirst = (T)st.e[0] * w[1] + (T)st.e[1] * w[2] + (T)st.e[2] * w[3] +
+                 (T)st.e[3] * w[4] + (T)st.e[4] * w[5];
+    if (last)    // row N-1 over f[N+1..N-3]
+        rlast = (T)st.esign * ((T)st.e[0] * w[C + 4] + (T)st.e[1] * w[C + 3] +
+                               (T)st.e[2] * w[C + 2] + (T)st.e[3] * w[C + 1] +
+                               (T)st.e[4] * w[C]);
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+        const T fm3 = w[k], fm2 = w[k + 1], fm1 = w[k + 2], f0 = w[k + 3];
+        const T fp1 = w[k + 4], fp2 = w[k + 5], fp3 = w[k + 6];
+        T r;
+        if constexpr (KIND == XD1) {
+            r = c1 * (fp1 - fm1) + c2 * (fp2 - fm2) + c3 * (fp3 - fm3);
+        } else if constexpr (KIND == XD2) {
+            r = c1 * (fp1 - 2 * f0 + fm1) + c2 * (fp2 - 2 * f0 + fm2) + c3 * (fp3 - 2 * f0 + fm3);
+        } else {
+            r = c0 * f0 + c1 * (fp1 + fm1) + c2 * (fp2 + fm2) + c3 * (fp3 + fm3);
+        }
+        if constexpr (BLK) {
+            // block edges and open-end rows: explicit stencil over f[-4 .. 4]
+            // (f[-4] exists from the second row of the chunk on, f[+4] up to
+            // the one before last; the tables put no weight where it does not)
+            if ((om >> k) & 1ull) {
+                const double* __restrict__ cf = tb.ocoef + 9 * oi;
+                T acc = (T)cf[1] * fm3 + (T)cf[2] * fm2 + (T)cf[3] * fm1 + (T)cf[4] * f0 +
+                        (T)cf[5] * fp1 + (T)cf[6] * fp2 + (T)cf[7] * fp3;
+                if (k >= 1) acc += (T)cf[0] * saved;
+                if (k + 7 <= C + 5) acc += (T)cf[8] * w[(k + 7 <= C + 5) ? k + 7 : 0];
+                r = acc;
+                oi++;
+            }
+            saved = fm3;
+        }
+        w[k] = r;  // w[k] is not read again by later rows
+    }
+    if (first) w[0] = rfirst;
+    if (last) w[C - 1] = rlast;
+}
+
+// forward recurrence y_i = r_i - l_i y_{i-1} over the chunk from the value
+// y entering it, in place
+template <typename T, typename FT, int C>
+__device__ __forceinline__ void xcol_forward(T (&w)[C + 6], T y, bool interior, T linf,
+                                             const FT* __restrict__ L)
+{
+#pragma clang fp contract(on)
+    if (interior) {
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+            y = w[k] - linf * y;
+            w[k] = y;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+            y = w[k] - (T)L[k] * y;
+            w[k] = y;
+        }
+    }
+}
+
+// backward recurrence x_i = y_i invd_i + q_i x_{i+1} over the chunk from
+// the value x entering it from above, in place
+template <typename T, typename FT, int C>
+__device__ __forceinline__ void xcol_backward(T (&w)[C + 6], T x, bool interior, T dinf, T qinf,
+                                              const FT* __restrict__ ID, const FT* __restrict__ Q)
+{
+#pragma clang fp contract(on)
+    if (interior) {
+#pragma unroll
+        for (int k = C - 1; k >= 0; k--) {
+            x = w[k] * dinf + qinf * x;
+            w[k] = x;
+        }
+    } else {
+#pragma unroll
+        for (int k = C - 1; k >= 0; k--) {
+            x = w[k] * (T)ID[k] + (T)Q[k] * x;
+            w[k] = x;
+        }
+    }
+}
+
+// Rows from tb.kconv to N-2 carry the factors' exact FP64 fixed point, so a
+// chunk inside that range uses the scalar constants (no loads).  (FP32 only:
+// in FP64 the second code path costs more registers than the table loads it
+// saves)
+#ifndef MHD2D_X_CONST
+#define MHD2D_X_CONST(T) (sizeof(T) == 4)
+#endif
+
+// x-derivative solve of a periodic line that stops after the two chunk
+// scans and stores, per chunk, the values entering it: y at the row before
+// it (forward) and x at the row after it (backward); k_xrhs_rk re-solves
+// the chunks from them.  A template value beside the XKind ones.
+constexpr int XD1_CARRY = 3;
+// filter solve whose input rows at chunk edges come from XSolveArgs::edge
+constexpr int XFILT_EDGE = 4;
+
 // One workgroup's compact solves: column strip bx, field v, instance z of
 // the launch described by `a`.  sh_in / sh_fw / sh_bw hold at least
 // XShared<W>::WORDS words, sh_half W words; x_dyn the factor tables (3 nx
@@ -196,7 +370,10 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs<T, TI>& a, int bx, 
                                             T* __restrict__ sh_bw, T* __restrict__ sh_half,
                                             unsigned char* x_dyn)
 {
+#pragma clang fp contract(on)
     if (a.rs && a.rs[z].stop) return;  // the run stopped (dt < 1e-8): uniform per block
+    constexpr bool CARRY = KIND == XD1_CARRY, EDGE = KIND == XFILT_EDGE;
+    constexpr int SK = CARRY ? XD1 : (EDGE ? XFILT : KIND);  // the stencil
     const int G = blockDim.x / W;
     const int c = threadIdx.x % W;
     const int g = threadIdx.x / W;
@@ -204,30 +381,16 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs<T, TI>& a, int bx, 
     const size_t ny = (size_t)a.ny;
     const size_t j = (size_t)bx * W + c;
     const TI* __restrict__ f = a.in[v] + z * a.bs_in + j;
-    T* __restrict__ o = a.out[v] + z * a.bs_out + j;
+    T* __restrict__ o = CARRY ? nullptr : a.out[v] + z * a.bs_out + j;
     const int s = g * C;
 
     constexpr int WP = XShared<W>::WP;
 
-    // rows s-3 .. s+C+2 of this column; only the 3-row halos can wrap in x
     T w[C + 6];
-    {
-        const TI* __restrict__ lo = f + (size_t)(s > 0 ? s - 3 : nx - 3) * ny;
-        const TI* __restrict__ hi = f + (size_t)(s + C < nx ? s + C : 0) * ny;
-        const TI* __restrict__ mid = f + (size_t)s * ny;
-        // A narrowing load (FP64 state into an FP32 derivative solve) first
-        // subtracts the column's row-0 value in TI: the operator annihilates
-        // constants, and the rounding then scales with the variation along
-        // the line instead of with a large uniform background.
-        TI c0 = 0;
-        if constexpr (!std::is_same<T, TI>::value) c0 = f[0];
-#pragma unroll
-        for (int k = 0; k < 3; k++) w[k] = (T)(lo[k * ny] - c0);
-#pragma unroll
-        for (int k = 0; k < C; k++) w[k + 3] = (T)(mid[k * ny] - c0);
-#pragma unroll
-        for (int k = 0; k < 3; k++) w[C + 3 + k] = (T)(hi[k * ny] - c0);
-    }
+    if constexpr (EDGE)
+        xcol_load_edge<T, C>(w, f, a.edge + z * a.bs_edge + v * a.edge_fs + j, g, G, ny);
+    else
+        xcol_load<T, TI, C>(w, f, s, nx, ny);
 
     // Factor rows.  With chunks of up to 32 rows (every line length now) the
     // three tables of the whole line go to LDS (3 nx words, dynamic) and the
@@ -259,65 +422,11 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs<T, TI>& a, int bx, 
         Q = a.tb.q + s;
     }
 
-    const XStencil& st = a.st;
-    const T c0 = (T)st.c0, c1 = (T)st.c1, c2 = (T)st.c2, c3 = (T)st.c3;
     const bool first = !BLK && (g == 0), last = !BLK && (g == G - 1);
-    unsigned long long om = 0;
-    int oi = 0;
-    if constexpr (BLK) {
-        om = a.tb.omask[g];
-        oi = a.tb.ofirst[g];
-    }
-    T saved = 0;  // f at row - 4 of the current row (w[k - 1] before it is overwritten)
-    T rfirst = 0, rlast = 0;
-    if (first)   // row 0 over f[-2..2]
-        rfirst = (T)st.e[0] * w[1] + (T)st.e[1] * w[2] + (T)st.e[2] * w[3] +
-                 (T)st.e[3] * w[4] + (T)st.e[4] * w[5];
-    if (last)    // row N-1 over f[N+1..N-3]
-        rlast = (T)st.esign * ((T)st.e[0] * w[C + 4] + (T)st.e[1] * w[C + 3] +
-                               (T)st.e[2] * w[C + 2] + (T)st.e[3] * w[C + 1] +
-                               (T)st.e[4] * w[C]);
-#pragma unroll
-    for (int k = 0; k < C; k++) {
-        const T fm3 = w[k], fm2 = w[k + 1], fm1 = w[k + 2], f0 = w[k + 3];
-        const T fp1 = w[k + 4], fp2 = w[k + 5], fp3 = w[k + 6];
-        T r;
-        if constexpr (KIND == XD1) {
-            r = c1 * (fp1 - fm1) + c2 * (fp2 - fm2) + c3 * (fp3 - fm3);
-        } else if constexpr (KIND == XD2) {
-            r = c1 * (fp1 - 2 * f0 + fm1) + c2 * (fp2 - 2 * f0 + fm2) + c3 * (fp3 - 2 * f0 + fm3);
-        } else {
-            r = c0 * f0 + c1 * (fp1 + fm1) + c2 * (fp2 + fm2) + c3 * (fp3 + fm3);
-        }
-        if constexpr (BLK) {
-            // block edges and open-end rows: explicit stencil over f[-4 .. 4]
-            // (f[-4] exists from the second row of the chunk on, f[+4] up to
-            // the one before last; the tables put no weight where it does not)
-            if ((om >> k) & 1ull) {
-                const double* __restrict__ cf = a.tb.ocoef + 9 * oi;
-                T acc = (T)cf[1] * fm3 + (T)cf[2] * fm2 + (T)cf[3] * fm1 + (T)cf[4] * f0 +
-                        (T)cf[5] * fp1 + (T)cf[6] * fp2 + (T)cf[7] * fp3;
-                if (k >= 1) acc += (T)cf[0] * saved;
-                if (k + 7 <= C + 5) acc += (T)cf[8] * w[(k + 7 <= C + 5) ? k + 7 : 0];
-                r = acc;
-                oi++;
-            }
-            saved = fm3;
-        }
-        w[k] = r;  // w[k] is not read again by later rows
-    }
-    if (first) w[0] = rfirst;
-    if (last) w[C - 1] = rlast;
+    xcol_rhs<T, C, SK, BLK>(w, a.st, a.tb, g, first, last);
 
     if constexpr (xsolve_lds_tables(C)) __syncthreads();  // the tables are in LDS
-    // Rows from tb.kconv to N-2 carry the factors' exact FP64 fixed point,
-    // so a chunk inside that range uses the scalar constants (no loads).
-    // (FP32 only: in FP64 the second code path costs more registers than
-    // the table loads it saves)
-#ifndef MHD2D_X_CONST
-#define MHD2D_X_CONST (sizeof(T) == 4)
-#endif
-    const bool interior = MHD2D_X_CONST && !BLK && (s >= a.tb.kconv) && !last;
+    const bool interior = MHD2D_X_CONST(T) && !BLK && (s >= a.tb.kconv) && !last;
     const T linf = (T)a.tb.linf, dinf = (T)a.tb.dinf, qinf = (T)a.tb.qinf;
 
     // ---- forward recurrence y_i = r_i - l_i y_{i-1}
@@ -405,6 +514,17 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs<T, TI>& a, int bx, 
     }
     __syncthreads();
     x = (g < G - 1) ? sh_bw[(g + 1) * WP + c] : (T)0;
+    if constexpr (CARRY) {
+        // the two values entering chunk g, [z][v][2][G][ny]
+        T* __restrict__ cy = a.carry + z * a.bs_carry + ((size_t)v * 2 * G + g) * ny + j;
+        cy[0] = (g > 0) ? sh_fw[(g - 1) * WP + c] : (T)0;
+        cy[(size_t)G * ny] = x;
+        return;
+    }
+    // (the same recurrences as xcol_forward / xcol_backward, written out
+    // here: through the helpers the compiler allocates 72 instead of 64
+    // VGPRs to the FP32 16-row solve, which then fits one workgroup per CU
+    // instead of two)
     if (interior) {
 #pragma unroll
         for (int k = C - 1; k >= 0; k--) {
@@ -421,7 +541,7 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs<T, TI>& a, int bx, 
     T* __restrict__ po = o + (size_t)s * ny;
     // row segments of at least 64 bytes; narrower ones (8 FP32 columns)
     // measured 1.5x slower as nontemporal stores
-    constexpr bool NT = (KIND == XFILT ? MHD2D_NT_XF : MHD2D_NT_X) && W * sizeof(T) >= 64;
+    constexpr bool NT = (SK == XFILT ? MHD2D_NT_XF : MHD2D_NT_X) && W * sizeof(T) >= 64;
     if (a.accumulate) {
         // second x-derivative onto the second y-derivative already there
         // (addSourceTerm's d2/dx2 + d2/dy2, calcRHS.c:498-531)
@@ -476,6 +596,19 @@ template <typename T, typename TI, int C, int W, bool BLK>
 static void launch_xsolve_kind(const XSolveArgs<T, TI>& a, dim3 grid, dim3 block, hipStream_t s)
 {
     const size_t shm = xsolve_lds_tables(C) ? 3 * (size_t)a.nx * sizeof(T) : 0;
+    if constexpr (!BLK) {
+        if (a.carry) {
+            hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD1_CARRY, false>), grid, block, shm, s, a);
+            return;
+        }
+        if constexpr (std::is_same<T, TI>::value) {
+            if (a.edge) {
+                hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XFILT_EDGE, false>), grid, block, shm, s,
+                                   a);
+                return;
+            }
+        }
+    }
     if constexpr (std::is_same<T, TI>::value) {
         switch (a.kind) {
         case XD1: hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD1, BLK>), grid, block, shm, s, a); break;
@@ -511,6 +644,10 @@ hipError_t launch_xsolve(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int 
     // converting inputs: derivatives only (they annihilate the subtracted
     // constant; the filter does not)
     if (!std::is_same<T, TI>::value && a.kind == XFILT) return hipErrorInvalidValue;
+    // carries: first derivatives of one periodic block, nothing accumulated;
+    // edge rows: filters of one periodic block
+    if (a.carry && (a.kind != XD1 || a.tb.omask || a.accumulate)) return hipErrorInvalidValue;
+    if (a.edge && (a.kind != XFILT || a.tb.omask || a.accumulate)) return hipErrorInvalidValue;
     int C, W, G;
     xsolve_shape(a.nx, sizeof(TI), &C, &W, &G);
     if (a.width) {
@@ -1094,6 +1231,42 @@ __device__ __forceinline__ void stp(T* p, const Pack<T, W>& v)
     }
 }
 
+// calcRHSInnerPoint (calcRHS.c:85-142) at one cell of ideal MHD: R of the 8
+// fields from the state uc, its x- and y-derivatives d_x, d_y (state slots;
+// d_x[BX] and d_y[BY] are not read), dEy/dx, dEz/dx (ex0, ex1) and dEx/dy,
+// dEz/dy (ey0, ey1)
+template <typename T>
+__device__ __forceinline__ void rhs_point(const T (&uc)[8], const T (&d_x)[8], const T (&d_y)[8],
+                                          T ex0, T ex1, T ey0, T ey1, T g, T (&R)[8])
+{
+#pragma clang fp contract(on)
+    const T rho = uc[RHO], ux = uc[UX], uy = uc[UY];
+    const T bx = uc[BX], by = uc[BY], bz = uc[BZ], pr = uc[PR];
+    const T divU = d_x[UX] + d_y[UY];
+    const T jx = d_y[BZ];
+    const T jy = -d_x[BZ];
+    const T jz = d_x[BY] - d_y[BX];
+    const T ir = (T)1 / rho;
+    R[RHO] = -rho * divU - ux * d_x[RHO] - uy * d_y[RHO];
+    R[UX] = -ux * d_x[UX] - uy * d_y[UX] - d_x[PR] * ir + jy * bz * ir - jz * by * ir;
+    R[UY] = -ux * d_x[UY] - uy * d_y[UY] - d_y[PR] * ir + jz * bx * ir - jx * bz * ir;
+    R[UZ] = -ux * d_x[UZ] - uy * d_y[UZ] + jx * by * ir - jy * bx * ir;
+    // calcRHS.c:135-137
+    R[BX] = -ey1;
+    R[BY] = ex1;
+    R[BZ] = ey0 - ex0;
+    R[PR] = -ux * d_x[PR] - uy * d_y[PR] - g * pr * divU;
+}
+
+// rkExecute (timeAdvance.c:31-35): u + cc R + dd R_prev (R_prev is zero in
+// stage 0)
+template <typename T>
+__device__ __forceinline__ T rk_update(T u, T R, T rp, T cc, T dd)
+{
+#pragma clang fp contract(on)
+    return u + cc * R + dd * rp;
+}
+
 // fused RHS + RK update; each thread handles W = 16 / sizeof(T) consecutive
 // cells (one 16-byte vector of each state field) per field (n is a multiple
 // of W: nx, ny are powers of two).  Derivatives and R_prev are read and
@@ -1149,29 +1322,17 @@ __device__ __forceinline__ void rhs_rk_body(const RhsRkArgs<T, TD>& a, int z, in
         Pack<T, W> R[8];
 #pragma unroll
         for (int w = 0; w < W; w++) {
-            // calcRHSInnerPoint, calcRHS.c:85-142
-            const T rho = u[RHO].x[w], ux = u[UX].x[w], uy = u[UY].x[w];
-            const T bx = u[BX].x[w], by = u[BY].x[w], bz = u[BZ].x[w], pr = u[PR].x[w];
-            T d_x[8], d_y[8];
+            T uc[8], d_x[8], d_y[8], Rc[8];
 #pragma unroll
             for (int f = 0; f < 8; f++) {
+                uc[f] = u[f].x[w];
                 d_x[f] = (T)dx[f].x[w];
                 d_y[f] = (T)dy[f].x[w];
             }
-            const T divU = d_x[UX] + d_y[UY];
-            const T jx = d_y[BZ];
-            const T jy = -d_x[BZ];
-            const T jz = d_x[BY] - d_y[BX];
-            const T ir = (T)1 / rho;
-            R[RHO].x[w] = -rho * divU - ux * d_x[RHO] - uy * d_y[RHO];
-            R[UX].x[w] = -ux * d_x[UX] - uy * d_y[UX] - d_x[PR] * ir + jy * bz * ir - jz * by * ir;
-            R[UY].x[w] = -ux * d_x[UY] - uy * d_y[UY] - d_y[PR] * ir + jz * bx * ir - jx * bz * ir;
-            R[UZ].x[w] = -ux * d_x[UZ] - uy * d_y[UZ] + jx * by * ir - jy * bx * ir;
-            // calcRHS.c:135-137; ex = (dEy/dx, dEz/dx), ey = (dEx/dy, dEz/dy)
-            R[BX].x[w] = -(T)ey[1].x[w];
-            R[BY].x[w] = (T)ex[1].x[w];
-            R[BZ].x[w] = (T)ey[0].x[w] - (T)ex[0].x[w];
-            R[PR].x[w] = -ux * d_x[PR] - uy * d_y[PR] - g * pr * divU;
+            rhs_point<T>(uc, d_x, d_y, (T)ex[0].x[w], (T)ex[1].x[w], (T)ey[0].x[w],
+                         (T)ey[1].x[w], g, Rc);
+#pragma unroll
+            for (int f = 0; f < 8; f++) R[f].x[w] = Rc[f];
         }
         Pack<T, W> rpsi;
         if (GLM) {
@@ -1237,7 +1398,7 @@ __device__ __forceinline__ void rhs_rk_body(const RhsRkArgs<T, TD>& a, int z, in
             Pack<TD, W> rs_out;
 #pragma unroll
             for (int w = 0; w < W; w++) {
-                o.x[w] = u[f].x[w] + cc * R[f].x[w] + dd * (use_prev ? (T)rp.x[w] : (T)0);
+                o.x[w] = rk_update<T>(u[f].x[w], R[f].x[w], use_prev ? (T)rp.x[w] : (T)0, cc, dd);
                 rs_out.x[w] = (TD)R[f].x[w];
             }
             stp<MHD2D_NT_R, W>(uz + f * n + p, o);
@@ -1270,6 +1431,167 @@ __global__ void __launch_bounds__(256)
     // starts on the instance this one touched last
     const int z = (int)(gridDim.y - 1 - blockIdx.y);
     rhs_rk_body<T, TD, MODE>(a, z, blockIdx.x, gridDim.x);
+}
+
+// ------------------------------------------ fused x-derivative + RHS + RK
+// Ideal MHD on one periodic x block.  A workgroup takes one chunk of C rows
+// (the chunks of the full x-solve) by WB columns of one instance.  It
+// re-solves the x-derivatives of the nine fields the RHS reads (in[0..8] of
+// the derivative arguments: rho, ux, uy, uz, By, Bz, p, Ey, Ez) on its chunk
+// from the carries k_xsolve<..., XD1_CARRY> stored, with the recurrences of
+// the full solve (so the values are the same bit for bit), into LDS; then
+// it applies the RHS + RK update to the chunk's cells.  The new state of
+// the chunk's first and last 3 rows, which the chunks above and below read
+// as halos meanwhile, goes to a small edge buffer (6 rows per chunk), the
+// rest back in place; the x-filter reads the edge rows from there.  The
+// derivatives are neither written to nor read back from HBM (18 words per
+// cell and stage in FP64).
+template <typename T>
+struct XRhsShape {
+    static constexpr int CELLS = sizeof(T) == 8 ? 512 : 1024;  // cells per workgroup
+    static constexpr int PW = 16 / (int)sizeof(T);             // cells per thread: 256 threads
+    static constexpr int NV = 9;                               // derivative fields
+};
+
+template <typename T, typename TD, int C>
+__global__ void __launch_bounds__(256) k_xrhs_rk(XRhsArgs<T, TD> a)
+{
+    using S = XRhsShape<T>;
+    constexpr int PW = S::PW, NV = S::NV;
+    constexpr int WB = S::CELLS / C;          // columns of the chunk
+    constexpr int NI = NV * WB;               // column solves
+    constexpr int IPT = (NI + 255) / 256;     // column solves per thread
+    // LDS words per field: the pad puts the fields that one wavefront writes
+    // together (same column, same row) on different banks
+    constexpr int FS = S::CELLS + WB;
+    // read back as PW-word vectors (16 or 8 bytes)
+    __shared__ __align__(16) TD sdx[NV * FS];
+    __shared__ TD stab[3 * C];
+    const XSolveArgs<TD, T>& xa = a.x;
+    const RhsRkArgs<T, TD>& ra = a.r;
+    const int bx = blockIdx.x, g = blockIdx.y;
+    // instances last to first, as k_rhs_rk (the carry solve before it runs
+    // them first to last)
+    const int z = (int)(gridDim.z - 1 - blockIdx.z);
+    const RunScalars* const rs = ra.rs + z;
+    if (rs->stop) return;  // the run stopped (dt < 1e-8): uniform per block
+    const int nx = xa.nx;
+    const size_t ny = (size_t)xa.ny;
+    const int G = nx / C;
+    const int s = g * C;
+    const bool first = g == 0, last = g == G - 1;
+
+    // the chunk's rows (with halos) and its two carries, for every column
+    // solve of this thread; then the factor rows of the chunk
+    TD w[IPT][C + 6];
+    TD yin[IPT], xin[IPT];
+#pragma unroll
+    for (int i = 0; i < IPT; i++) {
+        const int it = threadIdx.x + 256 * i;
+        if (it < NI) {
+            const int v = it / WB, c = it % WB;
+            const size_t j = (size_t)bx * WB + c;
+            xcol_load<TD, T, C>(w[i], xa.in[v] + z * xa.bs_in + j, s, nx, ny);
+            const TD* __restrict__ cy =
+                a.carry + z * a.bs_carry + ((size_t)v * 2 * G + g) * ny + j;
+            yin[i] = cy[0];
+            xin[i] = cy[(size_t)G * ny];
+        }
+    }
+    if (threadIdx.x < C) {
+        stab[threadIdx.x] = (TD)xa.tb.l[s + threadIdx.x];
+        stab[C + threadIdx.x] = (TD)xa.tb.invd[s + threadIdx.x];
+        stab[2 * C + threadIdx.x] = (TD)xa.tb.q[s + threadIdx.x];
+    }
+    __syncthreads();
+    const bool interior = MHD2D_X_CONST(TD) && (s >= xa.tb.kconv) && !last;
+    const TD linf = (TD)xa.tb.linf, dinf = (TD)xa.tb.dinf, qinf = (TD)xa.tb.qinf;
+#pragma unroll
+    for (int i = 0; i < IPT; i++) {
+        const int it = threadIdx.x + 256 * i;
+        if (it < NI) {
+            const int v = it / WB, c = it % WB;
+            xcol_rhs<TD, C, XD1, false>(w[i], xa.st, xa.tb, g, first, last);
+            xcol_forward<TD, TD, C>(w[i], yin[i], interior, linf, stab);
+            xcol_backward<TD, TD, C>(w[i], xin[i], interior, dinf, qinf, stab + C, stab + 2 * C);
+            TD* const d = sdx + v * FS + c;
+#pragma unroll
+            for (int k = 0; k < C; k++) d[k * WB] = w[i][k];
+        }
+    }
+    __syncthreads();
+
+    // RHS + RK of PW consecutive cells of one row of the chunk
+    const int kr = threadIdx.x / (WB / PW);
+    const int col = (threadIdx.x % (WB / PW)) * PW;
+    const size_t n = ra.n;
+    const size_t p = (size_t)(s + kr) * ny + (size_t)bx * WB + col;
+    const T cc = (T)rs->cc[ra.stage];
+    const T dd = (T)rs->dd[ra.stage];
+    T* const uz = ra.u + z * ra.bs_u;
+    // this row's destination: the edge buffer for the first and last 3 rows
+    // of the chunk (other chunks read them as halos), else the state in place
+    T* uo;
+    size_t po, fs;
+    if (kr < 3 || kr >= C - 3) {
+        uo = a.edge + z * a.bs_edge;
+        po = ((size_t)g * 6 + (kr < 3 ? kr : kr - C + 6)) * ny + (size_t)bx * WB + col;
+        fs = a.edge_fs;
+    } else {
+        uo = uz;
+        po = p;
+        fs = n;
+    }
+    TD* const rpz = ra.rprev + z * ra.bs_r;
+    const TD* const dyz = ra.dudy + z * ra.bs_d;
+    const TD* const eyz = ra.dEdy + z * ra.bs_d;
+    const T g_ = (T)ra.gamma;
+    const bool use_prev = ra.stage > 0;
+    Pack<T, PW> u[8];
+    Pack<TD, PW> dy[8], ey[2], dxv[NV];
+#pragma unroll
+    for (int f = 0; f < 8; f++) {
+        u[f] = ldp<PW>(uz + f * n + p);
+        dy[f] = ldn<PW>(dyz + f * n + p);
+    }
+#pragma unroll
+    for (int f = 0; f < 2; f++) ey[f] = ldn<PW>(eyz + f * n + p);
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+        dxv[v] = *reinterpret_cast<const Pack<TD, PW>*>(sdx + v * FS + kr * WB + col);
+    // derivative field v -> state slot (Bx is not differentiated)
+    constexpr int SLOT[7] = {RHO, UX, UY, UZ, BY, BZ, PR};
+    Pack<T, PW> R[8];
+#pragma unroll
+    for (int w = 0; w < PW; w++) {
+        T uc[8], d_x[8], d_y[8], Rc[8];
+#pragma unroll
+        for (int f = 0; f < 8; f++) {
+            uc[f] = u[f].x[w];
+            d_x[f] = (T)0;
+            d_y[f] = (T)dy[f].x[w];
+        }
+#pragma unroll
+        for (int v = 0; v < 7; v++) d_x[SLOT[v]] = (T)dxv[v].x[w];
+        rhs_point<T>(uc, d_x, d_y, (T)dxv[7].x[w], (T)dxv[8].x[w], (T)ey[0].x[w], (T)ey[1].x[w],
+                     g_, Rc);
+#pragma unroll
+        for (int f = 0; f < 8; f++) R[f].x[w] = Rc[f];
+    }
+#pragma unroll
+    for (int f = 0; f < 8; f++) {
+        Pack<TD, PW> rp;
+        if (use_prev) rp = ldn<PW>(rpz + f * n + p);
+        Pack<T, PW> o;
+        Pack<TD, PW> rs_out;
+#pragma unroll
+        for (int w = 0; w < PW; w++) {
+            o.x[w] = rk_update<T>(u[f].x[w], R[f].x[w], use_prev ? (T)rp.x[w] : (T)0, cc, dd);
+            rs_out.x[w] = (TD)R[f].x[w];
+        }
+        stp<MHD2D_NT_R, PW>(uo + f * fs + po, o);
+        if (ra.write_rprev) stp<MHD2D_NT_R, PW>(rpz + f * n + p, rs_out);
+    }
 }
 
 // calcCharacteristicsX (calcRHS.c:253-495) at one open boundary cell:
@@ -1631,6 +1953,34 @@ hipError_t launch_rhs_rk(const RhsRkArgs<T, TD>& a, hipStream_t s, int nb)
     return hipGetLastError();
 }
 
+int xrhs_supported(int nx, int ny, int elem_bytes)
+{
+    int C, W, G;
+    xsolve_shape(nx, elem_bytes, &C, &W, &G);  // C depends on nx only
+    const int WB = (elem_bytes == 8 ? 512 : 1024) / C;
+    return nx % C == 0 && G >= 2 && G <= 128 && ny % WB == 0;
+}
+
+template <typename T, typename TD>
+hipError_t launch_xrhs_rk(const XRhsArgs<T, TD>& a, hipStream_t s, int nb)
+{
+    if (nb < 1 || nb > 65535 || !a.carry || !a.edge || a.x.tb.omask || a.r.open_x || a.r.lap ||
+        a.r.ddivb || a.r.glm)
+        return hipErrorInvalidValue;
+    if (!xrhs_supported(a.x.nx, a.x.ny, sizeof(T))) return hipErrorInvalidValue;
+    int C, W, G;
+    xsolve_shape(a.x.nx, sizeof(T), &C, &W, &G);
+    const int WB = XRhsShape<T>::CELLS / C;
+    dim3 grid(a.x.ny / WB, G, nb), block(256);
+    switch (C) {
+    case 8: hipLaunchKernelGGL((k_xrhs_rk<T, TD, 8>), grid, block, 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_xrhs_rk<T, TD, 16>), grid, block, 0, s, a); break;
+    case 32: hipLaunchKernelGGL((k_xrhs_rk<T, TD, 32>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_rk(T* u, const T* dudt, T* dudtRK, double cc, double dd, size_t n,
                      hipStream_t s)
@@ -1953,5 +2303,11 @@ template hipError_t launch_rhs_rk<double, float>(const RhsRkArgs<double, float>&
                                                  int);
 template hipError_t launch_rhs_rk_open<double, float>(const RhsRkArgs<double, float>&,
                                                       hipStream_t, int);
+
+template hipError_t launch_xrhs_rk<double, double>(const XRhsArgs<double, double>&, hipStream_t,
+                                                   int);
+template hipError_t launch_xrhs_rk<float, float>(const XRhsArgs<float, float>&, hipStream_t, int);
+template hipError_t launch_xrhs_rk<double, float>(const XRhsArgs<double, float>&, hipStream_t,
+                                                  int);
 
 }  // namespace mhd2d
