@@ -230,6 +230,8 @@ int mhd2d_get_subbatch(const mhd2d_ctx *c);
  * arrays never reach device memory.  Results are bitwise those of on = 0.
  * Ideal MHD (no viscosity, resistivity or div B cleaning) on one periodic x
  * block with the tridiagonal x-derivative only: MHD2D_EINVAL otherwise.
+ * A context starts with it on where it applies in MHD2D_FP64 and MHD2D_FP32
+ * and off in MHD2D_MIXED; mhd2d_autotune keeps whichever runs faster.
  * mhd2d_get_xfuse returns the setting. */
 int mhd2d_set_xfuse(mhd2d_ctx *c, int on);
 int mhd2d_get_xfuse(const mhd2d_ctx *c);

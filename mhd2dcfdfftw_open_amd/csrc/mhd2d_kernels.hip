@@ -1481,6 +1481,42 @@ __global__ void __launch_bounds__(256) k_xrhs_rk(XRhsArgs<T, TD> a)
     const int s = g * C;
     const bool first = g == 0, last = g == G - 1;
 
+    // this thread's cells in the RHS + RK phase: PW consecutive cells of
+    // row kr of the chunk
+    const int kr = threadIdx.x / (WB / PW);
+    const int col = (threadIdx.x % (WB / PW)) * PW;
+    const size_t n = ra.n;
+    const size_t p = (size_t)(s + kr) * ny + (size_t)bx * WB + col;
+    T* const uz = ra.u + z * ra.bs_u;
+    TD* const rpz = ra.rprev + z * ra.bs_r;
+    const TD* const dyz = ra.dudy + z * ra.bs_d;
+    const TD* const eyz = ra.dEdy + z * ra.bs_d;
+    const bool use_prev = ra.stage > 0;
+    Pack<T, PW> u[8];
+    Pack<TD, PW> dy[8], ey[2], rp[8];
+    // the RHS phase's streaming inputs (y-derivatives, R_prev) are requested
+    // before the chunk solves, so that their latency overlaps the solves'
+    // own loads and arithmetic (MHD2D_XRHS_PREFETCH 2: the state as well)
+#ifndef MHD2D_XRHS_PREFETCH
+#define MHD2D_XRHS_PREFETCH 0
+#endif
+    auto load_streams = [&]() {
+#pragma unroll
+        for (int f = 0; f < 8; f++) dy[f] = ldn<PW>(dyz + f * n + p);
+#pragma unroll
+        for (int f = 0; f < 2; f++) ey[f] = ldn<PW>(eyz + f * n + p);
+        if (use_prev) {
+#pragma unroll
+            for (int f = 0; f < 8; f++) rp[f] = ldn<PW>(rpz + f * n + p);
+        }
+    };
+    auto load_state = [&]() {
+#pragma unroll
+        for (int f = 0; f < 8; f++) u[f] = ldp<PW>(uz + f * n + p);
+    };
+    if constexpr (MHD2D_XRHS_PREFETCH >= 1) load_streams();
+    if constexpr (MHD2D_XRHS_PREFETCH >= 2) load_state();
+
     // the chunk's rows (with halos) and its two carries, for every column
     // solve of this thread; then the factor rows of the chunk
     TD w[IPT][C + 6];
@@ -1521,14 +1557,11 @@ __global__ void __launch_bounds__(256) k_xrhs_rk(XRhsArgs<T, TD> a)
     }
     __syncthreads();
 
-    // RHS + RK of PW consecutive cells of one row of the chunk
-    const int kr = threadIdx.x / (WB / PW);
-    const int col = (threadIdx.x % (WB / PW)) * PW;
-    const size_t n = ra.n;
-    const size_t p = (size_t)(s + kr) * ny + (size_t)bx * WB + col;
+    // RHS + RK
+    if constexpr (MHD2D_XRHS_PREFETCH < 1) load_streams();
+    if constexpr (MHD2D_XRHS_PREFETCH < 2) load_state();
     const T cc = (T)rs->cc[ra.stage];
     const T dd = (T)rs->dd[ra.stage];
-    T* const uz = ra.u + z * ra.bs_u;
     // this row's destination: the edge buffer for the first and last 3 rows
     // of the chunk (other chunks read them as halos), else the state in place
     T* uo;
@@ -1542,20 +1575,8 @@ __global__ void __launch_bounds__(256) k_xrhs_rk(XRhsArgs<T, TD> a)
         po = p;
         fs = n;
     }
-    TD* const rpz = ra.rprev + z * ra.bs_r;
-    const TD* const dyz = ra.dudy + z * ra.bs_d;
-    const TD* const eyz = ra.dEdy + z * ra.bs_d;
     const T g_ = (T)ra.gamma;
-    const bool use_prev = ra.stage > 0;
-    Pack<T, PW> u[8];
-    Pack<TD, PW> dy[8], ey[2], dxv[NV];
-#pragma unroll
-    for (int f = 0; f < 8; f++) {
-        u[f] = ldp<PW>(uz + f * n + p);
-        dy[f] = ldn<PW>(dyz + f * n + p);
-    }
-#pragma unroll
-    for (int f = 0; f < 2; f++) ey[f] = ldn<PW>(eyz + f * n + p);
+    Pack<TD, PW> dxv[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++)
         dxv[v] = *reinterpret_cast<const Pack<TD, PW>*>(sdx + v * FS + kr * WB + col);
@@ -1580,13 +1601,11 @@ __global__ void __launch_bounds__(256) k_xrhs_rk(XRhsArgs<T, TD> a)
     }
 #pragma unroll
     for (int f = 0; f < 8; f++) {
-        Pack<TD, PW> rp;
-        if (use_prev) rp = ldn<PW>(rpz + f * n + p);
         Pack<T, PW> o;
         Pack<TD, PW> rs_out;
 #pragma unroll
         for (int w = 0; w < PW; w++) {
-            o.x[w] = rk_update<T>(u[f].x[w], R[f].x[w], use_prev ? (T)rp.x[w] : (T)0, cc, dd);
+            o.x[w] = rk_update<T>(u[f].x[w], R[f].x[w], use_prev ? (T)rp[f].x[w] : (T)0, cc, dd);
             rs_out.x[w] = (TD)R[f].x[w];
         }
         stp<MHD2D_NT_R, PW>(uo + f * fs + po, o);

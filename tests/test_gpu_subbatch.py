@@ -98,8 +98,10 @@ def test_group_size_bounds_and_profile(m):
     c.set_subbatch(1)
     c.run(1)
     p = c.profile_run(2)
-    # one y-sweep, x-derivative, RHS + RK and x-filter per stage, one dt
-    assert {k: v[1] for k, v in p.items()} == {"ysweep": 6, "xderiv": 6, "rhs_rk": 6,
+    # one y-sweep, x-derivative (or carry solve), RHS + RK (or the fused
+    # chunk kernel) and x-filter per stage, one dt
+    xd, rk = ("xcarry", "xrhs_rk") if c.xfuse else ("xderiv", "rhs_rk")
+    assert {k: v[1] for k, v in p.items()} == {"ysweep": 6, xd: 6, rk: 6,
                                                 "xfilter": 6, "dt_finalize": 2}
     assert all(v[0] > 0 for v in p.values())
 

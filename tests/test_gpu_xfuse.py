@@ -46,9 +46,8 @@ def _run(m, nx, ny, prec, fuse, steps, graph, B=1, sub=0, dis=None, hall=1):
         if dis is not None:
             c.set_di(b, dis[b])
         c.set_state(_state(m, nx, ny, 11 + b), inst=b)
-    if fuse:
-        c.set_xfuse(1)
-        assert c.xfuse == 1
+    c.set_xfuse(1 if fuse else 0)
+    assert c.xfuse == (1 if fuse else 0)
     if sub:
         c.set_subbatch(sub)
     c.run(steps, graph=graph)
@@ -116,6 +115,16 @@ def test_fused_against_oracle(m):
     assert n == steps
     assert abs(t - sim.time) <= 1e-12 * abs(sim.time)
     assert_fields_close(got, sim.state(), what="fused vs oracle")
+
+
+def test_fused_by_default_in_fp64_and_fp32(m):
+    for prec, want in (("fp64", 1), ("fp32", 1), ("mixed", 0)):
+        c = m.Context(256, 128, precision=_prec(m, prec), hall=1, di=0.05)
+        assert c.xfuse == want, prec
+        c.close()
+    c = m.Context(256, 128, precision=m.FP64, hall=1, di=0.05, viscosity=1e-3)
+    assert c.xfuse == 0
+    c.close()
 
 
 REFUSED = {
