@@ -129,39 +129,62 @@ def cpu_baseline(nx, ny, u0, hall, di, c1_steps=200, c1_steps_1t=40):
 COPY_BYTES = 1 << 30
 # every context first runs the x-solve shape autotuner (mhd2d_autotune)
 AUTOTUNE = True
+# fused x-derivative + RHS: None = the context's own choice (the autotuner's
+# where it runs), 0 / 1 = forced where it applies
+XFUSE = None
+
+
+def _apply_xfuse(ctx):
+    if XFUSE is not None:
+        try:
+            ctx.set_xfuse(XFUSE)
+        except Exception:
+            pass  # the path does not apply to this context
 
 
 def _pmc_rows(path):
-    """rows of a rocprofv3 counter_collection.csv: (kernel name, counter, value)"""
+    """rows of a rocprofv3 counter_collection.csv: (kernel name, counter,
+    value), of each kernel's dispatches with its largest grid only (the
+    one-field x-solves of the diagnostics share the step's kernel)"""
     import csv
-    out = []
+    rows = []
     with open(path) as f:
         for r in csv.DictReader(f):
             name = r.get("Kernel_Name") or r.get("Kernel-Name") or ""
             ctr = r.get("Counter_Name") or r.get("Counter-Name") or ""
             val = r.get("Counter_Value") or r.get("Counter-Value") or "0"
-            out.append((name, ctr, float(val)))
-    return out
+            grid = int(float(r.get("Grid_Size") or r.get("Grid-Size") or 0))
+            rows.append((name, ctr, float(val), grid))
+    gmax = {}
+    for name, _, _, grid in rows:
+        gmax[name] = max(gmax.get(name, 0), grid)
+    return [(name, ctr, val) for name, ctr, val, grid in rows if grid == gmax[name]]
 
 
 PMC_HEADER = """# HBM traffic per launch from rocprofv3 PMC counters, {nx}x{nx} FP64 turbulence
 # commands (GPU host, repo root $REPO; cd /tmp && export TMPDIR=/tmp first):
-#   rocprofv3 --pmc FETCH_SIZE --output-format csv -d $REPO/gpurun_out/pmc_fetch -o run -- python3 $REPO/bench.py --steps 2 --warmup 1 --no-cpu --no-forms --profile-steps 1 --graph 0
-#   rocprofv3 --pmc WRITE_SIZE --output-format csv -d $REPO/gpurun_out/pmc_write -o run -- python3 $REPO/bench.py --steps 2 --warmup 1 --no-cpu --no-forms --profile-steps 1 --graph 0
+#   rocprofv3 --pmc FETCH_SIZE --output-format csv -d $REPO/gpurun_out/pmc_fetch -o run -- python3 $REPO/bench.py --steps 2 --warmup 1 --no-cpu --no-forms --profile-steps 1 --graph 0 --no-autotune --xfuse 1
+#   rocprofv3 --pmc WRITE_SIZE --output-format csv -d $REPO/gpurun_out/pmc_write -o run -- python3 $REPO/bench.py --steps 2 --warmup 1 --no-cpu --no-forms --profile-steps 1 --graph 0 --no-autotune --xfuse 1
 #   rocprofv3 --pmc FETCH_SIZE --output-format csv -d $REPO/gpurun_out/pmc_copy_fetch -o run -- python3 $REPO/bench.py --copy-probe
 #   rocprofv3 --pmc WRITE_SIZE --output-format csv -d $REPO/gpurun_out/pmc_copy_write -o run -- python3 $REPO/bench.py --copy-probe
-#   python bench.py --summarize-pmc <the four run_counter_collection.csv>
+#   the first two again with --xfuse 0 into pmc_fetch_unfused, pmc_write_unfused
+#   python bench.py --summarize-pmc <the four run_counter_collection.csv> <the two unfused ones>
 # calibration: the 1 GiB copy kernel's counters scale FETCH by {fs:.3f}, WRITE by {ws:.3f}
-# per launch: median over dispatches; the RHS + RK: mean over whole steps
-# (its R_prev traffic differs by stage)
+# per launch: median over a kernel's full-size dispatches; the RHS + RK and
+# the fused chunk kernel: mean over whole steps (their R_prev traffic
+# differs by stage).  Fused path (the default in FP64): ysweep, xcarry,
+# xrhs_rk, xfilter (its chunk edge rows from the fused kernel's buffer);
+# unfused path: xderiv, rhs_rk, xfilter_unfused
 """
 
 
-def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, nx, ny):
+def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, nx, ny,
+                  fetch_unfused=None, write_unfused=None):
     """HBM bytes per launch of each kernel class from two --pmc passes
     (FETCH_SIZE, WRITE_SIZE in KiB per dispatch), scaled by what the same
     counters read for the copy kernel of known size (COPY_BYTES read and
-    COPY_BYTES written per launch)."""
+    COPY_BYTES written per launch); the fused path's classes from the first
+    pair of passes, the unfused path's from the optional second pair."""
     def per_kernel(path, counter):
         acc = {}
         for name, ctr, val in _pmc_rows(path):
@@ -169,10 +192,10 @@ def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, nx, ny):
                 continue
             acc.setdefault(name, []).append(val * 1024.0)
         # median over dispatches: drops the odd launches of the same kernel
-        # (first-stage y-sweep without filter, 1-field diagnostics solves);
-        # the RHS + RK moves different bytes per stage (R_prev is not read in
-        # stage 1 nor written in stage 3), so it takes the mean over its
-        # launches, whole steps of three each
+        # (first-stage y-sweep without filter); the RHS + RK moves different
+        # bytes per stage (R_prev is not read in stage 1 nor written in
+        # stage 3), so it takes the mean over its launches, whole steps of
+        # three each
         out = {}
         for k, v in acc.items():
             if "k_rhs_rk" in k or "k_xrhs_rk" in k:
@@ -188,45 +211,41 @@ def summarize_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, nx, ny):
     fscale = COPY_BYTES / cf[ck[0]] if ck and cf[ck[0]] > 0 else 1.0
     ck = [k for k in cw if "k_copy" in k]
     wscale = COPY_BYTES / cw[ck[0]] if ck and cw[ck[0]] > 0 else 1.0
-    fetch = per_kernel(fetch_csv, "FETCH_SIZE")
-    write = per_kernel(write_csv, "WRITE_SIZE")
-    ndisp = {}
-    for name, ctr, _ in _pmc_rows(fetch_csv):
-        if ctr == "FETCH_SIZE":
-            ndisp[name] = ndisp.get(name, 0) + 1
-    classes = {"ysweep": "k_ysweep", "xderiv": "k_xsolve", "rhs_rk": "k_rhs_rk",
-               "xfilter": "k_xsolve", "xcarry": "k_xsolve", "xrhs_rk": "k_xrhs_rk"}
     alg = kernel_alg_bytes(nx, ny)
+    alg["xfilter_unfused"] = alg["xfilter"]
+    # class: (kernel name part, k_xsolve KIND or None); KIND 0 derivative,
+    # 2 filter, 3 carry solve, 4 filter with edge rows
+    fused = {"ysweep": ("k_ysweep", None), "xcarry": ("k_xsolve", 3),
+             "xrhs_rk": ("k_xrhs_rk", None), "xfilter": ("k_xsolve", 4)}
+    unfused = {"xderiv": ("k_xsolve", 0), "rhs_rk": ("k_rhs_rk", None),
+               "xfilter_unfused": ("k_xsolve", 2)}
+    passes = [(fused, fetch_csv, write_csv)]
+    if fetch_unfused and write_unfused:
+        passes.append((unfused, fetch_unfused, write_unfused))
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
     rows = []
-    for cls, key in classes.items():
-        names = [k for k in fetch if key in k]
-        # k_xsolve<T, TI, C, W, KIND, BLK>: KIND 0 derivative, 2 filter, 3
-        # carry solve, 4 filter with its chunk edge rows from the fused path
-        if cls == "xderiv":
-            names = [k for k in names if ", 0, false>(" in k]
-        if cls == "xcarry":
-            names = [k for k in names if ", 3, false>(" in k]
-        if cls == "xfilter":
-            # the filter the timed steps ran: the one with more dispatches (the
-            # autotuner's steps run the other path too)
-            names = sorted([k for k in names if ", 2, false>(" in k or ", 4, false>(" in k],
-                           key=lambda k: -ndisp.get(k, 0))
-        if not names:
-            continue
-        k = names[0]
-        rd = fetch[k] * fscale
-        wr = write.get(k, 0.0) * wscale
-        rec = {"kernel": k, "workload": f"{nx}x{ny} FP64 turbulence",
-               "fetch_bytes_per_launch": rd, "write_bytes_per_launch": wr,
-               "bytes_per_launch": rd + wr, "alg_bytes_per_launch": alg[cls],
-               "traffic_over_alg": (rd + wr) / alg[cls],
-               "copy_scale": {"fetch": fscale, "write": wscale}}
-        with open(os.path.join(ROOT, "profiles", f"traffic_{cls}_{nx}.json"), "w") as f:
-            json.dump(rec, f, indent=1)
-        print(json.dumps(rec))
-        rows.append(f"{cls} | {rd:.3e} | {wr:.3e} | {rd + wr:.3e} | {alg[cls]:.3e} | "
-                    f"{(rd + wr) / alg[cls]:.3f}")
+    for classes, fcsv, wcsv in passes:
+        fetch = per_kernel(fcsv, "FETCH_SIZE")
+        write = per_kernel(wcsv, "WRITE_SIZE")
+        for cls, (key, kind) in classes.items():
+            names = [k for k in fetch if key + "<" in k]
+            if kind is not None:
+                names = [k for k in names if f", {kind}, false>(" in k]
+            if not names:
+                continue
+            k = names[0]
+            rd = fetch[k] * fscale
+            wr = write.get(k, 0.0) * wscale
+            rec = {"kernel": k, "workload": f"{nx}x{ny} FP64 turbulence",
+                   "fetch_bytes_per_launch": rd, "write_bytes_per_launch": wr,
+                   "bytes_per_launch": rd + wr, "alg_bytes_per_launch": alg[cls],
+                   "traffic_over_alg": (rd + wr) / alg[cls],
+                   "copy_scale": {"fetch": fscale, "write": wscale}}
+            with open(os.path.join(ROOT, "profiles", f"traffic_{cls}_{nx}.json"), "w") as f:
+                json.dump(rec, f, indent=1)
+            print(json.dumps(rec))
+            rows.append(f"{cls} | {rd:.3e} | {wr:.3e} | {rd + wr:.3e} | {alg[cls]:.3e} | "
+                        f"{(rd + wr) / alg[cls]:.3f}")
     with open(os.path.join(ROOT, "profiles", f"traffic_fp64_{nx}.txt"), "w") as f:
         f.write(PMC_HEADER.format(nx=nx, fs=fscale, ws=wscale))
         f.write("# kernel | fetch B/launch | write B/launch | total | algorithmic | total/alg\n")
@@ -259,6 +278,7 @@ def run_single(m, u0, nx, ny, prec, steps, warmup, graph, hall, di, **dissipatio
     ctx = m.Context(nx, ny, hall=hall, di=di, precision=_precision(m, prec), **dissipation)
     ctx.set_state(u0)
     ctx.x_shapes = ctx.autotune() if AUTOTUNE else None
+    _apply_xfuse(ctx)
     ctx.run(warmup, graph=graph)
     ctx.sync()
     ms = ctx.time_run(steps, graph=graph)
@@ -351,6 +371,8 @@ def sweep_alg_bytes(nx, ny, elem=8, delem=None, batch=1):
 # on one CU), or with the fused x-derivative + RHS the carry solve, the fused
 # chunk kernel and the filter x-solve
 SWEEP_KERNELS = {"ysweep": ("ysweep",), "xsweep": ("xderiv", "rhs_rk", "xfilter")}
+# traffic (PMC) class of each timed class where the two differ
+TRAFFIC_CLASS = {("xfilter", False): "xfilter_unfused"}
 SWEEP_KERNELS_FUSED = {"ysweep": ("ysweep",), "xsweep": ("xcarry", "xrhs_rk", "xfilter")}
 
 
@@ -375,8 +397,9 @@ def roofline_of(ctx, nx, ny, elem, delem, batch, profile_steps, copy_gbps, traff
             "frac": achieved / HBM_PEAK, "avg_ms_per_stage": st_ms[dom],
             "alg_bytes_per_stage": salg[dom],
             "frac_vs_copy_probe": achieved / (copy_gbps * 1e9), "traffic": None}
-    if traffic is not None and all(k in traffic for k in sk[dom]):
-        roof["traffic"] = sum(traffic[k] for k in sk[dom])
+    tk = [TRAFFIC_CLASS.get((k, sk is SWEEP_KERNELS_FUSED), k) for k in sk[dom]]
+    if traffic is not None and all(k in traffic for k in tk):
+        roof["traffic"] = sum(traffic[k] for k in tk)
     roof["by_sweep"] = {sw: {"avg_ms_per_stage": st_ms[sw], "alg_bytes_per_stage": salg[sw],
                              "GB/s": salg[sw] / (st_ms[sw] * 1e-3) / 1e9,
                              "frac": salg[sw] / (st_ms[sw] * 1e-3) / HBM_PEAK}
@@ -590,6 +613,9 @@ def main():
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default x-solve launch shapes")
+    ap.add_argument("--xfuse", choices=["auto", "0", "1"], default="auto",
+                    help="fused x-derivative + RHS: the context's (autotuned) choice, or "
+                         "forced off / on where it applies")
     ap.add_argument("--c3-only", action="store_true",
                     help="only the 4096^2 precision / x-derivative comparison (prints its JSON)")
     ap.add_argument("--c4-only", action="store_true",
@@ -598,15 +624,23 @@ def main():
                     help="only the 96-run uneven ensemble (prints its JSON; honours --graph)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the streaming-copy calibration kernel and print its rate")
-    ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV",
-                                                         "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
-                    help="turn rocprofv3 --pmc counter CSVs into profiles/traffic_*.json")
+    ap.add_argument("--summarize-pmc", nargs="+",
+                    metavar="CSV",
+                    help="turn rocprofv3 --pmc counter CSVs into profiles/traffic_*.json: "
+                         "FETCH, WRITE (fused path), COPY_FETCH, COPY_WRITE, then "
+                         "optionally FETCH, WRITE of the unfused path")
     args = ap.parse_args()
-    global AUTOTUNE
+    global AUTOTUNE, XFUSE
     AUTOTUNE = not args.no_autotune
+    XFUSE = None if args.xfuse == "auto" else int(args.xfuse)
 
     if args.summarize_pmc:
-        summarize_pmc(*args.summarize_pmc, nx=args.nx, ny=args.ny)
+        c = args.summarize_pmc
+        if len(c) not in (4, 6):
+            ap.error("--summarize-pmc takes 4 or 6 CSV files")
+        summarize_pmc(*c[:4], nx=args.nx, ny=args.ny,
+                      fetch_unfused=c[4] if len(c) == 6 else None,
+                      write_unfused=c[5] if len(c) == 6 else None)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -646,7 +680,8 @@ def main():
     traffic = None
     if args.precision == "fp64" and nx == 2048 and ny == 2048:
         traffic = {k: load_traffic(k)
-                   for k in ("ysweep", "xderiv", "rhs_rk", "xcarry", "xrhs_rk", "xfilter")}
+                   for k in ("ysweep", "xderiv", "rhs_rk", "xcarry", "xrhs_rk", "xfilter",
+                             "xfilter_unfused")}
         traffic = {k: v for k, v in traffic.items() if v is not None}
     roofline, kernels = roofline_of(ctx, nx, ny, elem, delem, 1, args.profile_steps, copy_gbps,
                                     traffic)
