@@ -282,9 +282,12 @@ __global__ void __launch_bounds__(256 * RH, 2 / RH) k_xgemm(XGemmArgs<TI> a)
     // this thread's U values and M pieces of chunk q into registers
     auto fetch = [&](int q) {
         const int k0 = chunk(q) * KC;
-        const TI* col = U + (size_t)(k0 + kpart * KT) * ny + mycol;
+        // unconditional loads (a column beyond a narrow grid reads column 0
+        // and is zeroed in the split): a conditional load per element became
+        // a branch with a wait for each one
+        const TI* col = U + (size_t)(k0 + kpart * KT) * ny + (cact ? mycol : 0);
 #pragma unroll
-        for (int e = 0; e < KT; e++) pre[e] = cact ? col[(size_t)e * ny] : (TI)0;
+        for (int e = 0; e < KT; e++) pre[e] = col[(size_t)e * ny];
 #pragma unroll
         for (int i = 0; i < MPT; i++) {
             const int pc = t + NT * i;  // piece: plane, row, k group
@@ -304,7 +307,7 @@ __global__ void __launch_bounds__(256 * RH, 2 / RH) k_xgemm(XGemmArgs<TI> a)
             s8v h, m, l;
 #pragma unroll
             for (int e = 0; e < 8; e++) {
-                const float x = (float)(pre[g * 8 + e] - c0);
+                const float x = cact ? (float)(pre[g * 8 + e] - c0) : 0.f;
                 const unsigned short hb = bf16_bits(x);
                 const float r1 = x - bf16_float(hb);
                 const unsigned short mb = bf16_bits(r1);
