@@ -473,6 +473,68 @@ def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50,
     return out
 
 
+def step_traffic_run(m, prec, steps, warmup):
+    """the 64 x 512^2 batch of linear waves in one precision, whole batch
+    per launch, default launch shapes, eager steps only: the program that
+    rocprofv3 --pmc runs to count HBM bytes per step (summarize with
+    --summarize-step-pmc)"""
+    enx = eny = 512
+    states, dis = ensemble_states(m, enx, eny, 64, 42)
+    ctx = m.Context(enx, eny, hall=1, precision=_precision(m, prec), batch=len(states))
+    for b in range(len(states)):
+        ctx.set_di(b, float(dis[b]))
+        ctx.set_state(states[b], inst=b)
+    ctx.run(warmup, graph=False)
+    ctx.sync()
+    ms = ctx.time_run(steps, graph=False)
+    ctx.sync()
+    out = {"workload": f"64 x {enx}x{eny} linear waves, {prec}, whole batch per launch, "
+                       "default launch shapes, eager steps",
+           "precision": prec, "steps_run": warmup + steps, "timed_steps": steps,
+           "ms_per_step": ms / steps,
+           "cell_updates_per_s": len(states) * enx * eny * steps / (ms * 1e-3),
+           "x_fused": ctx.xfuse}
+    ctx.close()
+    return out
+
+
+# kernels of an RK3 step (set-up and diagnostics kernels excluded)
+STEP_KERNELS = ("k_ysweep", "k_xsolve", "k_xrhs_rk", "k_rhs_rk", "k_dt_finalize")
+
+
+def summarize_step_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, steps):
+    """HBM bytes per RK3 step: FETCH_SIZE and WRITE_SIZE (KiB per dispatch)
+    summed over every dispatch of the step kernels and divided by the steps
+    the program ran, scaled by what the counters read for the copy kernel of
+    known size"""
+    import csv
+
+    def totals(path, counter):
+        per, copy = 0.0, []
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                name = r.get("Kernel_Name") or r.get("Kernel-Name") or ""
+                ctr = r.get("Counter_Name") or r.get("Counter-Name") or ""
+                if ctr != counter:
+                    continue
+                val = float(r.get("Counter_Value") or r.get("Counter-Value") or 0) * 1024.0
+                if "k_copy" in name:
+                    copy.append(val)
+                elif any(k in name for k in STEP_KERNELS):
+                    per += val
+        return per, copy
+
+    fetch, _ = totals(fetch_csv, "FETCH_SIZE")
+    write, _ = totals(write_csv, "WRITE_SIZE")
+    _, cf = totals(copy_fetch_csv, "FETCH_SIZE")
+    _, cw = totals(copy_write_csv, "WRITE_SIZE")
+    fscale = COPY_BYTES / float(np.median(cf)) if cf else 1.0
+    wscale = COPY_BYTES / float(np.median(cw)) if cw else 1.0
+    rd, wr = fetch * fscale / steps, write * wscale / steps
+    return {"fetch_bytes_per_step": rd, "write_bytes_per_step": wr,
+            "bytes_per_step": rd + wr, "fetch_scale": fscale, "write_scale": wscale}
+
+
 def batched_forms(m, value, args, copy_gbps):
     """the 64 x 512^2 batched ensemble in the three precisions (errors
     against the FP64 batch on four instances); value: the 2048^2 FP64 rate"""
@@ -624,6 +686,14 @@ def main():
                     help="only the 96-run uneven ensemble (prints its JSON; honours --graph)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the streaming-copy calibration kernel and print its rate")
+    ap.add_argument("--step-traffic", choices=["fp64", "fp32", "mixed"],
+                    help="only run the 64 x 512^2 batch for --warmup + --steps eager steps "
+                         "in this precision (the program for rocprofv3 --pmc byte counts "
+                         "per step); prints its JSON")
+    ap.add_argument("--summarize-step-pmc", nargs=4, metavar="CSV",
+                    help="HBM bytes per step from the counter CSVs of a --step-traffic run "
+                         "(FETCH, WRITE) and of --copy-probe (FETCH, WRITE); the program "
+                         "ran --warmup + --steps steps")
     ap.add_argument("--summarize-pmc", nargs="+",
                     metavar="CSV",
                     help="turn rocprofv3 --pmc counter CSVs into profiles/traffic_*.json: "
@@ -633,6 +703,11 @@ def main():
     global AUTOTUNE, XFUSE
     AUTOTUNE = not args.no_autotune
     XFUSE = None if args.xfuse == "auto" else int(args.xfuse)
+
+    if args.summarize_step_pmc:
+        c = args.summarize_step_pmc
+        print(json.dumps(summarize_step_pmc(*c, steps=args.warmup + args.steps)))
+        return
 
     if args.summarize_pmc:
         c = args.summarize_pmc
@@ -653,6 +728,10 @@ def main():
     if args.copy_probe:
         print(json.dumps({"copy_probe_bytes": COPY_BYTES,
                           "copy_gbps": m.copy_probe(COPY_BYTES, 5)}))
+        return
+
+    if args.step_traffic:
+        print(json.dumps(step_traffic_run(m, args.step_traffic, args.steps, args.warmup)))
         return
 
     if args.c3_only:
