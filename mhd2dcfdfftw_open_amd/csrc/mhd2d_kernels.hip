@@ -575,9 +575,15 @@ __global__ void __launch_bounds__(1024, MHD2D_X_MINB) k_xsolve(XSolveArgs<T, TI>
 
 void xsolve_shape(int nx, int elem_bytes, int* chunk, int* width, int* groups)
 {
-    // 32-row chunks at most (64-row ones need ~170 VGPRs in FP64, one
-    // 512-thread workgroup per CU); 4096 rows are 128 chunks of 8 columns
-    int C = nx / 64;
+    // 32 chunks per line, of 8 to 32 rows (64-row ones need ~170 VGPRs in
+    // FP64, one 512-thread workgroup per CU): longer lines take more chunks
+    // of 32 rows (4096 rows: 128 chunks of 8 columns).  Against 64 chunks
+    // of 8 rows, 512-row lines in 16-row chunks read 22 instead of 28 rows
+    // (with the 3-row halos) per 16 and run 512-thread workgroups, two to a
+    // CU: 64 x 512^2 +6 % FP64, +5 % mixed, +19 % FP32 per step; 1024-row
+    // lines in 32-row chunks: uneven ensemble +2.6 % FP64, unchanged mixed
+    // and FP32
+    int C = nx / 32;
     if (C < 8) C = 8;
     if (C > 32) C = 32;
     const int G = nx / C;
@@ -2147,9 +2153,10 @@ int xsolve_multi_code(int nx, int ny, int* chunk, int* width)
     xsolve_shape(nx, 8, &C, &W, &G);
     const int w = G > 0 ? 1024 / G : 0;
     int code = -1;
+    // the shapes xsolve_shape gives lines of 256 ... 4096 rows
     if (C == 8 && w == 32) code = 0;
-    else if (C == 8 && w == 16) code = 1;
-    else if (C == 16 && w == 16) code = 2;
+    else if (C == 16 && w == 32) code = 1;
+    else if (C == 32 && w == 32) code = 2;
     else if (C == 32 && w == 16) code = 3;
     else if (C == 32 && w == 8) code = 4;
     if (code < 0 || w * G != 1024 || ny % w) return -1;
@@ -2178,8 +2185,8 @@ __global__ void __launch_bounds__(1024, MHD2D_X_MINB)
     const int z = local / m.per[g];
     switch (m.code[g]) {
     case 0: xsolve_body<T, TI, 8, 32, KIND, false>(a, bx, v, z, sh_in, sh_fw, sh_bw, sh_half, x_dyn); break;
-    case 1: xsolve_body<T, TI, 8, 16, KIND, false>(a, bx, v, z, sh_in, sh_fw, sh_bw, sh_half, x_dyn); break;
-    case 2: xsolve_body<T, TI, 16, 16, KIND, false>(a, bx, v, z, sh_in, sh_fw, sh_bw, sh_half, x_dyn); break;
+    case 1: xsolve_body<T, TI, 16, 32, KIND, false>(a, bx, v, z, sh_in, sh_fw, sh_bw, sh_half, x_dyn); break;
+    case 2: xsolve_body<T, TI, 32, 32, KIND, false>(a, bx, v, z, sh_in, sh_fw, sh_bw, sh_half, x_dyn); break;
     case 3: xsolve_body<T, TI, 32, 16, KIND, false>(a, bx, v, z, sh_in, sh_fw, sh_bw, sh_half, x_dyn); break;
     case 4: xsolve_body<T, TI, 32, 8, KIND, false>(a, bx, v, z, sh_in, sh_fw, sh_bw, sh_half, x_dyn); break;
     default: break;

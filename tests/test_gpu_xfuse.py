@@ -57,7 +57,7 @@ def _run(m, nx, ny, prec, fuse, steps, graph, B=1, sub=0, dis=None, hall=1):
     return out
 
 
-# (nx, ny, precision): chunk C = nx / 64 clamped to 8..32 rows; a fused
+# (nx, ny, precision): chunk C = nx / 32 clamped to 8..32 rows; a fused
 # workgroup covers 512 / C columns (FP64 state) or 1024 / C (FP32)
 CASES = [
     (128, 64, "fp64"), (128, 64, "mixed"), (128, 128, "fp32"),
