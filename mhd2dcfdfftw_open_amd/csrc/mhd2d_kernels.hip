@@ -705,11 +705,17 @@ struct YElems {
 #define MHD2D_Y_ATTR
 #endif
 
+// threads of a y-sweep workgroup for lines shorter than that many threads
+// (several rows per workgroup): 256 measured within 1.4 % of 128 on the
+// batched and uneven ensembles and the FP32 2048^2 run
+#ifndef MHD2D_Y_THREADS
+#define MHD2D_Y_THREADS 128
+#endif
 template <typename T, int N>
 struct YShape {
     static constexpr int E = YElems<T>::E;
     static constexpr int TPR = FFTShape<N, E>::TPR;
-    static constexpr int ROWS = (TPR >= 128) ? 1 : 128 / TPR;
+    static constexpr int ROWS = (TPR >= MHD2D_Y_THREADS) ? 1 : MHD2D_Y_THREADS / TPR;
     static constexpr int THREADS = TPR * ROWS;
     static constexpr int LDSW = FFTShape<N, E>::LDS;
 };
@@ -2086,11 +2092,11 @@ __device__ __forceinline__ int multi_group(const MultiMap& m, int b)
     return g;
 }
 
-// line lengths a 128-thread y-sweep workgroup covers
+// line lengths a y-sweep workgroup of MHD2D_Y_THREADS threads covers
 template <typename T, int N>
 constexpr bool ymulti_ok()
 {
-    return YShape<T, N>::THREADS == 128;
+    return YShape<T, N>::THREADS == MHD2D_Y_THREADS;
 }
 template <typename T>
 constexpr int ymulti_lds()
@@ -2122,7 +2128,7 @@ int ysweep_multi_rows(int ny, int elem_bytes)
 }
 
 template <typename T, typename TD, int MODE>
-__global__ void __launch_bounds__(128) MHD2D_Y_ATTR
+__global__ void __launch_bounds__(MHD2D_Y_THREADS) MHD2D_Y_ATTR
     k_ysweep_multi(const YSweepArgs<T, TD>* __restrict__ tab, MultiMap m)
 {
     __shared__ T s_re[ymulti_lds<T>()];
@@ -2225,9 +2231,9 @@ hipError_t launch_ysweep_multi(const YSweepArgs<T, TD>* tab, const MultiMap& m, 
     const int nb = multi_blocks(m);
     if (m.ng < 1 || m.ng > MULTI_MAXG || nb < 1 || mode < 0 || mode > 1) return hipErrorInvalidValue;
     if (mode == 1)
-        hipLaunchKernelGGL((k_ysweep_multi<T, TD, 1>), dim3(nb), dim3(128), 0, s, tab, m);
+        hipLaunchKernelGGL((k_ysweep_multi<T, TD, 1>), dim3(nb), dim3(MHD2D_Y_THREADS), 0, s, tab, m);
     else
-        hipLaunchKernelGGL((k_ysweep_multi<T, TD, 0>), dim3(nb), dim3(128), 0, s, tab, m);
+        hipLaunchKernelGGL((k_ysweep_multi<T, TD, 0>), dim3(nb), dim3(MHD2D_Y_THREADS), 0, s, tab, m);
     return hipGetLastError();
 }
 
