@@ -1449,9 +1449,15 @@ struct EnsMulti {
     size_t off_y[3] = {0, 0, 0};  // y-sweeps: stage 0 without / with the pending filter, stages 1-2
     size_t off_xd = 0, off_x2 = 0, off_xf = 0, off_dt = 0;
     size_t off_r[3] = {0, 0, 0};  // RHS + RK of each stage
-    MultiMap my, mxd, mx2, mxf, mr, mdt;
+    // the fused x-derivative + RHS (every group ideal MHD on one periodic
+    // block): carry solve, fused kernel of each stage, x-filter with the
+    // chunk edge rows
+    size_t off_xc = 0, off_xfe = 0;
+    size_t off_xr[3] = {0, 0, 0};
+    MultiMap my, mxd, mx2, mxf, mr, mdt, mxr;
     size_t xdyn = 0;           // dynamic LDS of the x-solves: the largest group's tables
     int lap = 0;               // viscosity / resistivity in every group
+    int fused = 0;             // the fused x path
 };
 
 struct mhd2d_ens {
@@ -1494,6 +1500,22 @@ hipError_t build_multi_t(mhd2d_ens* e, bool* ok)
     }
     EnsMulti& M = e->multi;
     M.lap = lap;
+    // the fused x path where every group takes it; not in the mixed form,
+    // whose contexts keep the full x-derivative solve by default (the fused
+    // one-launch step measured 2.5 % slower there on the uneven ensemble)
+    bool fused = true;
+    for (auto* c : e->groups) fused = fused && xfuse_ok(c) && c->p.precision != MHD2D_MIXED;
+    if (fused) {
+        for (auto* c : e->groups) {
+            const hipError_t r = ensure_xedge(c);
+            if (r != hipSuccess) return r;
+        }
+    }
+    std::vector<XSolveArgs<TD, T>> xc;
+    std::vector<XRhsArgs<T, TD>> xr[3];
+    std::vector<XSolveArgs<T>> xfe;
+    MultiMap mxr;
+    mxr.ng = ng;
     std::vector<YSweepArgs<T, TD>> ya[3];
     std::vector<XSolveArgs<TD, T>> xd, x2;
     std::vector<XSolveArgs<T>> xf;
@@ -1549,6 +1571,11 @@ hipError_t build_multi_t(mhd2d_ens* e, bool* ok)
             nv2 = f1 - f0 + 1;
         }
         xf.push_back(xfilter_args<T>(c, false));
+        if (fused) {
+            xc.push_back(xcarry_args<T, TD>(c));
+            for (int st = 0; st < 3; st++) xr[st].push_back(xrhs_args<T, TD>(c, st));
+            xfe.push_back(xfilter_args<T>(c, true));
+        }
         xdyn = std::max(xdyn, 3 * (size_t)c->p.nx * std::max(sizeof(T), sizeof(TD)));
         const int rows = ysweep_multi_rows(c->p.ny, c->elem);
         const int B = c->B;
@@ -1572,6 +1599,11 @@ hipError_t build_multi_t(mhd2d_ens* e, bool* ok)
             set(mr, (int)b, 0, 0);
         }
         set(mdt, 1, 0, 0);
+        if (fused) {
+            // the fused kernel's tiles: column tiles x chunks per instance
+            const int tiles = c->p.ny / xrhs_tile_columns(c->p.nx, c->elem);
+            set(mxr, tiles * (c->p.nx / C), tiles, C);
+        }
     }
     // one device table: each argument set 256-byte aligned
     size_t off = 0;
@@ -1586,6 +1618,11 @@ hipError_t build_multi_t(mhd2d_ens* e, bool* ok)
     M.off_xf = take(xf.size() * sizeof(XSolveArgs<T>));
     for (int st = 0; st < 3; st++) M.off_r[st] = take(ra[st].size() * sizeof(RhsRkArgs<T, TD>));
     M.off_dt = take(dta.size() * sizeof(DtMultiArgs));
+    if (fused) {
+        M.off_xc = take(xc.size() * sizeof(XSolveArgs<TD, T>));
+        for (int st = 0; st < 3; st++) M.off_xr[st] = take(xr[st].size() * sizeof(XRhsArgs<T, TD>));
+        M.off_xfe = take(xfe.size() * sizeof(XSolveArgs<T>));
+    }
     std::vector<char> h(off, 0);
     for (int v = 0; v < 3; v++)
         std::memcpy(h.data() + M.off_y[v], ya[v].data(), ya[v].size() * sizeof(ya[v][0]));
@@ -1595,6 +1632,12 @@ hipError_t build_multi_t(mhd2d_ens* e, bool* ok)
     for (int st = 0; st < 3; st++)
         std::memcpy(h.data() + M.off_r[st], ra[st].data(), ra[st].size() * sizeof(ra[st][0]));
     std::memcpy(h.data() + M.off_dt, dta.data(), dta.size() * sizeof(dta[0]));
+    if (fused) {
+        std::memcpy(h.data() + M.off_xc, xc.data(), xc.size() * sizeof(xc[0]));
+        for (int st = 0; st < 3; st++)
+            std::memcpy(h.data() + M.off_xr[st], xr[st].data(), xr[st].size() * sizeof(xr[st][0]));
+        std::memcpy(h.data() + M.off_xfe, xfe.data(), xfe.size() * sizeof(xfe[0]));
+    }
     hipError_t r = hipMalloc(&M.tab, off);
     if (r != hipSuccess) return r;
     r = hipMemcpy(M.tab, h.data(), off, hipMemcpyHostToDevice);
@@ -1605,6 +1648,8 @@ hipError_t build_multi_t(mhd2d_ens* e, bool* ok)
     M.mxf = mxf;
     M.mr = mr;
     M.mdt = mdt;
+    M.mxr = mxr;
+    M.fused = fused ? 1 : 0;
     M.xdyn = xdyn;
     *ok = true;
     return hipSuccess;
@@ -1633,6 +1678,18 @@ hipError_t ens_step_multi_t(mhd2d_ens* e)
         r = launch_ysweep_multi<T, TD>((const YSweepArgs<T, TD>*)(tb + oy), M.my, mode, s);
         if (r == hipSuccess && st == 0)
             r = launch_dt_finalize_multi((const DtMultiArgs*)(tb + M.off_dt), M.mdt, s);
+        if (M.fused) {
+            if (r == hipSuccess)
+                r = launch_xsolve_multi<TD, T>((const XSolveArgs<TD, T>*)(tb + M.off_xc), M.mxd,
+                                               XD1_CARRY, M.xdyn, s);
+            if (r == hipSuccess)
+                r = launch_xrhs_rk_multi<T, TD>((const XRhsArgs<T, TD>*)(tb + M.off_xr[st]), M.mxr,
+                                                s);
+            if (r == hipSuccess)
+                r = launch_xsolve_multi<T, T>((const XSolveArgs<T>*)(tb + M.off_xfe), M.mxf,
+                                              XFILT_EDGE, M.xdyn, s);
+            continue;
+        }
         if (r == hipSuccess)
             r = launch_xsolve_multi<TD, T>((const XSolveArgs<TD, T>*)(tb + M.off_xd), M.mxd, XD1,
                                            M.xdyn, s);

@@ -18,6 +18,14 @@ struct RunScalars;
 
 // ---------------------------------------------------------------- x operators
 enum XKind { XD1 = 0, XD2 = 1, XFILT = 2, NXKIND = 3 };
+// two solve variants beside the XKind ones (template values of the x-solve
+// kernels): the x-derivative of a periodic line that stops after the two
+// chunk scans and stores, per chunk, the values entering it (y at the row
+// before it, x at the row after it), from which the fused x-derivative +
+// RHS re-solves the chunks; and the filter whose input rows at chunk edges
+// come from XSolveArgs::edge
+constexpr int XD1_CARRY = 3;
+constexpr int XFILT_EDGE = 4;
 
 // Scaled stencil weights of one x operator.  Interior rows:
 //   D1: r = c1 (f+1 - f-1) + c2 (f+2 - f-2) + c3 (f+3 - f-3)
@@ -303,6 +311,8 @@ hipError_t launch_xrhs_rk(const XRhsArgs<T, TD>& a, hipStream_t s, int nb = 1);
 // word): the x chunks are C = xsolve_shape's rows, a workgroup's columns
 // 512 / C (FP64 state) or 1024 / C (FP32) must divide ny
 int xrhs_supported(int nx, int ny, int elem_bytes);
+// columns of one workgroup tile of the fused kernel (elem_bytes: the state word)
+int xrhs_tile_columns(int nx, int elem_bytes);
 // the two open boundary lines (x = 0, x = Lx): calcRHSOpenBoundaryX with the
 // characteristic terms, then the same RK update
 template <typename T, typename TD = T>
@@ -353,6 +363,11 @@ hipError_t launch_xsolve_multi(const XSolveArgs<T, TI>* tab, const MultiMap& m, 
 template <typename T, typename TD>
 hipError_t launch_rhs_rk_multi(const RhsRkArgs<T, TD>* tab, const MultiMap& m, int mode,
                                hipStream_t s);
+// the fused x-derivative + RHS + RK: per group per[g] blocks per instance
+// (aux[g] column tiles of XRhsShape cells / code[g] columns, times the
+// chunks), code[g] = the chunk rows
+template <typename T, typename TD>
+hipError_t launch_xrhs_rk_multi(const XRhsArgs<T, TD>* tab, const MultiMap& m, hipStream_t s);
 hipError_t launch_dt_finalize_multi(const DtMultiArgs* tab, const MultiMap& m, hipStream_t s);
 // rows per y-sweep workgroup of a shape-group launch (0: ny not supported
 // there); elem_bytes: the state word

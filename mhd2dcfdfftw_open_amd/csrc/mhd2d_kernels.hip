@@ -352,13 +352,6 @@ __device__ __forceinline__ void xcol_backward(T (&w)[C + 6], T x, bool interior,
 #define MHD2D_X_CONST(T) (sizeof(T) == 4)
 #endif
 
-// x-derivative solve of a periodic line that stops after the two chunk
-// scans and stores, per chunk, the values entering it: y at the row before
-// it (forward) and x at the row after it (backward); k_xrhs_rk re-solves
-// the chunks from them.  A template value beside the XKind ones.
-constexpr int XD1_CARRY = 3;
-// filter solve whose input rows at chunk edges come from XSolveArgs::edge
-constexpr int XFILT_EDGE = 4;
 
 // One workgroup's compact solves: column strip bx, field v, instance z of
 // the launch described by `a`.  sh_in / sh_fw / sh_bw hold at least
@@ -1465,26 +1458,29 @@ struct XRhsShape {
     static constexpr int NV = 9;                               // derivative fields
 };
 
+template <typename T>
+constexpr int xrhs_fs(int C)
+{
+    // LDS words per field: the pad puts the fields that one wavefront writes
+    // together (same column, same row) on different banks
+    return XRhsShape<T>::CELLS + XRhsShape<T>::CELLS / C;
+}
+
+// One workgroup of the fused kernel: column tile bx, chunk g, instance z;
+// sdx holds at least XRhsShape<TD>::NV * xrhs_fs<T>(C) words (16-byte
+// aligned), stab 3 C
 template <typename T, typename TD, int C>
-__global__ void __launch_bounds__(256) k_xrhs_rk(XRhsArgs<T, TD> a)
+__device__ __forceinline__ void xrhs_body(const XRhsArgs<T, TD>& a, int bx, int g, int z,
+                                          TD* __restrict__ sdx, TD* __restrict__ stab)
 {
     using S = XRhsShape<T>;
     constexpr int PW = S::PW, NV = S::NV;
     constexpr int WB = S::CELLS / C;          // columns of the chunk
     constexpr int NI = NV * WB;               // column solves
     constexpr int IPT = (NI + 255) / 256;     // column solves per thread
-    // LDS words per field: the pad puts the fields that one wavefront writes
-    // together (same column, same row) on different banks
-    constexpr int FS = S::CELLS + WB;
-    // read back as PW-word vectors (16 or 8 bytes)
-    __shared__ __align__(16) TD sdx[NV * FS];
-    __shared__ TD stab[3 * C];
+    constexpr int FS = xrhs_fs<T>(C);
     const XSolveArgs<TD, T>& xa = a.x;
     const RhsRkArgs<T, TD>& ra = a.r;
-    const int bx = blockIdx.x, g = blockIdx.y;
-    // instances last to first, as k_rhs_rk (the carry solve before it runs
-    // them first to last)
-    const int z = (int)(gridDim.z - 1 - blockIdx.z);
     const RunScalars* const rs = ra.rs + z;
     if (rs->stop) return;  // the run stopped (dt < 1e-8): uniform per block
     const int nx = xa.nx;
@@ -1623,6 +1619,17 @@ __global__ void __launch_bounds__(256) k_xrhs_rk(XRhsArgs<T, TD> a)
         stp<MHD2D_NT_R, PW>(uo + f * fs + po, o);
         if (ra.write_rprev) stp<MHD2D_NT_R, PW>(rpz + f * n + p, rs_out);
     }
+}
+
+template <typename T, typename TD, int C>
+__global__ void __launch_bounds__(256) k_xrhs_rk(XRhsArgs<T, TD> a)
+{
+    // read back as PW-word vectors (16 or 8 bytes)
+    __shared__ __align__(16) TD sdx[XRhsShape<T>::NV * xrhs_fs<T>(C)];
+    __shared__ TD stab[3 * C];
+    // instances last to first, as k_rhs_rk (the carry solve before it runs
+    // them first to last)
+    xrhs_body<T, TD, C>(a, blockIdx.x, blockIdx.y, (int)(gridDim.z - 1 - blockIdx.z), sdx, stab);
 }
 
 // calcCharacteristicsX (calcRHS.c:253-495) at one open boundary cell:
@@ -1984,6 +1991,13 @@ hipError_t launch_rhs_rk(const RhsRkArgs<T, TD>& a, hipStream_t s, int nb)
     return hipGetLastError();
 }
 
+int xrhs_tile_columns(int nx, int elem_bytes)
+{
+    int C, W, G;
+    xsolve_shape(nx, elem_bytes, &C, &W, &G);
+    return (elem_bytes == 8 ? XRhsShape<double>::CELLS : XRhsShape<float>::CELLS) / C;
+}
+
 int xrhs_supported(int nx, int ny, int elem_bytes)
 {
     int C, W, G;
@@ -2212,6 +2226,30 @@ __global__ void __launch_bounds__(256)
     rhs_rk_body<T, TD, MODE>(tab[g], m.nb[g] - 1 - local / per, local % per, per);
 }
 
+// the fused x-derivative + RHS + RK over every group: per[g] blocks per
+// instance (aux[g] column tiles x the chunks), code[g] the chunk rows
+template <typename T, typename TD>
+__global__ void __launch_bounds__(256) k_xrhs_rk_multi(const XRhsArgs<T, TD>* __restrict__ tab,
+                                                       MultiMap m)
+{
+    // sized for the largest tile table: 8-row chunks
+    __shared__ __align__(16) TD sdx[XRhsShape<T>::NV * xrhs_fs<T>(8)];
+    __shared__ TD stab[3 * 32];
+    const int b = blockIdx.x;
+    const int g = multi_group(m, b);
+    const int local = b - m.first[g];
+    const int per = m.per[g], tiles = m.aux[g];
+    // instances last to first, as k_xrhs_rk
+    const int z = m.nb[g] - 1 - local / per;
+    const int bx = (local % per) % tiles, ch = (local % per) / tiles;
+    switch (m.code[g]) {
+    case 8: xrhs_body<T, TD, 8>(tab[g], bx, ch, z, sdx, stab); break;
+    case 16: xrhs_body<T, TD, 16>(tab[g], bx, ch, z, sdx, stab); break;
+    case 32: xrhs_body<T, TD, 32>(tab[g], bx, ch, z, sdx, stab); break;
+    default: break;
+    }
+}
+
 __global__ void k_dt_finalize_multi(const DtMultiArgs* __restrict__ tab, MultiMap m)
 {
     const int b = blockIdx.x;
@@ -2247,6 +2285,8 @@ hipError_t launch_xsolve_multi(const XSolveArgs<T, TI>* tab, const MultiMap& m, 
         switch (kind) {
         case XD1: hipLaunchKernelGGL((k_xsolve_multi<T, TI, XD1>), dim3(nb), dim3(1024), dyn_lds, s, tab, m); break;
         case XD2: hipLaunchKernelGGL((k_xsolve_multi<T, TI, XD2>), dim3(nb), dim3(1024), dyn_lds, s, tab, m); break;
+        case XD1_CARRY: hipLaunchKernelGGL((k_xsolve_multi<T, TI, XD1_CARRY>), dim3(nb), dim3(1024), dyn_lds, s, tab, m); break;
+        case XFILT_EDGE: hipLaunchKernelGGL((k_xsolve_multi<T, TI, XFILT_EDGE>), dim3(nb), dim3(1024), dyn_lds, s, tab, m); break;
         default: hipLaunchKernelGGL((k_xsolve_multi<T, TI, XFILT>), dim3(nb), dim3(1024), dyn_lds, s, tab, m); break;
         }
     } else {
@@ -2256,6 +2296,8 @@ hipError_t launch_xsolve_multi(const XSolveArgs<T, TI>* tab, const MultiMap& m, 
             hipLaunchKernelGGL((k_xsolve_multi<T, TI, XD1>), dim3(nb), dim3(1024), dyn_lds, s, tab, m);
         else if (kind == XD2)
             hipLaunchKernelGGL((k_xsolve_multi<T, TI, XD2>), dim3(nb), dim3(1024), dyn_lds, s, tab, m);
+        else if (kind == XD1_CARRY)
+            hipLaunchKernelGGL((k_xsolve_multi<T, TI, XD1_CARRY>), dim3(nb), dim3(1024), dyn_lds, s, tab, m);
         else
             return hipErrorInvalidValue;
     }
@@ -2272,6 +2314,15 @@ hipError_t launch_rhs_rk_multi(const RhsRkArgs<T, TD>* tab, const MultiMap& m, i
         hipLaunchKernelGGL((k_rhs_rk_multi<T, TD, 1>), dim3(nb), dim3(256), 0, s, tab, m);
     else
         hipLaunchKernelGGL((k_rhs_rk_multi<T, TD, 0>), dim3(nb), dim3(256), 0, s, tab, m);
+    return hipGetLastError();
+}
+
+template <typename T, typename TD>
+hipError_t launch_xrhs_rk_multi(const XRhsArgs<T, TD>* tab, const MultiMap& m, hipStream_t s)
+{
+    const int nb = multi_blocks(m);
+    if (m.ng < 1 || m.ng > MULTI_MAXG || nb < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_xrhs_rk_multi<T, TD>), dim3(nb), dim3(256), 0, s, tab, m);
     return hipGetLastError();
 }
 
@@ -2312,6 +2363,8 @@ MHD2D_INST(float)
 #define MHD2D_INST_MULTI(T, TD)                                                              \
     template hipError_t launch_ysweep_multi<T, TD>(const YSweepArgs<T, TD>*, const MultiMap&, \
                                                    int, hipStream_t);                        \
+    template hipError_t launch_xrhs_rk_multi<T, TD>(const XRhsArgs<T, TD>*, const MultiMap&,  \
+                                                    hipStream_t);                            \
     template hipError_t launch_rhs_rk_multi<T, TD>(const RhsRkArgs<T, TD>*, const MultiMap&, \
                                                    int, hipStream_t);
 MHD2D_INST_MULTI(double, double)

@@ -1,0 +1,139 @@
+"""The benchmark workloads at their full sizes on the GPU against the FP64
+oracle (its OpenMP build; results equal bit for bit to one thread):
+
+* 2048 x 2048 turbulence (seed 42, rms 0.1, 1 <= |k| <= 16, d_i = 0.02),
+  the fused FP64 path, 10 steps checked after every step: per-field
+  relative L2 <= 1e-12, dt and time;
+* 4096 x 4096 turbulence (the same seed): 2 FP64 steps at 1e-12, and the
+  mixed form within 1e-6 of the oracle after the same 2 steps;
+* the 64 x 512 x 512 batch of linear waves (d_i per instance): instances 0,
+  31 and 63 of one batched context after 20 steps;
+* the 96-run uneven ensemble (grid sizes drawn from 256 x 256, 512 x 256,
+  1024 x 1024, 2048 x 512): the first run of each size after 10 steps.
+
+The smaller grids, longer runs and every setting of the physics are covered
+by the other parity tests; these pin the workloads the measurements are
+quoted on, at the sizes they run.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from util import FP64_TOL, assert_fields_close, field_errors
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def m():
+    import mhd2dcfdfftw_open_amd as m
+    m.lib()
+    return m
+
+
+def test_turbulence_2048_every_step(m):
+    n, steps = 2048, 10
+    print(f"[workload] {n}^2 turbulence, FP64, {steps} steps", flush=True)
+    u0 = m.turbulence(n, n, seed=42, rms=0.1, kmin=1, kmax=16)
+    ctx = m.Context(n, n, hall=1, di=0.02)
+    ctx.set_state(u0)
+    worst = 0.0
+    try:
+        with orc.threaded():
+            S = orc.Sim(n, n, hall=1, di=0.02)
+            S.set_state(u0)
+            for k in range(1, steps + 1):
+                ctx.run(1, graph=True)
+                S.run(1)
+                e = assert_fields_close(ctx.state(), S.state(), what=f"step {k}")
+                worst = max(worst, e.max())
+                dt, t, nsteps = ctx.scalars()
+                assert nsteps == k
+                assert abs(dt - S.dt) <= FP64_TOL * S.dt, (k, dt, S.dt)
+                assert abs(t - S.time) <= FP64_TOL * S.time, (k, t, S.time)
+            del S
+    finally:
+        ctx.close()
+    print(f"[workload] {n}^2: worst per-field rel-L2 over {steps} steps {worst:.2e}")
+
+
+def test_turbulence_4096_fp64_and_mixed(m):
+    n, steps = 4096, 2
+    print(f"[workload] {n}^2 turbulence, FP64 and mixed, {steps} steps", flush=True)
+    u0 = m.turbulence(n, n, seed=42, rms=0.1, kmin=1, kmax=16)
+    with orc.threaded():
+        S = orc.Sim(n, n, hall=1, di=0.02)
+        S.set_state(u0)
+        S.run(steps)
+        want = S.state()
+        del S
+    for prec, tol in (("fp64", FP64_TOL), ("mixed", 1e-6)):
+        ctx = m.Context(n, n, hall=1, di=0.02,
+                        precision=m.FP64 if prec == "fp64" else m.MIXED)
+        try:
+            ctx.set_state(u0)
+            ctx.run(steps)
+            e = assert_fields_close(ctx.state(), want, tol=tol, what=prec)
+        finally:
+            ctx.close()
+        print(f"[workload] {n}^2 {prec}: per-field rel-L2 max {e.max():.2e}")
+
+
+def test_batch_64x512_instances(m):
+    n, B, steps = 512, 64, 20
+    print(f"[workload] {B} x {n}^2 linear waves, FP64, {steps} steps", flush=True)
+    rng = np.random.default_rng(42)
+    dis = rng.uniform(0.0, 0.2, B)
+    states = [m.linear_wave(n, n, seed=42 + b, amp=1e-4) for b in range(B)]
+    ctx = m.Context(n, n, hall=1, batch=B)
+    try:
+        for b in range(B):
+            ctx.set_di(b, float(dis[b]))
+            ctx.set_state(states[b], inst=b)
+        ctx.run(steps, graph=True)
+        ctx.sync()
+        with orc.threaded():
+            for b in (0, 31, 63):
+                S = orc.Sim(n, n, hall=1, di=float(dis[b]))
+                S.set_state(states[b])
+                S.run(steps)
+                e = assert_fields_close(ctx.state(b), S.state(), what=f"instance {b}")
+                print(f"[workload] instance {b}: per-field rel-L2 max {e.max():.2e}")
+                del S
+    finally:
+        ctx.close()
+
+
+def test_uneven_ensemble_one_run_per_size(m):
+    count, steps = 96, 10
+    print(f"[workload] {count}-run uneven ensemble, FP64, {steps} steps", flush=True)
+    rng = np.random.default_rng(42)
+    sizes = [(256, 256), (512, 256), (1024, 1024), (2048, 512)]
+    runs, states = [], []
+    for k in range(count):
+        nx, ny = sizes[rng.integers(0, 4)]
+        runs.append(dict(nx=nx, ny=ny, hall=1, di=float(rng.uniform(0, 0.1))))
+        states.append(m.turbulence(nx, ny, seed=42 + k, rms=float(rng.uniform(0.05, 0.2)),
+                                   kmax=16))
+    E = m.Ensemble(runs)
+    try:
+        assert E.groups == 4
+        for k in range(count):
+            E.set_state(k, states[k])
+        E.run(steps, graph=True)
+        E.sync()
+        first = {}
+        for k, r in enumerate(runs):
+            first.setdefault((r["nx"], r["ny"]), k)
+        assert len(first) == 4
+        with orc.threaded():
+            for (nx, ny), k in sorted(first.items()):
+                S = orc.Sim(nx, ny, hall=1, di=runs[k]["di"])
+                S.set_state(states[k])
+                S.run(steps)
+                e = field_errors(E.state(k), S.state())
+                print(f"[workload] run {k} ({nx}x{ny}): per-field rel-L2 max {e.max():.2e}")
+                assert np.all(e <= FP64_TOL), (k, e)
+                del S
+    finally:
+        E.close()
