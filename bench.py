@@ -535,6 +535,60 @@ def summarize_step_pmc(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, ste
             "bytes_per_step": rd + wr, "fetch_scale": fscale, "write_scale": wscale}
 
 
+def dropin_form(m, u0, nx, ny, hall, di, value, steps=10, warmup=2):
+    """the same workload through the reference's own entry points, called in
+    mhd.c's order (mhd.c:154-206) as its main loop would: one kernel per
+    entry point, host copies only on request (MHD2D_SYNC_EXPLICIT); each
+    step ends in calcDt, which hands dt back to the host"""
+    import time
+    L = m.lib()
+    G = m.RefGlobals(nx, ny, hall=hall, di=di)
+    try:
+        L.mhd2d_dropin_set_sync(1)
+        G.array("uu")[:8] = u0
+        P = G.ptr
+        nvar = G.nvar
+
+        def step(dt):
+            for irk in range(3):
+                L.sendrecvHaloPointsX()
+                L.derivX(nvar, P("uu"), P("uuHaloX0"), P("uuHaloX1"), P("dudx"), 0, 0)
+                L.derivY(nvar, P("uu"), P("dudy"), 0)
+                if hall == 1:   # d2B/dx2, d2B/dy2 (mhd.c:167-172, read only with eta > 0)
+                    L.derivXX(3, P("uu"), P("uuHaloX0"), P("uuHaloX1"), P("d2Bdx2"), 4, 4)
+                    L.derivYY(3, P("uu"), P("d2Bdy2"), 4)
+                L.calcEfield()
+                L.sendrecvHaloPointsEfieldX()
+                L.derivX(3, P("Efield"), P("EfieldHaloX0"), P("EfieldHaloX1"), P("dEdx"), 0, 0)
+                L.derivY(3, P("Efield"), P("dEdy"), 0)
+                L.calcRHS()
+                L.rkExecute(irk)
+                L.sendrecvHaloPointsX()
+                L.filtering()
+            dt = L.calcDt(dt)
+            L.rkInit(dt)
+            return dt
+
+        dt = L.calcDt(0.0)
+        L.rkInit(dt)
+        for _ in range(warmup):
+            dt = step(dt)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dt = step(dt)
+        sec = time.perf_counter() - t0
+        v = nx * ny * steps / sec
+        return {"workload": f"turbulence {nx}x{ny}, fp64, the reference's entry points in "
+                            "mhd.c's order (drop-in layer, host copies on request), "
+                            "incl. the d2B derivatives mhd.c computes with Hall = 1",
+                "cell_updates_per_s": v, "ms_per_step": sec * 1e3 / steps,
+                "timed_steps": steps, "fused_engine_vs_dropin": value / v}
+    finally:
+        L.mhd2d_dropin_set_sync(0)
+        L.mhd2d_dropin_release()
+        G.close()
+
+
 def batched_forms(m, value, args, copy_gbps):
     """the 64 x 512^2 batched ensemble in the three precisions (errors
     against the FP64 batch on four instances); value: the 2048^2 FP64 rate"""
@@ -834,6 +888,7 @@ def main():
                 "cell_updates_per_s": v, "ms_per_step": ms_p / args.steps,
                 "vs_ideal_fp64": v / value, "kernels": kern}
             c.close()
+        forms["dropin_fp64_" + str(nx)] = dropin_form(m, u0, nx, ny, hall, di, value)
         forms.update(batched_forms(m, value, args, copy_gbps))
         forms.update(uneven_forms(m, value, args))
         forms.update(c3_forms(m, value, args.profile_steps, copy_gbps, hall, di))
