@@ -200,7 +200,7 @@ int mhd2d_xderiv(mhd2d_ctx *c, int nv, const double *in, double *out);
  * field and split product); 0 for the tridiagonal form */
 long long mhd2d_xgemm_panels(const mhd2d_ctx *c);
 /* Autotuner (no counterpart in the reference): times every valid x-solve
- * launch shape - strip width of 8 or 16 y-lines per workgroup and the
+ * launch shape - strip width of 8, 16 or 32 y-lines per workgroup and the
  * XCD-contiguous workgroup order - for the derivative solves, then for the
  * filter, as reps whole RK3 steps of this context's own run, each from the
  * same state, and keeps the fastest.  Needs a state for every instance; the

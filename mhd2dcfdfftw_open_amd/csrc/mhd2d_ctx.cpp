@@ -1163,7 +1163,7 @@ hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
     int C0, W0, G0;
     xsolve_shape(c->p.nx, 8, &C0, &W0, &G0);
     std::vector<Cand> cands;
-    for (int w : {16, 8})
+    for (int w : {16, 8, 32})
         for (int r : {0, 1}) {
             if (w * G0 > 1024 || c->p.ny % w) continue;  // at most 1024 threads
             cands.push_back({w, r});

@@ -650,9 +650,10 @@ hipError_t launch_xsolve(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int 
     int C, W, G;
     xsolve_shape(a.nx, sizeof(TI), &C, &W, &G);
     if (a.width) {
-        // autotuned width: 8 or 16 columns, at most 1024 threads (128
-        // chunks only fit 8)
-        if ((a.width != 8 && a.width != 16) || a.width * G > 1024) return hipErrorInvalidValue;
+        // autotuned width: 8, 16 or 32 columns, at most 1024 threads (128
+        // chunks only fit 8, 64 chunks 16)
+        if ((a.width != 8 && a.width != 16 && a.width != 32) || a.width * G > 1024)
+            return hipErrorInvalidValue;
         W = a.width;
     }
     if (a.nx % C || G > 128 || G < 2 || W * G > 1024 || a.ny % W || nv < 1 || nv > 16)
@@ -664,6 +665,14 @@ hipError_t launch_xsolve(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int 
         case 8: return launch_xsolve_cw<T, TI, 8, 8>(b, nv, s, nb);
         case 16: return launch_xsolve_cw<T, TI, 16, 8>(b, nv, s, nb);
         case 32: return launch_xsolve_cw<T, TI, 32, 8>(b, nv, s, nb);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    if (W == 32) {
+        switch (C) {
+        case 8: return launch_xsolve_cw<T, TI, 8, 32>(b, nv, s, nb);
+        case 16: return launch_xsolve_cw<T, TI, 16, 32>(b, nv, s, nb);
+        case 32: return launch_xsolve_cw<T, TI, 32, 32>(b, nv, s, nb);
         default: return hipErrorInvalidValue;
         }
     }
