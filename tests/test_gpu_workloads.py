@@ -14,7 +14,14 @@ oracle (its OpenMP build; results equal bit for bit to one thread):
 The smaller grids, longer runs and every setting of the physics are covered
 by the other parity tests; these pin the workloads the measurements are
 quoted on, at the sizes they run.
+
+With MHD2D_LONG_TESTS=1 the same checks also run over each workload's full
+step count (2048^2 for 200 steps, checked every 20; batch instances for 500
+steps; the ensemble's runs for 200 steps): several minutes of checker time,
+so the default suite skips them.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -137,3 +144,109 @@ def test_uneven_ensemble_one_run_per_size(m):
                 del S
     finally:
         E.close()
+
+
+LONG = pytest.mark.skipif(not os.environ.get("MHD2D_LONG_TESTS"),
+                          reason="full step counts: minutes of checker time (MHD2D_LONG_TESTS=1)")
+
+
+@LONG
+def test_turbulence_2048_200_steps(m):
+    n, steps, every = 2048, 200, 20
+    print(f"[workload] {n}^2 turbulence, FP64, {steps} steps", flush=True)
+    u0 = m.turbulence(n, n, seed=42, rms=0.1, kmin=1, kmax=16)
+    ctx = m.Context(n, n, hall=1, di=0.02)
+    ctx.set_state(u0)
+    worst = 0.0
+    try:
+        with orc.threaded():
+            S = orc.Sim(n, n, hall=1, di=0.02)
+            S.set_state(u0)
+            for k in range(every, steps + 1, every):
+                ctx.run(every, graph=True)
+                S.run(every)
+                e = assert_fields_close(ctx.state(), S.state(), what=f"step {k}")
+                worst = max(worst, e.max())
+                dt, t, nsteps = ctx.scalars()
+                assert nsteps == k
+                assert abs(dt - S.dt) <= FP64_TOL * S.dt, (k, dt, S.dt)
+                assert abs(t - S.time) <= FP64_TOL * S.time, (k, t, S.time)
+                print(f"[workload] step {k}: per-field rel-L2 max {e.max():.2e}", flush=True)
+            del S
+    finally:
+        ctx.close()
+    print(f"[workload] {n}^2: worst per-field rel-L2 over {steps} steps {worst:.2e}")
+
+
+@LONG
+def test_batch_64x512_500_steps(m):
+    n, B, steps = 512, 64, 500
+    print(f"[workload] {B} x {n}^2 linear waves, FP64 and mixed, {steps} steps", flush=True)
+    rng = np.random.default_rng(42)
+    dis = rng.uniform(0.0, 0.2, B)
+    states = [m.linear_wave(n, n, seed=42 + b, amp=1e-4) for b in range(B)]
+    ctxs = {}
+    try:
+        for prec in ("fp64", "mixed"):
+            c = m.Context(n, n, hall=1, batch=B, precision=m.FP64 if prec == "fp64" else m.MIXED)
+            for b in range(B):
+                c.set_di(b, float(dis[b]))
+                c.set_state(states[b], inst=b)
+            c.run(steps, graph=True)
+            c.sync()
+            ctxs[prec] = c
+        with orc.threaded():
+            for b in (0, 63):
+                S = orc.Sim(n, n, hall=1, di=float(dis[b]))
+                S.set_state(states[b])
+                S.run(steps)
+                want = S.state()
+                del S
+                e = assert_fields_close(ctxs["fp64"].state(b), want, what=f"fp64 instance {b}")
+                em = assert_fields_close(ctxs["mixed"].state(b), want, tol=1e-6,
+                                         what=f"mixed instance {b}")
+                print(f"[workload] instance {b}: FP64 {e.max():.2e}, mixed {em.max():.2e}")
+    finally:
+        for c in ctxs.values():
+            c.close()
+
+
+@LONG
+def test_uneven_ensemble_200_steps(m):
+    count, steps = 96, 200
+    print(f"[workload] {count}-run uneven ensemble, FP64 and mixed, {steps} steps", flush=True)
+    rng = np.random.default_rng(42)
+    sizes = [(256, 256), (512, 256), (1024, 1024), (2048, 512)]
+    runs, states = [], []
+    for k in range(count):
+        nx, ny = sizes[rng.integers(0, 4)]
+        runs.append(dict(nx=nx, ny=ny, hall=1, di=float(rng.uniform(0, 0.1))))
+        states.append(m.turbulence(nx, ny, seed=42 + k, rms=float(rng.uniform(0.05, 0.2)),
+                                   kmax=16))
+    first = {}
+    for k, r in enumerate(runs):
+        first.setdefault((r["nx"], r["ny"]), k)
+    ens = {}
+    try:
+        for prec in ("fp64", "mixed"):
+            E = m.Ensemble(runs, precision=m.FP64 if prec == "fp64" else m.MIXED)
+            for k in range(count):
+                E.set_state(k, states[k])
+            E.run(steps, graph=True)
+            E.sync()
+            ens[prec] = E
+        with orc.threaded():
+            for (nx, ny), k in sorted(first.items()):
+                S = orc.Sim(nx, ny, hall=1, di=runs[k]["di"])
+                S.set_state(states[k])
+                S.run(steps)
+                want = S.state()
+                del S
+                e = field_errors(ens["fp64"].state(k), want)
+                em = field_errors(ens["mixed"].state(k), want)
+                print(f"[workload] run {k} ({nx}x{ny}): FP64 {e.max():.2e}, mixed {em.max():.2e}")
+                assert np.all(e <= FP64_TOL), (k, e)
+                assert np.all(em <= 1e-6), (k, em)
+    finally:
+        for E in ens.values():
+            E.close()
