@@ -16,9 +16,10 @@ by the other parity tests; these pin the workloads the measurements are
 quoted on, at the sizes they run.
 
 With MHD2D_LONG_TESTS=1 the same checks also run over each workload's full
-step count (2048^2 for 200 steps, checked every 20; batch instances for 500
-steps; the ensemble's runs for 200 steps): several minutes of checker time,
-so the default suite skips them.
+step count (2048^2 for 200 steps, checked every 20; 4096^2 for 100 steps,
+checked every 50 in FP64, mixed and the mixed GEMM form; batch instances for
+500 steps; the ensemble's runs for 200 steps): many minutes of checker
+time, so the default suite skips them.
 """
 import os
 
@@ -250,3 +251,43 @@ def test_uneven_ensemble_200_steps(m):
     finally:
         for E in ens.values():
             E.close()
+
+
+@LONG
+def test_turbulence_4096_100_steps_precision_forms(m):
+    """the precision comparison's workload: per-field relative L2 against the
+    oracle every 50 steps for 100 steps, FP64 (1e-12), mixed with the
+    tridiagonal x-derivative and mixed with the 6-product split-BF16 GEMM
+    (1e-6 each); the checker takes ~5 s a step at this size on 16 host
+    cores, so the 200 steps of the workload do not fit one run's time"""
+    n, steps, every = 4096, 100, 50
+    print(f"[workload] {n}^2 turbulence, FP64 / mixed / mixed GEMM, {steps} steps", flush=True)
+    u0 = m.turbulence(n, n, seed=42, rms=0.1, kmin=1, kmax=16)
+    forms = {"fp64": (dict(precision=m.FP64), FP64_TOL),
+             "mixed": (dict(precision=m.MIXED), 1e-6),
+             "mixed_gemm6": (dict(precision=m.MIXED, xgemm=6), 1e-6)}
+    ctxs = {}
+    try:
+        for name, (kw, _) in forms.items():
+            c = m.Context(n, n, hall=1, di=0.02, **kw)
+            c.set_state(u0)
+            ctxs[name] = c
+        with orc.threaded():
+            S = orc.Sim(n, n, hall=1, di=0.02)
+            S.set_state(u0)
+            for k in range(every, steps + 1, every):
+                for c in ctxs.values():
+                    c.run(every, graph=True)
+                for j in range(0, every, 10):
+                    S.run(10)
+                    print(f"[workload] checker at step {k - every + j + 10}", flush=True)
+                want = S.state()
+                for name, (_, tol) in forms.items():
+                    e = assert_fields_close(ctxs[name].state(), want, tol=tol,
+                                            what=f"{name} step {k}")
+                    print(f"[workload] step {k} {name}: per-field rel-L2 max {e.max():.2e}",
+                          flush=True)
+            del S
+    finally:
+        for c in ctxs.values():
+            c.close()
