@@ -23,6 +23,7 @@ device synchronisation on both sides.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -82,12 +83,32 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(nx, ny, u0, hall, di, c1_steps=200, c1_steps_1t=40):
+LSCPU_KEYS = ("Model name", "Vendor ID", "CPU family", "Model", "CPU(s)", "Socket(s)",
+              "Core(s) per socket", "Thread(s) per core", "CPU max MHz", "L3 cache")
+
+
+def _lscpu():
+    """the host CPU as lscpu reports it (the fields in LSCPU_KEYS), or None"""
+    try:
+        r = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20)
+    except (OSError, subprocess.SubprocessError):
+        return None
+    if r.returncode != 0:
+        return None
+    out = {}
+    for line in r.stdout.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in LSCPU_KEYS:
+            out[k.strip()] = v.strip()
+    return out or None
+
+
+def cpu_baseline(nx, ny, u0, hall, di, c1_steps=200, c1_steps_1t=100):
     """The FP64 C restatement (the checker) timed on this host's cores:
 
     * the Orszag-Tang workload (256 x 256, Hall CFL term with d_i = 0.05) with
       OpenMP over independent lines and cells, n threads, 200 RK3 steps, and
-      single-threaded for a bounded 40 steps (the same code; results equal
+      single-threaded for a bounded 100 steps (the same code; results equal
       bit for bit);
     * one RK3 step of the timed GPU workload (nx x ny turbulence), n threads.
     n = OMP_NUM_THREADS (at most 16, this host's share), else the CPU count."""
@@ -117,7 +138,7 @@ def cpu_baseline(nx, ny, u0, hall, di, c1_steps=200, c1_steps_1t=40):
             "sample": f"256x256 Orszag-Tang (d_i = 0.05 Hall CFL), {c1_steps} RK3 steps of the "
                       f"FP64 oracle restatement, OpenMP over lines and cells, {threads} threads "
                       f"({tn:.1f} s)",
-            "cpu_model": _cpu_model(), "cores_visible": os.cpu_count(),
+            "cpu_model": _cpu_model(), "lscpu": _lscpu(), "cores_visible": os.cpu_count(),
             "threads_1": {"value": v1, "steps": c1_steps_1t, "seconds": t1,
                           "workload": "256x256 Orszag-Tang, 1 thread"},
             "threads_n": {"value": vn, "threads": threads, "steps": c1_steps, "seconds": tn,
@@ -419,13 +440,15 @@ def roofline_of(ctx, nx, ny, elem, delem, batch, profile_steps, copy_gbps, traff
 
 def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50, chunks=4,
              graph=True):
-    """4096^2: mixed precision with the x-derivative as the tridiagonal solve
-    and as the dense A^-1 B through split-BF16 MFMA GEMMs (3 and 6
-    products), and plain FP32; per-field rel-L2 against FP64 every 50 steps
-    for 200 steps.  Each variant is timed over its 4 x 50 steps (hipEvents)."""
+    """4096^2: mixed precision (FP64 state) and plain FP32 (FP32 state), each
+    with the x-derivative as the tridiagonal solve and as the dense A^-1 B
+    through split-BF16 MFMA GEMMs (3 and 6 products); per-field rel-L2
+    against FP64 every 50 steps for 200 steps.  Each variant is timed over
+    its 4 x 50 steps (hipEvents)."""
     u0 = m.turbulence(n, n, seed=42, rms=0.1, kmin=1, kmax=16)
     variants = [("fp64", m.FP64, 0), ("mixed", m.MIXED, 0), ("mixed_gemm6", m.MIXED, 6),
-                ("mixed_gemm3", m.MIXED, 3), ("fp32", m.FP32, 0)]
+                ("mixed_gemm3", m.MIXED, 3), ("fp32", m.FP32, 0), ("fp32_gemm6", m.FP32, 6),
+                ("fp32_gemm3", m.FP32, 3)]
     ref_states, out = [], {}
     ref_value = None
     for name, prec, xg in variants:
@@ -453,8 +476,8 @@ def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50,
         rec = {"workload": f"turbulence {n}x{n}, {name}", "cell_updates_per_s": v,
                "ms_per_step": ms / (chunk * chunks), "speedup_vs_fp64_2048": v / value2048,
                "speedup_vs_fp64_4096": v / ref_value, "kernels": kern, "roofline": roof,
-               "step_roofline_frac": v * B_ALG["fp64" if name == "fp64" else
-                                                ("fp32" if name == "fp32" else "mixed")] / HBM_PEAK}
+               "step_roofline_frac": v * B_ALG["fp64" if prec == m.FP64 else
+                                                ("fp32" if prec == m.FP32 else "mixed")] / HBM_PEAK}
         if errs:
             rec["rel_l2_vs_fp64_every_50"] = errs
             rec["rel_l2_vs_fp64_max"] = max(errs)

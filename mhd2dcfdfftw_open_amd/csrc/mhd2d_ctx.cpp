@@ -205,6 +205,9 @@ hipError_t xderiv_fields(mhd2d_ctx* c, const T* in, TD* out, int nv, size_t bs_i
             g.klist2 = c->xg.klist2_d;
             g.kcount2 = c->xg.kcount2_d;
             g.kmax2 = c->xg.kmax2;
+            g.klist4 = c->xg.klist4_d;
+            g.kcount4 = c->xg.kcount4_d;
+            g.kmax4 = c->xg.kmax4;
             g.nsplit = c->xg.nsplit;
             g.rs = rs;
             g.nx = c->p.nx;

@@ -114,10 +114,14 @@ struct XGemmArgs {
     const int* klist = nullptr;
     const int* kcount = nullptr;
     int kmax = 0;
-    // the same for 128-row tiles: entries (chunk << 2) | half mask
+    // the same for 128- and 256-row tiles: entries (chunk << 4) | mask of
+    // the 64-row sub-tiles that multiply the chunk
     const int* klist2 = nullptr;
     const int* kcount2 = nullptr;
     int kmax2 = 0;
+    const int* klist4 = nullptr;
+    const int* kcount4 = nullptr;
+    int kmax4 = 0;
     int nsplit = 3;
     const RunScalars* rs = nullptr;
     int nx = 0, ny = 0;

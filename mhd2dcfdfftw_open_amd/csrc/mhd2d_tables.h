@@ -68,13 +68,15 @@ public:
     hipError_t build(const GridTables& tb, int nsplit);
     void release();
 
-    int nx = 0, ny = 0, nsplit = 0, kmax = 0, kmax2 = 0;
-    long long panels_total = 0;  // non-zero 64 x 16 panels over all row tiles
+    int nx = 0, ny = 0, nsplit = 0, kmax = 0, kmax2 = 0, kmax4 = 0;
+    long long panels_total = 0;  // non-zero 64 x 32 panels over all row tiles
     size_t bytes = 0;
     const unsigned short *mh = nullptr, *mm = nullptr, *ml = nullptr;
     const int *klist_d = nullptr, *kcount_d = nullptr;
-    // chunk lists of 128-row tiles, entries (chunk << 2) | half mask
+    // chunk lists of 128- and 256-row tiles, entries (chunk << 4) | mask of
+    // the 64-row sub-tiles that multiply the chunk
     const int *klist2_d = nullptr, *kcount2_d = nullptr;
+    const int *klist4_d = nullptr, *kcount4_d = nullptr;
 
 private:
     void* blob_ = nullptr;

@@ -130,50 +130,58 @@ hipError_t XGemmOp::build(const GridTables& tb, int nsplit_)
         panels_total += kcount[t];
         for (int q2 = 0; q2 < kcount[t]; q2++) klist[(size_t)t * kmax + q2] = lists[t][q2];
     }
-    // 128-row tiles (pairs of 64-row tiles): the union of the pair's chunk
-    // lists, each entry (chunk << 2) | (bit 0: the first half multiplies it,
-    // bit 1: the second half)
-    const int ntile2 = ntile / 2;
-    std::vector<std::vector<int>> lists2(ntile2);
-    kmax2 = 0;
-    for (int t = 0; t < ntile2; t++) {
-        std::vector<int> mask(npanel, 0);
-        for (int h = 0; h < 2; h++)
-            for (int c : lists[2 * t + h]) mask[c] |= 1 << h;
-        for (int c = 0; c < npanel; c++)
-            if (mask[c]) lists2[t].push_back((c << 2) | mask[c]);
-        kmax2 = std::max(kmax2, (int)lists2[t].size());
-    }
-    std::vector<int> klist2((size_t)std::max(1, ntile2 * kmax2), 0), kcount2(std::max(1, ntile2), 0);
-    for (int t = 0; t < ntile2; t++) {
-        kcount2[t] = (int)lists2[t].size();
-        for (int q2 = 0; q2 < kcount2[t]; q2++) klist2[(size_t)t * kmax2 + q2] = lists2[t][q2];
-    }
-    klist.insert(klist.end(), klist2.begin(), klist2.end());
-    const size_t k2off = (size_t)ntile * kmax;
+    // 128- and 256-row tiles (2 or 4 consecutive 64-row tiles): the union of
+    // their chunk lists, each entry (chunk << 4) | (bit h: the 64-row
+    // sub-tile h multiplies it)
+    auto group_lists = [&](int R, std::vector<int>& flat, std::vector<int>& count) {
+        const int nt = ntile / R;
+        std::vector<std::vector<int>> lr(nt);
+        int km = 0;
+        for (int t = 0; t < nt; t++) {
+            std::vector<int> mask(npanel, 0);
+            for (int h = 0; h < R; h++)
+                for (int c : lists[R * t + h]) mask[c] |= 1 << h;
+            for (int c = 0; c < npanel; c++)
+                if (mask[c]) lr[t].push_back((c << 4) | mask[c]);
+            km = std::max(km, (int)lr[t].size());
+        }
+        flat.assign((size_t)std::max(1, nt * km), 0);
+        count.assign(std::max(1, nt), 0);
+        for (int t = 0; t < nt; t++) {
+            count[t] = (int)lr[t].size();
+            for (int q2 = 0; q2 < count[t]; q2++) flat[(size_t)t * km + q2] = lr[t][q2];
+        }
+        return km;
+    };
+    std::vector<int> klist2, kcount2, klist4, kcount4;
+    kmax2 = group_lists(2, klist2, kcount2);
+    kmax4 = group_lists(4, klist4, kcount4);
+    // one int array: klist, klist2, klist4, kcount, kcount2, kcount4
+    const size_t o2 = klist.size(), o4 = o2 + klist2.size(), oc = o4 + klist4.size();
+    const size_t oc2 = oc + kcount.size(), oc4 = oc2 + kcount2.size();
+    std::vector<int> ints;
+    for (const auto* v : {&klist, &klist2, &klist4, &kcount, &kcount2, &kcount4})
+        ints.insert(ints.end(), v->begin(), v->end());
     const size_t pbytes = (size_t)used * NN * sizeof(unsigned short);
-    const size_t lbytes = (klist.size() + kcount.size() + kcount2.size()) * sizeof(int);
+    const size_t lbytes = ints.size() * sizeof(int);
     hipError_t e = hipMalloc(&blob_, pbytes + lbytes);
     if (e != hipSuccess) return e;
     e = hipMemcpy(blob_, planes.data(), pbytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
     char* base = (char*)blob_;
-    e = hipMemcpy(base + pbytes, klist.data(), klist.size() * sizeof(int), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return e;
-    e = hipMemcpy(base + pbytes + klist.size() * sizeof(int), kcount.data(),
-                  kcount.size() * sizeof(int), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return e;
-    e = hipMemcpy(base + pbytes + (klist.size() + kcount.size()) * sizeof(int), kcount2.data(),
-                  kcount2.size() * sizeof(int), hipMemcpyHostToDevice);
+    e = hipMemcpy(base + pbytes, ints.data(), lbytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
     const unsigned short* pl = (const unsigned short*)blob_;
     mh = pl;
     mm = pl + NN;
     ml = used == 3 ? pl + 2 * NN : nullptr;
-    klist_d = (const int*)(base + pbytes);
-    kcount_d = klist_d + klist.size();
-    klist2_d = ntile2 > 0 ? klist_d + k2off : nullptr;
-    kcount2_d = ntile2 > 0 ? kcount_d + kcount.size() : nullptr;
+    const int* id = (const int*)(base + pbytes);
+    klist_d = id;
+    klist2_d = ntile >= 2 ? id + o2 : nullptr;
+    klist4_d = ntile >= 4 ? id + o4 : nullptr;
+    kcount_d = id + oc;
+    kcount2_d = ntile >= 2 ? id + oc2 : nullptr;
+    kcount4_d = ntile >= 4 ? id + oc4 : nullptr;
     bytes = pbytes + lbytes;
     return hipSuccess;
 }
@@ -183,7 +191,7 @@ void XGemmOp::release()
     if (blob_) (void)hipFree(blob_);
     blob_ = nullptr;
     mh = mm = ml = nullptr;
-    klist_d = kcount_d = klist2_d = kcount2_d = nullptr;
+    klist_d = kcount_d = klist2_d = kcount2_d = klist4_d = kcount4_d = nullptr;
 }
 
 XGemmOp::~XGemmOp() { release(); }
@@ -192,14 +200,26 @@ typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef short s8v __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ unsigned short bf16_bits(float x)
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+
+// Two FP32 values split into BF16 planes, round to nearest even at each
+// level: p[0] their BF16 roundings, p[1] the roundings of what p[0] leaves,
+// p[2] (NP == 3) of what p[0] + p[1] leave; each plane packed in one dword
+// (first value in the low half).  The casts compile to gfx950's
+// v_cvt_pk_bf16_f32 (one instruction per plane and pair).
+template <int NP>
+__device__ __forceinline__ void bf16_split2(float x0, float x1, unsigned (&p)[3])
 {
-    const unsigned u = __float_as_uint(x);
-    return (unsigned short)((u + (((u >> 16) & 1u) + 0x7fffu)) >> 16);
-}
-__device__ __forceinline__ float bf16_float(unsigned short b)
-{
-    return __uint_as_float((unsigned)b << 16);
+    const bf2v h = {(__bf16)x0, (__bf16)x1};
+    const float r0 = x0 - (float)h[0], r1 = x1 - (float)h[1];
+    const bf2v m = {(__bf16)r0, (__bf16)r1};
+    p[0] = __builtin_bit_cast(unsigned, h);
+    p[1] = __builtin_bit_cast(unsigned, m);
+    if constexpr (NP == 3) {
+        const bf2v l = {(__bf16)(r0 - (float)m[0]), (__bf16)(r1 - (float)m[1])};
+        p[2] = __builtin_bit_cast(unsigned, l);
+    }
 }
 
 __device__ __forceinline__ f4v mfma_bf16(const s8v& a, const s8v& b, const f4v& c)
@@ -209,9 +229,10 @@ __device__ __forceinline__ f4v mfma_bf16(const s8v& a, const s8v& b, const f4v& 
 }
 
 // Workgroup: a (64 RH)-row x 256-column tile of D_v (one field v, one
-// instance), 4 RH waves of 64 x 64 each (4 x 4 blocks of 16 x 16, 16
-// accumulators).  For every 32-column chunk of M's row tile that is non-zero
-// in either 64-row half:
+// instance).  RH = 1: 4 waves of 64 x 64; RH = 2: 8 waves of 64 x 64; RH = 4:
+// 8 waves of 128 x 64 (4 x 4 or 8 x 4 blocks of 16 x 16 accumulators).  For
+// every 32-column chunk of M's row tile that is non-zero in any of its
+// 64-row sub-tiles:
 //  * the 32 x 256 block of U (rows = the chunk's columns) is read once by
 //    the workgroup (each column by RH threads), the column's row-0 value
 //    subtracted (the operator annihilates constants), split into BF16
@@ -220,24 +241,27 @@ __device__ __forceinline__ f4v mfma_bf16(const s8v& a, const s8v& b, const f4v& 
 //    Rows and columns are padded to 40 elements (80 bytes: the 16-byte
 //    fragment reads of 16 consecutive rows or columns then cover all 64
 //    banks once);
-//  * each wave whose half has the chunk reads its A and B fragments from
-//    LDS and runs the split products, small terms first;
+//  * each wave reads its A and B fragments from LDS and runs the split
+//    products of its sub-tiles that have the chunk, small terms first;
 //  * the next chunk's U and M blocks are loaded into registers while the
 //    MFMAs run.
-// With RH = 2 a U block serves 128 rows: the staging per output element
-// (loads, split, LDS writes) falls by about a third at nx = 4096.
+// Taller tiles share a U block among more rows: against 64-row tiles the
+// staging per output element (loads, split, LDS writes) falls by about a
+// third with 128 rows at nx = 4096, by two thirds with 256.
 // MFMA 16x16x32 operands: lane (r = lane % 16, g = lane / 16) holds 8
 // consecutive k of row r (A) or column r (B), k = 8 g .. 8 g + 7; the same k
 // mapping on both sides pairs the products correctly.  D[i][j] is in lane
 // (j + 16 * (i / 4)), element i % 4.
 template <typename TI, int NS, int RH>
-__global__ void __launch_bounds__(256 * RH, 2 / RH) k_xgemm(XGemmArgs<TI> a)
+__global__ void __launch_bounds__(RH == 1 ? 256 : 512, RH == 1 ? 2 : 1) k_xgemm(XGemmArgs<TI> a)
 {
-    constexpr int NT = 256 * RH;         // threads
+    constexpr int NT = RH == 1 ? 256 : 512;  // threads: 4 or 8 waves
+    constexpr int WRB = RH == 4 ? 8 : 4;     // 16-row blocks of a wave
+    constexpr int WSUB = WRB / 4;            // 64-row sub-tiles of a wave
     constexpr int NP = NS == 6 ? 3 : 2;  // planes of U and of M
     constexpr int CT = 256;              // columns per workgroup
     constexpr int KC = XGEMM_KC;         // k per chunk
-    constexpr int KT = KC / RH;          // k of a column staged by one thread
+    constexpr int KT = KC * 256 / NT;    // k of a column staged by one thread
     constexpr int CS = 40;               // LDS row / column stride (bf16 elements)
     constexpr int RT = 64 * RH;          // rows per workgroup
     constexpr int MP = NP * RT * (KC / 8);  // 16-byte pieces of an M block
@@ -251,7 +275,7 @@ __global__ void __launch_bounds__(256 * RH, 2 / RH) k_xgemm(XGemmArgs<TI> a)
     const int ctiles = (ny + CT - 1) / CT;
     const int it = blockIdx.x / ctiles, ct = blockIdx.x % ctiles;
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-    const int wh = w >> 2, wc = w & 3;        // this wave's row half and column block
+    const int wh = w >> 2, wc = w & 3;        // this wave's row group and column block
     const int lr = lane & 15, lg = lane >> 4;
     const int row0 = it * RT;
     const int colw = ct * CT + wc * 64;       // this wave's first column
@@ -265,18 +289,19 @@ __global__ void __launch_bounds__(256 * RH, 2 / RH) k_xgemm(XGemmArgs<TI> a)
     const unsigned short* const mpl[3] = {a.mh, a.mm, a.ml};
     const TI c0 = cact ? U[mycol] : (TI)0;
 
-    f4v acc[4][4];
+    f4v acc[WRB][4];
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++)
+    for (int rb = 0; rb < WRB; rb++)
 #pragma unroll
         for (int cb = 0; cb < 4; cb++) acc[rb][cb] = f4v{0.f, 0.f, 0.f, 0.f};
 
-    const int* __restrict__ kl = RH == 2 ? a.klist2 + (size_t)it * a.kmax2
-                                         : a.klist + (size_t)it * a.kmax;
-    const int nch = RH == 2 ? a.kcount2[it] : a.kcount[it];
-    // chunk index and the halves that multiply it
-    auto chunk = [&](int q) { return RH == 2 ? (kl[q] >> 2) : kl[q]; };
-    auto mask = [&](int q) { return RH == 2 ? (kl[q] & 3) : 1; };
+    const int* __restrict__ kl = RH == 4   ? a.klist4 + (size_t)it * a.kmax4
+                                 : RH == 2 ? a.klist2 + (size_t)it * a.kmax2
+                                           : a.klist + (size_t)it * a.kmax;
+    const int nch = RH == 4 ? a.kcount4[it] : (RH == 2 ? a.kcount2[it] : a.kcount[it]);
+    // chunk index and the 64-row sub-tiles that multiply it
+    auto chunk = [&](int q) { return RH == 1 ? kl[q] : (kl[q] >> 4); };
+    auto mask = [&](int q) { return RH == 1 ? 1 : (kl[q] & 15); };
     TI pre[KT];
     s8v mpre[MPT];
     // this thread's U values and M pieces of chunk q into registers
@@ -304,21 +329,21 @@ __global__ void __launch_bounds__(256 * RH, 2 / RH) k_xgemm(XGemmArgs<TI> a)
         const int sc = t % CT;
 #pragma unroll
         for (int g = 0; g < KT / 8; g++) {
-            s8v h, m, l;
+            u4v h, m, l;
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const float x = cact ? (float)(pre[g * 8 + e] - c0) : 0.f;
-                const unsigned short hb = bf16_bits(x);
-                const float r1 = x - bf16_float(hb);
-                const unsigned short mb = bf16_bits(r1);
-                h[e] = (short)hb;
-                m[e] = (short)mb;
-                if constexpr (NS == 6) l[e] = (short)bf16_bits(r1 - bf16_float(mb));
+            for (int e = 0; e < 8; e += 2) {
+                const float x0 = cact ? (float)(pre[g * 8 + e] - c0) : 0.f;
+                const float x1 = cact ? (float)(pre[g * 8 + e + 1] - c0) : 0.f;
+                unsigned p[3];
+                bf16_split2<NP>(x0, x1, p);
+                h[e / 2] = p[0];
+                m[e / 2] = p[1];
+                if constexpr (NS == 6) l[e / 2] = p[2];
             }
             const int ko = kpart * KT + g * 8;
-            *reinterpret_cast<s8v*>(su + (0 * CT + sc) * CS + ko) = h;
-            *reinterpret_cast<s8v*>(su + (1 * CT + sc) * CS + ko) = m;
-            if constexpr (NS == 6) *reinterpret_cast<s8v*>(su + (2 * CT + sc) * CS + ko) = l;
+            *reinterpret_cast<u4v*>(su + (0 * CT + sc) * CS + ko) = h;
+            *reinterpret_cast<u4v*>(su + (1 * CT + sc) * CS + ko) = m;
+            if constexpr (NS == 6) *reinterpret_cast<u4v*>(su + (2 * CT + sc) * CS + ko) = l;
         }
 #pragma unroll
         for (int i = 0; i < MPT; i++) {
@@ -329,7 +354,9 @@ __global__ void __launch_bounds__(256 * RH, 2 / RH) k_xgemm(XGemmArgs<TI> a)
             }
         }
         __syncthreads();
-        const bool mine = wact && ((mask(q) >> wh) & 1);
+        // the wave's 64-row sub-tiles that multiply chunk q
+        const int wmask = (mask(q) >> (wh * WSUB)) & ((1 << WSUB) - 1);
+        const bool mine = wact && wmask != 0;
         if (q + 1 < nch) fetch(q + 1);  // in flight during the MFMAs
         if (mine) {
             s8v bh[4], bm[4], bl[4];
@@ -342,8 +369,9 @@ __global__ void __launch_bounds__(256 * RH, 2 / RH) k_xgemm(XGemmArgs<TI> a)
                     bl[cb] = *reinterpret_cast<const s8v*>(su + (2 * CT + c) * CS + lg * 8);
             }
 #pragma unroll
-            for (int rb = 0; rb < 4; rb++) {
-                const int r = wh * 64 + rb * 16 + lr;
+            for (int rb = 0; rb < WRB; rb++) {
+                if (WSUB > 1 && !((wmask >> (rb / 4)) & 1)) continue;
+                const int r = wh * WRB * 16 + rb * 16 + lr;
                 const s8v ah = *reinterpret_cast<const s8v*>(sm + (0 * RT + r) * CS + lg * 8);
                 const s8v am = *reinterpret_cast<const s8v*>(sm + (1 * RT + r) * CS + lg * 8);
                 s8v al;
@@ -367,9 +395,9 @@ __global__ void __launch_bounds__(256 * RH, 2 / RH) k_xgemm(XGemmArgs<TI> a)
     if (!wact) return;
     // 16 lanes write 64 contiguous bytes of a row; nontemporal (the RHS
     // reads the derivatives once, later)
-    const int rw = row0 + wh * 64;
+    const int rw = row0 + wh * WRB * 16;
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++)
+    for (int rb = 0; rb < WRB; rb++)
 #pragma unroll
         for (int cb = 0; cb < 4; cb++)
 #pragma unroll
@@ -384,11 +412,18 @@ hipError_t launch_xgemm(const XGemmArgs<TI>& a, int nv, hipStream_t s, int nb)
     if (a.nx % 64 || a.ny % 64 || nv < 1 || nv > 16 || nb < 1 || nb > 65535 || !a.mh || !a.mm)
         return hipErrorInvalidValue;
     const int ctiles = (a.ny + 255) / 256;
-    // 128-row tiles where the lines hold whole pairs of 64-row tiles
-    const bool two = a.nx % 128 == 0 && a.klist2 && a.kcount2;
-    dim3 grid((a.nx / (two ? 128 : 64)) * ctiles, nv, nb), block(two ? 512 : 256);
+    // Row tiles: 256 rows for the 3-product form (the U staging per output
+    // halves against 128: 4096^2 mixed 0.71 -> 0.62 ms), 128 for 6 products
+    // (256-row tiles need 256 VGPRs there and measured 1.3x slower in the
+    // mixed form, 1.1x in FP32); lines that do not hold whole tiles take the
+    // next smaller ones
+    const int rh = (a.nsplit == 3 && a.nx % 256 == 0 && a.klist4 && a.kcount4) ? 4
+                   : (a.nx % 128 == 0 && a.klist2 && a.kcount2) ? 2 : 1;
+    dim3 grid((a.nx / (64 * rh)) * ctiles, nv, nb), block(rh == 1 ? 256 : 512);
     if (a.nsplit == 6 && !a.ml) return hipErrorInvalidValue;
-    if (two) {
+    if (rh == 4) {
+        hipLaunchKernelGGL((k_xgemm<TI, 3, 4>), grid, block, 0, s, a);
+    } else if (rh == 2) {
         if (a.nsplit == 6) hipLaunchKernelGGL((k_xgemm<TI, 6, 2>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((k_xgemm<TI, 3, 2>), grid, block, 0, s, a);
     } else {
