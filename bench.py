@@ -439,16 +439,19 @@ def roofline_of(ctx, nx, ny, elem, delem, batch, profile_steps, copy_gbps, traff
 
 
 def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50, chunks=4,
-             graph=True):
+             graph=True, only=None):
     """4096^2: mixed precision (FP64 state) and plain FP32 (FP32 state), each
     with the x-derivative as the tridiagonal solve and as the dense A^-1 B
     through split-BF16 MFMA GEMMs (3 and 6 products); per-field rel-L2
     against FP64 every 50 steps for 200 steps.  Each variant is timed over
-    its 4 x 50 steps (hipEvents)."""
+    its 4 x 50 steps (hipEvents).  only: the variants to run (by name);
+    errors and speed-ups against FP64 need "fp64" among them."""
     u0 = m.turbulence(n, n, seed=42, rms=0.1, kmin=1, kmax=16)
     variants = [("fp64", m.FP64, 0), ("mixed", m.MIXED, 0), ("mixed_gemm6", m.MIXED, 6),
                 ("mixed_gemm3", m.MIXED, 3), ("fp32", m.FP32, 0), ("fp32_gemm6", m.FP32, 6),
                 ("fp32_gemm3", m.FP32, 3)]
+    if only:
+        variants = [x for x in variants if x[0] in only]
     ref_states, out = [], {}
     ref_value = None
     for name, prec, xg in variants:
@@ -466,7 +469,7 @@ def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50,
             st = ctx.state()
             if name == "fp64":
                 ref_states.append(st)
-            else:
+            elif ref_states:
                 errs.append(max(field_errors(st, ref_states[k])))
         v = n * n * chunk * chunks / (ms * 1e-3)
         if name == "fp64":
@@ -475,11 +478,12 @@ def c3_forms(m, value2048, profile_steps, copy_gbps, hall, di, n=4096, chunk=50,
         roof, kern = roofline_of(ctx, n, n, pe, pd, 1, profile_steps, copy_gbps)
         rec = {"workload": f"turbulence {n}x{n}, {name}", "cell_updates_per_s": v,
                "ms_per_step": ms / (chunk * chunks), "speedup_vs_fp64_2048": v / value2048,
-               "speedup_vs_fp64_4096": v / ref_value, "kernels": kern, "roofline": roof,
+               "speedup_vs_fp64_4096": v / ref_value if ref_value else None,
+               "kernels": kern, "roofline": roof,
                "step_roofline_frac": v * B_ALG["fp64" if prec == m.FP64 else
                                                 ("fp32" if prec == m.FP32 else "mixed")] / HBM_PEAK}
         if errs:
-            rec["rel_l2_vs_fp64_every_50"] = errs
+            rec[f"rel_l2_vs_fp64_every_{chunk}"] = errs
             rec["rel_l2_vs_fp64_max"] = max(errs)
             rec["meets_1e-6"] = bool(max(errs) <= 1e-6)
         if xg:
@@ -757,6 +761,11 @@ def main():
                          "forced off / on where it applies")
     ap.add_argument("--c3-only", action="store_true",
                     help="only the 4096^2 precision / x-derivative comparison (prints its JSON)")
+    ap.add_argument("--c3-forms", default="",
+                    help="with --c3-only: comma-separated variants (fp64, mixed, mixed_gemm6, "
+                         "mixed_gemm3, fp32, fp32_gemm6, fp32_gemm3; default all)")
+    ap.add_argument("--c3-chunk", type=int, default=50,
+                    help="with --c3-only: steps between error checks (4 checks)")
     ap.add_argument("--c4-only", action="store_true",
                     help="only the 64 x 512^2 batched ensemble (prints its JSON; honours --graph)")
     ap.add_argument("--c5-only", action="store_true",
@@ -812,8 +821,10 @@ def main():
         return
 
     if args.c3_only:
+        only = [f for f in args.c3_forms.split(",") if f] or None
         print(json.dumps(c3_forms(m, float("nan"), args.profile_steps, m.copy_probe(COPY_BYTES, 5),
-                                  1, 0.02, graph=bool(args.graph))))
+                                  1, 0.02, chunk=args.c3_chunk, graph=bool(args.graph),
+                                  only=only)))
         return
 
     if args.c5_only:
