@@ -755,28 +755,32 @@ int mhd2d_copy_probe(size_t bytes, int iters, double* gbps)
 {
     if (!gbps || bytes < 16 || iters < 1) return fail(MHD2D_EINVAL, "bad argument");
     bytes &= ~(size_t)15;
-    void *a = nullptr, *b = nullptr;
-    hipStream_t s = nullptr;
-    hipEvent_t e0, e1;
-    HIPCHK(hipMalloc(&a, bytes));
-    HIPCHK(hipMalloc(&b, bytes));
-    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    HIPCHK(hipMemsetAsync(a, 0, bytes, s));
-    HIPCHK(launch_copy(a, b, bytes, s));  // warm-up
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, s));
-    for (int k = 0; k < iters; k++) HIPCHK(launch_copy(a, b, bytes, s));
-    HIPCHK(hipEventRecord(e1, s));
-    HIPCHK(hipEventSynchronize(e1));
+    // released on every return, also after a failed call
+    struct Res {
+        DevTemp<char> a, b;
+        hipStream_t s = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Res()
+        {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } r;
+    HIPCHK(hipMalloc(&r.a.p, bytes));
+    HIPCHK(hipMalloc(&r.b.p, bytes));
+    HIPCHK(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
+    HIPCHK(hipMemsetAsync(r.a.p, 0, bytes, r.s));
+    HIPCHK(launch_copy(r.a.p, r.b.p, bytes, r.s));  // warm-up
+    HIPCHK(hipEventCreate(&r.e0));
+    HIPCHK(hipEventCreate(&r.e1));
+    HIPCHK(hipEventRecord(r.e0, r.s));
+    for (int k = 0; k < iters; k++) HIPCHK(launch_copy(r.a.p, r.b.p, bytes, r.s));
+    HIPCHK(hipEventRecord(r.e1, r.s));
+    HIPCHK(hipEventSynchronize(r.e1));
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    HIPCHK(hipEventElapsedTime(&ms, r.e0, r.e1));
     *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(s);
-    (void)hipFree(a);
-    (void)hipFree(b);
     return MHD2D_OK;
 }
 
