@@ -815,11 +815,21 @@ __device__ __forceinline__ void ysweep_body(const YSweepArgs<T, TD>& a, int bx, 
     constexpr int PA[5] = {RHO, UX, UZ, BX, PSI_SLOT};
     constexpr int PB[5] = {PR, UY, BZ, BY, -1};
     constexpr int npairs = GLM ? 5 : 4;
+    // Ideal MHD (MODE 0): the last pair, (Bx, By), runs outside the loop and
+    // leaves its state values (filtered when the sweep filters, as loaded
+    // otherwise) in nxt, whose registers the next-pair load no longer needs;
+    // E takes them from there and reads four fields back instead of six.
+    // Its spectrum is saved in Z and differentiated there, so no more values
+    // are live than in the loop.  (The CFL loop, unrolled by 4 only, reads
+    // them back: a runtime index would move nxt to scratch memory.)
+    constexpr bool KEEPB = MODE == 0;
+    static_assert(!KEEPB || (PA[3] == BX && PB[3] == BY && npairs == 4),
+                  "the kept pair is (Bx, By), the last one");
+    cx<T> nxt[E];
     if (a.filter || a.derivs) {
         // The next pair is loaded while the last transform of the current
         // pair runs (its registers replace the saved spectrum Z), so the HBM
         // latency of every pair but the first is hidden behind FFT work.
-        cx<T> nxt[E];
 #pragma unroll
         for (int m = 0; m < E; m++) {
             const int k = t + m * TPR;
@@ -827,7 +837,7 @@ __device__ __forceinline__ void ysweep_body(const YSweepArgs<T, TD>& a, int bx, 
         }
         TD* const lapz = a.lap ? a.lap + z * a.bs_l : nullptr;
 #pragma unroll 1
-        for (int p = 0; p < npairs; p++) {
+        for (int p = 0; p < (KEEPB ? npairs - 1 : npairs); p++) {
             const int fa = PA[p], fb = PB[p];
             const bool two = !GLM || fb >= 0;
             T* ua = u + fa * n + row;
@@ -917,6 +927,68 @@ __device__ __forceinline__ void ysweep_body(const YSweepArgs<T, TD>& a, int bx, 
                 }
             }
         }
+        if constexpr (KEEPB) {
+            // the last pair (Bx, By): the same operations as the loop body
+            // without the Laplacian terms and with no next pair to load
+            T* const ua = u + BX * n + row;
+            T* const ub = u + BY * n + row;
+            int tq = t;
+            asm volatile("" : "+v"(tq));
+            cx<T> Z[E];
+#pragma unroll
+            for (int m = 0; m < E; m++) Z[m] = nxt[m];
+            fft_fwd<T, N, E>(Z, t, lre, lim, tw);
+            if (a.filter) {
+                cx<T> F[E];
+#pragma unroll
+                for (int m = 0; m < E; m++) F[m] = mul_scal(Z[m], (T)a.tb.mf[tq + m * TPR]);
+                fft_inv<T, N, E>(F, t, lre, lim, tw);
+#pragma unroll
+                for (int m = 0; m < E; m++) nxt[m] = F[m];
+                if (active) {
+#pragma unroll
+                    for (int m = 0; m < E; m++) {
+                        const int k = t + m * TPR;
+                        st_stream<MHD2D_NT_YS>(ua + k, F[m].re);
+                        st_stream<MHD2D_NT_YS>(ub + k, F[m].im);
+                    }
+                }
+            }
+            if (a.derivs) {
+                const double* __restrict__ m1 = a.filter ? a.tb.mfd : a.tb.md;
+#pragma unroll
+                for (int m = 0; m < E; m++) Z[m] = mul_i_scal(Z[m], (T)m1[tq + m * TPR]);
+                fft_inv<T, N, E>(Z, t, lre, lim, tw);
+                if (active) {
+                    TD* const da = dudy + BX * n + row;
+                    TD* const db = dudy + BY * n + row;
+#pragma unroll
+                    for (int m = 0; m < E; m++) {
+                        const int k = t + m * TPR;
+                        st_stream<MHD2D_NT_Y>(da + k, (TD)Z[m].re);
+                        st_stream<MHD2D_NT_Y>(db + k, (TD)Z[m].im);
+                    }
+                }
+            }
+        }
+    }
+
+    if (a.dt_limits) {
+        // CFL limits of the filtered state (stage 1; timeAdvance.c:50-136),
+        // ahead of E: E's other inputs are then read back from cache
+        DtParams dp = a.dp;
+        dp.di = rs->di;
+        double mn = INFINITY;
+        if (active) {
+#pragma unroll 4
+            for (int m = 0; m < E; m++) {
+                const size_t k = row + t + m * TPR;
+                mn = fmin(mn, cell_dt(u[RHO * n + k], u[UX * n + k], u[UY * n + k],
+                                      u[BX * n + k], u[BY * n + k], u[BZ * n + k],
+                                      u[PR * n + k], dp));
+            }
+        }
+        block_min_dt(mn, active, rs, red);
     }
 
     if (a.derivs) {
@@ -929,7 +1001,15 @@ __device__ __forceinline__ void ysweep_body(const YSweepArgs<T, TD>& a, int bx, 
         for (int m = 0; m < E; m++) {
             const size_t k = row + t + m * TPR;
             const T ux = u[UX * n + k], uy = u[UY * n + k], uz = u[UZ * n + k];
-            const T bx = u[BX * n + k], by = u[BY * n + k], bz = u[BZ * n + k];
+            T bx, by;
+            if constexpr (KEEPB) {
+                bx = nxt[m].re;
+                by = nxt[m].im;
+            } else {
+                bx = u[BX * n + k];
+                by = u[BY * n + k];
+            }
+            const T bz = u[BZ * n + k];
             const T ex = uz * by - uy * bz;
             const T ey = ux * bz - uz * bx;
             const T ez = uy * bx - ux * by;
@@ -951,23 +1031,6 @@ __device__ __forceinline__ void ysweep_body(const YSweepArgs<T, TD>& a, int bx, 
                 st_stream<MHD2D_NT_Y>(dEdy + 1 * n + k, (TD)v[m].im);
             }
         }
-    }
-
-    if (a.dt_limits) {
-        // CFL limits of the filtered state (stage 1; timeAdvance.c:50-136)
-        DtParams dp = a.dp;
-        dp.di = rs->di;
-        double mn = INFINITY;
-        if (active) {
-#pragma unroll 4
-            for (int m = 0; m < E; m++) {
-                const size_t k = row + t + m * TPR;
-                mn = fmin(mn, cell_dt(u[RHO * n + k], u[UX * n + k], u[UY * n + k],
-                                      u[BX * n + k], u[BY * n + k], u[BZ * n + k],
-                                      u[PR * n + k], dp));
-            }
-        }
-        block_min_dt(mn, active, rs, red);
     }
 }
 
