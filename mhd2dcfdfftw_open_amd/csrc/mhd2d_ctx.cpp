@@ -1158,9 +1158,11 @@ hipError_t time_launches(hipStream_t s, int reps, F f, float* ms)
 // order) of the derivative solves, then of the filter, is timed as whole RK3
 // steps of this context's own run (grid, precision, batch, settings), each
 // from the same saved state (state, R_prev, run scalars, pending filter),
-// which is restored afterwards.  A shape is judged where it runs: alone, the
-// kernels' own timings ranked the XCD orders differently from the step.  Two
-// interleaved rounds, each candidate keeps its faster mean.
+// which is restored afterwards; where the fused x-derivative + RHS applies,
+// once for each x path, and the faster path keeps its shapes.  A shape is
+// judged where it runs: alone, the kernels' own timings ranked the XCD
+// orders differently from the step.  Two interleaved rounds, each candidate
+// keeps its faster mean.
 template <typename T, typename TD>
 hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
 {
@@ -1204,54 +1206,61 @@ hipError_t autotune_impl(mhd2d_ctx* c, int reps, int* out)
         return r;
     };
     const int xw_f0 = c->xw_f, xr_f0 = c->xr_f;
-    int pick[2][2] = {{0, -1}, {0, -1}};  // [deriv, filter][width, order]
-    for (int phase = 0; phase < 2 && e == hipSuccess; phase++) {
-        std::vector<float> tm(cands.size(), 1e30f);
-        for (size_t it = 0; it < 2 * cands.size() && e == hipSuccess; it++) {
-            const size_t ci = it % cands.size();
-            if (phase == 0) {
-                c->xw_d = cands[ci].w;
-                c->xr_d = cands[ci].r;
-                c->xw_f = xw_f0;
-                c->xr_f = xr_f0;
-            } else {
-                c->xw_d = pick[0][0];
-                c->xr_d = pick[0][1];
-                c->xw_f = cands[ci].w;
-                c->xr_f = cands[ci].r;
-            }
-            float t = 0;
-            e = time_steps(&t);
-            if (e == hipSuccess) tm[ci] = std::min(tm[ci], t);
-        }
+    // shape search for one x path: the derivative shapes (the full solve,
+    // or the carry solve of the fused path), then the filter shapes with
+    // the chosen derivative shape; returns the best step time
+    auto search = [&](int (&pk)[2][2]) {
         float best = 1e30f;
-        for (size_t ci = 0; ci < cands.size(); ci++)
-            if (tm[ci] < best) {
-                best = tm[ci];
-                pick[phase][0] = cands[ci].w;
-                pick[phase][1] = cands[ci].r;
+        for (int phase = 0; phase < 2 && e == hipSuccess; phase++) {
+            std::vector<float> tm(cands.size(), 1e30f);
+            for (size_t it = 0; it < 2 * cands.size() && e == hipSuccess; it++) {
+                const size_t ci = it % cands.size();
+                if (phase == 0) {
+                    c->xw_d = cands[ci].w;
+                    c->xr_d = cands[ci].r;
+                    c->xw_f = xw_f0;
+                    c->xr_f = xr_f0;
+                } else {
+                    c->xw_d = pk[0][0];
+                    c->xr_d = pk[0][1];
+                    c->xw_f = cands[ci].w;
+                    c->xr_f = cands[ci].r;
+                }
+                float t = 0;
+                e = time_steps(&t);
+                if (e == hipSuccess) tm[ci] = std::min(tm[ci], t);
             }
-    }
-    // the fused x-derivative + RHS, where it applies: the faster of the two
-    // (it moves ~13 words per cell and stage less, but its chunk kernel
-    // streams less evenly than the RHS + RK alone; which wins depends on
-    // the grid, the precision and the batch)
+            best = 1e30f;
+            for (size_t ci = 0; ci < cands.size(); ci++)
+                if (tm[ci] < best) {
+                    best = tm[ci];
+                    pk[phase][0] = cands[ci].w;
+                    pk[phase][1] = cands[ci].r;
+                }
+        }
+        return best;
+    };
+    // the fused x-derivative + RHS, where it applies, against the unfused
+    // path, each with its own shapes (the carry solve and the full
+    // derivative solve prefer different strips): the faster pair wins.  The
+    // fused path moves ~13 words per cell and stage less, but its chunk
+    // kernel streams less evenly than the RHS + RK alone; which wins depends
+    // on the grid, the precision and the batch.
     const int fuse0 = c->xfuse;
     int pick_fuse = fuse0;
+    int pick[2][2] = {{0, -1}, {0, -1}};  // [deriv, filter][width, order]
     if (e == hipSuccess && xfuse_ok(c) && ensure_xedge(c) == hipSuccess) {
-        c->xw_d = pick[0][0];
-        c->xr_d = pick[0][1];
-        c->xw_f = pick[1][0];
-        c->xr_f = pick[1][1];
-        float tm[2] = {1e30f, 1e30f};
-        for (int it = 0; it < 4 && e == hipSuccess; it++) {
-            c->xfuse = it % 2;
-            float t = 0;
-            e = time_steps(&t);
-            if (e == hipSuccess) tm[it % 2] = std::min(tm[it % 2], t);
+        int pk[2][2][2] = {{{0, -1}, {0, -1}}, {{0, -1}, {0, -1}}};
+        float tf[2] = {1e30f, 1e30f};
+        for (int f = 0; f < 2 && e == hipSuccess; f++) {
+            c->xfuse = f;
+            tf[f] = search(pk[f]);
         }
-        pick_fuse = tm[1] < tm[0] ? 1 : 0;
+        pick_fuse = tf[1] < tf[0] ? 1 : 0;
+        std::memcpy(pick, pk[pick_fuse], sizeof(pick));
         c->xfuse = pick_fuse;
+    } else {
+        (void)search(pick);
     }
     // instances per launch (batches only): the whole batch, then groups of
     // half as many while a group keeps at least 2^21 cells (fewer leave the

@@ -138,6 +138,18 @@ __device__ __forceinline__ void st_stream(T* p, T v)
 
 // ============================================================== x solves
 constexpr bool xsolve_lds_tables(int C) { return C <= 32; }
+// Factor rows of a line in LDS.  With one periodic block the rows kconv ..
+// nx-2 all hold the factors' FP64 fixed point, so the table keeps rows 0 ..
+// K-1 (K: kconv rounded up to whole chunks of C rows), one chunk of copies of
+// the fixed point and the last chunk's rows, K + 2 C rows instead of nx (at
+// nx = 2048, C = 32: 96 or 160 of 2048 rows); chunk g reads from
+// xtable_base.  Other lines keep all nx rows.
+__host__ __device__ inline int xtable_head(int kconv, int C) { return (kconv + C - 1) / C * C; }
+__host__ __device__ inline int xtable_rows(int nx, int kconv, int C, bool blk)
+{
+    const int K = xtable_head(kconv, C);
+    return (!blk && K + 2 * C < nx) ? K + 2 * C : nx;
+}
 #ifndef MHD2D_X_MINB
 #define MHD2D_X_MINB 1
 #endif
@@ -386,8 +398,8 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs<T, TI>& a, int bx, 
         xcol_load<T, TI, C>(w, f, s, nx, ny);
 
     // Factor rows.  With chunks of up to 32 rows (every line length now) the
-    // three tables of the whole line go to LDS (3 nx words, dynamic) and the
-    // recurrences read them as broadcasts instead of 6 C vector loads per
+    // three tables of the line go to LDS (xtable_rows words each, dynamic)
+    // and the recurrences read them as broadcasts instead of 6 C vector loads per
     // thread (measured -8 % x-solve time at nx = 2048, -11 % at 512, -8 % at
     // 4096 with 128 chunks).  The fill is
     // issued right behind the line's loads, so both wait out one memory
@@ -398,17 +410,24 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs<T, TI>& a, int bx, 
     const FT* ID;
     const FT* Q;
     if constexpr (xsolve_lds_tables(C)) {
+        const int TR = xtable_rows(nx, a.tb.kconv, C, BLK);
+        const int K = xtable_head(a.tb.kconv, C);
+        const bool compact = TR < nx;
         T* const tl = reinterpret_cast<T*>(x_dyn);
-        T* const tid = tl + nx;
-        T* const tqq = tid + nx;
-        for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-            tl[i] = (T)a.tb.l[i];
-            tid[i] = (T)a.tb.invd[i];
-            tqq[i] = (T)a.tb.q[i];
+        T* const tid = tl + TR;
+        T* const tqq = tid + TR;
+        for (int i = threadIdx.x; i < TR; i += blockDim.x) {
+            // row of entry i: the head rows, the fixed point (row nx - 3,
+            // equal bit for bit to rows kconv .. nx - 2), the last chunk
+            const int r = (!compact || i < K) ? i : (i < K + C ? nx - 3 : i + nx - 2 * C - K);
+            tl[i] = (T)a.tb.l[r];
+            tid[i] = (T)a.tb.invd[r];
+            tqq[i] = (T)a.tb.q[r];
         }
-        L = tl + s;
-        ID = tid + s;
-        Q = tqq + s;
+        const int base = !compact ? s : (g == G - 1 ? K + C : (s >= K ? K : s));
+        L = tl + base;
+        ID = tid + base;
+        Q = tqq + base;
     } else {
         L = a.tb.l + s;
         ID = a.tb.invd + s;
@@ -594,7 +613,8 @@ void xsolve_shape(int nx, int elem_bytes, int* chunk, int* width, int* groups)
 template <typename T, typename TI, int C, int W, bool BLK>
 static void launch_xsolve_kind(const XSolveArgs<T, TI>& a, dim3 grid, dim3 block, hipStream_t s)
 {
-    const size_t shm = xsolve_lds_tables(C) ? 3 * (size_t)a.nx * sizeof(T) : 0;
+    const size_t shm =
+        xsolve_lds_tables(C) ? 3 * (size_t)xtable_rows(a.nx, a.tb.kconv, C, BLK) * sizeof(T) : 0;
     if constexpr (!BLK) {
         if (a.carry) {
             hipLaunchKernelGGL((k_xsolve<T, TI, C, W, XD1_CARRY, false>), grid, block, shm, s, a);
