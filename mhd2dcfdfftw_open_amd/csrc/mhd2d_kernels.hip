@@ -1560,8 +1560,9 @@ constexpr int xrhs_fs(int C)
 
 // One workgroup of the fused kernel: column tile bx, chunk g, instance z;
 // sdx holds at least XRhsShape<TD>::NV * xrhs_fs<T>(C) words (16-byte
-// aligned), stab 3 C
-template <typename T, typename TD, int C>
+// aligned), stab 3 C.  EARLY: the RHS phase's streaming inputs are requested
+// before the chunk solves.
+template <typename T, typename TD, int C, bool EARLY>
 __device__ __forceinline__ void xrhs_body(const XRhsArgs<T, TD>& a, int bx, int g, int z,
                                           TD* __restrict__ sdx, TD* __restrict__ stab)
 {
@@ -1594,9 +1595,10 @@ __device__ __forceinline__ void xrhs_body(const XRhsArgs<T, TD>& a, int bx, int 
     const bool use_prev = ra.stage > 0;
     Pack<T, PW> u[8];
     Pack<TD, PW> dy[8], ey[2], rp[8];
-    // the RHS phase's streaming inputs (y-derivatives, R_prev) are requested
-    // before the chunk solves, so that their latency overlaps the solves'
-    // own loads and arithmetic (MHD2D_XRHS_PREFETCH 2: the state as well)
+    // EARLY: the RHS phase's streaming inputs (y-derivatives, R_prev) are
+    // requested before the chunk solves, so that their latency overlaps the
+    // solves' own loads and arithmetic (MHD2D_XRHS_PREFETCH 2: the state as
+    // well)
 #ifndef MHD2D_XRHS_PREFETCH
 #define MHD2D_XRHS_PREFETCH 0
 #endif
@@ -1614,7 +1616,8 @@ __device__ __forceinline__ void xrhs_body(const XRhsArgs<T, TD>& a, int bx, int 
 #pragma unroll
         for (int f = 0; f < 8; f++) u[f] = ldp<PW>(uz + f * n + p);
     };
-    if constexpr (MHD2D_XRHS_PREFETCH >= 1) load_streams();
+    constexpr bool PF1 = EARLY || MHD2D_XRHS_PREFETCH >= 1;
+    if constexpr (PF1) load_streams();
     if constexpr (MHD2D_XRHS_PREFETCH >= 2) load_state();
 
     // the chunk's rows (with halos) and its two carries, for every column
@@ -1658,7 +1661,7 @@ __device__ __forceinline__ void xrhs_body(const XRhsArgs<T, TD>& a, int bx, int 
     __syncthreads();
 
     // RHS + RK
-    if constexpr (MHD2D_XRHS_PREFETCH < 1) load_streams();
+    if constexpr (!PF1) load_streams();
     if constexpr (MHD2D_XRHS_PREFETCH < 2) load_state();
     const T cc = (T)rs->cc[ra.stage];
     const T dd = (T)rs->dd[ra.stage];
@@ -1721,7 +1724,13 @@ __global__ void __launch_bounds__(256) k_xrhs_rk(XRhsArgs<T, TD> a)
     __shared__ TD stab[3 * C];
     // instances last to first, as k_rhs_rk (the carry solve before it runs
     // them first to last)
-    xrhs_body<T, TD, C>(a, blockIdx.x, blockIdx.y, (int)(gridDim.z - 1 - blockIdx.z), sdx, stab);
+    // FP64: the streaming inputs early (2048^2 -2.7 % kernel time, the
+    // uneven ensemble's per-group step -2 %); the mixed and FP32 instances
+    // and the multi-shape kernel measured slower with them early (more
+    // VGPRs, fewer waves)
+    constexpr bool EARLY = sizeof(T) == 8 && sizeof(TD) == 8;
+    xrhs_body<T, TD, C, EARLY>(a, blockIdx.x, blockIdx.y, (int)(gridDim.z - 1 - blockIdx.z), sdx,
+                               stab);
 }
 
 // calcCharacteristicsX (calcRHS.c:253-495) at one open boundary cell:
@@ -2335,9 +2344,9 @@ __global__ void __launch_bounds__(256) k_xrhs_rk_multi(const XRhsArgs<T, TD>* __
     const int z = m.nb[g] - 1 - local / per;
     const int bx = (local % per) % tiles, ch = (local % per) / tiles;
     switch (m.code[g]) {
-    case 8: xrhs_body<T, TD, 8>(tab[g], bx, ch, z, sdx, stab); break;
-    case 16: xrhs_body<T, TD, 16>(tab[g], bx, ch, z, sdx, stab); break;
-    case 32: xrhs_body<T, TD, 32>(tab[g], bx, ch, z, sdx, stab); break;
+    case 8: xrhs_body<T, TD, 8, false>(tab[g], bx, ch, z, sdx, stab); break;
+    case 16: xrhs_body<T, TD, 16, false>(tab[g], bx, ch, z, sdx, stab); break;
+    case 32: xrhs_body<T, TD, 32, false>(tab[g], bx, ch, z, sdx, stab); break;
     default: break;
     }
 }
