@@ -142,8 +142,9 @@ constexpr bool xsolve_lds_tables(int C) { return C <= 32; }
 // nx-2 all hold the factors' FP64 fixed point, so the table keeps rows 0 ..
 // K-1 (K: kconv rounded up to whole chunks of C rows), one chunk of copies of
 // the fixed point and the last chunk's rows, K + 2 C rows instead of nx (at
-// nx = 2048, C = 32: 96 or 160 of 2048 rows); chunk g reads from
-// xtable_base.  Other lines keep all nx rows.
+// nx = 2048, C = 32: 96 or 160 of 2048 rows; xsolve_body maps each chunk to
+// its rows).  Lines with several x blocks, and lines too short for the
+// compact table to be smaller, keep all nx rows.
 __host__ __device__ inline int xtable_head(int kconv, int C) { return (kconv + C - 1) / C * C; }
 __host__ __device__ inline int xtable_rows(int nx, int kconv, int C, bool blk)
 {
