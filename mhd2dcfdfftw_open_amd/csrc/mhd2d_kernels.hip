@@ -1194,8 +1194,10 @@ __device__ __forceinline__ void rhs_cell(const T* __restrict__ u, const T* __res
     rho_out = rho;
 }
 
+// 256-thread launches: without the bound the compiler assumed 1024 threads
+// and capped the kernel at 128 VGPRs (6 spilled)
 template <typename T>
-__global__ void k_rhs(RhsArgs<T> a)
+__global__ void __launch_bounds__(256) k_rhs(RhsArgs<T> a)
 {
     const size_t n = a.n;
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
