@@ -1727,11 +1727,12 @@ __global__ void __launch_bounds__(256) k_xrhs_rk(XRhsArgs<T, TD> a)
     __shared__ TD stab[3 * C];
     // instances last to first, as k_rhs_rk (the carry solve before it runs
     // them first to last)
-    // FP64: the streaming inputs early (2048^2 -2.7 % kernel time, the
-    // uneven ensemble's per-group step -2 %); the mixed and FP32 instances
-    // and the multi-shape kernel measured slower with them early (more
-    // VGPRs, fewer waves)
-    constexpr bool EARLY = sizeof(T) == 8 && sizeof(TD) == 8;
+    // FP64 with 32-row chunks (lines of >= 1024 rows): the streaming inputs
+    // early (2048^2 -2.7 % kernel time, the uneven ensemble's per-group step
+    // -2 %); the mixed and FP32 instances, shorter chunks (the 64 x 512^2
+    // batch: +6.6 % per step) and the multi-shape kernel measured slower
+    // with them early (more VGPRs, fewer waves)
+    constexpr bool EARLY = sizeof(T) == 8 && sizeof(TD) == 8 && C == 32;
     xrhs_body<T, TD, C, EARLY>(a, blockIdx.x, blockIdx.y, (int)(gridDim.z - 1 - blockIdx.z), sdx,
                                stab);
 }
