@@ -857,8 +857,10 @@ __device__ __forceinline__ void ysweep_body(const YSweepArgs<T, TD>& a, int bx, 
             nxt[m] = cx<T>{u[PA[0] * n + row + k], u[PB[0] * n + row + k]};
         }
         TD* const lapz = a.lap ? a.lap + z * a.bs_l : nullptr;
-#pragma unroll 1
-        for (int p = 0; p < (KEEPB ? npairs - 1 : npairs); p++) {
+        // one field pair; NEXT: load the next pair during the last transform
+        // (a compile-time flag, so the loop body has no conditional load)
+        auto pair = [&](int p, auto next_tag) {
+            constexpr bool NEXT = decltype(next_tag)::value;
             const int fa = PA[p], fb = PB[p];
             const bool two = !GLM || fb >= 0;
             T* ua = u + fa * n + row;
@@ -924,7 +926,7 @@ __device__ __forceinline__ void ysweep_body(const YSweepArgs<T, TD>& a, int bx, 
 #pragma unroll
                 for (int m = 0; m < E; m++) v[m] = mul_i_scal(Z[m], (T)m1[tq + m * TPR]);
             }
-            if (p + 1 < npairs) {
+            if (NEXT) {
                 const T* na = u + PA[p + 1] * n + row;
                 const bool two_n = !GLM || PB[p + 1] >= 0;
                 const T* nb = u + (two_n ? PB[p + 1] : PA[p + 1]) * n + row;
@@ -947,7 +949,13 @@ __device__ __forceinline__ void ysweep_body(const YSweepArgs<T, TD>& a, int bx, 
                     }
                 }
             }
-        }
+        };
+#pragma unroll 1
+        for (int p = 0; p < npairs - 1; p++) pair(p, std::true_type{});
+        // the last pair: with the Laplacian terms or the GLM line, the loop
+        // body without the next-pair load (ideal MHD keeps its (Bx, By)
+        // values, below)
+        if constexpr (!KEEPB) pair(npairs - 1, std::false_type{});
         if constexpr (KEEPB) {
             // the last pair (Bx, By): the same operations as the loop body
             // without the Laplacian terms and with no next pair to load
