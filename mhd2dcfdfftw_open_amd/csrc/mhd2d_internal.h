@@ -48,7 +48,7 @@ struct XTables {
     const double* q;      // [nx]
     const double* pf;     // [G] product of -l over each chunk
     const double* qb;     // [G] product of q over each chunk
-    int kconv;            // l, invd, q equal (bitwise) their fixed point on rows kconv..nx-2
+    int kconv;            // l, invd, q equal (bitwise) their fixed point on rows kconv..nx-3
     double linf, dinf, qinf;
     // x split into blocks solved separately (the reference's MPI ranks),
     // possibly with open x ends: rows whose right-hand side is not the

@@ -139,7 +139,7 @@ __device__ __forceinline__ void st_stream(T* p, T v)
 // ============================================================== x solves
 constexpr bool xsolve_lds_tables(int C) { return C <= 32; }
 // Factor rows of a line in LDS.  With one periodic block the rows kconv ..
-// nx-2 all hold the factors' FP64 fixed point, so the table keeps rows 0 ..
+// nx-3 all hold the factors' FP64 fixed point, so the table keeps rows 0 ..
 // K-1 (K: kconv rounded up to whole chunks of C rows), one chunk of copies of
 // the fixed point and the last chunk's rows, K + 2 C rows instead of nx (at
 // nx = 2048, C = 32: 96 or 160 of 2048 rows; xsolve_body maps each chunk to
@@ -357,8 +357,8 @@ __device__ __forceinline__ void xcol_backward(T (&w)[C + 6], T x, bool interior,
     }
 }
 
-// Rows from tb.kconv to N-2 carry the factors' exact FP64 fixed point, so a
-// chunk inside that range uses the scalar constants (no loads).  (FP32 only:
+// Rows from tb.kconv to N-3 carry the factors' exact FP64 fixed point, so a
+// chunk inside that range (never the last chunk) uses the scalar constants (no loads).  (FP32 only:
 // in FP64 the second code path costs more registers than the table loads it
 // saves)
 #ifndef MHD2D_X_CONST
@@ -419,7 +419,7 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs<T, TI>& a, int bx, 
         T* const tqq = tid + TR;
         for (int i = threadIdx.x; i < TR; i += blockDim.x) {
             // row of entry i: the head rows, the fixed point (row nx - 3,
-            // equal bit for bit to rows kconv .. nx - 2), the last chunk
+            // equal bit for bit to rows kconv .. nx - 3), the last chunk
             const int r = (!compact || i < K) ? i : (i < K + C ? nx - 3 : i + nx - 2 * C - K);
             tl[i] = (T)a.tb.l[r];
             tid[i] = (T)a.tb.invd[r];

@@ -340,8 +340,10 @@ hipError_t GridTables::build(int nx_, int ny_, double Lx_, double Ly_, int elem_
     const double* d = (const double*)dst;
     for (int k = 0; k < NXKIND; k++) {
         // first row from which l, invd and q stay bitwise equal to their
-        // values at row nx-3 up to row nx-2 (the Thomas factors' FP64 fixed
-        // point: row 25 for the derivatives, 87 for the filter)
+        // values at row nx-3 up to that row (the Thomas factors' FP64 fixed
+        // point: row 25 for the derivatives, 87 for the filter); the kernels
+        // take the fixed point only in chunks other than the last, whose rows
+        // end before nx-3
         const double* L = p + off_l[k];
         const double* ID = p + off_id[k];
         const double* Q = p + off_q[k];
