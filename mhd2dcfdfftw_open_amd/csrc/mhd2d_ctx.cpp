@@ -1493,7 +1493,8 @@ namespace {
 // group does not fit it (cleaning, open x, x blocks, GEMM x-derivative, a
 // line length outside the multi-shape kernels, or groups that differ in
 // having the Laplacian terms).  Groups go largest first, so the longest
-// blocks start first.
+// blocks start first (the x-solves take the groups in reverse order, to
+// start on the lines the kernel before them touched last).
 template <typename T, typename TD>
 hipError_t build_multi_t(mhd2d_ens* e, bool* ok)
 {
