@@ -706,27 +706,34 @@ hipError_t launch_xsolve(const XSolveArgs<T, TI>& a, int nv, hipStream_t s, int 
 }
 
 // ============================================================== y operators
-// elements per thread of the row FFTs: 8 keeps an FP64 line plus its
-// spectrum within ~100 VGPRs; FP32 lines take 16
+// elements per thread of the row FFTs.  FP64 lines up to 2048 points: 8
+// (a line plus its spectrum within ~100 VGPRs, 4 waves per SIMD; 4 elements
+// and 6 waves measured +30 % y-sweep time at 2048^2, 16 elements and 2
+// waves +11 %); 4096-point FP64 lines: 16 (2 LDS exchanges per transform
+// instead of 3, at 2 waves per SIMD: -6.5 % y-sweep time at 4096^2).
+// FP32 lines take 16.
 #ifndef MHD2D_Y_E64
 #define MHD2D_Y_E64 8
 #endif
-template <typename T>
+#ifndef MHD2D_Y_E64_4096
+#define MHD2D_Y_E64_4096 16
+#endif
+template <typename T, int N>
 struct YElems {
-    static constexpr int E = sizeof(T) == 8 ? MHD2D_Y_E64 : 16;
+    static constexpr int E = sizeof(T) == 8 ? (N >= 4096 ? MHD2D_Y_E64_4096 : MHD2D_Y_E64) : 16;
 };
 
 // occupancy request for the fused y-sweep: 4 waves per SIMD (128 VGPRs, a
 // few spills) measured -17 % y-sweep time at 4096^2 and neutral at 2048^2
-// (0: compiler's choice)
+// with 8 elements per thread; 2 waves (256 VGPRs) with 16 FP64 elements
 #ifndef MHD2D_Y_WAVES
 #define MHD2D_Y_WAVES 4
 #endif
-#if MHD2D_Y_WAVES > 0
-#define MHD2D_Y_ATTR __attribute__((amdgpu_waves_per_eu(MHD2D_Y_WAVES)))
-#else
-#define MHD2D_Y_ATTR
-#endif
+template <typename T, int N>
+constexpr int ysweep_waves()
+{
+    return (sizeof(T) == 8 && YElems<T, N>::E == 16) ? 2 : MHD2D_Y_WAVES;
+}
 
 // threads of a y-sweep workgroup for lines shorter than that many threads
 // (several rows per workgroup): 256 measured within 1.4 % of 128 on the
@@ -736,7 +743,7 @@ struct YElems {
 #endif
 template <typename T, int N>
 struct YShape {
-    static constexpr int E = YElems<T>::E;
+    static constexpr int E = YElems<T, N>::E;
     static constexpr int TPR = FFTShape<N, E>::TPR;
     static constexpr int ROWS = (TPR >= MHD2D_Y_THREADS) ? 1 : MHD2D_Y_THREADS / TPR;
     static constexpr int THREADS = TPR * ROWS;
@@ -1064,7 +1071,8 @@ __device__ __forceinline__ void ysweep_body(const YSweepArgs<T, TD>& a, int bx, 
 }
 
 template <typename T, typename TD, int N, int MODE>
-__global__ void __launch_bounds__((YShape<T, N>::THREADS)) MHD2D_Y_ATTR k_ysweep(YSweepArgs<T, TD> a)
+__global__ void __launch_bounds__((YShape<T, N>::THREADS))
+    __attribute__((amdgpu_waves_per_eu(ysweep_waves<T, N>()))) k_ysweep(YSweepArgs<T, TD> a)
 {
     using Y = YShape<T, N>;
     __shared__ T s_re[Y::ROWS * Y::LDSW];
@@ -2242,20 +2250,29 @@ constexpr int ymulti_lds()
     return m;
 }
 
+// rows per workgroup of the multi-shape y-sweep, 0 where the line length
+// does not fit its MHD2D_Y_THREADS threads
+template <int N>
+static int ymulti_rows(int elem_bytes)
+{
+    if (elem_bytes == 8) return ymulti_ok<double, N>() ? YShape<double, N>::ROWS : 0;
+    return ymulti_ok<float, N>() ? YShape<float, N>::ROWS : 0;
+}
+
 int ysweep_multi_rows(int ny, int elem_bytes)
 {
     switch (ny) {
-    case 128: return elem_bytes == 8 ? YShape<double, 128>::ROWS : YShape<float, 128>::ROWS;
-    case 256: return elem_bytes == 8 ? YShape<double, 256>::ROWS : YShape<float, 256>::ROWS;
-    case 512: return elem_bytes == 8 ? YShape<double, 512>::ROWS : YShape<float, 512>::ROWS;
-    case 1024: return elem_bytes == 8 ? YShape<double, 1024>::ROWS : YShape<float, 1024>::ROWS;
-    case 2048: return elem_bytes == 8 ? 0 : YShape<float, 2048>::ROWS;
+    case 128: return ymulti_rows<128>(elem_bytes);
+    case 256: return ymulti_rows<256>(elem_bytes);
+    case 512: return ymulti_rows<512>(elem_bytes);
+    case 1024: return ymulti_rows<1024>(elem_bytes);
+    case 2048: return ymulti_rows<2048>(elem_bytes);
     default: return 0;
     }
 }
 
 template <typename T, typename TD, int MODE>
-__global__ void __launch_bounds__(MHD2D_Y_THREADS) MHD2D_Y_ATTR
+__global__ void __launch_bounds__(MHD2D_Y_THREADS) __attribute__((amdgpu_waves_per_eu(MHD2D_Y_WAVES)))
     k_ysweep_multi(const YSweepArgs<T, TD>* __restrict__ tab, MultiMap m)
 {
     __shared__ T s_re[ymulti_lds<T>()];
@@ -2268,11 +2285,21 @@ __global__ void __launch_bounds__(MHD2D_Y_THREADS) MHD2D_Y_ATTR
     const int bx = local % m.per[g];
     // instances last to first, as in k_ysweep
     const int z = m.nb[g] - 1 - local / m.per[g];
+    // only the line lengths ysweep_multi_rows admits (a workgroup of
+    // MHD2D_Y_THREADS threads, shared memory sized for them)
     switch (a.ny) {
-    case 128: ysweep_body<T, TD, 128, MODE>(a, bx, z, s_re, s_im, red); break;
-    case 256: ysweep_body<T, TD, 256, MODE>(a, bx, z, s_re, s_im, red); break;
-    case 512: ysweep_body<T, TD, 512, MODE>(a, bx, z, s_re, s_im, red); break;
-    case 1024: ysweep_body<T, TD, 1024, MODE>(a, bx, z, s_re, s_im, red); break;
+    case 128:
+        if constexpr (ymulti_ok<T, 128>()) ysweep_body<T, TD, 128, MODE>(a, bx, z, s_re, s_im, red);
+        break;
+    case 256:
+        if constexpr (ymulti_ok<T, 256>()) ysweep_body<T, TD, 256, MODE>(a, bx, z, s_re, s_im, red);
+        break;
+    case 512:
+        if constexpr (ymulti_ok<T, 512>()) ysweep_body<T, TD, 512, MODE>(a, bx, z, s_re, s_im, red);
+        break;
+    case 1024:
+        if constexpr (ymulti_ok<T, 1024>()) ysweep_body<T, TD, 1024, MODE>(a, bx, z, s_re, s_im, red);
+        break;
     case 2048:
         if constexpr (ymulti_ok<T, 2048>()) ysweep_body<T, TD, 2048, MODE>(a, bx, z, s_re, s_im, red);
         break;
