@@ -54,12 +54,15 @@ CASES = [
     (128, 32, 3, False, 0.0, 0.0, 2, 1, 5),
     (64, 32, 2, True, 0.0, 0.0, 2, 0, 6),
     (128, 32, 3, True, 2e-3, 1e-3, 2, 1, 5),
-    # longer y-lines: the y-sweep's Laplacian and GLM instances at 512, 1024
-    # and 2048 points (one row per workgroup at 2048)
+    # longer y-lines: the y-sweep's Laplacian and GLM instances at 512, 1024,
+    # 2048 and 4096 points (one row per workgroup from 2048 on; 16 elements
+    # per thread at 4096)
     (64, 512, 1, False, 2e-3, 1e-3, 0, 1, 4),
     (32, 2048, 1, False, 1e-3, 1e-3, 0, 0, 3),
     (32, 1024, 1, False, 0.0, 0.0, 2, 1, 3),
     (32, 2048, 1, False, 1e-3, 1e-3, 2, 0, 3),
+    (32, 4096, 1, False, 1e-3, 1e-3, 0, 1, 3),
+    (32, 4096, 1, False, 1e-3, 1e-3, 2, 0, 3),
 ]
 
 

@@ -100,3 +100,24 @@ def test_mixed_small_amplitude_wave_within_1e6(m, seed):
     e = field_errors(mix.state(), ref.state())
     print(f"[mixed] wave: max rel-L2 {e.max():.2e}")
     assert e.max() <= MIXED_TOL, e
+
+
+@pytest.mark.parametrize("ny,clean", [(2048, 0), (4096, 0), (4096, 2)])
+def test_mixed_dissipative_long_lines(m, ny, clean):
+    # the y-sweep's mixed Laplacian and GLM instances on 2048- and 4096-point
+    # lines against the FP64 device path (held to the oracle per step on
+    # the same lines): the 8 MHD fields within the mixed-form tolerance
+    nx, steps = 32, 20
+    print(f"[mixed] {nx}x{ny} viscous + resistive, cleaning {clean}, {steps} steps", flush=True)
+    u0 = m.turbulence(nx, ny, seed=7, kmax=4, rms=0.1)
+    if clean == 2:
+        x = np.arange(nx) * 2 * np.pi / nx
+        u0 = u0.copy()
+        u0[4] += 0.01 * np.sin(2 * x)[:, None]
+        u0 = np.concatenate([u0, np.zeros((1, nx, ny))])
+    kw = dict(hall=1, di=0.05, viscosity=1e-3, resistivity=1e-3, divbclean=clean)
+    ref = _run(m, u0, nx, ny, m.FP64, steps, **kw)
+    mix = _run(m, u0, nx, ny, m.MIXED, steps, **kw)
+    e = field_errors(mix.state()[:8], ref.state()[:8])
+    print(f"[mixed] {nx}x{ny} cleaning {clean}: max rel-L2 {e.max():.2e}")
+    assert e.max() <= MIXED_TOL, e
