@@ -922,6 +922,14 @@ def main():
                 "cell_updates_per_s": v, "ms_per_step": ms_p / args.steps,
                 "vs_ideal_fp64": v / value, "kernels": kern}
             c.close()
+        # the reference's own CPU-runnable case, the CPU baseline's workload:
+        # 256 x 256 Orszag-Tang with the Hall CFL term (d_i = 0.05), 200 steps
+        ot = m.orszag_tang(256, 256)
+        v, ms_p, c = run_single(m, ot, 256, 256, "fp64", 200, args.warmup, graph, 1, 0.05)
+        forms["orszag_tang_256_fp64"] = {
+            "workload": "256x256 Orszag-Tang, d_i = 0.05, 200 RK3 steps, fp64",
+            "cell_updates_per_s": v, "ms_per_step": ms_p / 200}
+        c.close()
         forms["dropin_fp64_" + str(nx)] = dropin_form(m, u0, nx, ny, hall, di, value)
         forms.update(batched_forms(m, value, args, copy_gbps))
         forms.update(uneven_forms(m, value, args))
@@ -929,6 +937,14 @@ def main():
         out["forms"] = forms
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nx, ny, u0, hall, di)
+        ot_form = out.get("forms", {}).get("orszag_tang_256_fp64")
+        if ot_form:
+            # the GPU on the CPU baseline's own workload (a reported ratio)
+            g = ot_form["cell_updates_per_s"]
+            out["cpu_baseline"]["gpu_same_workload"] = {
+                "value": g, "workload": ot_form["workload"],
+                "vs_threads_n": g / out["cpu_baseline"]["value"],
+                "vs_threads_1": g / out["cpu_baseline"]["threads_1"]["value"]}
     print(json.dumps(out))
 
 
