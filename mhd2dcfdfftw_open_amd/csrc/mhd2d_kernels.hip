@@ -741,11 +741,14 @@ constexpr int ysweep_waves()
 #ifndef MHD2D_Y_THREADS
 #define MHD2D_Y_THREADS 128
 #endif
-template <typename T, int N>
+#ifndef MHD2D_Y_SMALL_GRID
+#define MHD2D_Y_SMALL_GRID 1024
+#endif
+template <typename T, int N, int THR = MHD2D_Y_THREADS>
 struct YShape {
     static constexpr int E = YElems<T, N>::E;
     static constexpr int TPR = FFTShape<N, E>::TPR;
-    static constexpr int ROWS = (TPR >= MHD2D_Y_THREADS) ? 1 : MHD2D_Y_THREADS / TPR;
+    static constexpr int ROWS = (TPR >= THR) ? 1 : THR / TPR;
     static constexpr int THREADS = TPR * ROWS;
     static constexpr int LDSW = FFTShape<N, E>::LDS;
 };
@@ -810,13 +813,13 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS)) k_yapply(YApplyArgs<T
 // the GLM psi line (separate instances, so each keeps its own registers)
 // One workgroup of the y-sweep: rows bx * ROWS .. of instance z; s_re and
 // s_im hold ROWS * LDSW words each, red 16 doubles.
-template <typename T, typename TD, int N, int MODE>
+template <typename T, typename TD, int N, int MODE, int THR = MHD2D_Y_THREADS>
 __device__ __forceinline__ void ysweep_body(const YSweepArgs<T, TD>& a, int bx, int z,
                                             T* __restrict__ s_re, T* __restrict__ s_im,
                                             double* red)
 {
     constexpr bool LAP = MODE >= 1, GLM = MODE == 2;
-    using Y = YShape<T, N>;
+    using Y = YShape<T, N, THR>;
     constexpr int TPR = Y::TPR;
     constexpr int E = Y::E;
     RunScalars* const rs = a.rs + z;
@@ -1070,11 +1073,11 @@ __device__ __forceinline__ void ysweep_body(const YSweepArgs<T, TD>& a, int bx, 
     }
 }
 
-template <typename T, typename TD, int N, int MODE>
-__global__ void __launch_bounds__((YShape<T, N>::THREADS))
+template <typename T, typename TD, int N, int MODE, int THR>
+__global__ void __launch_bounds__((YShape<T, N, THR>::THREADS))
     __attribute__((amdgpu_waves_per_eu(ysweep_waves<T, N>()))) k_ysweep(YSweepArgs<T, TD> a)
 {
-    using Y = YShape<T, N>;
+    using Y = YShape<T, N, THR>;
     __shared__ T s_re[Y::ROWS * Y::LDSW];
     __shared__ T s_im[Y::ROWS * Y::LDSW];
     __shared__ double red[16];
@@ -1083,7 +1086,7 @@ __global__ void __launch_bounds__((YShape<T, N>::THREADS))
     // still holds (with the same order in k_rhs_rk, -4 % step time for
     // 2 x 2048^2 and 8 x 1024^2 batches; the results do not depend on it)
     const int z = (int)(gridDim.y - 1 - blockIdx.y);
-    ysweep_body<T, TD, N, MODE>(a, blockIdx.x, z, s_re, s_im, red);
+    ysweep_body<T, TD, N, MODE, THR>(a, blockIdx.x, z, s_re, s_im, red);
 }
 
 template <typename T, typename TD, int N>
@@ -1099,12 +1102,31 @@ static hipError_t launch_y_n(const YApplyArgs<T>* ya, const YSweepArgs<T, TD>* y
             return hipGetLastError();
         }
     }
+    // Short lines on a small grid (fewer than MHD2D_Y_SMALL_GRID workgroups
+    // of MHD2D_Y_THREADS threads): 64-thread workgroups, half the rows each,
+    // twice as many to spread over the CUs (256^2: -9 % step time, the
+    // y-sweep 0.031 -> 0.027 ms; larger grids keep the wider workgroups:
+    // the 64 x 512^2 batch took +3 % with 64 threads).  Each row's
+    // arithmetic is the same, so are the results.
+    using Ys = YShape<T, N, 64>;
+    if constexpr (N >= 128 && Ys::ROWS != Y::ROWS) {
+        if ((size_t)grid.x * nb < MHD2D_Y_SMALL_GRID) {
+            const dim3 gs((nx + Ys::ROWS - 1) / Ys::ROWS, nb), bs(Ys::THREADS);
+            if (ys->glm)
+                hipLaunchKernelGGL((k_ysweep<T, TD, N, 2, 64>), gs, bs, 0, s, *ys);
+            else if (ys->lap)
+                hipLaunchKernelGGL((k_ysweep<T, TD, N, 1, 64>), gs, bs, 0, s, *ys);
+            else
+                hipLaunchKernelGGL((k_ysweep<T, TD, N, 0, 64>), gs, bs, 0, s, *ys);
+            return hipGetLastError();
+        }
+    }
     if (ys->glm)
-        hipLaunchKernelGGL((k_ysweep<T, TD, N, 2>), grid, block, 0, s, *ys);
+        hipLaunchKernelGGL((k_ysweep<T, TD, N, 2, MHD2D_Y_THREADS>), grid, block, 0, s, *ys);
     else if (ys->lap)
-        hipLaunchKernelGGL((k_ysweep<T, TD, N, 1>), grid, block, 0, s, *ys);
+        hipLaunchKernelGGL((k_ysweep<T, TD, N, 1, MHD2D_Y_THREADS>), grid, block, 0, s, *ys);
     else
-        hipLaunchKernelGGL((k_ysweep<T, TD, N, 0>), grid, block, 0, s, *ys);
+        hipLaunchKernelGGL((k_ysweep<T, TD, N, 0, MHD2D_Y_THREADS>), grid, block, 0, s, *ys);
     return hipGetLastError();
 }
 
